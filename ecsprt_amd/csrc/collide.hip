@@ -1,0 +1,323 @@
+// Elastic ion-neutral collisions, ScatterElastic (src/collisions.c:9-167), on the GPU.
+//
+// Pre-drawn mode reproduces the reference's serial loop exactly: attempts are
+// processed in attempt order per particle, and the loop's stop at the Nc-th
+// success becomes a global cut found by a scan over per-attempt outcomes.
+//   1. ip of every attempt; group attempts by ip (counting sort, lists sorted by
+//      attempt index).
+//   2. pass 1: each particle replays its attempts in order and records each
+//      attempt's outcome (rejected / collided / refused).
+//   3. exclusive scans of the outcomes give the cut: the attempt after the Nc-th
+//      success.  Ten refusals before the cut is the reference's Abort.
+//   4. pass 2 replays the attempts below the cut and writes the velocities.
+#include <algorithm>
+
+#include "common.hpp"
+#include "philox.hpp"
+#include "pic_math.hpp"
+
+namespace ecsprt {
+
+CollideConsts collide_consts_of(const ecsprt_params &p);
+
+// x86-64 double -> int32 (cvttsd2si): out of range or NaN gives INT_MIN.
+static int x86_double_to_int(double v) {
+  if (!(v > -2147483649.0 && v < 2147483648.0)) return (int)0x80000000u;
+  return (int)v;
+}
+
+// Nc, src/collisions.c:48-63.
+int collision_count(const ecsprt_params *p, long np, double dt) {
+  double fc = p->nui * dt;
+  double tmp = (p->vi0x - p->vn0x) * (p->vi0x - p->vn0x) + (p->vi0y - p->vn0y) * (p->vi0y - p->vn0y) +
+               (p->vi0z - p->vn0z) * (p->vi0z - p->vn0z);
+  double vrm = 4.0 * p->viT + sqrt(tmp);
+  int Nc = x86_double_to_int(((double)np * fc * vrm) / ((p->vnT + p->vn0) * sqrt(p->mn / p->mi)));
+  if (Nc > np) Nc = (int)np;
+  else if (Nc < 0) Nc = 0;
+  return Nc;
+}
+
+// ---- exclusive scan (int -> long long), 1024 elements per block --------------
+__global__ void k_scan_local(const int *in, long long *out, long long *bsum, long n) {
+  __shared__ long long wsum[16];
+  const long base = (long)blockIdx.x * 1024;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  long long v[4], s = 0;
+  for (int q = 0; q < 4; ++q) {
+    long i = base + tid * 4 + q;
+    v[q] = (i < n) ? in[i] : 0;
+    s += v[q];
+  }
+  long long x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    long long y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wid] = x;
+  __syncthreads();
+  if (tid == 0) {
+    long long acc = 0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) {
+      long long t = wsum[w];
+      wsum[w] = acc;
+      acc += t;
+    }
+    bsum[blockIdx.x] = acc;
+  }
+  __syncthreads();
+  long long e = x - s + wsum[wid];
+  for (int q = 0; q < 4; ++q) {
+    long i = base + tid * 4 + q;
+    if (i < n) out[i] = e;
+    e += v[q];
+  }
+}
+
+__global__ void k_scan_add(long long *out, const long long *boff, long n) {
+  const long base = (long)blockIdx.x * 1024;
+  const long long add = boff[blockIdx.x];
+  for (int q = threadIdx.x; q < 1024; q += blockDim.x) {
+    long i = base + q;
+    if (i < n) out[i] += add;
+  }
+}
+
+__global__ void k_i64_to_i32(const long long *in, int *out, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = (int)in[i];
+}
+
+// out[i] = sum_{j<i} in[j]; returns nothing, total available as out[n-1]+in[n-1].
+static void scan_exclusive(ecsprt_hip_ctx *c, const int *in, long long *out, long n) {
+  if (n <= 0) return;
+  const long nb = (n + 1023) / 1024;
+  long long *bsum = nullptr, *boff = nullptr;
+  ECS_HIP(hipMallocAsync((void **)&bsum, sizeof(long long) * nb, c->stream));
+  k_scan_local<<<nb, 256, 0, c->stream>>>(in, out, bsum, n);
+  ECS_HIP(hipGetLastError());
+  if (nb > 1) {
+    ECS_HIP(hipMallocAsync((void **)&boff, sizeof(long long) * nb, c->stream));
+    int *bsum32 = nullptr;
+    ECS_HIP(hipMallocAsync((void **)&bsum32, sizeof(int) * nb, c->stream));
+    k_i64_to_i32<<<(nb + 255) / 256, 256, 0, c->stream>>>(bsum, bsum32, nb);
+    scan_exclusive(c, bsum32, boff, nb);
+    k_scan_add<<<nb, 256, 0, c->stream>>>(out, boff, n);
+    ECS_HIP(hipGetLastError());
+    ECS_HIP(hipFreeAsync(bsum32, c->stream));
+    ECS_HIP(hipFreeAsync(boff, c->stream));
+  }
+  ECS_HIP(hipFreeAsync(bsum, c->stream));
+}
+
+// ---- pre-drawn kernels --------------------------------------------------------
+__global__ void k_att_count(const ecsprt_attempt *att, long na, long np, int *ipa, int *cnt) {
+  for (long a = blockIdx.x * (long)blockDim.x + threadIdx.x; a < na; a += (long)gridDim.x * blockDim.x) {
+    int ip = (int)pick_ip(np, att[a].u_ip);
+    ipa[a] = ip;
+    atomicAdd(&cnt[ip], 1);
+  }
+}
+
+__global__ void k_att_place(long na, const int *ipa, const long long *off, int *fill, int *list) {
+  for (long a = blockIdx.x * (long)blockDim.x + threadIdx.x; a < na; a += (long)gridDim.x * blockDim.x) {
+    int ip = ipa[a];
+    long long pos = off[ip] + atomicAdd(&fill[ip], 1);
+    list[pos] = (int)a;
+  }
+}
+
+__global__ void k_att_sort(long np, const int *cnt, const long long *off, int *list) {
+  for (long ip = blockIdx.x * (long)blockDim.x + threadIdx.x; ip < np; ip += (long)gridDim.x * blockDim.x) {
+    int n = cnt[ip];
+    if (n < 2) continue;
+    int *L = list + off[ip];
+    for (int i = 1; i < n; ++i) {
+      int v = L[i], j = i - 1;
+      while (j >= 0 && L[j] > v) {
+        L[j + 1] = L[j];
+        --j;
+      }
+      L[j + 1] = v;
+    }
+  }
+}
+
+__global__ void k_slot_of_id(ParticlesF64 A, const int *base, const int *count, int *slot_of_id) {
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  for (long q = threadIdx.x; q < n; q += blockDim.x) slot_of_id[A.id[b + q]] = (int)(b + q);
+}
+
+// mode 0: record outcomes; mode 1: apply attempts below the cut.
+__global__ void k_att_replay(int mode, long np, const ecsprt_attempt *att, const int *cnt, const long long *off,
+                             const int *list, const int *slot_of_id, ParticlesF64 A, CollideConsts k,
+                             int *succ, int *fail, const long long *cut_ptr) {
+  const long long cut = mode ? *cut_ptr : 0;
+  for (long ip = blockIdx.x * (long)blockDim.x + threadIdx.x; ip < np; ip += (long)gridDim.x * blockDim.x) {
+    const int n = cnt[ip];
+    if (n == 0) continue;
+    const int s = slot_of_id[ip];
+    double v[3] = {A.vx[s], A.vy[s], A.vz[s]};
+    const int *L = list + off[ip];
+    for (int q = 0; q < n; ++q) {
+      const int a = L[q];
+      if (mode && a >= cut) break;
+      const ecsprt_attempt r = att[a];
+      double vf[3];
+      const int res = collide_one(k, r.g, r.u_acc1, r.u_acc2, r.u_z, r.u_phi, v, vf);
+      if (res == 1) {
+        v[0] = vf[0];
+        v[1] = vf[1];
+        v[2] = vf[2];
+      }
+      if (!mode) {
+        succ[a] = res == 1;
+        fail[a] = res == 2;
+      }
+    }
+    if (mode) {
+      A.vx[s] = v[0];
+      A.vy[s] = v[1];
+      A.vz[s] = v[2];
+    }
+  }
+}
+
+// res[0] = cut, res[1] = successes in total, res[2] = refusals before the cut.
+__global__ void k_att_cut(long na, int Nc, const int *succ, const long long *succ_ex, const int *fail,
+                          const long long *fail_ex, long long *res) {
+  for (long a = blockIdx.x * (long)blockDim.x + threadIdx.x; a < na; a += (long)gridDim.x * blockDim.x) {
+    if (succ[a] && succ_ex[a] + 1 == Nc) res[0] = a + 1;
+    if (a == na - 1) res[1] = succ_ex[a] + succ[a];
+  }
+  (void)fail;
+  (void)fail_ex;
+}
+
+__global__ void k_att_fail_before(long na, const long long *cut_ptr, const int *fail, const long long *fail_ex,
+                                  long long *res) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    long long cut = *cut_ptr;
+    if (cut >= na) res[2] = fail_ex[na - 1] + fail[na - 1];
+    else res[2] = fail_ex[cut];
+  }
+}
+
+__global__ void k_set_i64(long long *p, long long v) { *p = v; }
+
+void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
+  const long np = c->np, na = c->n_att;
+  const int Nc = collision_count(&c->p, np, dt);
+  c->last_Nc = Nc;
+  c->last_Ns = 0;
+  c->last_Nf = 0;
+  c->last_attempts = 0;
+  if (Nc == 0) return;
+  ECS_CHECK(na > 0 && c->d_att, ECSPRT_ERR_ATTEMPTS, "no pre-drawn attempt records set");
+  hipStream_t st = c->stream;
+  int *cnt, *fill, *ipa, *list, *slot, *succ, *fail;
+  long long *off, *succ_ex, *fail_ex, *res;
+  ECS_HIP(hipMallocAsync((void **)&cnt, sizeof(int) * np, st));
+  ECS_HIP(hipMallocAsync((void **)&fill, sizeof(int) * np, st));
+  ECS_HIP(hipMallocAsync((void **)&slot, sizeof(int) * np, st));
+  ECS_HIP(hipMallocAsync((void **)&off, sizeof(long long) * np, st));
+  ECS_HIP(hipMallocAsync((void **)&ipa, sizeof(int) * na, st));
+  ECS_HIP(hipMallocAsync((void **)&list, sizeof(int) * na, st));
+  ECS_HIP(hipMallocAsync((void **)&succ, sizeof(int) * na, st));
+  ECS_HIP(hipMallocAsync((void **)&fail, sizeof(int) * na, st));
+  ECS_HIP(hipMallocAsync((void **)&succ_ex, sizeof(long long) * na, st));
+  ECS_HIP(hipMallocAsync((void **)&fail_ex, sizeof(long long) * na, st));
+  ECS_HIP(hipMallocAsync((void **)&res, sizeof(long long) * 3, st));
+  ECS_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * np, st));
+  ECS_HIP(hipMemsetAsync(fill, 0, sizeof(int) * np, st));
+  ECS_HIP(hipMemsetAsync(succ, 0, sizeof(int) * na, st));
+  ECS_HIP(hipMemsetAsync(fail, 0, sizeof(int) * na, st));
+  const int gb = (int)std::min<long>((std::max(na, np) + 255) / 256, 65536);
+  k_att_count<<<gb, 256, 0, st>>>(c->d_att, na, np, ipa, cnt);
+  scan_exclusive(c, cnt, off, np);
+  k_att_place<<<gb, 256, 0, st>>>(na, ipa, off, fill, list);
+  k_att_sort<<<gb, 256, 0, st>>>(np, cnt, off, list);
+  k_slot_of_id<<<c->ntiles, 256, 0, st>>>(c->A, c->d_base, c->d_count, slot);
+  CollideConsts k = collide_consts_of(c->p);
+  k_att_replay<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, nullptr);
+  scan_exclusive(c, succ, succ_ex, na);
+  scan_exclusive(c, fail, fail_ex, na);
+  k_set_i64<<<1, 1, 0, st>>>(res, (long long)na);
+  k_att_cut<<<gb, 256, 0, st>>>(na, Nc, succ, succ_ex, fail, fail_ex, res);
+  k_att_fail_before<<<1, 1, 0, st>>>(na, res, fail, fail_ex, res);
+  long long h[3];
+  ECS_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st));
+  ECS_HIP(hipStreamSynchronize(st));
+  const long long cut = h[0], total_succ = h[1], fails = h[2];
+  int rc = ECSPRT_OK;
+  std::string msg;
+  if (fails >= 10) {
+    rc = ECSPRT_ERR_COLLISION;
+    msg = "failed to collide " + std::to_string(fails) + " ion-neutral pairs";
+  } else if (total_succ < Nc) {
+    rc = ECSPRT_ERR_ATTEMPTS;
+    msg = "pre-drawn records exhausted after " + std::to_string(total_succ) + " of " + std::to_string(Nc) +
+          " collisions";
+  } else {
+    k_att_replay<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, res);
+    ECS_HIP(hipGetLastError());
+  }
+  void *bufs[] = {cnt, fill, slot, off, ipa, list, succ, fail, succ_ex, fail_ex, res};
+  for (void *b : bufs) ECS_HIP(hipFreeAsync(b, st));
+  ECS_HIP(hipStreamSynchronize(st));
+  c->last_Ns = std::min<long long>(total_succ, Nc);
+  c->last_Nf = fails;
+  c->last_attempts = rc == ECSPRT_OK ? cut : na;
+  if (rc != ECSPRT_OK) throw Error{rc, msg};
+}
+
+// Acceptance estimate for the Philox mode: one trial attempt per particle.
+__global__ void k_pilot(ParticlesF64 A, const int *base, const int *count, CollideConsts k, uint32_t key0,
+                        uint32_t key1, unsigned long long *ctr) {
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  unsigned long long ns = 0, na = 0;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    long s = b + q;
+    u32x4 c1 = {{(uint32_t)A.id[s], 0xFFFFFFFFu, 1u, 0x9170F00Du}};
+    u32x4 r1 = philox4x32_10(c1, key0, key1);
+    u32x4 c2 = {{(uint32_t)A.id[s], 0xFFFFFFFFu, 2u, 0x9170F00Du}};
+    u32x4 r2 = philox4x32_10(c2, key0, key1);
+    float g[3];
+    {
+      float u1 = u01(r1.v[0]), u2 = u01(r1.v[1]), u3 = u01(r1.v[2]), u4 = u01(r1.v[3]);
+      float ra = sqrtf(-2.0f * logf(u1)), rb = sqrtf(-2.0f * logf(u3));
+      g[0] = ra * cosf(6.283185307179586f * u2);
+      g[1] = ra * sinf(6.283185307179586f * u2);
+      g[2] = rb * cosf(6.283185307179586f * u4);
+    }
+    double vi[3] = {A.vx[s], A.vy[s], A.vz[s]}, vf[3];
+    int res = collide_one(k, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
+    na++;
+    ns += res == 1;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    ns += __shfl_down(ns, o);
+    na += __shfl_down(na, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&ctr[6], na);
+    atomicAdd(&ctr[7], ns);
+  }
+}
+
+void run_collide_philox_pilot(ecsprt_hip_ctx *c, double) {
+  unsigned long long zero[2] = {0, 0};
+  ECS_HIP(hipMemcpyAsync(c->d_ctr + 6, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+  CollideConsts k = collide_consts_of(c->p);
+  k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, (uint32_t)c->o.seed,
+                                             (uint32_t)(c->o.seed >> 32), c->d_ctr);
+  ECS_HIP(hipGetLastError());
+  unsigned long long h[2];
+  ECS_HIP(hipMemcpyAsync(h, c->d_ctr + 6, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipStreamSynchronize(c->stream));
+  c->acc_est = h[0] ? (double)h[1] / (double)h[0] : 0.0;
+}
+
+}  // namespace ecsprt

@@ -1,0 +1,160 @@
+// Shared internals of libecsprt_hip: context layout, tile geometry, error plumbing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ecsprt_hip.h"
+
+namespace ecsprt {
+
+// Error raised inside the library; entry points convert it to an int code.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define ECS_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      throw ::ecsprt::Error{ECSPRT_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    }                                                                                  \
+  } while (0)
+
+#define ECS_CHECK(cond, code, msg)                        \
+  do {                                                    \
+    if (!(cond)) throw ::ecsprt::Error{(code), (msg)};    \
+  } while (0)
+
+// Structure-of-arrays particle storage (FP64).
+struct ParticlesF64 {
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  int *id = nullptr;
+};
+
+// Geometry of the cell tiling.  A tile is a T^3 brick of cells (clipped at the upper
+// grid faces).  Tile t's particles live in slots [base[t], base[t]+count[t]) of the
+// particle arrays; capacity cap[t] leaves room for immigrants.
+struct TileGeom {
+  int Nx, Ny, Nz;
+  int T;          // cells per tile edge
+  int ntx, nty, ntz;
+  int H;          // deposit halo (cells) of the accumulation region
+  int RP;         // potential region edge: T+4 nodes, origin at tile origin - 1
+  int RA;         // accumulation region edge: T+1+2H nodes, origin at tile origin - H
+  int split;      // workgroups per tile
+};
+
+struct PushConsts {
+  double dx, dy, dz;
+  double t, s;
+  double Escale[3];
+};
+
+struct DriftConsts {
+  double dt;
+  double L[3], r0[3], r1[3];
+};
+
+// Per-axis fold table: for global node i, up to FOLD_MAXP (tile index, local index) pairs.
+constexpr int FOLD_MAXP = 4;
+
+struct Solver;  // solver.hip
+
+}  // namespace ecsprt
+
+struct ecsprt_hip_ctx {
+  ecsprt_params p;
+  ecsprt_options o;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+
+  // tiling
+  ecsprt::TileGeom g{};
+  int ntiles = 0;
+  long np = 0;
+  long pcap = 0;
+  std::vector<int> h_base, h_cap;
+  int *d_base = nullptr, *d_cap = nullptr, *d_count = nullptr, *d_next = nullptr;
+  ecsprt::ParticlesF64 A, B, E;  // current, next, emigrants
+  ecsprt::ParticlesF64 S;        // spill for tiles that overflow their capacity
+  long scap = 0;
+  // counters: [0] emigrants, [1] spilled, [2] lost, [3] collision attempts,
+  // [4] collision successes, [5] collision refusals (accumulated in long long below)
+  unsigned long long *d_ctr = nullptr;
+  unsigned long long *h_ctr = nullptr;  // pinned mirror
+
+  // grid vectors
+  long nnode = 0;
+  double *d_phi = nullptr;      // [Nz][Ny][Nx]
+  double *d_moments = nullptr;  // [Nz][Ny][Nx][4]
+  double *d_fold = nullptr;     // per workgroup accumulation regions
+  double *d_ovf = nullptr;      // [Nz][Ny][Nx][4] overflow deposits (global atomics)
+  double *d_rhs = nullptr;      // [Nz][Ny][Nx]
+  int *d_foldtab = nullptr;     // 3 axes x (count, pairs)
+  int foldtab_off[3] = {0, 0, 0};
+  size_t lds_bytes = 0;
+
+  // staging (host<->device AoS)
+  double *d_stage = nullptr;
+  long stage_cap = 0;
+  int *d_key = nullptr;
+  long key_cap = 0;
+
+  // collisions
+  ecsprt_attempt *d_att = nullptr;
+  long att_cap = 0, n_att = 0;
+  int last_Nc = 0;
+  long last_Ns = 0, last_Nf = 0, last_attempts = 0;
+  double acc_est = -1.0;  // Philox mode: acceptance fraction estimate
+  uint64_t step_index = 0;
+
+  // solver
+  ecsprt::Solver *solver = nullptr;
+  bool operator_built = false;
+  int last_iters = 0;
+  double last_rnorm = 0.0, last_bnorm = 0.0;
+
+  // timing
+  hipEvent_t ev[8] = {};
+  double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+};
+
+namespace ecsprt {
+
+// particles.hip
+void tiles_init(ecsprt_hip_ctx *c);
+void tiles_free(ecsprt_hip_ctx *c);
+void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool ids_from_stage);
+void particles_to_stage(ecsprt_hip_ctx *c);
+void particles_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k);
+void run_push(ecsprt_hip_ctx *c, double dt);
+void run_drift_rebin(ecsprt_hip_ctx *c, double dt);
+void run_deposit(ecsprt_hip_ctx *c);
+void run_fused_particles(ecsprt_hip_ctx *c, double dt);
+void run_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
+void run_apply_bc_aos(ecsprt_hip_ctx *c, long np, double *pos);
+void ensure_stage(ecsprt_hip_ctx *c, long np);
+
+// collide.hip
+void run_collide_predrawn(ecsprt_hip_ctx *c, double dt);
+void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt);
+int collision_count(const ecsprt_params *p, long np, double dt);
+
+// solver.hip
+void solver_create(ecsprt_hip_ctx *c);
+void solver_destroy(ecsprt_hip_ctx *c);
+void solver_build_operator(ecsprt_hip_ctx *c);
+void solver_rhs(ecsprt_hip_ctx *c, double *b);
+void solver_matmult(ecsprt_hip_ctx *c, const double *x, double *y);
+void solver_pcapply(ecsprt_hip_ctx *c, const double *r, double *z);
+void solver_remove_mean(ecsprt_hip_ctx *c, double *v);
+int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x);
+
+}  // namespace ecsprt

@@ -1,0 +1,908 @@
+// Particle stages of the hot path on gfx950: Boris push with LDS-staged field
+// gather, drift + periodic wrap + re-binning, cloud-in-cell deposit into per-tile
+// LDS accumulators with a deterministic fold, and the fused single-pass timestep.
+//
+// Particles are binned by tile (a T^3 brick of cells) in structure-of-arrays FP64
+// storage, so a workgroup streams one tile's particles with coalesced loads while
+// the potential it gathers from and the moments it deposits into sit in LDS.
+#include <algorithm>
+#include <numeric>
+
+#include "common.hpp"
+#include "philox.hpp"
+#include "pic_math.hpp"
+
+namespace ecsprt {
+
+enum : int { M_PUSH = 1, M_DRIFT = 2, M_DEPOSIT = 4, M_REBIN = 8, M_COLLIDE = 16 };
+
+struct TileArgs {
+  TileGeom g;
+  ParticlesF64 src, dst, emi;
+  const int *base, *count, *cap;
+  int *next;
+  unsigned long long *ctr;
+  const double *phi;
+  double *fold, *ovf;
+  PushConsts pc;
+  DriftConsts dc;
+  CollideConsts cc;
+  uint32_t key0, key1, step;
+  double lambda;
+};
+
+__device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg) {
+  int ci = (int)xg, cj = (int)yg, ck = (int)zg;
+  if (ci >= g.Nx) ci = g.Nx - 1;
+  if (cj >= g.Ny) cj = g.Ny - 1;
+  if (ck >= g.Nz) ck = g.Nz - 1;
+  return ((ck / g.T) * g.nty + cj / g.T) * g.ntx + ci / g.T;
+}
+
+template <typename T>
+__device__ inline T wave_append(bool pred, T *counter) {
+  unsigned long long m = __ballot(pred);
+  int lane = __lane_id();
+  unsigned long long below = m & ((1ull << lane) - 1ull);
+  int rank = __popcll(below);
+  int n = __popcll(m);
+  T base = 0;
+  if (lane == 0 && n) base = atomicAdd(counter, (T)n);
+  base = __shfl(base, 0);
+  return base + (T)rank;
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  return v;
+}
+
+__device__ inline void store_particle(const ParticlesF64 &d, long i, double x, double y, double z,
+                                      double vx, double vy, double vz, int id) {
+  d.x[i] = x;
+  d.y[i] = y;
+  d.z[i] = z;
+  d.vx[i] = vx;
+  d.vy[i] = vy;
+  d.vz[i] = vz;
+  d.id[i] = id;
+}
+
+// Box-Muller pair from two 32-bit draws (statistical collision mode only).
+__device__ inline void box_muller(uint32_t a, uint32_t b, float &g0, float &g1) {
+  float u1 = u01(a), u2 = u01(b);
+  float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincosf(6.283185307179586f * u2, &s, &c);
+  g0 = r * c;
+  g1 = r * s;
+}
+
+// Philox collision attempts for one particle: Poisson(lambda) attempts, each the
+// reference's scattering formula on fresh draws.  Counter = (id, step, attempt, tag).
+__device__ inline void collide_philox(const TileArgs &a, int pid, double &vx, double &vy, double &vz,
+                                      unsigned &natt, unsigned &nsucc, unsigned &nfail) {
+  u32x4 c0 = {{(uint32_t)pid, a.step, 0u, 0xC0111DE5u}};
+  u32x4 r0 = philox4x32_10(c0, a.key0, a.key1);
+  float u = u01(r0.v[0]);
+  double lam = a.lambda;
+  double pk = exp(-lam), F = pk;
+  int k = 0;
+  while ((double)u > F && k < 24) {
+    ++k;
+    pk *= lam / k;
+    F += pk;
+  }
+  for (int at = 0; at < k; ++at) {
+    u32x4 c1 = {{(uint32_t)pid, a.step, (uint32_t)(at + 1), 0xA77E3B71u}};
+    u32x4 c2 = {{(uint32_t)pid, a.step, (uint32_t)(at + 1), 0x5CA77E12u}};
+    u32x4 r1 = philox4x32_10(c1, a.key0, a.key1);
+    u32x4 r2 = philox4x32_10(c2, a.key0, a.key1);
+    float g[3], spare;
+    box_muller(r1.v[0], r1.v[1], g[0], g[1]);
+    box_muller(r1.v[2], r1.v[3], g[2], spare);
+    double vi[3] = {vx, vy, vz}, vf[3];
+    int res = collide_one(a.cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
+    natt++;
+    if (res == 1) {
+      vx = vf[0];
+      vy = vf[1];
+      vz = vf[2];
+      nsucc++;
+    } else if (res == 2) {
+      nfail++;
+    }
+  }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
+  extern __shared__ double smem[];
+  const TileGeom g = a.g;
+  const int nwg = gridDim.x;
+  int w = blockIdx.x;
+  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+  // contiguous run of tiles so neighbouring tiles share an L2.
+  if ((nwg & 7) == 0) w = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
+  const int t = w / g.split, s = w % g.split;
+  const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
+  const int i0 = tx * g.T, j0 = ty * g.T, k0 = tz * g.T;
+  const int ex = min(g.T, g.Nx - i0), ey = min(g.T, g.Ny - j0), ez = min(g.T, g.Nz - k0);
+  const int RP = g.RP, RA = g.RA, H = g.H;
+  const int RP3 = RP * RP * RP, RA3 = RA * RA * RA;
+  const int RAx = ex + 1 + 2 * H, RAy = ey + 1 + 2 * H, RAz = ez + 1 + 2 * H;
+  double *sphi = smem;
+  double *sacc = smem + ((MODE & M_PUSH) ? RP3 : 0);
+  const int tid = threadIdx.x;
+
+  if (MODE & M_PUSH) {
+    for (int q = tid; q < RP3; q += blockDim.x) {
+      int li = q % RP, lj = (q / RP) % RP, lk = q / (RP * RP);
+      int gi = wrap_index(i0 - 1 + li, g.Nx);
+      int gj = wrap_index(j0 - 1 + lj, g.Ny);
+      int gk = wrap_index(k0 - 1 + lk, g.Nz);
+      sphi[q] = a.phi[((long)gk * g.Ny + gj) * g.Nx + gi];
+    }
+  }
+  if (MODE & M_DEPOSIT) {
+    for (int q = tid; q < 4 * RA3; q += blockDim.x) sacc[q] = 0.0;
+  }
+  __syncthreads();
+
+  const long cnt = a.count[t];
+  const long beg = a.base[t] + cnt * s / g.split;
+  const long end = a.base[t] + cnt * (s + 1) / g.split;
+  const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
+  unsigned natt = 0, nsucc = 0, nfail = 0;
+
+  for (long p0 = beg; p0 < end; p0 += blockDim.x) {
+    const long p = p0 + tid;
+    const bool valid = p < end;
+    double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
+    int pid = 0;
+    if (valid) {
+      px = a.src.x[p];
+      py = a.src.y[p];
+      pz = a.src.z[p];
+      vx = a.src.vx[p];
+      vy = a.src.vy[p];
+      vz = a.src.vz[p];
+      pid = a.src.id[p];
+    }
+    if ((MODE & M_PUSH) && valid) {
+      // BorisMoverBz3D body, src/velocities.c:342-356.
+      double E[3];
+      diff_vec3d_lds(sphi, RP, i0 - 1, j0 - 1, k0 - 1, g.Nx, g.Ny, g.Nz, px / dx, py / dy, pz / dz, E);
+      double vxm = vx + a.pc.Escale[0] * E[0];
+      double vym = vy + a.pc.Escale[1] * E[1];
+      double vzm = vz + a.pc.Escale[2] * E[2];
+      double vxt = vxm + vym * a.pc.t;
+      double vyp = vym - vxt * a.pc.s;
+      double vxp = vxt + vyp * a.pc.t;
+      vx = vxp;
+      vy = vyp;
+      vz = vzm;
+    }
+    if ((MODE & M_COLLIDE) && valid) collide_philox(a, pid, vx, vy, vz, natt, nsucc, nfail);
+    bool lost = false;
+    int ntile = t;
+    if ((MODE & M_DRIFT) && valid) {
+      // UpdatePositions drift (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28).
+      px += vx * a.dc.dt;
+      py += vy * a.dc.dt;
+      pz += vz * a.dc.dt;
+      if (px < a.dc.r0[0]) px += a.dc.L[0];
+      if (px >= a.dc.r1[0]) px -= a.dc.L[0];
+      if (py < a.dc.r0[1]) py += a.dc.L[1];
+      if (py >= a.dc.r1[1]) py -= a.dc.L[1];
+      if (pz < a.dc.r0[2]) pz += a.dc.L[2];
+      if (pz >= a.dc.r1[2]) pz -= a.dc.L[2];
+      lost = !(px >= a.dc.r0[0] && px <= a.dc.r1[0] && py >= a.dc.r0[1] && py <= a.dc.r1[1] &&
+               pz >= a.dc.r0[2] && pz <= a.dc.r1[2]);
+      if (!lost) ntile = tile_of(g, px / dx, py / dy, pz / dz);
+    }
+    if ((MODE & M_DEPOSIT) && valid && !lost) {
+      // Collect3DFluidMoments body, src/moments.c:154-205.
+      double x = px / dx, y = py / dy, z = pz / dz;
+      int ixl = (int)x, iyl = (int)y, izl = (int)z;
+      double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+      double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+      double wzh = z - (double)izl, wzl = 1.0 - wzh;
+      double w8[8];
+      w8[0] = wzh * wyh * wxh;
+      w8[1] = wzl * wyh * wxh;
+      w8[2] = wzh * wyl * wxh;
+      w8[3] = wzl * wyl * wxh;
+      w8[4] = wzh * wyh * wxl;
+      w8[5] = wzl * wyh * wxl;
+      w8[6] = wzh * wyl * wxl;
+      w8[7] = wzl * wyl * wxl;
+      int la = ixl - (i0 - H), lb = iyl - (j0 - H), lc = izl - (k0 - H);
+      if (la < 0) la += g.Nx; else if (la > RAx - 2) la -= g.Nx;
+      if (lb < 0) lb += g.Ny; else if (lb > RAy - 2) lb -= g.Ny;
+      if (lc < 0) lc += g.Nz; else if (lc > RAz - 2) lc -= g.Nz;
+      const bool inreg = la >= 0 && la <= RAx - 2 && lb >= 0 && lb <= RAy - 2 && lc >= 0 && lc <= RAz - 2;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+        const double w = w8[c];
+        if (inreg) {
+          const int idx = ((lc + az) * RA + (lb + ay)) * RA + (la + ax);
+          atomicAdd(&sacc[idx], w);
+          atomicAdd(&sacc[RA3 + idx], vx * w);
+          atomicAdd(&sacc[2 * RA3 + idx], vy * w);
+          atomicAdd(&sacc[3 * RA3 + idx], vz * w);
+        } else {
+          const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
+                            wrap_index(ixl + ax, g.Nx);
+          unsafeAtomicAdd(&a.ovf[node * 4 + 0], w);
+          unsafeAtomicAdd(&a.ovf[node * 4 + 1], vx * w);
+          unsafeAtomicAdd(&a.ovf[node * 4 + 2], vy * w);
+          unsafeAtomicAdd(&a.ovf[node * 4 + 3], vz * w);
+        }
+      }
+    }
+    if (MODE & M_REBIN) {
+      const bool stay = valid && !lost && ntile == t;
+      const bool emig = valid && !lost && ntile != t;
+      const int sslot = wave_append<int>(stay, &a.next[t]);
+      const unsigned long long eslot = wave_append<unsigned long long>(emig, &a.ctr[0]);
+      if (stay) store_particle(a.dst, a.base[t] + sslot, px, py, pz, vx, vy, vz, pid);
+      if (emig) store_particle(a.emi, (long)eslot, px, py, pz, vx, vy, vz, pid);
+      if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
+    } else if ((MODE & (M_PUSH | M_COLLIDE)) && valid) {
+      a.src.vx[p] = vx;
+      a.src.vy[p] = vy;
+      a.src.vz[p] = vz;
+    }
+  }
+  if (MODE & M_COLLIDE) {
+    unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+    if (__lane_id() == 0) {
+      if (s0) atomicAdd(&a.ctr[3], s0);
+      if (s1) atomicAdd(&a.ctr[4], s1);
+      if (s2) atomicAdd(&a.ctr[5], s2);
+    }
+  }
+  if (MODE & M_DEPOSIT) {
+    __syncthreads();
+    double *out = a.fold + (size_t)w * 4 * RA3;
+    for (int q = tid; q < 4 * RA3; q += blockDim.x) out[q] = sacc[q];
+  }
+}
+
+// Emigrants -> destination tiles (slots after the stayers), spill when a tile is full.
+__global__ void k_rebin(TileGeom g, ParticlesF64 emi, ParticlesF64 dst, ParticlesF64 spill, long scap,
+                        const int *base, const int *cap, int *next, unsigned long long *ctr, double dx,
+                        double dy, double dz) {
+  const unsigned long long n = ctr[0];
+  for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
+       e += (unsigned long long)gridDim.x * blockDim.x) {
+    double x = emi.x[e], y = emi.y[e], z = emi.z[e];
+    int t = tile_of(g, x / dx, y / dy, z / dz);
+    int slot = atomicAdd(&next[t], 1);
+    if (slot < cap[t]) {
+      store_particle(dst, (long)base[t] + slot, x, y, z, emi.vx[e], emi.vy[e], emi.vz[e], emi.id[e]);
+    } else {
+      unsigned long long sp = atomicAdd(&ctr[1], 1ull);
+      if ((long)sp < scap) store_particle(spill, (long)sp, x, y, z, emi.vx[e], emi.vy[e], emi.vz[e], emi.id[e]);
+      else atomicAdd(&ctr[2], 1ull);
+    }
+  }
+}
+
+__global__ void k_finalize_counts(int ntiles, int *count, int *next, const int *cap) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ntiles) {
+    count[t] = min(next[t], cap[t]);
+    next[t] = 0;
+  }
+}
+
+// Sum of every workgroup region covering a node, in a fixed order, plus the
+// overflow deposits: the moments in FluidNode layout.
+__global__ void k_fold(TileGeom g, const double *__restrict__ fold, const double *__restrict__ ovf,
+                       const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m) {
+  const long nnode = (long)g.Nx * g.Ny * g.Nz;
+  const int RA = g.RA, RA3 = RA * RA * RA;
+  for (long node = blockIdx.x * (long)blockDim.x + threadIdx.x; node < nnode; node += (long)gridDim.x * blockDim.x) {
+    const int i = node % g.Nx, j = (node / g.Nx) % g.Ny, k = node / ((long)g.Nx * g.Ny);
+    const int *tx = tab + offx, *ty = tab + offy, *tz = tab + offz;
+    const int cx = tx[i], cy = ty[j], cz = tz[k];
+    const int *px = tx + g.Nx + i * FOLD_MAXP * 2;
+    const int *py = ty + g.Ny + j * FOLD_MAXP * 2;
+    const int *pz = tz + g.Nz + k * FOLD_MAXP * 2;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int a = 0; a < cz; ++a)
+      for (int b = 0; b < cy; ++b)
+        for (int c = 0; c < cx; ++c) {
+          const int t = (pz[2 * a] * g.nty + py[2 * b]) * g.ntx + px[2 * c];
+          const int loc = (pz[2 * a + 1] * RA + py[2 * b + 1]) * RA + px[2 * c + 1];
+          for (int s = 0; s < g.split; ++s) {
+            const double *r = fold + (size_t)(t * g.split + s) * 4 * RA3 + loc;
+            acc[0] += r[0];
+            acc[1] += r[RA3];
+            acc[2] += r[2 * RA3];
+            acc[3] += r[3 * RA3];
+          }
+        }
+    for (int d = 0; d < 4; ++d) m[node * 4 + d] = acc[d] + ovf[node * 4 + d];
+  }
+}
+
+__global__ void k_fill_f64(double *p, long n, double v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_i32(int *p, long n, int v) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u64(unsigned long long *p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+// Binning from an AoS staging buffer (pos[np][3] then vel[np][3]); staging index = id.
+__global__ void k_bin_count(TileGeom g, const double *stage, long np, double dx, double dy, double dz, DriftConsts dc,
+                            int *key, int *cnt, unsigned long long *ctr) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    double x = stage[p * 3], y = stage[p * 3 + 1], z = stage[p * 3 + 2];
+    bool in = x >= dc.r0[0] && x <= dc.r1[0] && y >= dc.r0[1] && y <= dc.r1[1] && z >= dc.r0[2] && z <= dc.r1[2];
+    if (in) {
+      int t = tile_of(g, x / dx, y / dy, z / dz);
+      key[2 * p] = t;
+      key[2 * p + 1] = atomicAdd(&cnt[t], 1);
+    } else {
+      key[2 * p] = -1;
+      atomicAdd(&ctr[2], 1ull);
+    }
+  }
+}
+
+__global__ void k_bin_scatter(const double *stage, long np, const int *key, const int *base, ParticlesF64 A) {
+  const double *vel = stage + 3 * np;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    int t = key[2 * p];
+    if (t < 0) continue;
+    long slot = (long)base[t] + key[2 * p + 1];
+    store_particle(A, slot, stage[p * 3], stage[p * 3 + 1], stage[p * 3 + 2], vel[p * 3], vel[p * 3 + 1],
+                   vel[p * 3 + 2], (int)p);
+  }
+}
+
+__global__ void k_unbin(ParticlesF64 A, const int *base, const int *count, double *stage, long np) {
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  double *vel = stage + 3 * np;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    long s = b + q;
+    long id = A.id[s];
+    stage[id * 3] = A.x[s];
+    stage[id * 3 + 1] = A.y[s];
+    stage[id * 3 + 2] = A.z[s];
+    vel[id * 3] = A.vx[s];
+    vel[id * 3 + 1] = A.vy[s];
+    vel[id * 3 + 2] = A.vz[s];
+  }
+}
+
+__global__ void k_unbin_list(ParticlesF64 S, long n, double *stage, long np) {
+  double *vel = stage + 3 * np;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    long id = S.id[q];
+    stage[id * 3] = S.x[q];
+    stage[id * 3 + 1] = S.y[q];
+    stage[id * 3 + 2] = S.z[q];
+    vel[id * 3] = S.vx[q];
+    vel[id * 3 + 1] = S.vy[q];
+    vel[id * 3 + 2] = S.vz[q];
+  }
+}
+
+// Initial conditions on the device: uniform y, z; x from the inverse CDF of
+// 1 + amp*sin(2*pi*k*x/L) (Newton iterations from the uniform guess); Maxwellian
+// velocities from Box-Muller pairs.  Counter = (particle index, 0x1C1C, 0, 0).
+__global__ void k_init(double *stage, long np, double Lx, double Ly, double Lz, double x0, double y0, double z0,
+                       double amp, int kk, double vtx, double vty, double vtz, double v0x, double v0y, double v0z,
+                       uint32_t key0, uint32_t key1) {
+  double *vel = stage + 3 * np;
+  const double twopi = 6.283185307179586;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    u32x4 c0 = {{(uint32_t)p, (uint32_t)(p >> 32), 0x1C1Cu, 0u}};
+    u32x4 c1 = {{(uint32_t)p, (uint32_t)(p >> 32), 0x1C1Cu, 1u}};
+    u32x4 r0 = philox4x32_10(c0, key0, key1);
+    u32x4 r1 = philox4x32_10(c1, key0, key1);
+    // 53-bit uniforms in [0, 1) for positions.
+    double ux = ((double)(r0.v[0] >> 5) * 67108864.0 + (double)(r0.v[1] >> 6)) * (1.0 / 9007199254740992.0);
+    double uy = ((double)(r0.v[2] >> 5) * 67108864.0 + (double)(r0.v[3] >> 6)) * (1.0 / 9007199254740992.0);
+    double uz = ((double)(r1.v[0] >> 5) * 67108864.0 + (double)(r1.v[1] >> 6)) * (1.0 / 9007199254740992.0);
+    double xs = ux;
+    if (amp != 0.0 && kk != 0) {
+      const double kw = twopi * kk;
+      for (int it = 0; it < 8; ++it) {
+        double F = xs - amp / kw * (cos(kw * xs) - 1.0) - ux;
+        double dF = 1.0 + amp * sin(kw * xs);
+        xs -= F / dF;
+      }
+      xs = xs - floor(xs);
+    }
+    double x = x0 + xs * Lx, y = y0 + uy * Ly, z = z0 + uz * Lz;
+    if (x >= x0 + Lx) x = x0;
+    if (y >= y0 + Ly) y = y0;
+    if (z >= z0 + Lz) z = z0;
+    stage[p * 3] = x;
+    stage[p * 3 + 1] = y;
+    stage[p * 3 + 2] = z;
+    double u1 = ((double)r1.v[2] + 0.5) * (1.0 / 4294967296.0);
+    double u2 = ((double)r1.v[3] + 0.5) * (1.0 / 4294967296.0);
+    u32x4 c2 = {{(uint32_t)p, (uint32_t)(p >> 32), 0x1C1Cu, 2u}};
+    u32x4 r2 = philox4x32_10(c2, key0, key1);
+    double u3 = ((double)r2.v[0] + 0.5) * (1.0 / 4294967296.0);
+    double u4 = ((double)r2.v[1] + 0.5) * (1.0 / 4294967296.0);
+    double ra = sqrt(-2.0 * log(u1)), rb = sqrt(-2.0 * log(u3));
+    vel[p * 3] = vtx * ra * cos(twopi * u2) + v0x;
+    vel[p * 3 + 1] = vty * ra * sin(twopi * u2) + v0y;
+    vel[p * 3 + 2] = vtz * rb * cos(twopi * u4) + v0z;
+  }
+}
+
+__global__ void k_cell_counts(TileGeom g, ParticlesF64 A, const int *base, const int *count, double dx, double dy,
+                              double dz, int *counts) {
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    long s = b + q;
+    int i = wrap_index((int)(A.x[s] / dx), g.Nx);
+    int j = wrap_index((int)(A.y[s] / dy), g.Ny);
+    int k = wrap_index((int)(A.z[s] / dz), g.Nz);
+    atomicAdd(&counts[((long)k * g.Ny + j) * g.Nx + i], 1);
+  }
+}
+
+__global__ void k_apply_bc_aos(double *pos, long np, DriftConsts dc) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    for (int d = 0; d < 3; ++d) {
+      double r = pos[p * 3 + d];
+      if (r < dc.r0[d]) r += dc.L[d];
+      if (r >= dc.r1[d]) r -= dc.L[d];
+      pos[p * 3 + d] = r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+static int grid_for(long n, int block) {
+  long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (int)b;
+}
+
+static void fill_f64(ecsprt_hip_ctx *c, double *p, long n, double v) {
+  k_fill_f64<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
+  ECS_HIP(hipGetLastError());
+}
+static void fill_i32(ecsprt_hip_ctx *c, int *p, long n, int v) {
+  k_fill_i32<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
+  ECS_HIP(hipGetLastError());
+}
+static void zero_ctr(ecsprt_hip_ctx *c) {
+  k_fill_u64<<<1, 64, 0, c->stream>>>(c->d_ctr, 8);
+  ECS_HIP(hipGetLastError());
+}
+
+static DriftConsts drift_consts(const ecsprt_params &p, double dt) {
+  DriftConsts d;
+  d.dt = dt;
+  d.L[0] = p.Lx;
+  d.L[1] = p.Ly;
+  d.L[2] = p.Lz;
+  d.r0[0] = p.x0;
+  d.r0[1] = p.y0;
+  d.r0[2] = p.z0;
+  d.r1[0] = p.x1;
+  d.r1[1] = p.y1;
+  d.r1[2] = p.z1;
+  return d;
+}
+
+// BorisMoverBz3D scalars, src/velocities.c:307-320.
+static PushConsts push_consts(const ecsprt_params &p, double dt) {
+  PushConsts k;
+  double h[3] = {1.0 / p.dx, 1.0 / p.dy, 1.0 / p.dz};
+  double tscale = 0.5 * (p.qi / p.mi) * dt;
+  k.t = tscale * p.B0;
+  k.s = 2.0 * k.t / (1 + k.t * k.t);
+  for (int d = 0; d < 3; ++d) k.Escale[d] = -0.5 * h[d] * tscale;
+  k.dx = p.dx;
+  k.dy = p.dy;
+  k.dz = p.dz;
+  return k;
+}
+
+static void free_particles(ParticlesF64 &P) {
+  (void)hipFree(P.x);
+  (void)hipFree(P.y);
+  (void)hipFree(P.z);
+  (void)hipFree(P.vx);
+  (void)hipFree(P.vy);
+  (void)hipFree(P.vz);
+  (void)hipFree(P.id);
+  P = ParticlesF64{};
+}
+
+static void alloc_particles(ParticlesF64 &P, long n) {
+  free_particles(P);
+  n = std::max(n, 1L);
+  hipError_t e = hipSuccess;
+  double **f[6] = {&P.x, &P.y, &P.z, &P.vx, &P.vy, &P.vz};
+  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMalloc(f[i], sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(&P.id, sizeof(int) * n);
+  if (e != hipSuccess) {
+    free_particles(P);
+    throw Error{ECSPRT_ERR_NOMEM, "particle arrays: " + std::string(hipGetErrorString(e))};
+  }
+}
+
+void ensure_stage(ecsprt_hip_ctx *c, long np) {
+  if (c->stage_cap >= np && c->d_stage) return;
+  (void)hipFree(c->d_stage);
+  c->d_stage = nullptr;
+  c->stage_cap = 0;
+  if (hipMalloc(&c->d_stage, sizeof(double) * 6 * std::max(np, 1L)) != hipSuccess)
+    throw Error{ECSPRT_ERR_NOMEM, "staging buffer"};
+  c->stage_cap = np;
+}
+
+static void ensure_keys(ecsprt_hip_ctx *c, long np) {
+  if (c->key_cap >= np && c->d_key) return;
+  (void)hipFree(c->d_key);
+  c->d_key = nullptr;
+  if (hipMalloc(&c->d_key, sizeof(int) * 2 * std::max(np, 1L)) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "keys"};
+  c->key_cap = np;
+}
+
+template <int MODE>
+static void set_lds_attr() {
+  static bool done = false;
+  if (!done) {
+    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    done = true;
+  }
+}
+
+static size_t lds_for(const TileGeom &g, int mode) {
+  size_t b = 0;
+  if (mode & M_PUSH) b += sizeof(double) * g.RP * g.RP * g.RP;
+  if (mode & M_DEPOSIT) b += sizeof(double) * 4 * g.RA * g.RA * g.RA;
+  return b;
+}
+
+void tiles_init(ecsprt_hip_ctx *c) {
+  const ecsprt_params &p = c->p;
+  TileGeom &g = c->g;
+  g.Nx = p.Nx;
+  g.Ny = p.Ny;
+  g.Nz = p.Nz;
+  int T = c->o.tile;
+  if (T <= 0) {
+    long n8 = (long)((p.Nx + 7) / 8) * ((p.Ny + 7) / 8) * ((p.Nz + 7) / 8);
+    T = n8 >= 512 ? 8 : 4;
+  }
+  g.T = T;
+  g.H = c->o.halo > 0 ? c->o.halo : 2;
+  g.ntx = (p.Nx + T - 1) / T;
+  g.nty = (p.Ny + T - 1) / T;
+  g.ntz = (p.Nz + T - 1) / T;
+  g.RP = T + 4;
+  g.RA = T + 1 + 2 * g.H;
+  c->ntiles = g.ntx * g.nty * g.ntz;
+  int split = c->o.split;
+  if (split <= 0) {
+    split = (1024 + c->ntiles - 1) / c->ntiles;
+    if (split > 32) split = 32;
+    if (split < 1) split = 1;
+  }
+  g.split = split;
+  size_t lds = lds_for(g, M_PUSH | M_DEPOSIT);
+  ECS_CHECK(lds <= 160 * 1024, ECSPRT_ERR_ARG, "tile/halo too large for 160 KB of LDS");
+  int block = c->o.block > 0 ? c->o.block : 1024;
+  ECS_CHECK(block % 64 == 0 && block <= 1024, ECSPRT_ERR_ARG, "block must be a multiple of 64, <= 1024");
+  c->lds_bytes = lds;
+
+  // Fold tables.
+  std::vector<int> tab;
+  const int Ns[3] = {p.Nx, p.Ny, p.Nz};
+  const int nts[3] = {g.ntx, g.nty, g.ntz};
+  for (int ax = 0; ax < 3; ++ax) {
+    const int N = Ns[ax];
+    c->foldtab_off[ax] = (int)tab.size();
+    std::vector<int> cnt(N, 0), pairs((size_t)N * FOLD_MAXP * 2, 0);
+    for (int ti = 0; ti < nts[ax]; ++ti) {
+      int i0 = ti * T, ext = std::min(T, N - i0), R = ext + 1 + 2 * g.H;
+      for (int a = 0; a < R; ++a) {
+        int node = wrap_index(i0 - g.H + a, N);
+        ECS_CHECK(cnt[node] < FOLD_MAXP, ECSPRT_ERR_ARG, "grid too small for the tile halo");
+        pairs[(size_t)node * FOLD_MAXP * 2 + 2 * cnt[node]] = ti;
+        pairs[(size_t)node * FOLD_MAXP * 2 + 2 * cnt[node] + 1] = a;
+        cnt[node]++;
+      }
+    }
+    tab.insert(tab.end(), cnt.begin(), cnt.end());
+    tab.insert(tab.end(), pairs.begin(), pairs.end());
+  }
+  ECS_HIP(hipMalloc(&c->d_foldtab, sizeof(int) * tab.size()));
+  ECS_HIP(hipMemcpy(c->d_foldtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
+
+  const long nreg = (long)c->ntiles * g.split;
+  const size_t fold_bytes = sizeof(double) * 4 * (size_t)g.RA * g.RA * g.RA * nreg;
+  if (hipMalloc(&c->d_fold, fold_bytes) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "fold buffer"};
+  ECS_HIP(hipMalloc(&c->d_ovf, sizeof(double) * 4 * c->nnode));
+  ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * 8));
+  ECS_HIP(hipHostMalloc((void **)&c->h_ctr, sizeof(unsigned long long) * 8, hipHostMallocDefault));
+  ECS_HIP(hipMalloc(&c->d_base, sizeof(int) * c->ntiles));
+  ECS_HIP(hipMalloc(&c->d_cap, sizeof(int) * c->ntiles));
+  ECS_HIP(hipMalloc(&c->d_count, sizeof(int) * c->ntiles));
+  ECS_HIP(hipMalloc(&c->d_next, sizeof(int) * c->ntiles));
+  fill_i32(c, c->d_count, c->ntiles, 0);
+  fill_i32(c, c->d_next, c->ntiles, 0);
+  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  zero_ctr(c);
+  set_lds_attr<M_PUSH>();
+  set_lds_attr<M_DEPOSIT>();
+  set_lds_attr<M_DRIFT | M_REBIN>();
+  set_lds_attr<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>();
+  set_lds_attr<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>();
+}
+
+void tiles_free(ecsprt_hip_ctx *c) {
+  free_particles(c->A);
+  free_particles(c->B);
+  free_particles(c->E);
+  free_particles(c->S);
+  (void)hipFree(c->d_fold);
+  (void)hipFree(c->d_ovf);
+  (void)hipFree(c->d_ctr);
+  if (c->h_ctr) (void)hipHostFree(c->h_ctr);
+  (void)hipFree(c->d_base);
+  (void)hipFree(c->d_cap);
+  (void)hipFree(c->d_count);
+  (void)hipFree(c->d_next);
+  (void)hipFree(c->d_foldtab);
+  (void)hipFree(c->d_stage);
+  (void)hipFree(c->d_key);
+}
+
+static void read_ctr(ecsprt_hip_ctx *c) {
+  ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipStreamSynchronize(c->stream));
+}
+
+void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
+  ECS_CHECK(np > 0 && np < 2147483647L, ECSPRT_ERR_ARG, "np must be in [1, 2^31-1)");
+  ensure_keys(c, np);
+  const TileGeom &g = c->g;
+  const ecsprt_params &p = c->p;
+  zero_ctr(c);
+  fill_i32(c, c->d_next, c->ntiles, 0);
+  DriftConsts dc = drift_consts(p, 0.0);
+  k_bin_count<<<grid_for(np, 256), 256, 0, c->stream>>>(g, c->d_stage, np, p.dx, p.dy, p.dz, dc, c->d_key, c->d_next,
+                                                         c->d_ctr);
+  ECS_HIP(hipGetLastError());
+  std::vector<int> cnt(c->ntiles);
+  ECS_HIP(hipMemcpyAsync(cnt.data(), c->d_next, sizeof(int) * c->ntiles, hipMemcpyDeviceToHost, c->stream));
+  read_ctr(c);
+  const long lost = (long)c->h_ctr[2];
+  // Capacities with slack for immigrants; bases in tile order.
+  const double slack = c->o.slack > 0 ? c->o.slack : 0.25;
+  c->h_base.assign(c->ntiles, 0);
+  c->h_cap.assign(c->ntiles, 0);
+  long total = 0;
+  for (int t = 0; t < c->ntiles; ++t) {
+    long cap = cnt[t] + (long)(slack * cnt[t]) + 64;
+    cap = (cap + 15) / 16 * 16;
+    c->h_base[t] = (int)total;
+    c->h_cap[t] = (int)cap;
+    total += cap;
+  }
+  ECS_CHECK(total < 2147483647L, ECSPRT_ERR_ARG, "particle capacity exceeds int32 slots");
+  if (total > c->pcap || !c->A.x) {
+    alloc_particles(c->A, total);
+    alloc_particles(c->B, total);
+    c->pcap = total;
+  }
+  if (!c->E.x || c->np != np) {
+    alloc_particles(c->E, np);
+    c->scap = std::max(np / 64, 65536L);
+    alloc_particles(c->S, c->scap);
+  }
+  ECS_HIP(hipMemcpyAsync(c->d_base, c->h_base.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice, c->stream));
+  ECS_HIP(hipMemcpyAsync(c->d_cap, c->h_cap.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice, c->stream));
+  ECS_HIP(hipMemcpyAsync(c->d_count, c->d_next, sizeof(int) * c->ntiles, hipMemcpyDeviceToDevice, c->stream));
+  fill_i32(c, c->d_next, c->ntiles, 0);
+  k_bin_scatter<<<grid_for(np, 256), 256, 0, c->stream>>>(c->d_stage, np, c->d_key, c->d_base, c->A);
+  ECS_HIP(hipGetLastError());
+  ECS_HIP(hipStreamSynchronize(c->stream));
+  // Losing a particle is the reference's Abort (src/boundaries.c:101-109): the
+  // context drops its particle state, since ids no longer cover [0, np).
+  c->np = lost == 0 ? np : 0;
+  ECS_CHECK(lost == 0, ECSPRT_ERR_COUNT_CHANGED,
+            "particles outside the periodic domain were dropped (" + std::to_string(lost) + ")");
+}
+
+void particles_to_stage(ecsprt_hip_ctx *c) {
+  ensure_stage(c, c->np);
+  k_unbin<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->d_stage, c->np);
+  ECS_HIP(hipGetLastError());
+}
+
+void particles_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k) {
+  ensure_stage(c, np);
+  const ecsprt_params &p = c->p;
+  k_init<<<grid_for(np, 256), 256, 0, c->stream>>>(c->d_stage, np, p.Lx, p.Ly, p.Lz, p.x0, p.y0, p.z0, amp, k,
+                                                    p.viTx, p.viTy, p.viTz, p.vi0x, p.vi0y, p.vi0z,
+                                                    (uint32_t)seed, (uint32_t)(seed >> 32));
+  ECS_HIP(hipGetLastError());
+  particles_bin_from_stage(c, np, true);
+}
+
+static TileArgs make_args(ecsprt_hip_ctx *c) {
+  TileArgs a{};
+  a.g = c->g;
+  a.src = c->A;
+  a.dst = c->B;
+  a.emi = c->E;
+  a.base = c->d_base;
+  a.count = c->d_count;
+  a.cap = c->d_cap;
+  a.next = c->d_next;
+  a.ctr = c->d_ctr;
+  a.phi = c->d_phi;
+  a.fold = c->d_fold;
+  a.ovf = c->d_ovf;
+  a.pc = push_consts(c->p, 0.0);
+  return a;
+}
+
+template <int MODE>
+static void launch_tile(ecsprt_hip_ctx *c, const TileArgs &a) {
+  const int block = c->o.block > 0 ? c->o.block : 1024;
+  const size_t lds = lds_for(c->g, MODE);
+  k_tile<MODE><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  ECS_HIP(hipGetLastError());
+}
+
+static void rebuild_from_spill(ecsprt_hip_ctx *c, long nspill) {
+  // A tile overflowed its capacity: gather everything by id and re-bin with new
+  // capacities.
+  ensure_stage(c, c->np);
+  k_unbin<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->d_stage, c->np);
+  ECS_HIP(hipGetLastError());
+  if (nspill > 0) {
+    k_unbin_list<<<grid_for(nspill, 256), 256, 0, c->stream>>>(c->S, nspill, c->d_stage, c->np);
+    ECS_HIP(hipGetLastError());
+  }
+  particles_bin_from_stage(c, c->np, true);
+}
+
+// After a re-binning kernel: immigrants, counts, swap, conservation check.
+static void finish_rebin(ecsprt_hip_ctx *c) {
+  const TileGeom &g = c->g;
+  const ecsprt_params &p = c->p;
+  k_rebin<<<2048, 256, 0, c->stream>>>(g, c->E, c->B, c->S, c->scap, c->d_base, c->d_cap, c->d_next, c->d_ctr, p.dx,
+                                       p.dy, p.dz);
+  ECS_HIP(hipGetLastError());
+  k_finalize_counts<<<(c->ntiles + 255) / 256, 256, 0, c->stream>>>(c->ntiles, c->d_count, c->d_next, c->d_cap);
+  ECS_HIP(hipGetLastError());
+  std::swap(c->A, c->B);
+}
+
+static void check_after_rebin(ecsprt_hip_ctx *c) {
+  read_ctr(c);
+  const long lost = (long)c->h_ctr[2];
+  const long nspill = (long)c->h_ctr[1];
+  if (lost > 0) {
+    c->np = 0;
+    throw Error{ECSPRT_ERR_COUNT_CHANGED,
+                "total number of ions changed (" + std::to_string(-lost) + ") under periodic boundaries"};
+  }
+  if (nspill > 0) rebuild_from_spill(c, nspill);
+}
+
+void run_push(ecsprt_hip_ctx *c, double dt) {
+  TileArgs a = make_args(c);
+  a.pc = push_consts(c->p, dt);
+  launch_tile<M_PUSH>(c, a);
+}
+
+void run_drift_rebin(ecsprt_hip_ctx *c, double dt) {
+  zero_ctr(c);
+  TileArgs a = make_args(c);
+  a.dc = drift_consts(c->p, dt);
+  launch_tile<M_DRIFT | M_REBIN>(c, a);
+  finish_rebin(c);
+  check_after_rebin(c);
+}
+
+static void launch_fold(ecsprt_hip_ctx *c) {
+  k_fold<<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, c->d_foldtab, c->foldtab_off[0],
+                                                           c->foldtab_off[1], c->foldtab_off[2], c->d_moments);
+  ECS_HIP(hipGetLastError());
+}
+
+void run_deposit(ecsprt_hip_ctx *c) {
+  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  TileArgs a = make_args(c);
+  launch_tile<M_DEPOSIT>(c, a);
+  launch_fold(c);
+}
+
+static CollideConsts collide_consts(const ecsprt_params &p) {
+  CollideConsts k;
+  k.vn0[0] = p.vn0x;
+  k.vn0[1] = p.vn0y;
+  k.vn0[2] = p.vn0z;
+  k.vnT = p.vnT;
+  k.mi = p.mi;
+  k.mn = p.mn;
+  k.M = p.mn + p.mi;
+  double tmp = (p.vi0x - p.vn0x) * (p.vi0x - p.vn0x) + (p.vi0y - p.vn0y) * (p.vi0y - p.vn0y) +
+               (p.vi0z - p.vn0z) * (p.vi0z - p.vn0z);
+  k.vrm = 4.0 * p.viT + sqrt(tmp);
+  k.viT = p.viT;
+  return k;
+}
+
+CollideConsts collide_consts_of(const ecsprt_params &p) { return collide_consts(p); }
+
+void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
+  zero_ctr(c);
+  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  TileArgs a = make_args(c);
+  a.pc = push_consts(c->p, dt);
+  a.dc = drift_consts(c->p, dt);
+  const bool coll = c->o.collide_mode == ECSPRT_COLLIDE_PHILOX && c->p.mn > 0.0;
+  if (coll) {
+    if (c->acc_est <= 0.0) run_collide_philox_pilot(c, dt);
+    int Nc = collision_count(&c->p, c->np, dt);
+    c->last_Nc = Nc;
+    a.cc = collide_consts(c->p);
+    a.key0 = (uint32_t)c->o.seed;
+    a.key1 = (uint32_t)(c->o.seed >> 32);
+    a.step = (uint32_t)c->step_index;
+    a.lambda = (c->acc_est > 0.0 && c->np > 0) ? (double)Nc / (c->acc_est * (double)c->np) : 0.0;
+    launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
+  } else {
+    launch_tile<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
+  }
+  if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[1], c->stream));
+  finish_rebin(c);
+  launch_fold(c);
+  if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[2], c->stream));
+  check_after_rebin(c);
+  if (coll) {
+    const long natt = (long)c->h_ctr[3], ns = (long)c->h_ctr[4], nf = (long)c->h_ctr[5];
+    c->last_attempts = natt;
+    c->last_Ns = ns;
+    c->last_Nf = nf;
+    if (natt > 0) c->acc_est = (double)ns / (double)natt;
+    ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
+  }
+  c->step_index++;
+}
+
+void run_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
+  fill_i32(c, d_counts, c->nnode, 0);
+  k_cell_counts<<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_base, c->d_count, c->p.dx, c->p.dy, c->p.dz,
+                                                   d_counts);
+  ECS_HIP(hipGetLastError());
+}
+
+void run_apply_bc_aos(ecsprt_hip_ctx *c, long np, double *pos) {
+  DriftConsts dc = drift_consts(c->p, 0.0);
+  k_apply_bc_aos<<<grid_for(np, 256), 256, 0, c->stream>>>(pos, np, dc);
+  ECS_HIP(hipGetLastError());
+}
+
+}  // namespace ecsprt

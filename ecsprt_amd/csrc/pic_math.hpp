@@ -1,0 +1,174 @@
+// Per-particle arithmetic of the hot path, shared by the tile kernels and the
+// collision kernels.  Operation order follows the reference line by line so that
+// FP64 results are bit-identical to a non-contracting CPU build (the library is
+// compiled with -ffp-contract=off).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "common.hpp"
+
+namespace ecsprt {
+
+// Field gather, DifferenceVector3D (src/vector-math.c:596-722), from a potential
+// region staged in LDS.  F holds nodes [oi, oi+RP) x [oj, oj+RP) x [ok, ok+RP) of the
+// periodic potential (global, unwrapped indices).  Branch choices use the global
+// indices exactly as the reference does; the interpolation order keeps the
+// reference's y->wyh, z->wxh, x->wzh weight assignment (vector-math.c:645-651).
+__device__ inline void diff_vec3d_lds(const double *__restrict__ F, int RP, int oi, int oj, int ok,
+                                      int Nx, int Ny, int Nz, double x0, double y0, double z0,
+                                      double f[3]) {
+#define PL(kk, jj, ii) F[(((kk)-ok) * RP + ((jj)-oj)) * RP + ((ii)-oi)]
+  int ixl = (int)x0, ixh = ixl + 1;
+  int iyl = (int)y0, iyh = iyl + 1;
+  int izl = (int)z0, izh = izl + 1;
+  double wxh = x0 - (double)ixl;
+  double wyh = y0 - (double)iyl;
+  double wzh = z0 - (double)izl;
+  double hhh, lhh, hlh, llh, hhl, lhl, hll, lll;
+  double whh, whl, wlh, wll, Ewh, Ewl;
+  // Particles reach this function only from inside their tile, so ixl >= 0 and the
+  // reference's forward-difference branch (vector-math.c:625-631) cannot occur.
+  hhl = PL(izh, iyh, ixl + 1) - PL(izh, iyh, ixl - 1);
+  lhl = PL(izl, iyh, ixl + 1) - PL(izl, iyh, ixl - 1);
+  hll = PL(izh, iyl, ixl + 1) - PL(izh, iyl, ixl - 1);
+  lll = PL(izl, iyl, ixl + 1) - PL(izl, iyl, ixl - 1);
+  if (ixh < Nx) {
+    hhh = PL(izh, iyh, ixh + 1) - PL(izh, iyh, ixh - 1);
+    lhh = PL(izl, iyh, ixh + 1) - PL(izl, iyh, ixh - 1);
+    hlh = PL(izh, iyl, ixh + 1) - PL(izh, iyl, ixh - 1);
+    llh = PL(izl, iyl, ixh + 1) - PL(izl, iyl, ixh - 1);
+  } else {
+    hhh = +3.0 * PL(izh, iyh, ixh) - 4.0 * PL(izh, iyh, ixh - 1) + 1.0 * PL(izh, iyh, ixh - 2);
+    lhh = +3.0 * PL(izl, iyh, ixh) - 4.0 * PL(izl, iyh, ixh - 1) + 1.0 * PL(izl, iyh, ixh - 2);
+    hlh = +3.0 * PL(izh, iyl, ixh) - 4.0 * PL(izh, iyl, ixh - 1) + 1.0 * PL(izh, iyl, ixh - 2);
+    llh = +3.0 * PL(izl, iyl, ixh) - 4.0 * PL(izl, iyl, ixh - 1) + 1.0 * PL(izl, iyl, ixh - 2);
+  }
+  whh = hlh + wyh * (hhh - hlh);
+  whl = hll + wyh * (hhl - hll);
+  wlh = llh + wyh * (lhh - llh);
+  wll = lll + wyh * (lhl - lll);
+  Ewh = wlh + wxh * (whh - wlh);
+  Ewl = wll + wxh * (whl - wll);
+  f[0] = Ewl + wzh * (Ewh - Ewl);
+
+  hlh = PL(izh, iyl + 1, ixh) - PL(izh, iyl - 1, ixh);
+  llh = PL(izl, iyl + 1, ixh) - PL(izl, iyl - 1, ixh);
+  hll = PL(izh, iyl + 1, ixl) - PL(izh, iyl - 1, ixl);
+  lll = PL(izl, iyl + 1, ixl) - PL(izl, iyl - 1, ixl);
+  if (iyh < Ny) {
+    hhh = PL(izh, iyh + 1, ixh) - PL(izh, iyh - 1, ixh);
+    lhh = PL(izl, iyh + 1, ixh) - PL(izl, iyh - 1, ixh);
+    hhl = PL(izh, iyh + 1, ixl) - PL(izh, iyh - 1, ixl);
+    lhl = PL(izl, iyh + 1, ixl) - PL(izl, iyh - 1, ixl);
+  } else {
+    hhh = +3.0 * PL(izh, iyh, ixh) - 4.0 * PL(izh, iyh - 1, ixh) + 1.0 * PL(izh, iyh - 2, ixh);
+    lhh = +3.0 * PL(izl, iyh, ixh) - 4.0 * PL(izl, iyh - 1, ixh) + 1.0 * PL(izl, iyh - 2, ixh);
+    hhl = +3.0 * PL(izh, iyh, ixl) - 4.0 * PL(izh, iyh - 1, ixl) + 1.0 * PL(izh, iyh - 2, ixl);
+    lhl = +3.0 * PL(izl, iyh, ixl) - 4.0 * PL(izl, iyh - 1, ixl) + 1.0 * PL(izl, iyh - 2, ixl);
+  }
+  whh = hlh + wyh * (hhh - hlh);
+  whl = hll + wyh * (hhl - hll);
+  wlh = llh + wyh * (lhh - llh);
+  wll = lll + wyh * (lhl - lll);
+  Ewh = wlh + wxh * (whh - wlh);
+  Ewl = wll + wxh * (whl - wll);
+  f[1] = Ewl + wzh * (Ewh - Ewl);
+
+  lhh = PL(izl + 1, iyh, ixh) - PL(izl - 1, iyh, ixh);
+  llh = PL(izl + 1, iyl, ixh) - PL(izl - 1, iyl, ixh);
+  lhl = PL(izl + 1, iyh, ixl) - PL(izl - 1, iyh, ixl);
+  lll = PL(izl + 1, iyl, ixl) - PL(izl - 1, iyl, ixl);
+  if (izh < Nz) {
+    hhh = PL(izh + 1, iyh, ixh) - PL(izh - 1, iyh, ixh);
+    hlh = PL(izh + 1, iyl, ixh) - PL(izh - 1, iyl, ixh);
+    hhl = PL(izh + 1, iyh, ixl) - PL(izh - 1, iyh, ixl);
+    hll = PL(izh + 1, iyl, ixl) - PL(izh - 1, iyl, ixl);
+  } else {
+    hhh = +3.0 * PL(izh, iyh, ixh) - 4.0 * PL(izh - 1, iyh, ixh) + 1.0 * PL(izh - 2, iyh, ixh);
+    hlh = +3.0 * PL(izh, iyl, ixh) - 4.0 * PL(izh - 1, iyl, ixh) + 1.0 * PL(izh - 2, iyl, ixh);
+    hhl = +3.0 * PL(izh, iyh, ixl) - 4.0 * PL(izh - 1, iyh, ixl) + 1.0 * PL(izh - 2, iyh, ixl);
+    hll = +3.0 * PL(izh, iyl, ixl) - 4.0 * PL(izh - 1, iyl, ixl) + 1.0 * PL(izh - 2, iyl, ixl);
+  }
+  whh = hlh + wyh * (hhh - hlh);
+  whl = hll + wyh * (hhl - hll);
+  wlh = llh + wyh * (lhh - llh);
+  wll = lll + wyh * (lhl - lll);
+  Ewh = wlh + wxh * (whh - wlh);
+  Ewl = wll + wxh * (whl - wll);
+  f[2] = Ewl + wzh * (Ewh - Ewl);
+#undef PL
+}
+
+// Collision constants (src/collisions.c:18-52).
+struct CollideConsts {
+  double vn0[3];
+  double vnT, mi, mn, M, vrm, viT;
+};
+
+// One attempt of the ScatterElastic loop body (src/collisions.c:77-154) on ion
+// velocity vi.  Returns 0 rejected by the acceptance draw, 1 collided (vf written),
+// 2 refused because |vf|/viT > 10 (collisions.c:141).
+__device__ inline int collide_one(const CollideConsts &k, const float g[3], float u_acc1, float u_acc2,
+                                  float u_z, float u_phi, const double vi[3], double vf[3]) {
+  double vn[3], vr[3], vc[3], u2[3];
+  double tmp = 0.0;
+  for (int d = 0; d < 3; d++) {
+    vn[d] = g[d] * k.vnT + k.vn0[d];
+    vr[d] = vi[d] - vn[d];
+    tmp += vr[d] * vr[d];
+  }
+  double vrr = sqrt(tmp);
+  float acc = u_acc1 * u_acc2;
+  if (!(acc < vrr * vrr / (k.vrm * k.vrm))) return 0;
+  tmp = 0.0;
+  for (int d = 0; d < 3; d++) {
+    vc[d] = (vi[d] * k.mi + vn[d] * k.mn) / k.M;
+    tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
+  }
+  double vcr = sqrt(tmp);
+  double u0z = 2.0 * u_z - 1.0;
+  double uperp = sqrt(1.0 - u0z * u0z);
+  double uphi = 2 * 3.14159265358979323846 * u_phi;
+  double u0x = uperp * cos(uphi);
+  double u0y = uperp * sin(uphi);
+  double costht = vr[2] / vrr;
+  double sintht = sqrt(1.0 - costht * costht);
+  double cosphi = vr[0] / (vrr * sintht);
+  double sinphi = vr[1] / (vrr * sintht);
+  double u1x = u0z * sintht + u0x * costht;
+  double u1y = u0y;
+  double u1z = u0z * costht - u0x * sintht;
+  u2[0] = u1x * cosphi - u1y * sinphi;
+  u2[1] = u1x * sinphi + u1y * cosphi;
+  u2[2] = u1z;
+  tmp = 0.0;
+  double w[3];
+  for (int d = 0; d < 3; d++) {
+    w[d] = vc[d] + vcr * u2[d];
+    tmp += w[d] * w[d];
+  }
+  double vfr = sqrt(tmp);
+  double ratio = vfr / k.viT;
+  if (ratio > 10) return 2;
+  vf[0] = w[0];
+  vf[1] = w[1];
+  vf[2] = w[2];
+  return 1;
+}
+
+// (PetscInt)(np*ran3(&seed)): the int*float product is a float (src/collisions.c:72);
+// a product rounding up to np is clamped to np-1.
+__host__ __device__ inline long pick_ip(long np, float u) {
+  long ip = (long)(int)((float)np * u);
+  if (ip >= np) ip = np - 1;
+  if (ip < 0) ip = 0;
+  return ip;
+}
+
+__host__ __device__ inline int wrap_index(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+}  // namespace ecsprt

@@ -1,0 +1,226 @@
+/*
+ * ecsprt_hip.h -- C ABI of the MI355X (gfx950) implementation of ECSPRT's
+ * per-timestep hybrid-PIC hot path.
+ *
+ * Every entry point returns an int error code (0 = success), the analogue of
+ * PetscErrorCode; the library never aborts the process.  Paths in citations are
+ * relative to the ECSPRT reference repository root.
+ *
+ * Ownership: the library owns the particle state (tile-binned structure of arrays)
+ * and the grid vectors on the device between calls.  Host arrays cross the boundary
+ * only through the explicit get/set calls, in the reference's layouts:
+ *   particles  AoS double[np][3] (DMSwarm coordinate and "velocity" fields, src/setup.c:494)
+ *   moments    double[Nz][Ny][Nx][4] = {n, Gx, Gy, Gz} (FluidNode, src/ecsprt.h:194-199)
+ *   potential  double[Nz][Ny][Nx]
+ * Device-vector entry points (matmult, pcapply, remove_mean, rhs) take device
+ * pointers to [Nz][Ny][Nx] vectors.  All work is ordered on the context's stream;
+ * a context is used by one host thread at a time.
+ */
+#ifndef ECSPRT_HIP_H
+#define ECSPRT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Error codes. */
+#define ECSPRT_OK                 0
+#define ECSPRT_ERR_ARG            1 /* SETERRQ(PETSC_ERR_ARG_WRONG) analogue */
+#define ECSPRT_ERR_COLLISION      2 /* >=10 refused collisions: src/collisions.c:141-148 Abort */
+#define ECSPRT_ERR_COUNT_CHANGED  3 /* particle lost under periodic BC: src/boundaries.c:101-109 Abort */
+#define ECSPRT_ERR_HIP            4 /* HIP runtime error */
+#define ECSPRT_ERR_NOT_CONVERGED  5 /* KSP reached max_it without meeting rtol */
+#define ECSPRT_ERR_ATTEMPTS       6 /* pre-drawn collision records exhausted before Nc successes */
+#define ECSPRT_ERR_NOMEM          7 /* device allocation failed */
+#define ECSPRT_ERR_STATE          8 /* call made before the state it needs was set */
+
+/* Command-line parameters of the hot path (subset of CLI, src/ecsprt.h:137-191). */
+typedef struct {
+  int    Nx, Ny, Nz;
+  double dx, dy, dz;
+  double x0, y0, z0, x1, y1, z1;
+  double B0, E0, n0;
+  double qi, mi, nui;
+  double vi0x, vi0y, vi0z;
+  double viTx, viTy, viTz;
+  double Ti;
+  double nue, Te;
+  double veTx, veTy, veTz;
+  double mn, vn0x, vn0y, vn0z, Tn;
+} ecsprt_cli;
+
+/* Derived context scalars (Context, src/ecsprt.h:118-134, as SetUpContext fills them). */
+typedef struct {
+  int    Nx, Ny, Nz;
+  double dx, dy, dz, Lx, Ly, Lz, x0, y0, z0, x1, y1, z1;
+  double B0, E0, n0;
+  double qi, mi, nui, vi0x, vi0y, vi0z, vi0, viTx, viTy, viTz, viT, Ti, Omega_i, kappa_i;
+  double qe, me, nue, Te, veT, Omega_e, kappa_e;
+  double mn, vn0x, vn0y, vn0z, vn0, vnT, Tn;
+  double scale;
+  double gammaT;
+} ecsprt_params;
+
+/* One collision attempt's random numbers in the reference's draw order
+ * (src/collisions.c:72,80,88,100,102).  u_acc1*u_acc2 is the acceptance draw:
+ * PetscSqr(ran3(&seed)) expands to two ran3 calls. */
+typedef struct {
+  float u_ip;
+  float g[3];
+  float u_acc1, u_acc2;
+  float u_z, u_phi;
+} ecsprt_attempt;
+
+/* Collision modes. */
+#define ECSPRT_COLLIDE_OFF      0
+#define ECSPRT_COLLIDE_PREDRAWN 1 /* records from ecsprt_hip_set_attempts; per-particle parity */
+#define ECSPRT_COLLIDE_PHILOX   2 /* counter-based Philox4x32-10 stream; statistical parity */
+
+/* Operator modes for the potential solve. */
+#define ECSPRT_OPERATOR_FROZEN  0 /* built at the first solve only (KSPSetComputeOperators, src/pic.c:192) */
+#define ECSPRT_OPERATOR_REBUILD 1 /* rebuilt from the current density at every solve */
+
+/* Preconditioners. */
+#define ECSPRT_PC_NONE   0
+#define ECSPRT_PC_JACOBI 1
+#define ECSPRT_PC_GMG    2
+
+/* Precision of the potential solve. */
+#define ECSPRT_SOLVE_FP64  0
+#define ECSPRT_SOLVE_MIXED 1 /* FP32 GMRES+GMG inside FP64 iterative refinement */
+
+typedef struct {
+  int      tile;           /* cells per tile edge (0 = 8) */
+  int      halo;           /* deposit halo in cells for the fused step (0 = 2) */
+  int      block;          /* threads per tile workgroup (0 = 1024) */
+  int      split;          /* workgroups per tile (0 = auto) */
+  double   slack;          /* tile capacity slack fraction (0 = 0.25) */
+  int      collide_mode;   /* ECSPRT_COLLIDE_* */
+  uint64_t seed;           /* Philox key */
+  int      operator_mode;  /* ECSPRT_OPERATOR_* */
+  int      initial_guess;  /* 0 = zero every solve (ComputeInitialPhi, src/potential.c:20-29), 1 = previous phi */
+  int      pc_type;        /* ECSPRT_PC_* */
+  int      solve_precision;/* ECSPRT_SOLVE_* */
+  int      ksp_restart;    /* GMRES restart (0 = 30) */
+  int      ksp_max_it;     /* (0 = 10000, PETSc's default) */
+  double   ksp_rtol;       /* (0 = 1e-5, PETSc's default) */
+  double   ksp_atol;       /* (0 = 1e-50) */
+  int      mg_pre, mg_post;/* smoothing sweeps (0 = 2 each) */
+  int      mg_coarse_sweeps;/* sweeps on the coarsest level (0 = 50) */
+  double   inner_rtol;     /* mixed precision: inner FP32 solve tolerance (0 = 1e-4) */
+  int      timing;         /* 1 = record per-stage HIP events */
+} ecsprt_options;
+
+typedef struct ecsprt_hip_ctx ecsprt_hip_ctx;
+
+/* ---- set-up --------------------------------------------------------------- */
+
+/* SetUpContext + NormalizeGrid (src/setup.c:116-271, 296-346). */
+int ecsprt_hip_setup_context(const ecsprt_cli *cli, ecsprt_params *out);
+/* Fill opt with defaults. */
+int ecsprt_hip_default_options(ecsprt_options *opt);
+/* Create a context on the current HIP device: grid vectors, solver, no particles.
+ * Replaces CreateSwarmDM/CreateFluidDM/CreatePotentialDM/KSPCreate (src/pic.c:179-192). */
+int ecsprt_hip_create(const ecsprt_params *p, const ecsprt_options *opt, ecsprt_hip_ctx **out);
+int ecsprt_hip_destroy(ecsprt_hip_ctx *ctx);
+
+/* ---- particle state -------------------------------------------------------- */
+
+/* Upload np particles from host AoS arrays (DMSwarm layout); particle ids are the
+ * array indices.  Returns ECSPRT_ERR_COUNT_CHANGED if any coordinate is outside
+ * the domain (what DMSwarmMigrate would drop). */
+int ecsprt_hip_set_particles(ecsprt_hip_ctx *ctx, long np, const double *pos, const double *vel);
+/* Download particles to host AoS arrays in id order. */
+int ecsprt_hip_get_particles(ecsprt_hip_ctx *ctx, double *pos, double *vel);
+/* Generate np particles on the device: positions uniform in the box with an optional
+ * x-density perturbation 1 + amp*sin(2*pi*k*x/Lx) (inverse CDF), velocities
+ * Maxwellian with per-component thermal speeds viT{x,y,z} and drift vi0{x,y,z}. */
+int ecsprt_hip_init_particles(ecsprt_hip_ctx *ctx, long np, uint64_t seed, double amp, int k);
+long ecsprt_hip_num_particles(const ecsprt_hip_ctx *ctx);
+/* Per-cell counts of (int)(pos/d) per axis, host int[Nz][Ny][Nx]. */
+int ecsprt_hip_cell_counts(ecsprt_hip_ctx *ctx, int *counts);
+
+/* ---- plugin slots (src/global.h:27-32, bound in src/setup.c:99-114) ----------- */
+
+/* PushFunc: BorisMoverBz3D (src/velocities.c:281-371). */
+int ecsprt_hip_push(double dt, void *ctx);
+/* CollideFunc: ScatterElastic (src/collisions.c:9-167), in the context's collide mode. */
+int ecsprt_hip_collide(int ndim, double dt, void *ctx);
+/* BCFunc: ApplyBC (src/boundaries.c:6-34) on device AoS arrays pos[np][3] (vel unused). */
+int ecsprt_hip_apply_bc(int np, double *pos, double *vel, void *ctx);
+/* CollectFunc: Collect3DFluidMoments (src/moments.c:108-226). */
+int ecsprt_hip_collect(void *ctx);
+
+/* ---- direct entry points ---------------------------------------------------- */
+
+/* UpdateVelocities (src/velocities.c:375-390): push, then collide if mn > 0. */
+int ecsprt_hip_update_velocities(int ndim, double dt, void *ctx);
+/* UpdatePositions (src/positions.c:436-472): drift, ApplyBC, re-binning and the
+ * count-conservation check of ApplyBCAndMigrate (src/boundaries.c:57-113). */
+int ecsprt_hip_update_positions(int ndim, double dt, void *ctx);
+/* ComputePotential (src/potential.c:6-16): operator (per operator mode), RHS, KSP solve. */
+int ecsprt_hip_compute_potential(void *ctx);
+/* One fused timestep in the order of src/pic.c:246-258: push, collide (Philox
+ * mode only), drift, wrap, deposit in one pass over the particles, then the solve. */
+int ecsprt_hip_step(void *ctx, double dt);
+
+/* ---- linear system (src/lhs.c, src/rhs.c, src/potential.c) ------------------ */
+
+/* LHSFunc analogue: ComputeFullLHS3D (src/lhs.c:337-505) captured from the current
+ * moments as the matrix-free operator used by matmult and the solver. */
+int ecsprt_hip_lhs(void *ctx);
+/* RHSFunc analogue: ComputeFullRHS3D (src/rhs.c:259-373) into device vector b, mean removed. */
+int ecsprt_hip_rhs(void *ctx, double *b);
+/* MATSHELL MatMult: y = A x with the captured operator (device vectors). */
+int ecsprt_hip_matmult(void *shell, const double *x, double *y);
+/* PCShell apply: z = M^-1 r, one geometric-multigrid V-cycle (device vectors). */
+int ecsprt_hip_pcapply(void *shell, const double *r, double *z);
+/* MatNullSpaceRemove with the constant null space (device vector). */
+int ecsprt_hip_remove_mean(void *ctx, double *v);
+/* KSPSolve analogue on device vectors: A phi = b, phi overwritten (initial guess per options). */
+int ecsprt_hip_ksp_solve(void *ctx, const double *b, double *phi);
+
+/* ---- host access to grid vectors --------------------------------------------- */
+
+int ecsprt_hip_get_moments(ecsprt_hip_ctx *ctx, double *moments);
+int ecsprt_hip_set_moments(ecsprt_hip_ctx *ctx, const double *moments);
+int ecsprt_hip_get_potential(ecsprt_hip_ctx *ctx, double *phi);
+int ecsprt_hip_set_potential(ecsprt_hip_ctx *ctx, const double *phi);
+int ecsprt_hip_get_forcing(ecsprt_hip_ctx *ctx, double *b);
+
+/* ---- collisions ---------------------------------------------------------------- */
+
+/* Set pre-drawn attempt records (host array, attempt order) for the next collide. */
+int ecsprt_hip_set_attempts(ecsprt_hip_ctx *ctx, const ecsprt_attempt *att, long n);
+/* Nc from the reference formula (src/collisions.c:58-63, x86 int-conversion semantics). */
+int ecsprt_hip_collision_count(const ecsprt_params *p, long np, double dt);
+/* Counters of the last collide: Nc, successes, refusals, attempts consumed. */
+int ecsprt_hip_collision_stats(ecsprt_hip_ctx *ctx, int *Nc, long *Ns, long *Nf, long *attempts);
+
+/* ---- diagnostics ----------------------------------------------------------------- */
+
+/* Last solve: iterations, final true-residual norm, ||b||. */
+int ecsprt_hip_solver_stats(ecsprt_hip_ctx *ctx, int *iters, double *rnorm, double *bnorm);
+/* Per-stage times of the last call that recorded them (ms): [0] push or fused particle
+ * kernel, [1] collide, [2] drift+rebin, [3] deposit+fold, [4] rhs, [5] solve. */
+int ecsprt_hip_stage_times(ecsprt_hip_ctx *ctx, double *ms);
+/* Block until the context's stream is idle. */
+int ecsprt_hip_synchronize(ecsprt_hip_ctx *ctx);
+/* Text of the last error on this context. */
+const char *ecsprt_hip_last_error(const ecsprt_hip_ctx *ctx);
+
+/* Device memory helpers for callers without their own allocator. */
+int ecsprt_hip_dmalloc(void **ptr, long bytes);
+int ecsprt_hip_dfree(void *ptr);
+int ecsprt_hip_h2d(void *dst, const void *src, long bytes);
+int ecsprt_hip_d2h(void *dst, const void *src, long bytes);
+
+/* Philox4x32-10 (Salmon et al., SC'11) on the host, for known-answer tests. */
+void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,886 @@
+/*
+ * ecsprt_oracle.c -- CPU restatement of ECSPRT's hybrid-PIC hot path (TEST INFRASTRUCTURE).
+ *
+ * This file is the parity oracle for the HIP library.  It is loaded only by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg; the product never links it.
+ * Each function cites the reference function it restates (paths relative to the
+ * reference repository root).  Build: oracle/Makefile (gcc -O3 -ffp-contract=off).
+ *
+ * How each function is pinned (the reference itself cannot be built here: every
+ * hot-path file includes petsc.h and PETSc is absent, so no oracle/_ref exists):
+ *   oracle_difference_vector3d  closed forms: phi = x^2 on nodes exposes the x/z
+ *                               weight swap; quadratic phi exposes the one-sided
+ *                               last-cell branch; linear phi gives 2*slope*dx.
+ *   oracle_boris_bz3d           constant phi: exact rotation by 2*atan(t), |v_perp|
+ *                               preserved, vz unchanged; t=0: pure half-kick.
+ *   oracle_update_positions /   exact arithmetic; edge cases L-ulp, -ulp (wraps to L),
+ *   oracle_apply_bc             >= L, NaN.
+ *   oracle_collect_moments      sum(n)=np, sum(G)=sum(v), node-centred particle hits one
+ *                               node, brute-force numpy CIC.
+ *   oracle_cell_counts          numpy histogram of floor(pos/dx).
+ *   oracle_ran1                 Park-Miller minimal standard: 10000th state from seed 1
+ *                               is 1043618065 (Park & Miller 1988); outputs are a
+ *                               shuffle of the LCG stream.
+ *   oracle_ran3                 lagged-subtraction identity x_n = x_{n-55} - x_{n-24}
+ *                               (mod 1e9) on consecutive outputs.
+ *   oracle_gasdev               polar-method identity and N(0,1) moments.
+ *   oracle_collision_count      hand-computed cases incl. int overflow -> 0.
+ *   oracle_collide_one          centre-of-mass invariants: |vf-vc| = vcr, pair momentum
+ *                               and energy conserved by the implied neutral velocity.
+ *   oracle_full_lhs3d_*         zero row and column sums; uniform n reduces to
+ *                               scale*n0*(Lxx+Lyy+(1+k^2)Lzz) solved exactly by FFT;
+ *                               scipy sparse solve of the assembled matrix.
+ *   oracle_full_rhs3d           closed form for sinusoidal n and G; zero mean.
+ *   oracle_sinusoidal_rhs3d     closed form cos products.
+ *   oracle_setup_context        hand-computed derived coefficients.
+ *   oracle_scatter_elastic_*    predrawn and NR-stream paths agree when fed the NR
+ *                               log; invariants above per collision.
+ */
+#include "ecsprt_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <float.h>
+
+static inline int wrapi(int i, int n)
+{
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+static inline long nidx(int i, int j, int k, int Nx, int Ny, int Nz)
+{
+  return ((long)wrapi(k, Nz) * Ny + wrapi(j, Ny)) * Nx + wrapi(i, Nx);
+}
+
+/* ------------------------------------------------------------------------- */
+/* setup.c:116-271 (SetUpContext) and setup.c:296-346 (NormalizeGrid).        */
+/* ------------------------------------------------------------------------- */
+void oracle_setup_context(const oracle_cli *c, oracle_params *p)
+{
+  double tmp;
+  memset(p, 0, sizeof(*p));
+  p->qe = -OR_Q;
+  p->me = OR_ME;
+  p->n0 = c->n0;
+  p->B0 = c->B0;
+  p->E0 = c->E0;
+  p->qi = c->qi;
+  p->mi = c->mi;
+  p->vi0x = c->vi0x; p->vi0y = c->vi0y; p->vi0z = c->vi0z;
+  p->viTx = c->viTx; p->viTy = c->viTy; p->viTz = c->viTz;
+  p->Ti = c->Ti;
+  p->nui = c->nui;
+  p->nue = c->nue;
+  p->Te = c->Te;
+  p->mn = c->mn;
+  p->vn0x = c->vn0x; p->vn0y = c->vn0y; p->vn0z = c->vn0z;
+  p->Tn = c->Tn;
+
+  p->Nx = c->Nx > 0 ? c->Nx : 7;
+  p->Ny = c->Ny > 0 ? c->Ny : 7;
+  p->Nz = c->Nz > 0 ? c->Nz : 7;
+  p->dx = c->dx; p->dy = c->dy; p->dz = c->dz;
+  p->x0 = c->x0; p->y0 = c->y0; p->z0 = c->z0;
+  p->x1 = c->x1; p->y1 = c->y1; p->z1 = c->z1;
+  /* NormalizeGrid, setup.c:319-342 */
+  if (p->dx <= 0.0) { p->Lx = p->x1 - p->x0; p->dx = p->Lx / (double)p->Nx; }
+  else { p->Lx = p->dx * (double)p->Nx; p->x0 = 0.0; p->x1 = p->Lx; }
+  if (p->dy <= 0.0) { p->Ly = p->y1 - p->y0; p->dy = p->Ly / (double)p->Ny; }
+  else { p->Ly = p->dy * (double)p->Ny; p->y0 = 0.0; p->y1 = p->Ly; }
+  if (p->dz <= 0.0) { p->Lz = p->z1 - p->z0; p->dz = p->Lz / (double)p->Nz; }
+  else { p->Lz = p->dz * (double)p->Nz; p->z0 = 0.0; p->z1 = p->Lz; }
+
+  /* setup.c:222-230 */
+  p->Omega_e = fabs(p->qe * p->B0 / p->me);
+  p->Omega_i = fabs(p->qi * p->B0 / p->mi);
+  p->kappa_e = p->Omega_e / p->nue;
+  p->kappa_i = p->Omega_i / p->nui;
+  p->scale = 1.0 / (1 + p->kappa_e * p->kappa_e);
+
+  /* setup.c:233-235 */
+  p->vi0 = sqrt(p->vi0x * p->vi0x + p->vi0y * p->vi0y + p->vi0z * p->vi0z);
+  p->vn0 = sqrt(p->vn0x * p->vn0x + p->vn0y * p->vn0y + p->vn0z * p->vn0z);
+
+  /* setup.c:238-247 (electrons) */
+  if (c->veTx != 0.0 || c->veTy != 0.0 || c->veTz != 0.0) {
+    p->veT = sqrt(c->veTx * c->veTx + c->veTy * c->veTy + c->veTz * c->veTz);
+    p->Te = (0.5 * p->me / OR_KB) * (p->veT * p->veT);
+  } else {
+    tmp = sqrt(2.0 * OR_KB * p->Te / p->me) / 3.0;
+    p->veT = sqrt(tmp * tmp + tmp * tmp + tmp * tmp);
+  }
+  /* setup.c:250-259 (ions) */
+  if (p->viTx != 0.0 || p->viTy != 0.0 || p->viTz != 0.0) {
+    p->viT = sqrt(p->viTx * p->viTx + p->viTy * p->viTy + p->viTz * p->viTz);
+    p->Ti = (0.5 * p->mi / OR_KB) * (p->viT * p->viT);
+  } else {
+    tmp = sqrt(2.0 * OR_KB * p->Ti / p->mi) / 3.0;
+    p->viTx = tmp; p->viTy = tmp; p->viTz = tmp;
+    p->viT = sqrt(p->viTx * p->viTx + p->viTy * p->viTy + p->viTz * p->viTz);
+  }
+  /* setup.c:262-268 (neutrals) */
+  if (p->Tn == -1.0) p->Tn = p->Ti;
+  p->vnT = sqrt(2.0 * OR_KB * p->Tn / p->mn);
+  /* setup.c:271 */
+  p->gammaT = 1.0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* vector-math.c:596-722 (DifferenceVector3D).  F is the unghosted potential;  */
+/* the ghost width-2 local array of the reference is emulated by wrapping.    */
+/* The interpolation keeps the reference's ordering: y with wyh, then z with  */
+/* wxh, then x with wzh (vector-math.c:645-651).                              */
+/* ------------------------------------------------------------------------- */
+#define PHI(kk, jj, ii) F[nidx((ii), (jj), (kk), Nx, Ny, Nz)]
+
+void oracle_difference_vector3d(const double *F, int Nx, int Ny, int Nz,
+                                double x0, double y0, double z0, double f[3])
+{
+  int ixl, ixh, iyl, iyh, izl, izh;
+  double wxh, wyh, wzh;
+  double hhh, lhh, hlh, llh, hhl, lhl, hll, lll;
+  double whh, whl, wlh, wll, Ewh, Ewl;
+
+  ixl = (int)x0; ixh = ixl + 1; wxh = x0 - (double)ixl;
+  iyl = (int)y0; iyh = iyl + 1; wyh = y0 - (double)iyl;
+  izl = (int)z0; izh = izl + 1; wzh = z0 - (double)izl;
+
+  /* x component, vector-math.c:625-657 */
+  if (ixl >= 0) {
+    hhl = PHI(izh, iyh, ixl + 1) - PHI(izh, iyh, ixl - 1);
+    lhl = PHI(izl, iyh, ixl + 1) - PHI(izl, iyh, ixl - 1);
+    hll = PHI(izh, iyl, ixl + 1) - PHI(izh, iyl, ixl - 1);
+    lll = PHI(izl, iyl, ixl + 1) - PHI(izl, iyl, ixl - 1);
+  } else {
+    hhl = -1.0 * PHI(izh, iyh, ixl + 2) + 4.0 * PHI(izh, iyh, ixl + 1) - 3.0 * PHI(izh, iyh, ixl);
+    lhl = -1.0 * PHI(izl, iyh, ixl + 2) + 4.0 * PHI(izl, iyh, ixl + 1) - 3.0 * PHI(izl, iyh, ixl);
+    hll = -1.0 * PHI(izh, iyl, ixl + 2) + 4.0 * PHI(izh, iyl, ixl + 1) - 3.0 * PHI(izh, iyl, ixl);
+    lll = -1.0 * PHI(izl, iyl, ixl + 2) + 4.0 * PHI(izl, iyl, ixl + 1) - 3.0 * PHI(izl, iyl, ixl);
+  }
+  if (ixh < Nx) {
+    hhh = PHI(izh, iyh, ixh + 1) - PHI(izh, iyh, ixh - 1);
+    lhh = PHI(izl, iyh, ixh + 1) - PHI(izl, iyh, ixh - 1);
+    hlh = PHI(izh, iyl, ixh + 1) - PHI(izh, iyl, ixh - 1);
+    llh = PHI(izl, iyl, ixh + 1) - PHI(izl, iyl, ixh - 1);
+  } else {
+    hhh = +3.0 * PHI(izh, iyh, ixh) - 4.0 * PHI(izh, iyh, ixh - 1) + 1.0 * PHI(izh, iyh, ixh - 2);
+    lhh = +3.0 * PHI(izl, iyh, ixh) - 4.0 * PHI(izl, iyh, ixh - 1) + 1.0 * PHI(izl, iyh, ixh - 2);
+    hlh = +3.0 * PHI(izh, iyl, ixh) - 4.0 * PHI(izh, iyl, ixh - 1) + 1.0 * PHI(izh, iyl, ixh - 2);
+    llh = +3.0 * PHI(izl, iyl, ixh) - 4.0 * PHI(izl, iyl, ixh - 1) + 1.0 * PHI(izl, iyl, ixh - 2);
+  }
+  whh = hlh + wyh * (hhh - hlh);
+  whl = hll + wyh * (hhl - hll);
+  wlh = llh + wyh * (lhh - llh);
+  wll = lll + wyh * (lhl - lll);
+  Ewh = wlh + wxh * (whh - wlh);
+  Ewl = wll + wxh * (whl - wll);
+  f[0] = Ewl + wzh * (Ewh - Ewl);
+
+  /* y component, vector-math.c:659-691 */
+  if (iyl >= 0) {
+    hlh = PHI(izh, iyl + 1, ixh) - PHI(izh, iyl - 1, ixh);
+    llh = PHI(izl, iyl + 1, ixh) - PHI(izl, iyl - 1, ixh);
+    hll = PHI(izh, iyl + 1, ixl) - PHI(izh, iyl - 1, ixl);
+    lll = PHI(izl, iyl + 1, ixl) - PHI(izl, iyl - 1, ixl);
+  } else {
+    hlh = -1.0 * PHI(izh, iyl + 2, ixh) + 4.0 * PHI(izh, iyl + 1, ixh) - 3.0 * PHI(izh, iyl, ixh);
+    llh = -1.0 * PHI(izl, iyl + 2, ixh) + 4.0 * PHI(izl, iyl + 1, ixh) - 3.0 * PHI(izl, iyl, ixh);
+    hll = -1.0 * PHI(izh, iyl + 2, ixl) + 4.0 * PHI(izh, iyl + 1, ixl) - 3.0 * PHI(izh, iyl, ixl);
+    lll = -1.0 * PHI(izl, iyl + 2, ixl) + 4.0 * PHI(izl, iyl + 1, ixl) - 3.0 * PHI(izl, iyl, ixl);
+  }
+  if (iyh < Ny) {
+    hhh = PHI(izh, iyh + 1, ixh) - PHI(izh, iyh - 1, ixh);
+    lhh = PHI(izl, iyh + 1, ixh) - PHI(izl, iyh - 1, ixh);
+    hhl = PHI(izh, iyh + 1, ixl) - PHI(izh, iyh - 1, ixl);
+    lhl = PHI(izl, iyh + 1, ixl) - PHI(izl, iyh - 1, ixl);
+  } else {
+    hhh = +3.0 * PHI(izh, iyh, ixh) - 4.0 * PHI(izh, iyh - 1, ixh) + 1.0 * PHI(izh, iyh - 2, ixh);
+    lhh = +3.0 * PHI(izl, iyh, ixh) - 4.0 * PHI(izl, iyh - 1, ixh) + 1.0 * PHI(izl, iyh - 2, ixh);
+    hhl = +3.0 * PHI(izh, iyh, ixl) - 4.0 * PHI(izh, iyh - 1, ixl) + 1.0 * PHI(izh, iyh - 2, ixl);
+    lhl = +3.0 * PHI(izl, iyh, ixl) - 4.0 * PHI(izl, iyh - 1, ixl) + 1.0 * PHI(izl, iyh - 2, ixl);
+  }
+  whh = hlh + wyh * (hhh - hlh);
+  whl = hll + wyh * (hhl - hll);
+  wlh = llh + wyh * (lhh - llh);
+  wll = lll + wyh * (lhl - lll);
+  Ewh = wlh + wxh * (whh - wlh);
+  Ewl = wll + wxh * (whl - wll);
+  f[1] = Ewl + wzh * (Ewh - Ewl);
+
+  /* z component, vector-math.c:693-719 */
+  if (izl >= 0) {
+    lhh = PHI(izl + 1, iyh, ixh) - PHI(izl - 1, iyh, ixh);
+    llh = PHI(izl + 1, iyl, ixh) - PHI(izl - 1, iyl, ixh);
+    lhl = PHI(izl + 1, iyh, ixl) - PHI(izl - 1, iyh, ixl);
+    lll = PHI(izl + 1, iyl, ixl) - PHI(izl - 1, iyl, ixl);
+  } else {
+    lhh = -1.0 * PHI(izl + 2, iyh, ixh) + 4.0 * PHI(izl + 1, iyh, ixh) - 3.0 * PHI(izl, iyh, ixh);
+    llh = -1.0 * PHI(izl + 2, iyl, ixh) + 4.0 * PHI(izl + 1, iyl, ixh) - 3.0 * PHI(izl, iyl, ixh);
+    lhl = -1.0 * PHI(izl + 2, iyh, ixl) + 4.0 * PHI(izl + 1, iyh, ixl) - 3.0 * PHI(izl, iyh, ixl);
+    lll = -1.0 * PHI(izl + 2, iyl, ixl) + 4.0 * PHI(izl + 1, iyl, ixl) - 3.0 * PHI(izl, iyl, ixl);
+  }
+  if (izh < Nz) {
+    hhh = PHI(izh + 1, iyh, ixh) - PHI(izh - 1, iyh, ixh);
+    hlh = PHI(izh + 1, iyl, ixh) - PHI(izh - 1, iyl, ixh);
+    hhl = PHI(izh + 1, iyh, ixl) - PHI(izh - 1, iyh, ixl);
+    hll = PHI(izh + 1, iyl, ixl) - PHI(izh - 1, iyl, ixl);
+  } else {
+    hhh = +3.0 * PHI(izh, iyh, ixh) - 4.0 * PHI(izh - 1, iyh, ixh) + 1.0 * PHI(izh - 2, iyh, ixh);
+    hlh = +3.0 * PHI(izh, iyl, ixh) - 4.0 * PHI(izh - 1, iyl, ixh) + 1.0 * PHI(izh - 2, iyl, ixh);
+    hhl = +3.0 * PHI(izh, iyh, ixl) - 4.0 * PHI(izh - 1, iyh, ixl) + 1.0 * PHI(izh - 2, iyh, ixl);
+    hll = +3.0 * PHI(izh, iyl, ixl) - 4.0 * PHI(izh - 1, iyl, ixl) + 1.0 * PHI(izh - 2, iyl, ixl);
+  }
+  whh = hlh + wyh * (hhh - hlh);
+  whl = hll + wyh * (hhl - hll);
+  wlh = llh + wyh * (lhh - llh);
+  wll = lll + wyh * (lhl - lll);
+  Ewh = wlh + wxh * (whh - wlh);
+  Ewl = wll + wxh * (whl - wll);
+  f[2] = Ewl + wzh * (Ewh - Ewl);
+}
+#undef PHI
+
+/* velocities.c:281-371 (BorisMoverBz3D): first half E-kick, shear rotation, no
+ * second half-kick.  Coordinates are divided by dx (velocities.c:342-344).    */
+void oracle_boris_bz3d(const oracle_params *p, double dt, long np,
+                       const double *pos, double *vel, const double *phi)
+{
+  double h[3] = {1.0 / p->dx, 1.0 / p->dy, 1.0 / p->dz};
+  double tscale = 0.5 * (p->qi / p->mi) * dt;
+  double t = tscale * p->B0;
+  double s = 2.0 * t / (1 + t * t);
+  double Escale[3];
+  double E[3];
+  long ip;
+  int d;
+  for (d = 0; d < 3; d++) Escale[d] = -0.5 * h[d] * tscale;
+  for (ip = 0; ip < np; ip++) {
+    double x = pos[ip * 3 + 0] / p->dx;
+    double y = pos[ip * 3 + 1] / p->dy;
+    double z = pos[ip * 3 + 2] / p->dz;
+    double vxm, vym, vz, vxt, vxp, vyp;
+    oracle_difference_vector3d(phi, p->Nx, p->Ny, p->Nz, x, y, z, E);
+    vxm = vel[ip * 3 + 0] + Escale[0] * E[0];
+    vym = vel[ip * 3 + 1] + Escale[1] * E[1];
+    vz  = vel[ip * 3 + 2] + Escale[2] * E[2];
+    vxt = vxm + vym * t;
+    vyp = vym - vxt * s;
+    vxp = vxt + vyp * t;
+    vel[ip * 3 + 0] = vxp;
+    vel[ip * 3 + 1] = vyp;
+    vel[ip * 3 + 2] = vz;
+  }
+}
+
+/* positions.c:455-459 (UpdatePositions drift). */
+void oracle_update_positions(double dt, long np, double *pos, const double *vel)
+{
+  long ip;
+  int d;
+  for (ip = 0; ip < np; ip++)
+    for (d = 0; d < 3; d++) pos[ip * 3 + d] += vel[ip * 3 + d] * dt;
+}
+
+/* boundaries.c:6-34 (ApplyBC, all-periodic): one wrap per axis. */
+void oracle_apply_bc(const oracle_params *p, long np, double *pos)
+{
+  double L[3] = {p->Lx, p->Ly, p->Lz};
+  double r0[3] = {p->x0, p->y0, p->z0};
+  double r1[3] = {p->x1, p->y1, p->z1};
+  long ip;
+  int d;
+  for (ip = 0; ip < np; ip++) {
+    for (d = 0; d < 3; d++) {
+      double r = pos[ip * 3 + d];
+      if (r < r0[d]) r += L[d];
+      if (r >= r1[d]) r -= L[d];
+      pos[ip * 3 + d] = r;
+    }
+  }
+}
+
+/* Particles DMSwarmMigrate would drop (boundaries.c:92, 101-109): a coordinate
+ * outside [x0, x1] or NaN.  A coordinate equal to x1 (a -ulp wrap) is kept: the
+ * cell DM's point location treats the closed upper face as inside.            */
+long oracle_count_outside(const oracle_params *p, long np, const double *pos)
+{
+  double r0[3] = {p->x0, p->y0, p->z0};
+  double r1[3] = {p->x1, p->y1, p->z1};
+  long ip, n = 0;
+  int d;
+  for (ip = 0; ip < np; ip++) {
+    int out = 0;
+    for (d = 0; d < 3; d++) {
+      double r = pos[ip * 3 + d];
+      if (!(r >= r0[d] && r <= r1[d])) out = 1;
+    }
+    n += out;
+  }
+  return n;
+}
+
+/* moments.c:108-226 (Collect3DFluidMoments).  Accumulation into the ghost-width-1
+ * local array followed by the ADD_VALUES fold equals accumulation at the wrapped
+ * node; a coordinate equal to L (node N+1, outside the reference's ghosted array)
+ * is wrapped the same way.                                                     */
+void oracle_collect_moments(const oracle_params *p, long np, const double *pos,
+                            const double *vel, double *m)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+  long ncell = (long)Nx * Ny * Nz;
+  long ip;
+  memset(m, 0, sizeof(double) * 4 * ncell);
+  for (ip = 0; ip < np; ip++) {
+    double x = pos[ip * 3 + 0] / p->dx;
+    double y = pos[ip * 3 + 1] / p->dy;
+    double z = pos[ip * 3 + 2] / p->dz;
+    int ixl = (int)x, ixh = ixl + 1;
+    int iyl = (int)y, iyh = iyl + 1;
+    int izl = (int)z, izh = izl + 1;
+    double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+    double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+    double wzh = z - (double)izl, wzl = 1.0 - wzh;
+    double w8[8];
+    long n8[8];
+    int c, d;
+    w8[0] = wzh * wyh * wxh; n8[0] = nidx(ixh, iyh, izh, Nx, Ny, Nz);
+    w8[1] = wzl * wyh * wxh; n8[1] = nidx(ixh, iyh, izl, Nx, Ny, Nz);
+    w8[2] = wzh * wyl * wxh; n8[2] = nidx(ixh, iyl, izh, Nx, Ny, Nz);
+    w8[3] = wzl * wyl * wxh; n8[3] = nidx(ixh, iyl, izl, Nx, Ny, Nz);
+    w8[4] = wzh * wyh * wxl; n8[4] = nidx(ixl, iyh, izh, Nx, Ny, Nz);
+    w8[5] = wzl * wyh * wxl; n8[5] = nidx(ixl, iyh, izl, Nx, Ny, Nz);
+    w8[6] = wzh * wyl * wxl; n8[6] = nidx(ixl, iyl, izh, Nx, Ny, Nz);
+    w8[7] = wzl * wyl * wxl; n8[7] = nidx(ixl, iyl, izl, Nx, Ny, Nz);
+    for (c = 0; c < 8; c++) m[n8[c] * 4 + 0] += w8[c];
+    for (d = 0; d < 3; d++) {
+      double w = vel[ip * 3 + d];
+      for (c = 0; c < 8; c++) m[n8[c] * 4 + 1 + d] += w * w8[c];
+    }
+  }
+}
+
+/* Per-cell particle counts from (int)(pos/d) per axis, the cell the reference's
+ * deposit assigns (moments.c:154-174); coordinates equal to L count in cell 0. */
+void oracle_cell_counts(const oracle_params *p, long np, const double *pos, int *counts)
+{
+  long ip;
+  memset(counts, 0, sizeof(int) * (long)p->Nx * p->Ny * p->Nz);
+  for (ip = 0; ip < np; ip++) {
+    int i = (int)(pos[ip * 3 + 0] / p->dx);
+    int j = (int)(pos[ip * 3 + 1] / p->dy);
+    int k = (int)(pos[ip * 3 + 2] / p->dz);
+    counts[nidx(i, j, k, p->Nx, p->Ny, p->Nz)]++;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* random.c: NR 2nd-edition generators with their static state made explicit. */
+/* ------------------------------------------------------------------------- */
+#define NR_IA 16807
+#define NR_IM 2147483647
+#define NR_AM (1.0 / NR_IM)
+#define NR_IQ 127773
+#define NR_IR 2836
+#define NR_NTAB 32
+#define NR_NDIV (1 + (NR_IM - 1) / NR_NTAB)
+#define NR_RNMX (1.0 - DBL_EPSILON)
+#define NR_MBIG 1000000000
+#define NR_MSEED 161803398
+#define NR_MZ 0
+#define NR_FAC (1.0 / NR_MBIG)
+
+void oracle_nr_init(oracle_nr_state *s) { memset(s, 0, sizeof(*s)); }
+
+long oracle_getseed(int rank) { return (long)(-(rank + 1) * 12345); } /* random.c:292-295 */
+
+/* random.c:24-72 (ran1). */
+float oracle_ran1(oracle_nr_state *s, long *idum)
+{
+  int j;
+  long k;
+  float temp;
+  if (*idum <= 0 || !s->r1_iy) {
+    if (-(*idum) < 1) *idum = 1;
+    else *idum = -(*idum);
+    for (j = NR_NTAB + 7; j >= 0; j--) {
+      k = (*idum) / NR_IQ;
+      *idum = NR_IA * (*idum - k * NR_IQ) - NR_IR * k;
+      if (*idum < 0) *idum += NR_IM;
+      if (j < NR_NTAB) s->r1_iv[j] = *idum;
+    }
+    s->r1_iy = s->r1_iv[0];
+  }
+  k = (*idum) / NR_IQ;
+  *idum = NR_IA * (*idum - k * NR_IQ) - NR_IR * k;
+  if (*idum < 0) *idum += NR_IM;
+  j = (int)(s->r1_iy / NR_NDIV);
+  s->r1_iy = s->r1_iv[j];
+  s->r1_iv[j] = *idum;
+  if ((temp = NR_AM * s->r1_iy) > NR_RNMX) return NR_RNMX;
+  return temp;
+}
+
+/* random.c:94-154 (ran3). */
+float oracle_ran3(oracle_nr_state *s, long *idum)
+{
+  long mj, mk;
+  int i, ii, k;
+  if (*idum < 0 || s->r3_iff == 0) {
+    s->r3_iff = 1;
+    mj = labs(NR_MSEED - labs(*idum));
+    mj %= NR_MBIG;
+    s->r3_ma[55] = mj;
+    mk = 1;
+    for (i = 1; i <= 54; i++) {
+      ii = (21 * i) % 55;
+      s->r3_ma[ii] = mk;
+      mk = mj - mk;
+      if (mk < NR_MZ) mk += NR_MBIG;
+      mj = s->r3_ma[ii];
+    }
+    for (k = 1; k <= 4; k++) {
+      for (i = 1; i <= 55; i++) {
+        s->r3_ma[i] -= s->r3_ma[1 + (i + 30) % 55];
+        if (s->r3_ma[i] < NR_MZ) s->r3_ma[i] += NR_MBIG;
+      }
+    }
+    s->r3_inext = 0;
+    s->r3_inextp = 31;
+    *idum = 1;
+  }
+  if (++s->r3_inext == 56) s->r3_inext = 1;
+  if (++s->r3_inextp == 56) s->r3_inextp = 1;
+  mj = s->r3_ma[s->r3_inext] - s->r3_ma[s->r3_inextp];
+  if (mj < NR_MZ) mj += NR_MBIG;
+  s->r3_ma[s->r3_inext] = mj;
+  return mj * NR_FAC;
+}
+
+/* random.c:172-204 (gasdev, polar Box-Muller over ran1). */
+float oracle_gasdev(oracle_nr_state *s, long *idum)
+{
+  float fac, rsq, v1, v2;
+  if (*idum < 0) s->g_iset = 0;
+  if (s->g_iset == 0) {
+    do {
+      v1 = 2.0 * oracle_ran1(s, idum) - 1.0;
+      v2 = 2.0 * oracle_ran1(s, idum) - 1.0;
+      rsq = v1 * v1 + v2 * v2;
+    } while (rsq >= 1.0 || rsq == 0.0);
+    fac = sqrt(-2.0 * log(rsq) / rsq);
+    s->g_gset = v1 * fac;
+    s->g_iset = 1;
+    return v2 * fac;
+  }
+  s->g_iset = 0;
+  return s->g_gset;
+}
+
+/* ------------------------------------------------------------------------- */
+/* collisions.c:9-167 (ScatterElastic).                                      */
+/* ------------------------------------------------------------------------- */
+
+/* x86-64 double -> int32 conversion (cvttsd2si): out-of-range and NaN give INT_MIN. */
+static int x86_double_to_int(double v)
+{
+  if (!(v > -2147483649.0 && v < 2147483648.0)) return (int)0x80000000u;
+  return (int)v;
+}
+
+static double vrm_of(const oracle_params *p)
+{
+  double vi0[3] = {p->vi0x, p->vi0y, p->vi0z};
+  double vn0[3] = {p->vn0x, p->vn0y, p->vn0z};
+  double tmp = 0.0;
+  int d;
+  for (d = 0; d < 3; d++) tmp += (vi0[d] - vn0[d]) * (vi0[d] - vn0[d]);
+  return 4.0 * p->viT + sqrt(tmp); /* collisions.c:48-52 */
+}
+
+/* collisions.c:58-63 */
+int oracle_collision_count(const oracle_params *p, long np, double dt)
+{
+  double fc = p->nui * dt;
+  double vrm = vrm_of(p);
+  int Nc = x86_double_to_int(((double)np * fc * vrm) / ((p->vnT + p->vn0) * sqrt(p->mn / p->mi)));
+  if (Nc > np) Nc = (int)np;
+  else if (Nc < 0) Nc = 0;
+  return Nc;
+}
+
+/* One attempt of the collisions.c:70-157 loop body on velocity vi_in. */
+void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const double vi_in[3],
+                        double vf_out[3], int *accepted, int *failed)
+{
+  double vn0[3] = {p->vn0x, p->vn0y, p->vn0z};
+  double vnT = p->vnT, mi = p->mi, mn = p->mn, M = mn + mi;
+  double vrm = vrm_of(p);
+  double vi[3], vn[3], vr[3], vc[3], u2[3], vf[3];
+  double tmp = 0.0, vrr, vcr, u0x, u0y, u0z, uperp, uphi;
+  double costht, sintht, cosphi, sinphi, u1x, u1y, u1z, vfr, ratio;
+  float acc;
+  int d;
+  *accepted = 0;
+  *failed = 0;
+  for (d = 0; d < 3; d++) {
+    vi[d] = vi_in[d];
+    vn[d] = a->g[d] * vnT + vn0[d];
+    vr[d] = vi[d] - vn[d];
+    tmp += vr[d] * vr[d];
+  }
+  vrr = sqrt(tmp);
+  acc = a->u_acc1 * a->u_acc2; /* PetscSqr(ran3(&seed)): two draws, float product */
+  if (!(acc < vrr * vrr / (vrm * vrm))) {
+    for (d = 0; d < 3; d++) vf_out[d] = vi_in[d];
+    return;
+  }
+  *accepted = 1;
+  tmp = 0.0;
+  for (d = 0; d < 3; d++) {
+    vc[d] = (vi[d] * mi + vn[d] * mn) / M;
+    tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
+  }
+  vcr = sqrt(tmp);
+  u0z = 2.0 * a->u_z - 1.0;
+  uperp = sqrt(1.0 - u0z * u0z);
+  uphi = 2 * 3.14159265358979323846 * a->u_phi;
+  u0x = uperp * cos(uphi);
+  u0y = uperp * sin(uphi);
+  costht = vr[2] / vrr;
+  sintht = sqrt(1.0 - costht * costht);
+  cosphi = vr[0] / (vrr * sintht);
+  sinphi = vr[1] / (vrr * sintht);
+  u1x = u0z * sintht + u0x * costht;
+  u1y = u0y;
+  u1z = u0z * costht - u0x * sintht;
+  u2[0] = u1x * cosphi - u1y * sinphi;
+  u2[1] = u1x * sinphi + u1y * cosphi;
+  u2[2] = u1z;
+  tmp = 0.0;
+  for (d = 0; d < 3; d++) {
+    vf[d] = vc[d] + vcr * u2[d];
+    tmp += vf[d] * vf[d];
+  }
+  vfr = sqrt(tmp);
+  ratio = vfr / p->viT;
+  if (ratio > 10) {
+    *failed = 1;
+    for (d = 0; d < 3; d++) vf_out[d] = vi_in[d];
+    return;
+  }
+  for (d = 0; d < 3; d++) vf_out[d] = vf[d];
+}
+
+static long pick_ip(long np, float u)
+{
+  /* (PetscInt)(np*ran3(&seed)): int*float is a float product (collisions.c:72).
+   * A product that rounds up to np would index past the array in the reference;
+   * both implementations clamp it to np-1. */
+  long ip = (long)(int)((float)np * u);
+  if (ip >= np) ip = np - 1;
+  if (ip < 0) ip = 0;
+  return ip;
+}
+
+/* Return codes: 0 ok, 2 collision failure (collisions.c:145-147 Abort), 3 records
+ * exhausted before Nc successes. */
+int oracle_scatter_elastic_predrawn(const oracle_params *p, double dt, long np, double *vel,
+                                    const oracle_attempt *att, long n_att, int Nc,
+                                    long *n_used, int *Ns_out, int *Nf_out)
+{
+  int Ns = 0, Nf = 0, rc = 0;
+  long a = 0;
+  (void)dt;
+  while (Ns < Nc) {
+    long ip;
+    int acc, fail;
+    double vf[3];
+    if (a >= n_att) { rc = 3; break; }
+    ip = pick_ip(np, att[a].u_ip);
+    oracle_collide_one(p, &att[a], &vel[ip * 3], vf, &acc, &fail);
+    a++;
+    if (!acc) continue;
+    if (fail) {
+      Nf++;
+      if (Nf >= 10) { rc = 2; break; }
+    } else {
+      vel[ip * 3 + 0] = vf[0];
+      vel[ip * 3 + 1] = vf[1];
+      vel[ip * 3 + 2] = vf[2];
+      Ns++;
+    }
+  }
+  if (n_used) *n_used = a;
+  if (Ns_out) *Ns_out = Ns;
+  if (Nf_out) *Nf_out = Nf;
+  return rc;
+}
+
+/* The reference stream: ScatterElastic with ran3/gasdev drawn in the reference's
+ * order from the caller-held static state, starting from a local seed (getseed).
+ * Each consumed attempt is appended to log (u_z, u_phi = -1 when not drawn).    */
+int oracle_scatter_elastic_nr(const oracle_params *p, double dt, long np, double *vel,
+                              oracle_nr_state *st, long seed, oracle_attempt *log,
+                              long log_cap, long *n_log, int *Ns_out, int *Nf_out)
+{
+  int Nc = oracle_collision_count(p, np, dt);
+  double vrm = vrm_of(p);
+  int Ns = 0, Nf = 0, rc = 0;
+  long nl = 0;
+  (void)vrm;
+  while (Ns < Nc) {
+    oracle_attempt a;
+    long ip;
+    int d, acc, fail;
+    double vf[3], tmp = 0.0, vrr;
+    a.u_ip = oracle_ran3(st, &seed);
+    ip = pick_ip(np, a.u_ip);
+    for (d = 0; d < 3; d++) a.g[d] = oracle_gasdev(st, &seed);
+    a.u_acc1 = oracle_ran3(st, &seed);
+    a.u_acc2 = oracle_ran3(st, &seed);
+    a.u_z = -1.0f;
+    a.u_phi = -1.0f;
+    /* The acceptance decision needs vrr before the angle draws. */
+    {
+      double vn0[3] = {p->vn0x, p->vn0y, p->vn0z};
+      for (d = 0; d < 3; d++) {
+        double vr = vel[ip * 3 + d] - (a.g[d] * p->vnT + vn0[d]);
+        tmp += vr * vr;
+      }
+      vrr = sqrt(tmp);
+    }
+    if ((float)(a.u_acc1 * a.u_acc2) < vrr * vrr / (vrm_of(p) * vrm_of(p))) {
+      a.u_z = oracle_ran3(st, &seed);
+      a.u_phi = oracle_ran3(st, &seed);
+    }
+    if (log && nl < log_cap) log[nl] = a;
+    nl++;
+    oracle_collide_one(p, &a, &vel[ip * 3], vf, &acc, &fail);
+    if (!acc) continue;
+    if (fail) {
+      Nf++;
+      if (Nf >= 10) { rc = 2; break; }
+    } else {
+      vel[ip * 3 + 0] = vf[0];
+      vel[ip * 3 + 1] = vf[1];
+      vel[ip * 3 + 2] = vf[2];
+      Ns++;
+    }
+  }
+  if (n_log) *n_log = nl;
+  if (Ns_out) *Ns_out = Ns;
+  if (Nf_out) *Nf_out = Nf;
+  return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* lhs.c:337-505 (ComputeFullLHS3D): 11 coefficients per row in the reference's */
+/* column order (i+1), (i-1), (j+1), (j-1), (k+1), (k-1), (i+1,j+1), (i+1,j-1), */
+/* (i-1,j+1), (i-1,j-1), (i,j,k) (lhs.c:426-479).                              */
+/* ------------------------------------------------------------------------- */
+static const int lhs_off[11][3] = {
+  {+1, 0, 0}, {-1, 0, 0}, {0, +1, 0}, {0, -1, 0}, {0, 0, +1}, {0, 0, -1},
+  {+1, +1, 0}, {+1, -1, 0}, {-1, +1, 0}, {-1, -1, 0}, {0, 0, 0}};
+
+void oracle_full_lhs3d_coeffs(const oracle_params *p, const double *m, double *coef)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+  double kappa = p->kappa_e;
+  double hxx = 1.0 / (2.0 * p->dx * p->dx);
+  double hyy = 1.0 / (2.0 * p->dy * p->dy);
+  double hzz = 1.0 / (2.0 * p->dz * p->dz);
+  double hod = 1.0 / (8.0 * p->dy * p->dx);
+  double sxx = p->scale * hxx;
+  double syy = p->scale * hyy;
+  double szz = p->scale * (1 + kappa * kappa) * hzz;
+  double sod = p->scale * kappa * hod;
+  int i, j, k;
+  for (k = 0; k < Nz; k++)
+    for (j = 0; j < Ny; j++)
+      for (i = 0; i < Nx; i++) {
+        long r = ((long)k * Ny + j) * Nx + i;
+        double nijk = m[nidx(i, j, k, Nx, Ny, Nz) * 4];
+        double nmjk = m[nidx(i - 1, j, k, Nx, Ny, Nz) * 4];
+        double npjk = m[nidx(i + 1, j, k, Nx, Ny, Nz) * 4];
+        double nimk = m[nidx(i, j - 1, k, Nx, Ny, Nz) * 4];
+        double nipk = m[nidx(i, j + 1, k, Nx, Ny, Nz) * 4];
+        double nijm = m[nidx(i, j, k - 1, Nx, Ny, Nz) * 4];
+        double nijp = m[nidx(i, j, k + 1, Nx, Ny, Nz) * 4];
+        double *c = &coef[r * 11];
+        c[6] = sod * (nipk - npjk);
+        c[7] = sod * (npjk - nimk);
+        c[8] = sod * (nmjk - nipk);
+        c[9] = sod * (nimk - nmjk);
+        c[0] = sxx * (npjk + nijk) + sod * (nipk - nimk);
+        c[1] = sxx * (nijk + nmjk) - sod * (nipk - nimk);
+        c[2] = syy * (nipk + nijk) - sod * (npjk - nmjk);
+        c[3] = syy * (nijk + nimk) + sod * (npjk - nmjk);
+        c[4] = szz * (nijp + nijk);
+        c[5] = szz * (nijk + nijm);
+        c[10] = -(sxx * (npjk + 2 * nijk + nmjk) + syy * (nipk + 2 * nijk + nimk) +
+                  szz * (nijp + 2 * nijk + nijm));
+      }
+}
+
+/* CSR with 11 entries per row, columns in the reference order (not sorted). */
+void oracle_full_lhs3d_csr(const oracle_params *p, const double *m,
+                           int *rowptr, int *cols, double *vals)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+  long n = (long)Nx * Ny * Nz, r;
+  int i, j, k, e;
+  oracle_full_lhs3d_coeffs(p, m, vals);
+  for (k = 0; k < Nz; k++)
+    for (j = 0; j < Ny; j++)
+      for (i = 0; i < Nx; i++) {
+        r = ((long)k * Ny + j) * Nx + i;
+        rowptr[r] = (int)(r * 11);
+        for (e = 0; e < 11; e++)
+          cols[r * 11 + e] = (int)nidx(i + lhs_off[e][0], j + lhs_off[e][1], k + lhs_off[e][2], Nx, Ny, Nz);
+      }
+  rowptr[n] = (int)(n * 11);
+}
+
+void oracle_full_lhs3d_apply(const oracle_params *p, const double *m, const double *x, double *y)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+  long n = (long)Nx * Ny * Nz;
+  double *coef = (double *)malloc(sizeof(double) * 11 * n);
+  int i, j, k, e;
+  oracle_full_lhs3d_coeffs(p, m, coef);
+  for (k = 0; k < Nz; k++)
+    for (j = 0; j < Ny; j++)
+      for (i = 0; i < Nx; i++) {
+        long r = ((long)k * Ny + j) * Nx + i;
+        double acc = 0.0;
+        for (e = 0; e < 11; e++)
+          acc += coef[r * 11 + e] * x[nidx(i + lhs_off[e][0], j + lhs_off[e][1], k + lhs_off[e][2], Nx, Ny, Nz)];
+        y[r] = acc;
+      }
+  free(coef);
+}
+
+/* lhs.c:100-191 (ComputeLaplacianLHS3D), the 7-point test operator. */
+void oracle_laplacian_lhs3d_apply(const oracle_params *p, const double *x, double *y)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+  double vpjk = 1.0 / (p->dx * p->dx), vmjk = vpjk;
+  double vipk = 1.0 / (p->dy * p->dy), vimk = vipk;
+  double vijp = 1.0 / (p->dz * p->dz), vijm = vijp;
+  double vijk = -(vpjk + vipk + vijp + vmjk + vimk + vijm);
+  int i, j, k;
+  for (k = 0; k < Nz; k++)
+    for (j = 0; j < Ny; j++)
+      for (i = 0; i < Nx; i++) {
+        long r = ((long)k * Ny + j) * Nx + i;
+        y[r] = vpjk * x[nidx(i + 1, j, k, Nx, Ny, Nz)] + vmjk * x[nidx(i - 1, j, k, Nx, Ny, Nz)] +
+               vipk * x[nidx(i, j + 1, k, Nx, Ny, Nz)] + vimk * x[nidx(i, j - 1, k, Nx, Ny, Nz)] +
+               vijp * x[nidx(i, j, k + 1, Nx, Ny, Nz)] + vijm * x[nidx(i, j, k - 1, Nx, Ny, Nz)] +
+               vijk * x[r];
+      }
+}
+
+/* MatNullSpaceRemove with the constant null space: subtract the mean. */
+void oracle_remove_mean(long n, double *v)
+{
+  double s = 0.0;
+  long i;
+  for (i = 0; i < n; i++) s += v[i];
+  s /= (double)n;
+  for (i = 0; i < n; i++) v[i] -= s;
+}
+
+/* rhs.c:259-373 (ComputeFullRHS3D). */
+void oracle_full_rhs3d(const oracle_params *p, const double *m, double *b)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz;
+  double kappa = p->kappa_e, E0 = p->E0, scale = p->scale;
+  double cth = p->gammaT * OR_KB * p->Te / OR_Q;
+  double cG = (1 + kappa * kappa) * p->me * p->nue / OR_Q;
+  double hx = 1.0 / (2.0 * p->dx), hy = 1.0 / (2.0 * p->dy), hz = 1.0 / (2.0 * p->dz);
+  double hxx = 1.0 / (p->dx * p->dx), hyy = 1.0 / (p->dy * p->dy), hzz = 1.0 / (p->dz * p->dz);
+  int i, j, k;
+#define MN(ii, jj, kk, dof) m[nidx((ii), (jj), (kk), Nx, Ny, Nz) * 4 + (dof)]
+  for (k = 0; k < Nz; k++)
+    for (j = 0; j < Ny; j++)
+      for (i = 0; i < Nx; i++) {
+        double nijk = MN(i, j, k, 0), npjk = MN(i + 1, j, k, 0), nmjk = MN(i - 1, j, k, 0);
+        double nipk = MN(i, j + 1, k, 0), nimk = MN(i, j - 1, k, 0);
+        double nijp = MN(i, j, k + 1, 0), nijm = MN(i, j, k - 1, 0);
+        double dndx = hx * (npjk - nmjk);
+        double dndy = hy * (nipk - nimk);
+        double d2ndxx = hxx * (npjk - 2 * nijk + nmjk);
+        double d2ndyy = hyy * (nipk - 2 * nijk + nimk);
+        double d2ndzz = hzz * (nijp - 2 * nijk + nijm);
+        double dGxdx = hx * (MN(i + 1, j, k, 1) - MN(i - 1, j, k, 1));
+        double dGydy = hy * (MN(i, j + 1, k, 2) - MN(i, j - 1, k, 2));
+        double dGzdz = hz * (MN(i, j, k + 1, 3) - MN(i, j, k - 1, 3));
+        double Eterm = (dndy - kappa * dndx) * E0;
+        double Pterm = cth * (d2ndxx + d2ndyy + (1 + kappa * kappa) * d2ndzz);
+        double Gterm = cG * (dGxdx + dGydy + dGzdz);
+        b[((long)k * Ny + j) * Nx + i] = scale * (Eterm + Pterm + Gterm);
+      }
+#undef MN
+  oracle_remove_mean((long)Nx * Ny * Nz, b);
+}
+
+/* rhs.c:86-148 (ComputeSinusoidalRHS3D). */
+void oracle_sinusoidal_rhs3d(const oracle_params *p, double *b)
+{
+  int Nx = p->Nx, Ny = p->Ny, Nz = p->Nz, i, j, k;
+  const double PI = 3.14159265358979323846;
+  for (k = 0; k < Nz; k++)
+    for (j = 0; j < Ny; j++)
+      for (i = 0; i < Nx; i++) {
+        double x = ((double)i + 0.5) * p->dx;
+        double y = ((double)j + 0.5) * p->dy;
+        double z = ((double)k + 0.5) * p->dz;
+        double Cx = cos(2 * PI * x), Cy = cos(2 * PI * y), Cz = cos(2 * PI * z);
+        double val = Cx * Cy * Cz;
+        b[((long)k * Ny + j) * Nx + i] = p->n0 * val * p->scale;
+      }
+  oracle_remove_mean((long)Nx * Ny * Nz, b);
+}
+
+/* ------------------------------------------------------------------------- */
+/* CPU baseline loop: src/pic.c:246-254 order with the potential held fixed.   */
+/* ------------------------------------------------------------------------- */
+static double now_s(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int oracle_particle_steps(const oracle_params *p, double dt, int nt, long np, double *pos,
+                          double *vel, const double *phi, double *moments,
+                          oracle_nr_state *st, int collide, double t[4])
+{
+  int it, rc = 0;
+  t[0] = t[1] = t[2] = t[3] = 0.0;
+  for (it = 0; it < nt; it++) {
+    double a = now_s(), b, c, d, e;
+    int Ns, Nf;
+    long nl;
+    oracle_boris_bz3d(p, dt, np, pos, vel, phi);
+    b = now_s();
+    if (collide && p->mn > 0.0) {
+      rc = oracle_scatter_elastic_nr(p, dt, np, vel, st, oracle_getseed(0), NULL, 0, &nl, &Ns, &Nf);
+      if (rc) return rc;
+    }
+    c = now_s();
+    oracle_update_positions(dt, np, pos, vel);
+    oracle_apply_bc(p, np, pos);
+    d = now_s();
+    oracle_collect_moments(p, np, pos, vel, moments);
+    e = now_s();
+    t[0] += b - a;
+    t[1] += c - b;
+    t[2] += d - c;
+    t[3] += e - d;
+  }
+  return rc;
+}

@@ -1,0 +1,73 @@
+"""The C-ABI library loads, exports every declared symbol, and its host-only
+functions agree with the oracle (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from ecsprt_amd import _lib, configs
+from ecsprt_amd import pic
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "ecsprt_hip.h")).read()
+    return sorted(set(re.findall(r"\b(ecsprt_hip_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.load()
+    names = declared_symbols()
+    assert len(names) >= 40
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_struct_layouts_match_header():
+    # Field order and count of the ctypes mirrors follow the header structs.
+    text = open(os.path.join(ROOT, "include", "ecsprt_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    for cname, mirror in (("ecsprt_params", _lib.Params), ("ecsprt_cli", _lib.CLI),
+                          ("ecsprt_options", _lib.Options)):
+        body = re.search(r"typedef struct \{([^{}]*?)\} " + cname + ";", text, re.S).group(1)
+        fields = re.findall(r"\b(\w+)\s*[,;]", body)
+        assert [f for f, _ in mirror._fields_] == fields, cname
+
+
+def test_philox_known_answers():
+    """Random123 known-answer vectors for Philox4x32-10."""
+    assert pic.philox4x32_10([0, 0, 0, 0], [0, 0]) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert pic.philox4x32_10([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2) == [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert pic.philox4x32_10([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0]) == \
+        [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+@pytest.mark.parametrize("wl", [configs.C0, configs.C1, configs.small(12)])
+def test_setup_context_matches_oracle(wl):
+    a = pic.setup_context(**wl.cli)
+    b = oracle.setup_context(**wl.cli)
+    for name, _ in a._fields_:
+        assert getattr(a, name) == getattr(b, name), name
+
+
+@pytest.mark.parametrize("Tn_over_Ti,nui,np_", [(1.0, 0.02, 10 ** 6), (0.01, 0.01, 2097152), (1e-30, 0.01, 10 ** 6),
+                                                 (100.0, 5.0, 1000)])
+def test_collision_count_matches_oracle(Tn_over_Ti, nui, np_):
+    cli = configs.base_cli(16, nui=nui, Tn_over_Ti=Tn_over_Ti)
+    a = pic.setup_context(**cli)
+    b = oracle.setup_context(**cli)
+    assert pic.collision_count(a, np_, 1.0) == oracle.collision_count(b, np_, 1.0)
+
+
+def test_workload_collision_counts():
+    """The counts the workloads run with (recorded in the design notes)."""
+    p0 = oracle.setup_context(**configs.C0.cli)
+    p1 = oracle.setup_context(**configs.C1.cli)
+    n0 = oracle.collision_count(p0, configs.C0.np, configs.C0.dt)
+    n1 = oracle.collision_count(p1, configs.C1.np, configs.C1.dt)
+    assert 0.3 * configs.C0.np < n0 < 0.5 * configs.C0.np
+    assert 0.05 * configs.C1.np < n1 < 0.1 * configs.C1.np

@@ -1,0 +1,201 @@
+"""GPU parity of the particle stages against the oracle (through the C ABI)."""
+import numpy as np
+import pytest
+
+import oracle
+from ecsprt_amd import configs, pic, _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make(wl, seed=1, **opt):
+    p = pic.setup_context(**wl.cli)
+    po = oracle.setup_context(**wl.cli)
+    o = pic.default_options(**opt)
+    ctx = pic.Context(p, o)
+    rng = np.random.default_rng(seed)
+    n = wl.np
+    pos = np.column_stack([rng.uniform(0, po.Lx, n), rng.uniform(0, po.Ly, n), rng.uniform(0, po.Lz, n)])
+    vel = rng.normal(size=(n, 3)) * np.array([po.viTx, po.viTy, po.viTz])
+    return ctx, po, pos, vel
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("N,ppc,tile", [(12, 8, 4), (16, 6, 8), (20, 4, 8), (9, 10, 4)])
+def test_push_parity(N, ppc, tile):
+    say(f"push parity N={N} ppc={ppc} tile={tile}")
+    wl = configs.small(N, ppc)
+    ctx, po, pos, vel = make(wl, tile=tile)
+    rng = np.random.default_rng(2)
+    phi = rng.normal(size=(N, N, N))
+    ctx.set_particles(pos, vel)
+    ctx.set_potential(phi)
+    ctx.push(0.9)
+    gpos, gvel = ctx.get_particles()
+    ovel = vel.copy()
+    oracle.boris(po, 0.9, pos, ovel, phi)
+    np.testing.assert_array_equal(gpos, pos)
+    assert rel(gvel, ovel) <= 1e-12
+    say("  max |dv| =", np.abs(gvel - ovel).max())
+    ctx.close()
+
+
+@pytest.mark.parametrize("N,ppc,tile", [(12, 8, 4), (16, 6, 8), (10, 5, 8)])
+def test_drift_wrap_rebin_parity(N, ppc, tile):
+    say(f"drift parity N={N} ppc={ppc} tile={tile}")
+    wl = configs.small(N, ppc)
+    ctx, po, pos, vel = make(wl, tile=tile)
+    vel *= 2.5  # many particles leave their tile and wrap
+    ctx.set_particles(pos, vel)
+    for step in range(3):
+        ctx.update_positions(1.0)
+        oracle.update_positions(1.0, pos, vel)
+        oracle.apply_bc(po, pos)
+        gpos, gvel = ctx.get_particles()
+        np.testing.assert_array_equal(gpos, pos)
+        np.testing.assert_array_equal(gvel, vel)
+        np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
+    assert ctx.num_particles == wl.np
+    ctx.close()
+
+
+@pytest.mark.parametrize("N,ppc,tile", [(12, 8, 4), (16, 6, 8), (9, 7, 4), (24, 2, 8)])
+def test_deposit_parity(N, ppc, tile):
+    say(f"deposit parity N={N} ppc={ppc} tile={tile}")
+    wl = configs.small(N, ppc)
+    ctx, po, pos, vel = make(wl, tile=tile)
+    ctx.set_particles(pos, vel)
+    ctx.collect()
+    gm = ctx.get_moments()
+    om = oracle.collect_moments(po, pos, vel)
+    for d in range(4):
+        assert rel(gm[..., d], om[..., d]) <= 1e-12, d
+    assert gm[..., 0].sum() == pytest.approx(wl.np, rel=1e-12)
+    ctx.close()
+
+
+def test_deposit_edge_positions():
+    say("deposit edge positions")
+    wl = configs.small(8, 1)
+    ctx, po, pos, vel = make(wl, tile=4)
+    L = po.Lx
+    edge = np.array([[0.0, 0.0, 0.0], [np.nextafter(L, 0), 3.0, 7.5], [4.0, np.nextafter(L, 0), 0.0],
+                     [7.999, 7.999, 7.999], [4.0, 4.0, 4.0]])
+    pos[:len(edge)] = edge
+    ctx.set_particles(pos, vel)
+    ctx.collect()
+    om = oracle.collect_moments(po, pos, vel)
+    gm = ctx.get_moments()
+    assert rel(gm, om) <= 1e-12
+    np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
+    ctx.close()
+
+
+def test_stage_sequence_many_steps_with_spills():
+    """Push, drift+rebin, deposit for several steps with tiny tile capacity slack so
+    that tiles overflow and the spill/rebuild path runs."""
+    say("stage sequence with spills")
+    wl = configs.small(12, 6)
+    ctx, po, pos, vel = make(wl, tile=4, slack=1e-9)
+    rng = np.random.default_rng(3)
+    phi = 0.05 * rng.normal(size=(12, 12, 12))
+    ctx.set_particles(pos, vel)
+    ctx.set_potential(phi)
+    for step in range(4):
+        ctx.push(1.0)
+        ctx.update_positions(1.0)
+        ctx.collect()
+        oracle.boris(po, 1.0, pos, vel, phi)
+        oracle.update_positions(1.0, pos, vel)
+        oracle.apply_bc(po, pos)
+    gpos, gvel = ctx.get_particles()
+    assert rel(gpos, pos) <= 1e-12
+    assert rel(gvel, vel) <= 1e-12
+    om = oracle.collect_moments(po, pos, vel)
+    assert rel(ctx.get_moments(), om) <= 1e-12
+    ctx.close()
+
+
+def test_fused_particle_pass_matches_stages():
+    """ecsprt_hip_step with collisions off equals push, drift, wrap, deposit."""
+    say("fused step vs stages")
+    wl = configs.small(16, 6)
+    ctx, po, pos, vel = make(wl, tile=8, collide_mode=_lib.COLLIDE_OFF, ksp_rtol=1e-10,
+                             operator_mode=_lib.OPERATOR_REBUILD)
+    rng = np.random.default_rng(4)
+    phi = 0.05 * rng.normal(size=(16, 16, 16))
+    phi -= phi.mean()
+    ctx.set_particles(pos, vel)
+    ctx.set_potential(phi)
+    ctx.step(1.0)
+    oracle.boris(po, 1.0, pos, vel, phi)
+    oracle.update_positions(1.0, pos, vel)
+    oracle.apply_bc(po, pos)
+    om = oracle.collect_moments(po, pos, vel)
+    gpos, gvel = ctx.get_particles()
+    assert rel(gvel, vel) <= 1e-12
+    assert rel(gpos, pos) <= 1e-12
+    gm = ctx.get_moments()
+    assert rel(gm, om) <= 1e-12
+    # the solve that followed used those moments
+    b = oracle.full_rhs(po, gm)
+    x = oracle.solve_sparse(po, gm, b)
+    gphi = ctx.get_potential()
+    assert rel(gphi, x) <= 1e-7
+    ctx.close()
+
+
+def test_count_changed_is_an_error():
+    say("count changed error")
+    wl = configs.small(8, 2)
+    ctx, po, pos, vel = make(wl, tile=4)
+    ctx.set_particles(pos, vel)
+    vel_fast = np.zeros_like(vel)
+    vel_fast[0] = [3 * po.Lx, 0, 0]  # leaves the domain after one wrap
+    ctx.set_particles(pos, vel_fast)
+    with pytest.raises(pic.HipError) as e:
+        ctx.update_positions(1.0)
+    assert e.value.code == _lib.ERR_COUNT_CHANGED
+    ctx.close()
+
+
+def test_apply_bc_device_aos():
+    say("apply_bc on device AoS")
+    wl = configs.small(8, 1)
+    ctx, po, pos, vel = make(wl, tile=4)
+    L = po.Lx
+    x = np.array([[L + 0.5, -0.5, 3.0], [-1e-300, L, 2 * L + 1], [np.nextafter(L, 0), 0.0, -0.25]])
+    dv = pic.DeviceVector(x.size)
+    dv.upload(x)
+    ctx.apply_bc_device(dv.ptr, x.shape[0])
+    got = dv.download().reshape(-1, 3)
+    oracle.apply_bc(po, x)
+    np.testing.assert_array_equal(got, x)
+    dv.free()
+    ctx.close()
+
+
+def test_device_init_statistics():
+    say("device initial conditions")
+    wl = configs.small(16, 20)
+    p = pic.setup_context(**wl.cli)
+    ctx = pic.Context(p, pic.default_options())
+    ctx.init_particles(wl.np, 7, 0.2, 2)
+    pos, vel = ctx.get_particles()
+    assert ctx.num_particles == wl.np
+    assert pos.min() >= 0 and pos.max() < 16
+    np.testing.assert_allclose(vel.mean(axis=0), 0, atol=0.02)
+    np.testing.assert_allclose(vel.std(axis=0), 1, atol=0.02)
+    # x density follows 1 + 0.2 sin(2 pi 2 x / L)
+    h, _ = np.histogram(pos[:, 0], bins=32, range=(0, 16))
+    xc = (np.arange(32) + 0.5) * 0.5
+    expect = wl.np / 32 * (1 + 0.2 * np.sin(2 * np.pi * 2 * xc / 16))
+    assert np.abs(h - expect).max() < 5 * np.sqrt(expect.max())
+    ctx.close()
