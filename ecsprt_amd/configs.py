@@ -16,8 +16,10 @@ KB = 1.3807e-23
 
 
 def electron_cli(B0, kappa_e):
+    """nu_e giving kappa_e = Omega_e/nu_e; an unmagnetised case (B0 = 0) keeps nu_e = 1."""
     omega_e = Q * B0 / ME
-    return {"nue": omega_e / kappa_e, "Te": Q / KB}
+    nue = omega_e / kappa_e if omega_e > 0 and kappa_e > 0 else 1.0
+    return {"nue": nue, "Te": Q / KB}
 
 
 def ion_temperature(viT, mi=1.0):

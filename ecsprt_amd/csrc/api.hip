@@ -351,8 +351,8 @@ int ecsprt_hip_step(void *opts, double dt) {
     compute_potential(c);
     if (c->o.timing) {
       ECS_HIP(hipStreamSynchronize(c->stream));
-      c->stage_ms[0] = ev_ms(c, 0, 1);
-      c->stage_ms[1] = 0.0;
+      c->stage_ms[0] = ev_ms(c, 6, 1);
+      c->stage_ms[1] = ev_ms(c, 0, 6);
       c->stage_ms[2] = ev_ms(c, 1, 2);
       c->stage_ms[3] = 0.0;
       c->stage_ms[4] = ev_ms(c, 3, 4);

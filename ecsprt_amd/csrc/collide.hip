@@ -272,31 +272,19 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
   if (rc != ECSPRT_OK) throw Error{rc, msg};
 }
 
-// Acceptance estimate for the Philox mode: one trial attempt per particle.
+// Acceptance estimate for the Philox mode: a dry run of the per-particle Poisson
+// process (velocities are not written) on a stream independent of the real steps.
 __global__ void k_pilot(ParticlesF64 A, const int *base, const int *count, CollideConsts k, uint32_t key0,
-                        uint32_t key1, unsigned long long *ctr) {
+                        uint32_t key1, uint32_t tag, double lam, unsigned long long *ctr) {
   const int t = blockIdx.x;
   const long b = base[t], n = count[t];
-  unsigned long long ns = 0, na = 0;
+  unsigned natt = 0, nsucc = 0, nfail = 0;
   for (long q = threadIdx.x; q < n; q += blockDim.x) {
-    long s = b + q;
-    u32x4 c1 = {{(uint32_t)A.id[s], 0xFFFFFFFFu, 1u, 0x9170F00Du}};
-    u32x4 r1 = philox4x32_10(c1, key0, key1);
-    u32x4 c2 = {{(uint32_t)A.id[s], 0xFFFFFFFFu, 2u, 0x9170F00Du}};
-    u32x4 r2 = philox4x32_10(c2, key0, key1);
-    float g[3];
-    {
-      float u1 = u01(r1.v[0]), u2 = u01(r1.v[1]), u3 = u01(r1.v[2]), u4 = u01(r1.v[3]);
-      float ra = sqrtf(-2.0f * logf(u1)), rb = sqrtf(-2.0f * logf(u3));
-      g[0] = ra * cosf(6.283185307179586f * u2);
-      g[1] = ra * sinf(6.283185307179586f * u2);
-      g[2] = rb * cosf(6.283185307179586f * u4);
-    }
-    double vi[3] = {A.vx[s], A.vy[s], A.vz[s]}, vf[3];
-    int res = collide_one(k, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
-    na++;
-    ns += res == 1;
+    const long s = b + q;
+    double vx = A.vx[s], vy = A.vy[s], vz = A.vz[s];
+    collide_philox(k, key0, key1, tag, lam, A.id[s], vx, vy, vz, natt, nsucc, nfail);
   }
+  unsigned long long na = natt, ns = nsucc;
   for (int o = 32; o > 0; o >>= 1) {
     ns += __shfl_down(ns, o);
     na += __shfl_down(na, o);
@@ -307,17 +295,25 @@ __global__ void k_pilot(ParticlesF64 A, const int *base, const int *count, Colli
   }
 }
 
-void run_collide_philox_pilot(ecsprt_hip_ctx *c, double) {
+static double pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam) {
   unsigned long long zero[2] = {0, 0};
   ECS_HIP(hipMemcpyAsync(c->d_ctr + 6, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
   CollideConsts k = collide_consts_of(c->p);
   k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, (uint32_t)c->o.seed,
-                                             (uint32_t)(c->o.seed >> 32), c->d_ctr);
+                                             (uint32_t)(c->o.seed >> 32), tag, lam, c->d_ctr);
   ECS_HIP(hipGetLastError());
   unsigned long long h[2];
   ECS_HIP(hipMemcpyAsync(h, c->d_ctr + 6, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   ECS_HIP(hipStreamSynchronize(c->stream));
-  c->acc_est = h[0] ? (double)h[1] / (double)h[0] : 0.0;
+  return h[0] ? (double)h[1] / (double)h[0] : 0.0;
+}
+
+void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt) {
+  // First pass at lambda = 1, second at the lambda the first implies.
+  const int Nc = collision_count(&c->p, c->np, dt);
+  double q = pilot_pass(c, 0xFFFFFFF0u, 1.0);
+  if (q > 0 && Nc > 0) q = pilot_pass(c, 0xFFFFFFF1u, (double)Nc / (q * (double)c->np));
+  c->acc_est = q;
 }
 
 }  // namespace ecsprt

@@ -68,53 +68,6 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
   d.id[i] = id;
 }
 
-// Box-Muller pair from two 32-bit draws (statistical collision mode only).
-__device__ inline void box_muller(uint32_t a, uint32_t b, float &g0, float &g1) {
-  float u1 = u01(a), u2 = u01(b);
-  float r = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincosf(6.283185307179586f * u2, &s, &c);
-  g0 = r * c;
-  g1 = r * s;
-}
-
-// Philox collision attempts for one particle: Poisson(lambda) attempts, each the
-// reference's scattering formula on fresh draws.  Counter = (id, step, attempt, tag).
-__device__ inline void collide_philox(const TileArgs &a, int pid, double &vx, double &vy, double &vz,
-                                      unsigned &natt, unsigned &nsucc, unsigned &nfail) {
-  u32x4 c0 = {{(uint32_t)pid, a.step, 0u, 0xC0111DE5u}};
-  u32x4 r0 = philox4x32_10(c0, a.key0, a.key1);
-  float u = u01(r0.v[0]);
-  double lam = a.lambda;
-  double pk = exp(-lam), F = pk;
-  int k = 0;
-  while ((double)u > F && k < 24) {
-    ++k;
-    pk *= lam / k;
-    F += pk;
-  }
-  for (int at = 0; at < k; ++at) {
-    u32x4 c1 = {{(uint32_t)pid, a.step, (uint32_t)(at + 1), 0xA77E3B71u}};
-    u32x4 c2 = {{(uint32_t)pid, a.step, (uint32_t)(at + 1), 0x5CA77E12u}};
-    u32x4 r1 = philox4x32_10(c1, a.key0, a.key1);
-    u32x4 r2 = philox4x32_10(c2, a.key0, a.key1);
-    float g[3], spare;
-    box_muller(r1.v[0], r1.v[1], g[0], g[1]);
-    box_muller(r1.v[2], r1.v[3], g[2], spare);
-    double vi[3] = {vx, vy, vz}, vf[3];
-    int res = collide_one(a.cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
-    natt++;
-    if (res == 1) {
-      vx = vf[0];
-      vy = vf[1];
-      vz = vf[2];
-      nsucc++;
-    } else if (res == 2) {
-      nfail++;
-    }
-  }
-}
-
 template <int MODE>
 __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   extern __shared__ double smem[];
@@ -183,7 +136,8 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       vy = vyp;
       vz = vzm;
     }
-    if ((MODE & M_COLLIDE) && valid) collide_philox(a, pid, vx, vy, vz, natt, nsucc, nfail);
+    if ((MODE & M_COLLIDE) && valid)
+      collide_philox(a.cc, a.key0, a.key1, a.step, a.lambda, pid, vx, vy, vz, natt, nsucc, nfail);
     bool lost = false;
     int ntile = t;
     if ((MODE & M_DRIFT) && valid) {
@@ -872,8 +826,10 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
     a.key1 = (uint32_t)(c->o.seed >> 32);
     a.step = (uint32_t)c->step_index;
     a.lambda = (c->acc_est > 0.0 && c->np > 0) ? (double)Nc / (c->acc_est * (double)c->np) : 0.0;
+    if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
     launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
   } else {
+    if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
     launch_tile<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
   }
   if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[1], c->stream));

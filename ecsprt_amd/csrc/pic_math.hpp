@@ -7,6 +7,7 @@
 #include <math.h>
 
 #include "common.hpp"
+#include "philox.hpp"
 
 namespace ecsprt {
 
@@ -155,6 +156,54 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   vf[1] = w[1];
   vf[2] = w[2];
   return 1;
+}
+
+// Box-Muller pair from two 32-bit draws (statistical collision mode only).
+__device__ inline void box_muller(uint32_t a, uint32_t b, float &g0, float &g1) {
+  float u1 = u01(a), u2 = u01(b);
+  float r = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincosf(6.283185307179586f * u2, &s, &c);
+  g0 = r * c;
+  g1 = r * s;
+}
+
+// Statistical collision mode: a Poisson(lambda) number of attempts on one ion, each
+// the reference's scattering formula on Philox draws keyed by (id, step, attempt).
+// With lambda = Nc / (np * acceptance) the expected number of collisions is Nc.
+__device__ inline void collide_philox(const CollideConsts &cc, uint32_t key0, uint32_t key1, uint32_t step,
+                                      double lam, int pid, double &vx, double &vy, double &vz, unsigned &natt,
+                                      unsigned &nsucc, unsigned &nfail) {
+  u32x4 c0 = {{(uint32_t)pid, step, 0u, 0xC0111DE5u}};
+  u32x4 r0 = philox4x32_10(c0, key0, key1);
+  float u = u01(r0.v[0]);
+  double pk = exp(-lam), F = pk;
+  int k = 0;
+  while ((double)u > F && k < 24) {
+    ++k;
+    pk *= lam / k;
+    F += pk;
+  }
+  for (int at = 0; at < k; ++at) {
+    u32x4 c1 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0xA77E3B71u}};
+    u32x4 c2 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0x5CA77E12u}};
+    u32x4 r1 = philox4x32_10(c1, key0, key1);
+    u32x4 r2 = philox4x32_10(c2, key0, key1);
+    float g[3], spare;
+    box_muller(r1.v[0], r1.v[1], g[0], g[1]);
+    box_muller(r1.v[2], r1.v[3], g[2], spare);
+    double vi[3] = {vx, vy, vz}, vf[3];
+    int res = collide_one(cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
+    natt++;
+    if (res == 1) {
+      vx = vf[0];
+      vy = vf[1];
+      vz = vf[2];
+      nsucc++;
+    } else if (res == 2) {
+      nfail++;
+    }
+  }
 }
 
 // (PetscInt)(np*ran3(&seed)): the int*float product is a float (src/collisions.c:72);

@@ -664,6 +664,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       *converged = true;
       break;
     }
+    if (!std::isfinite(beta)) break;
     if (it >= maxit) break;
     k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.r, g.V[0], g.dh, n, 1);
     std::fill(gv.begin(), gv.end(), 0.0);
@@ -685,6 +686,10 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       ECS_HIP(hipStreamSynchronize(st));
       for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = g.hpin[q] + g.hpin[(m + 2) + q];
       const double hn = std::sqrt(std::max(g.hpin[j + 1], 0.0));
+      if (!std::isfinite(hn)) {
+        it = maxit;
+        break;
+      }
       H[(size_t)(j + 1) * m + j] = hn;
       // w/hn -> V[j+1]
       if (hn > 0) k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.w, g.V[j + 1], g.dh + (j + 1), n, 1);
@@ -843,6 +848,7 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
         conv = true;
         break;
       }
+      if (!std::isfinite(rnorm)) break;
       if (iters >= maxit) break;
       k_convert<double, float><<<gsz(n), 256, 0, c->stream>>>(s->r64, s->b32, n, 1.0 / rnorm);
       k_fill<float><<<gsz(n), 256, 0, c->stream>>>(s->x32, n, 0.0f);

@@ -203,8 +203,10 @@ int ecsprt_hip_collision_stats(ecsprt_hip_ctx *ctx, int *Nc, long *Ns, long *Nf,
 
 /* Last solve: iterations, final true-residual norm, ||b||. */
 int ecsprt_hip_solver_stats(ecsprt_hip_ctx *ctx, int *iters, double *rnorm, double *bnorm);
-/* Per-stage times of the last call that recorded them (ms): [0] push or fused particle
- * kernel, [1] collide, [2] drift+rebin, [3] deposit+fold, [4] rhs, [5] solve. */
+/* Per-stage times of the last call that recorded them (ms, HIP events on the context's
+ * stream, options.timing = 1).  ecsprt_hip_step: [0] fused particle kernel, [1] set-up
+ * before it, [2] re-binning and fold, [4] operator + RHS, [5] KSP solve.  Stage calls:
+ * [0] push, [3] deposit + fold, [4] operator + RHS, [5] KSP solve. */
 int ecsprt_hip_stage_times(ecsprt_hip_ctx *ctx, double *ms);
 /* Block until the context's stream is idle. */
 int ecsprt_hip_synchronize(ecsprt_hip_ctx *ctx);
