@@ -53,8 +53,8 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     drv_src = os.path.join(HERE, "host", "pic_main.c")
     if os.path.exists(drv_src):
-        _run(["gcc", "-O2", "-std=c99", "-Wall", f"-I{os.path.join(ROOT, 'include')}", drv_src,
-              "-o", DRIVER, f"-L{HERE}", "-lecsprt_hip", f"-Wl,-rpath,{HERE}", "-lm"])
+        _run(["gcc", "-O2", "-std=c99", "-D_POSIX_C_SOURCE=199309L", "-Wall", f"-I{os.path.join(ROOT, 'include')}", drv_src,
+              "-o", DRIVER, f"-L{HERE}", "-lecsprt_hip", "-Wl,-rpath,$ORIGIN", "-lm"])
     if verbose:
         print("built", LIB)
     return LIB

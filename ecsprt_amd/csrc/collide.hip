@@ -295,6 +295,7 @@ __global__ void k_pilot(ParticlesF64 A, const int *base, const int *count, Colli
   }
 }
 
+// Returns the number of collisions of one dry run.
 static double pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam) {
   unsigned long long zero[2] = {0, 0};
   ECS_HIP(hipMemcpyAsync(c->d_ctr + 6, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
@@ -305,15 +306,19 @@ static double pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam) {
   unsigned long long h[2];
   ECS_HIP(hipMemcpyAsync(h, c->d_ctr + 6, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   ECS_HIP(hipStreamSynchronize(c->stream));
-  return h[0] ? (double)h[1] / (double)h[0] : 0.0;
+  return (double)h[1];
 }
 
+// Initial attempt rate: dry runs of the Poisson process, each rescaling lambda by
+// Nc / (collisions it produced), starting from one attempt per collision.
 void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt) {
-  // First pass at lambda = 1, second at the lambda the first implies.
   const int Nc = collision_count(&c->p, c->np, dt);
-  double q = pilot_pass(c, 0xFFFFFFF0u, 1.0);
-  if (q > 0 && Nc > 0) q = pilot_pass(c, 0xFFFFFFF1u, (double)Nc / (q * (double)c->np));
-  c->acc_est = q;
+  double lam = c->np > 0 ? (double)Nc / (double)c->np : 0.0;
+  for (int it = 0; it < 4 && Nc > 0 && lam > 0; ++it) {
+    const double s = pilot_pass(c, 0xFFFFFFF0u + it, lam);
+    lam *= s > 0 ? std::min(16.0, std::max(0.25, (double)Nc / s)) : 16.0;
+  }
+  c->coll_lambda = lam;
 }
 
 }  // namespace ecsprt

@@ -112,7 +112,7 @@ struct ecsprt_hip_ctx {
   long att_cap = 0, n_att = 0;
   int last_Nc = 0;
   long last_Ns = 0, last_Nf = 0, last_attempts = 0;
-  double acc_est = -1.0;  // Philox mode: acceptance fraction estimate
+  double coll_lambda = -1.0;  // Philox mode: mean attempts per ion per step
   uint64_t step_index = 0;
 
   // solver

@@ -18,7 +18,8 @@ enum : int { M_PUSH = 1, M_DRIFT = 2, M_DEPOSIT = 4, M_REBIN = 8, M_COLLIDE = 16
 
 struct TileArgs {
   TileGeom g;
-  ParticlesF64 src, dst, emi;
+  ParticlesF64 src, dst, emi, spill;
+  long scap;
   const int *base, *count, *cap;
   int *next;
   unsigned long long *ctr;
@@ -39,19 +40,6 @@ __device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg
   return ((ck / g.T) * g.nty + cj / g.T) * g.ntx + ci / g.T;
 }
 
-template <typename T>
-__device__ inline T wave_append(bool pred, T *counter) {
-  unsigned long long m = __ballot(pred);
-  int lane = __lane_id();
-  unsigned long long below = m & ((1ull << lane) - 1ull);
-  int rank = __popcll(below);
-  int n = __popcll(m);
-  T base = 0;
-  if (lane == 0 && n) base = atomicAdd(counter, (T)n);
-  base = __shfl(base, 0);
-  return base + (T)rank;
-}
-
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
   return v;
@@ -68,9 +56,34 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
   d.id[i] = id;
 }
 
+// Offset of tile coordinate b relative to a on a periodic axis of n tiles:
+// -1, 0, +1, or 2 for anything further.
+__device__ inline int tile_offset(int b, int a, int n) {
+  if (b == a) return 0;
+  if (b == (a + 1 == n ? 0 : a + 1)) return 1;
+  if (b == (a == 0 ? n - 1 : a - 1)) return -1;
+  return 2;
+}
+
+// Place a particle whose destination slot was reserved: into the destination tile
+// when there is room, else into the spill list (the host re-bins after the step).
+__device__ inline void place(const TileArgs &a, int dt, int slot, double x, double y, double z, double vx,
+                             double vy, double vz, int id) {
+  if (slot < a.cap[dt]) {
+    store_particle(a.dst, (long)a.base[dt] + slot, x, y, z, vx, vy, vz, id);
+  } else {
+    const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
+    if ((long)sp < a.scap) store_particle(a.spill, (long)sp, x, y, z, vx, vy, vz, id);
+    else atomicAdd(&a.ctr[2], 1ull);
+  }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   extern __shared__ double smem[];
+  // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
+  // and the slot range reserved in each.
+  __shared__ int s_cnt[27], s_base[27], s_nbr[27];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
   int w = blockIdx.x;
@@ -99,6 +112,10 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   }
   if (MODE & M_DEPOSIT) {
     for (int q = tid; q < 4 * RA3; q += blockDim.x) sacc[q] = 0.0;
+  }
+  if ((MODE & M_REBIN) && tid < 27) {
+    const int ox = tid % 3 - 1, oy = (tid / 3) % 3 - 1, oz = tid / 9 - 1;
+    s_nbr[tid] = (wrap_index(tz + oz, g.ntz) * g.nty + wrap_index(ty + oy, g.nty)) * g.ntx + wrap_index(tx + ox, g.ntx);
   }
   __syncthreads();
 
@@ -197,13 +214,28 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       }
     }
     if (MODE & M_REBIN) {
-      const bool stay = valid && !lost && ntile == t;
-      const bool emig = valid && !lost && ntile != t;
-      const int sslot = wave_append<int>(stay, &a.next[t]);
-      const unsigned long long eslot = wave_append<unsigned long long>(emig, &a.ctr[0]);
-      if (stay) store_particle(a.dst, a.base[t] + sslot, px, py, pz, vx, vy, vz, pid);
-      if (emig) store_particle(a.emi, (long)eslot, px, py, pz, vx, vy, vz, pid);
+      // Destination code: neighbour index 0..26, 27 = further than one tile, 28 = none.
+      int code = 28;
+      if (valid && !lost) {
+        const int nx = ntile % g.ntx, ny = (ntile / g.ntx) % g.nty, nz = ntile / (g.ntx * g.nty);
+        const int ox = tile_offset(nx, tx, g.ntx), oy = tile_offset(ny, ty, g.nty), oz = tile_offset(nz, tz, g.ntz);
+        code = (ox < 2 && oy < 2 && oz < 2) ? ((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1) : 27;
+      }
+      if (tid < 27) s_cnt[tid] = 0;
+      __syncthreads();
+      int rank = 0;
+      if (code < 27) rank = atomicAdd(&s_cnt[code], 1);
+      __syncthreads();
+      if (tid < 27 && s_cnt[tid] > 0) s_base[tid] = atomicAdd(&a.next[s_nbr[tid]], s_cnt[tid]);
+      __syncthreads();
+      if (code < 27) {
+        place(a, s_nbr[code], s_base[code] + rank, px, py, pz, vx, vy, vz, pid);
+      } else if (code == 27) {
+        const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
+        store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
+      }
       if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
+      __syncthreads();
     } else if ((MODE & (M_PUSH | M_COLLIDE)) && valid) {
       a.src.vx[p] = vx;
       a.src.vy[p] = vy;
@@ -707,6 +739,8 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
   a.src = c->A;
   a.dst = c->B;
   a.emi = c->E;
+  a.spill = c->S;
+  a.scap = c->scap;
   a.base = c->d_base;
   a.count = c->d_count;
   a.cap = c->d_cap;
@@ -817,15 +851,16 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
   a.pc = push_consts(c->p, dt);
   a.dc = drift_consts(c->p, dt);
   const bool coll = c->o.collide_mode == ECSPRT_COLLIDE_PHILOX && c->p.mn > 0.0;
+  int Nc = 0;
   if (coll) {
-    if (c->acc_est <= 0.0) run_collide_philox_pilot(c, dt);
-    int Nc = collision_count(&c->p, c->np, dt);
+    Nc = collision_count(&c->p, c->np, dt);
     c->last_Nc = Nc;
+    if (c->coll_lambda <= 0.0) run_collide_philox_pilot(c, dt);
     a.cc = collide_consts(c->p);
     a.key0 = (uint32_t)c->o.seed;
     a.key1 = (uint32_t)(c->o.seed >> 32);
     a.step = (uint32_t)c->step_index;
-    a.lambda = (c->acc_est > 0.0 && c->np > 0) ? (double)Nc / (c->acc_est * (double)c->np) : 0.0;
+    a.lambda = Nc > 0 ? c->coll_lambda : 0.0;
     if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
     launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
   } else {
@@ -842,7 +877,8 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
     c->last_attempts = natt;
     c->last_Ns = ns;
     c->last_Nf = nf;
-    if (natt > 0) c->acc_est = (double)ns / (double)natt;
+    // Steer the attempt rate so that the expected number of collisions is Nc.
+    if (ns > 0 && Nc > 0) c->coll_lambda *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
     ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
   }
   c->step_index++;
