@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
+    ap.add_argument("--collide", default="philox", choices=["philox", "off"])
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     ap.add_argument("--profile-steps", type=int, default=0,
                     help="run only this many steps without the extras (for rocprofv3)")
     return ap.parse_args()
@@ -91,6 +94,24 @@ def cpu_baseline(wl, target_s=15.0):
     }
 
 
+def copy_probe(gib, reps=5):
+    """hipMemcpy device-to-device of `gib` GiB: achievable HBM bandwidth (read + write)."""
+    import ctypes
+    from ecsprt_amd import _lib
+    L = _lib.load()
+    n = int(gib * (1 << 30))
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.ecsprt_hip_dmalloc(ctypes.byref(a), n) == 0 and L.ecsprt_hip_dmalloc(ctypes.byref(b), n) == 0
+    ms = ctypes.c_double()
+    best = 1e30
+    for _ in range(reps):
+        assert L.ecsprt_hip_d2d(b, a, n, ctypes.byref(ms)) == 0
+        best = min(best, ms.value)
+    L.ecsprt_hip_dfree(a)
+    L.ecsprt_hip_dfree(b)
+    return {"copy_bytes": n, "best_ms": best, "read_plus_write_GBs": 2 * n / (best * 1e-3) / 1e9}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -98,9 +119,13 @@ def main():
                                    "note, not an implemented path)"}))
         sys.exit(2)
     from ecsprt_amd import pic, _lib
+    if args.copy_probe:
+        print(json.dumps(copy_probe(args.copy_probe)))
+        return
     wl = workload(args)
     p = pic.setup_context(**wl.cli)
-    opts = pic.default_options(collide_mode=_lib.COLLIDE_PHILOX, ksp_rtol=wl.rtol, timing=1,
+    cmode = _lib.COLLIDE_PHILOX if args.collide == "philox" else _lib.COLLIDE_OFF
+    opts = pic.default_options(collide_mode=cmode, ksp_rtol=wl.rtol, timing=1,
                                tile=args.tile, block=args.block, split=args.split)
     ctx = pic.Context(p, opts)
     npart = wl.np
@@ -153,7 +178,7 @@ def main():
         "data": "synthetic (device Philox initial conditions)",
         "config": {"workload": wl.name, "grid": [p.Nx, p.Ny, p.Nz], "ions_per_cell": wl.ppc, "np": npart,
                    "dt": wl.dt, "Omega_i_dt": p.Omega_i * wl.dt, "nu_dt": p.nui * wl.dt, "E0": p.E0,
-                   "kappa_e": p.kappa_e, "ksp_rtol": wl.rtol, "collisions": "Philox statistical mode",
+                   "kappa_e": p.kappa_e, "ksp_rtol": wl.rtol, "collisions": "Philox statistical mode" if args.collide == "philox" else "off",
                    "operator": "frozen after the first solve", "initial_guess": "zero each solve"},
         "particle_steps_per_s_particle_stages": npart / (particle_ms * 1e-3),
         "phi_solve_ms": float(np.mean(solve_ms)) + float(np.mean(rhs_ms)),

@@ -104,6 +104,7 @@ SIGNATURES = {
     "ecsprt_hip_dfree": (ctypes.c_int, [_P]),
     "ecsprt_hip_h2d": (ctypes.c_int, [_P, _P, ctypes.c_long]),
     "ecsprt_hip_d2h": (ctypes.c_int, [_P, _P, ctypes.c_long]),
+    "ecsprt_hip_d2d": (ctypes.c_int, [_P, _P, ctypes.c_long, _D]),
     "ecsprt_hip_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                         ctypes.POINTER(ctypes.c_uint32)]),
 }

@@ -519,6 +519,22 @@ int ecsprt_hip_d2h(void *dst, const void *src, long bytes) {
   return hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost) == hipSuccess ? ECSPRT_OK : ECSPRT_ERR_HIP;
 }
 
+int ecsprt_hip_d2d(void *dst, const void *src, long bytes, double *ms) {
+  hipEvent_t a, b;
+  if (hipEventCreate(&a) != hipSuccess) return ECSPRT_ERR_HIP;
+  if (hipEventCreate(&b) != hipSuccess) return ECSPRT_ERR_HIP;
+  int rc = ECSPRT_OK;
+  if (hipEventRecord(a, 0) != hipSuccess || hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice, 0) != hipSuccess ||
+      hipEventRecord(b, 0) != hipSuccess || hipEventSynchronize(b) != hipSuccess)
+    rc = ECSPRT_ERR_HIP;
+  float t = 0.f;
+  if (rc == ECSPRT_OK && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = ECSPRT_ERR_HIP;
+  if (ms) *ms = t;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
 void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
   ecsprt::u32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
   ecsprt::u32x4 r = ecsprt::philox4x32_10(c, key[0], key[1]);

@@ -54,7 +54,15 @@ struct PushConsts {
   double dx, dy, dz;
   double t, s;
   double Escale[3];
+  // When every cell width is a power of two, pos*inv equals pos/d exactly and the
+  // kernels multiply instead of dividing.
+  double inv[3];
+  int pow2;
 };
+
+__host__ __device__ inline double cell_coord(double v, double d, double inv, int pow2) {
+  return pow2 ? v * inv : v / d;
+}
 
 struct DriftConsts {
   double dt;
@@ -84,6 +92,7 @@ struct ecsprt_hip_ctx {
   int *d_base = nullptr, *d_cap = nullptr, *d_count = nullptr, *d_next = nullptr;
   ecsprt::ParticlesF64 A, B, E;  // current, next, emigrants
   ecsprt::ParticlesF64 S;        // spill for tiles that overflow their capacity
+  ecsprt::ParticlesF64 D;        // ions deferred to the collision phase (tile layout of A)
   long scap = 0;
   // counters: [0] emigrants, [1] spilled, [2] lost, [3] collision attempts,
   // [4] collision successes, [5] collision refusals (accumulated in long long below)

@@ -18,8 +18,9 @@ enum : int { M_PUSH = 1, M_DRIFT = 2, M_DEPOSIT = 4, M_REBIN = 8, M_COLLIDE = 16
 
 struct TileArgs {
   TileGeom g;
-  ParticlesF64 src, dst, emi, spill;
+  ParticlesF64 src, dst, emi, spill, def;
   long scap;
+  PoissonTable pt;
   const int *base, *count, *cap;
   int *next;
   unsigned long long *ctr;
@@ -29,7 +30,6 @@ struct TileArgs {
   DriftConsts dc;
   CollideConsts cc;
   uint32_t key0, key1, step;
-  double lambda;
 };
 
 __device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg) {
@@ -78,71 +78,211 @@ __device__ inline void place(const TileArgs &a, int dt, int slot, double x, doub
   }
 }
 
+// Per-workgroup tile constants and LDS pointers used by the particle tail.
+struct TileWG {
+  int t, tx, ty, tz, i0, j0, k0;
+  int RA, RA3, RAx, RAy, RAz, H;
+  double *sacc;
+  int *s_cnt, *s_base, *s_nbr;
+};
+
+// Drift + wrap, deposit, and re-binning of one particle per thread.  Every thread
+// of the workgroup calls this in uniform control flow (re-binning uses barriers).
 template <int MODE>
-__global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
+__device__ inline void particle_tail(const TileArgs &a, const TileWG &w, bool valid, long p, double px, double py,
+                                     double pz, double vx, double vy, double vz, int pid) {
+  const TileGeom &g = a.g;
+  const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
+  const int tid = threadIdx.x;
+  bool lost = false;
+  int ntile = w.t;
+  if ((MODE & M_DRIFT) && valid) {
+    // UpdatePositions drift (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28).
+    px += vx * a.dc.dt;
+    py += vy * a.dc.dt;
+    pz += vz * a.dc.dt;
+    if (px < a.dc.r0[0]) px += a.dc.L[0];
+    if (px >= a.dc.r1[0]) px -= a.dc.L[0];
+    if (py < a.dc.r0[1]) py += a.dc.L[1];
+    if (py >= a.dc.r1[1]) py -= a.dc.L[1];
+    if (pz < a.dc.r0[2]) pz += a.dc.L[2];
+    if (pz >= a.dc.r1[2]) pz -= a.dc.L[2];
+    lost = !(px >= a.dc.r0[0] && px <= a.dc.r1[0] && py >= a.dc.r0[1] && py <= a.dc.r1[1] && pz >= a.dc.r0[2] &&
+             pz <= a.dc.r1[2]);
+  }
+  // Normalised coordinates, x = pos/dx (src/moments.c:154-156), shared by the tile
+  // lookup and the deposit.
+  const double x = cell_coord(px, dx, a.pc.inv[0], a.pc.pow2);
+  const double y = cell_coord(py, dy, a.pc.inv[1], a.pc.pow2);
+  const double z = cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2);
+  if ((MODE & M_DRIFT) && valid && !lost) ntile = tile_of(g, x, y, z);
+  if ((MODE & M_DEPOSIT) && valid && !lost) {
+    // Collect3DFluidMoments body, src/moments.c:154-205.
+    int ixl = (int)x, iyl = (int)y, izl = (int)z;
+    double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+    double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+    double wzh = z - (double)izl, wzl = 1.0 - wzh;
+    double w8[8];
+    w8[0] = wzh * wyh * wxh;
+    w8[1] = wzl * wyh * wxh;
+    w8[2] = wzh * wyl * wxh;
+    w8[3] = wzl * wyl * wxh;
+    w8[4] = wzh * wyh * wxl;
+    w8[5] = wzl * wyh * wxl;
+    w8[6] = wzh * wyl * wxl;
+    w8[7] = wzl * wyl * wxl;
+    int la = ixl - (w.i0 - w.H), lb = iyl - (w.j0 - w.H), lc = izl - (w.k0 - w.H);
+    if (la < 0) la += g.Nx; else if (la > w.RAx - 2) la -= g.Nx;
+    if (lb < 0) lb += g.Ny; else if (lb > w.RAy - 2) lb -= g.Ny;
+    if (lc < 0) lc += g.Nz; else if (lc > w.RAz - 2) lc -= g.Nz;
+    const bool inreg = la >= 0 && la <= w.RAx - 2 && lb >= 0 && lb <= w.RAy - 2 && lc >= 0 && lc <= w.RAz - 2;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+      const double wt = w8[c];
+      if (inreg) {
+        const int idx = ((lc + az) * w.RA + (lb + ay)) * w.RA + (la + ax);
+        atomicAdd(&w.sacc[idx], wt);
+        atomicAdd(&w.sacc[w.RA3 + idx], vx * wt);
+        atomicAdd(&w.sacc[2 * w.RA3 + idx], vy * wt);
+        atomicAdd(&w.sacc[3 * w.RA3 + idx], vz * wt);
+      } else {
+        const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
+                          wrap_index(ixl + ax, g.Nx);
+        unsafeAtomicAdd(&a.ovf[node * 4 + 0], wt);
+        unsafeAtomicAdd(&a.ovf[node * 4 + 1], vx * wt);
+        unsafeAtomicAdd(&a.ovf[node * 4 + 2], vy * wt);
+        unsafeAtomicAdd(&a.ovf[node * 4 + 3], vz * wt);
+      }
+    }
+  }
+  if (MODE & M_REBIN) {
+    // Destination code: neighbour index 0..26, 27 = further than one tile, 28 = none.
+    int code = 28;
+    if (valid && !lost) {
+      const int nx = ntile % g.ntx, ny = (ntile / g.ntx) % g.nty, nz = ntile / (g.ntx * g.nty);
+      const int ox = tile_offset(nx, w.tx, g.ntx), oy = tile_offset(ny, w.ty, g.nty), oz = tile_offset(nz, w.tz, g.ntz);
+      code = (ox < 2 && oy < 2 && oz < 2) ? ((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1) : 27;
+    }
+    if (tid < 27) w.s_cnt[tid] = 0;
+    __syncthreads();
+    int rank = 0;
+    if (code < 27) rank = atomicAdd(&w.s_cnt[code], 1);
+    __syncthreads();
+    if (tid < 27 && w.s_cnt[tid] > 0) w.s_base[tid] = atomicAdd(&a.next[w.s_nbr[tid]], w.s_cnt[tid]);
+    __syncthreads();
+    if (code < 27) {
+      place(a, w.s_nbr[code], w.s_base[code] + rank, px, py, pz, vx, vy, vz, pid);
+    } else if (code == 27) {
+      const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
+      store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
+    }
+    if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
+    __syncthreads();
+  } else if ((MODE & M_PUSH) && valid) {
+    a.src.vx[p] = vx;
+    a.src.vy[p] = vy;
+    a.src.vz[p] = vz;
+  }
+}
+
+template <int MODE, int MAXT>
+__global__ void __launch_bounds__(MAXT) k_tile(TileArgs a) {
+  static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ double smem[];
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
-  // and the slot range reserved in each.
-  __shared__ int s_cnt[27], s_base[27], s_nbr[27];
+  // the slot range reserved in each; number of ions deferred for collisions.
+  __shared__ int s_cnt[27], s_base[27], s_nbr[27], s_ndef;
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
-  int w = blockIdx.x;
+  int wi = blockIdx.x;
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so neighbouring tiles share an L2.
-  if ((nwg & 7) == 0) w = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
-  const int t = w / g.split, s = w % g.split;
-  const int tx = t % g.ntx, ty = (t / g.ntx) % g.nty, tz = t / (g.ntx * g.nty);
-  const int i0 = tx * g.T, j0 = ty * g.T, k0 = tz * g.T;
-  const int ex = min(g.T, g.Nx - i0), ey = min(g.T, g.Ny - j0), ez = min(g.T, g.Nz - k0);
-  const int RP = g.RP, RA = g.RA, H = g.H;
-  const int RP3 = RP * RP * RP, RA3 = RA * RA * RA;
-  const int RAx = ex + 1 + 2 * H, RAy = ey + 1 + 2 * H, RAz = ez + 1 + 2 * H;
+  if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
+  TileWG w;
+  w.t = wi / g.split;
+  const int s = wi % g.split;
+  w.tx = w.t % g.ntx;
+  w.ty = (w.t / g.ntx) % g.nty;
+  w.tz = w.t / (g.ntx * g.nty);
+  w.i0 = w.tx * g.T;
+  w.j0 = w.ty * g.T;
+  w.k0 = w.tz * g.T;
+  w.H = g.H;
+  w.RA = g.RA;
+  w.RA3 = g.RA * g.RA * g.RA;
+  w.RAx = min(g.T, g.Nx - w.i0) + 1 + 2 * g.H;
+  w.RAy = min(g.T, g.Ny - w.j0) + 1 + 2 * g.H;
+  w.RAz = min(g.T, g.Nz - w.k0) + 1 + 2 * g.H;
+  w.s_cnt = s_cnt;
+  w.s_base = s_base;
+  w.s_nbr = s_nbr;
+  const int RP = g.RP, RP3 = RP * RP * RP;
   double *sphi = smem;
-  double *sacc = smem + ((MODE & M_PUSH) ? RP3 : 0);
+  w.sacc = smem + ((MODE & M_PUSH) ? RP3 : 0);
   const int tid = threadIdx.x;
 
   if (MODE & M_PUSH) {
     for (int q = tid; q < RP3; q += blockDim.x) {
       int li = q % RP, lj = (q / RP) % RP, lk = q / (RP * RP);
-      int gi = wrap_index(i0 - 1 + li, g.Nx);
-      int gj = wrap_index(j0 - 1 + lj, g.Ny);
-      int gk = wrap_index(k0 - 1 + lk, g.Nz);
+      int gi = wrap_index(w.i0 - 1 + li, g.Nx);
+      int gj = wrap_index(w.j0 - 1 + lj, g.Ny);
+      int gk = wrap_index(w.k0 - 1 + lk, g.Nz);
       sphi[q] = a.phi[((long)gk * g.Ny + gj) * g.Nx + gi];
     }
   }
   if (MODE & M_DEPOSIT) {
-    for (int q = tid; q < 4 * RA3; q += blockDim.x) sacc[q] = 0.0;
+    for (int q = tid; q < 4 * w.RA3; q += blockDim.x) w.sacc[q] = 0.0;
   }
   if ((MODE & M_REBIN) && tid < 27) {
     const int ox = tid % 3 - 1, oy = (tid / 3) % 3 - 1, oz = tid / 9 - 1;
-    s_nbr[tid] = (wrap_index(tz + oz, g.ntz) * g.nty + wrap_index(ty + oy, g.nty)) * g.ntx + wrap_index(tx + ox, g.ntx);
+    s_nbr[tid] = (wrap_index(w.tz + oz, g.ntz) * g.nty + wrap_index(w.ty + oy, g.nty)) * g.ntx +
+                 wrap_index(w.tx + ox, g.ntx);
   }
+  if (tid == 0) s_ndef = 0;
   __syncthreads();
 
-  const long cnt = a.count[t];
-  const long beg = a.base[t] + cnt * s / g.split;
-  const long end = a.base[t] + cnt * (s + 1) / g.split;
+  const long cnt = a.count[w.t];
+  const long beg = a.base[w.t] + cnt * s / g.split;
+  const long end = a.base[w.t] + cnt * (s + 1) / g.split;
   const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
   unsigned natt = 0, nsucc = 0, nfail = 0;
 
+  // Phase 1.  Software pipeline: the next batch's particles load while this one computes.
+  double nxp = 0, nyp = 0, nzp = 0, nvx = 0, nvy = 0, nvz = 0;
+  int nid = 0;
+  if (beg + tid < end) {
+    const long p = beg + tid;
+    nxp = a.src.x[p];
+    nyp = a.src.y[p];
+    nzp = a.src.z[p];
+    nvx = a.src.vx[p];
+    nvy = a.src.vy[p];
+    nvz = a.src.vz[p];
+    nid = a.src.id[p];
+  }
   for (long p0 = beg; p0 < end; p0 += blockDim.x) {
     const long p = p0 + tid;
     const bool valid = p < end;
-    double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
-    int pid = 0;
-    if (valid) {
-      px = a.src.x[p];
-      py = a.src.y[p];
-      pz = a.src.z[p];
-      vx = a.src.vx[p];
-      vy = a.src.vy[p];
-      vz = a.src.vz[p];
-      pid = a.src.id[p];
+    double px = nxp, py = nyp, pz = nzp, vx = nvx, vy = nvy, vz = nvz;
+    const int pid = nid;
+    {
+      const long pn = p + blockDim.x;
+      if (pn < end) {
+        nxp = a.src.x[pn];
+        nyp = a.src.y[pn];
+        nzp = a.src.z[pn];
+        nvx = a.src.vx[pn];
+        nvy = a.src.vy[pn];
+        nvz = a.src.vz[pn];
+        nid = a.src.id[pn];
+      }
     }
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body, src/velocities.c:342-356.
       double E[3];
-      diff_vec3d_lds(sphi, RP, i0 - 1, j0 - 1, k0 - 1, g.Nx, g.Ny, g.Nz, px / dx, py / dy, pz / dz, E);
+      diff_vec3d_lds(sphi, RP, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, cell_coord(px, dx, a.pc.inv[0], a.pc.pow2),
+                     cell_coord(py, dy, a.pc.inv[1], a.pc.pow2), cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2), E);
       double vxm = vx + a.pc.Escale[0] * E[0];
       double vym = vy + a.pc.Escale[1] * E[1];
       double vzm = vz + a.pc.Escale[2] * E[2];
@@ -153,96 +293,40 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       vy = vyp;
       vz = vzm;
     }
-    if ((MODE & M_COLLIDE) && valid)
-      collide_philox(a.cc, a.key0, a.key1, a.step, a.lambda, pid, vx, vy, vz, natt, nsucc, nfail);
-    bool lost = false;
-    int ntile = t;
-    if ((MODE & M_DRIFT) && valid) {
-      // UpdatePositions drift (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28).
-      px += vx * a.dc.dt;
-      py += vy * a.dc.dt;
-      pz += vz * a.dc.dt;
-      if (px < a.dc.r0[0]) px += a.dc.L[0];
-      if (px >= a.dc.r1[0]) px -= a.dc.L[0];
-      if (py < a.dc.r0[1]) py += a.dc.L[1];
-      if (py >= a.dc.r1[1]) py -= a.dc.L[1];
-      if (pz < a.dc.r0[2]) pz += a.dc.L[2];
-      if (pz >= a.dc.r1[2]) pz -= a.dc.L[2];
-      lost = !(px >= a.dc.r0[0] && px <= a.dc.r1[0] && py >= a.dc.r0[1] && py <= a.dc.r1[1] &&
-               pz >= a.dc.r0[2] && pz <= a.dc.r1[2]);
-      if (!lost) ntile = tile_of(g, px / dx, py / dy, pz / dz);
+    bool defer = false;
+    if ((MODE & M_COLLIDE) && valid && poisson_attempts(a.pt, a.key0, a.key1, a.step, pid) > 0) {
+      // About one ion in seven has a collision attempt this step.  It is set aside
+      // (post-push state, in this workgroup's own slot range) and finished in
+      // phase 2 with every lane busy.
+      defer = true;
+      const int q = atomicAdd(&s_ndef, 1);
+      store_particle(a.def, beg + q, px, py, pz, vx, vy, vz, pid);
     }
-    if ((MODE & M_DEPOSIT) && valid && !lost) {
-      // Collect3DFluidMoments body, src/moments.c:154-205.
-      double x = px / dx, y = py / dy, z = pz / dz;
-      int ixl = (int)x, iyl = (int)y, izl = (int)z;
-      double wxh = x - (double)ixl, wxl = 1.0 - wxh;
-      double wyh = y - (double)iyl, wyl = 1.0 - wyh;
-      double wzh = z - (double)izl, wzl = 1.0 - wzh;
-      double w8[8];
-      w8[0] = wzh * wyh * wxh;
-      w8[1] = wzl * wyh * wxh;
-      w8[2] = wzh * wyl * wxh;
-      w8[3] = wzl * wyl * wxh;
-      w8[4] = wzh * wyh * wxl;
-      w8[5] = wzl * wyh * wxl;
-      w8[6] = wzh * wyl * wxl;
-      w8[7] = wzl * wyl * wxl;
-      int la = ixl - (i0 - H), lb = iyl - (j0 - H), lc = izl - (k0 - H);
-      if (la < 0) la += g.Nx; else if (la > RAx - 2) la -= g.Nx;
-      if (lb < 0) lb += g.Ny; else if (lb > RAy - 2) lb -= g.Ny;
-      if (lc < 0) lc += g.Nz; else if (lc > RAz - 2) lc -= g.Nz;
-      const bool inreg = la >= 0 && la <= RAx - 2 && lb >= 0 && lb <= RAy - 2 && lc >= 0 && lc <= RAz - 2;
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
-        const double w = w8[c];
-        if (inreg) {
-          const int idx = ((lc + az) * RA + (lb + ay)) * RA + (la + ax);
-          atomicAdd(&sacc[idx], w);
-          atomicAdd(&sacc[RA3 + idx], vx * w);
-          atomicAdd(&sacc[2 * RA3 + idx], vy * w);
-          atomicAdd(&sacc[3 * RA3 + idx], vz * w);
-        } else {
-          const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
-                            wrap_index(ixl + ax, g.Nx);
-          unsafeAtomicAdd(&a.ovf[node * 4 + 0], w);
-          unsafeAtomicAdd(&a.ovf[node * 4 + 1], vx * w);
-          unsafeAtomicAdd(&a.ovf[node * 4 + 2], vy * w);
-          unsafeAtomicAdd(&a.ovf[node * 4 + 3], vz * w);
-        }
-      }
-    }
-    if (MODE & M_REBIN) {
-      // Destination code: neighbour index 0..26, 27 = further than one tile, 28 = none.
-      int code = 28;
-      if (valid && !lost) {
-        const int nx = ntile % g.ntx, ny = (ntile / g.ntx) % g.nty, nz = ntile / (g.ntx * g.nty);
-        const int ox = tile_offset(nx, tx, g.ntx), oy = tile_offset(ny, ty, g.nty), oz = tile_offset(nz, tz, g.ntz);
-        code = (ox < 2 && oy < 2 && oz < 2) ? ((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1) : 27;
-      }
-      if (tid < 27) s_cnt[tid] = 0;
-      __syncthreads();
-      int rank = 0;
-      if (code < 27) rank = atomicAdd(&s_cnt[code], 1);
-      __syncthreads();
-      if (tid < 27 && s_cnt[tid] > 0) s_base[tid] = atomicAdd(&a.next[s_nbr[tid]], s_cnt[tid]);
-      __syncthreads();
-      if (code < 27) {
-        place(a, s_nbr[code], s_base[code] + rank, px, py, pz, vx, vy, vz, pid);
-      } else if (code == 27) {
-        const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
-        store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
-      }
-      if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
-      __syncthreads();
-    } else if ((MODE & (M_PUSH | M_COLLIDE)) && valid) {
-      a.src.vx[p] = vx;
-      a.src.vy[p] = vy;
-      a.src.vz[p] = vz;
-    }
+    particle_tail<MODE>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
+    // Phase 2: the deferred ions, collided then drifted, deposited and re-binned.
+    __syncthreads();
+    const int ndef = s_ndef;
+    for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
+      const int q = q0 + tid;
+      const bool valid = q < ndef;
+      double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
+      int pid = 0;
+      if (valid) {
+        const long d = beg + q;
+        px = a.def.x[d];
+        py = a.def.y[d];
+        pz = a.def.z[d];
+        vx = a.def.vx[d];
+        vy = a.def.vy[d];
+        vz = a.def.vz[d];
+        pid = a.def.id[d];
+        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
+        collide_attempts(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+      }
+      particle_tail<MODE>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
+    }
     unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
     if (__lane_id() == 0) {
       if (s0) atomicAdd(&a.ctr[3], s0);
@@ -252,8 +336,8 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   }
   if (MODE & M_DEPOSIT) {
     __syncthreads();
-    double *out = a.fold + (size_t)w * 4 * RA3;
-    for (int q = tid; q < 4 * RA3; q += blockDim.x) out[q] = sacc[q];
+    double *out = a.fold + (size_t)wi * 4 * w.RA3;
+    for (int q = tid; q < 4 * w.RA3; q += blockDim.x) out[q] = w.sacc[q];
   }
 }
 
@@ -504,6 +588,13 @@ static PushConsts push_consts(const ecsprt_params &p, double dt) {
   k.dx = p.dx;
   k.dy = p.dy;
   k.dz = p.dz;
+  const double d3[3] = {p.dx, p.dy, p.dz};
+  k.pow2 = 1;
+  for (int d = 0; d < 3; ++d) {
+    int e;
+    k.pow2 &= std::frexp(d3[d], &e) == 0.5 ? 1 : 0;
+    k.inv[d] = 1.0 / d3[d];
+  }
   return k;
 }
 
@@ -549,16 +640,26 @@ static void ensure_keys(ecsprt_hip_ctx *c, long np) {
   c->key_cap = np;
 }
 
-template <int MODE>
+template <int MODE, int MAXT>
 static void set_lds_attr() {
   static bool done = false;
   if (!done) {
-    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    // 160 KB of LDS per workgroup on gfx950, less what the kernel declares statically.
+    hipFuncAttributes fa;
+    ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, MAXT>));
+    const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
+    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, MAXT>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     done = true;
   }
 }
 
-static size_t lds_for(const TileGeom &g, int mode) {
+template <int MODE>
+static void set_lds_attrs() {
+  set_lds_attr<MODE, 1024>();
+  set_lds_attr<MODE, 512>();
+}
+
+static size_t lds_for(const TileGeom &g, int mode, int) {
   size_t b = 0;
   if (mode & M_PUSH) b += sizeof(double) * g.RP * g.RP * g.RP;
   if (mode & M_DEPOSIT) b += sizeof(double) * 4 * g.RA * g.RA * g.RA;
@@ -591,10 +692,10 @@ void tiles_init(ecsprt_hip_ctx *c) {
     if (split < 1) split = 1;
   }
   g.split = split;
-  size_t lds = lds_for(g, M_PUSH | M_DEPOSIT);
-  ECS_CHECK(lds <= 160 * 1024, ECSPRT_ERR_ARG, "tile/halo too large for 160 KB of LDS");
   int block = c->o.block > 0 ? c->o.block : 1024;
   ECS_CHECK(block % 64 == 0 && block <= 1024, ECSPRT_ERR_ARG, "block must be a multiple of 64, <= 1024");
+  size_t lds = lds_for(g, M_PUSH | M_COLLIDE | M_DEPOSIT, block);
+  ECS_CHECK(lds <= 159 * 1024, ECSPRT_ERR_ARG, "tile/halo too large for 160 KB of LDS");
   c->lds_bytes = lds;
 
   // Fold tables.
@@ -635,11 +736,11 @@ void tiles_init(ecsprt_hip_ctx *c) {
   fill_i32(c, c->d_next, c->ntiles, 0);
   fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
   zero_ctr(c);
-  set_lds_attr<M_PUSH>();
-  set_lds_attr<M_DEPOSIT>();
-  set_lds_attr<M_DRIFT | M_REBIN>();
-  set_lds_attr<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>();
-  set_lds_attr<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>();
+  set_lds_attrs<M_PUSH>();
+  set_lds_attrs<M_DEPOSIT>();
+  set_lds_attrs<M_DRIFT | M_REBIN>();
+  set_lds_attrs<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>();
+  set_lds_attrs<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>();
 }
 
 void tiles_free(ecsprt_hip_ctx *c) {
@@ -647,6 +748,7 @@ void tiles_free(ecsprt_hip_ctx *c) {
   free_particles(c->B);
   free_particles(c->E);
   free_particles(c->S);
+  free_particles(c->D);
   (void)hipFree(c->d_fold);
   (void)hipFree(c->d_ovf);
   (void)hipFree(c->d_ctr);
@@ -696,6 +798,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
   if (total > c->pcap || !c->A.x) {
     alloc_particles(c->A, total);
     alloc_particles(c->B, total);
+    if (c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) alloc_particles(c->D, total);
     c->pcap = total;
   }
   if (!c->E.x || c->np != np) {
@@ -741,6 +844,7 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
   a.emi = c->E;
   a.spill = c->S;
   a.scap = c->scap;
+  a.def = c->D;
   a.base = c->d_base;
   a.count = c->d_count;
   a.cap = c->d_cap;
@@ -756,8 +860,11 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
 template <int MODE>
 static void launch_tile(ecsprt_hip_ctx *c, const TileArgs &a) {
   const int block = c->o.block > 0 ? c->o.block : 1024;
-  const size_t lds = lds_for(c->g, MODE);
-  k_tile<MODE><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  const size_t lds = lds_for(c->g, MODE, block);
+  // The launch bound fixes the register budget: 1024 threads leave 128 VGPRs per
+  // lane, 512 threads leave 256.
+  if (block <= 512) k_tile<MODE, 512><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  else k_tile<MODE, 1024><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
   ECS_HIP(hipGetLastError());
 }
 
@@ -860,7 +967,7 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
     a.key0 = (uint32_t)c->o.seed;
     a.key1 = (uint32_t)(c->o.seed >> 32);
     a.step = (uint32_t)c->step_index;
-    a.lambda = Nc > 0 ? c->coll_lambda : 0.0;
+    a.pt = poisson_table(Nc > 0 ? c->coll_lambda : 0.0);
     if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
     launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
   } else {

@@ -114,6 +114,7 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
                                   float u_z, float u_phi, const double vi[3], double vf[3]) {
   double vn[3], vr[3], vc[3], u2[3];
   double tmp = 0.0;
+#pragma unroll
   for (int d = 0; d < 3; d++) {
     vn[d] = g[d] * k.vnT + k.vn0[d];
     vr[d] = vi[d] - vn[d];
@@ -123,6 +124,7 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   float acc = u_acc1 * u_acc2;
   if (!(acc < vrr * vrr / (k.vrm * k.vrm))) return 0;
   tmp = 0.0;
+#pragma unroll
   for (int d = 0; d < 3; d++) {
     vc[d] = (vi[d] * k.mi + vn[d] * k.mn) / k.M;
     tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
@@ -145,6 +147,7 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   u2[2] = u1z;
   tmp = 0.0;
   double w[3];
+#pragma unroll
   for (int d = 0; d < 3; d++) {
     w[d] = vc[d] + vcr * u2[d];
     tmp += w[d] * w[d];
@@ -168,22 +171,27 @@ __device__ inline void box_muller(uint32_t a, uint32_t b, float &g0, float &g1) 
   g1 = r * s;
 }
 
-// Statistical collision mode: a Poisson(lambda) number of attempts on one ion, each
-// the reference's scattering formula on Philox draws keyed by (id, step, attempt).
-// With lambda = Nc / (np * acceptance) the expected number of collisions is Nc.
-__device__ inline void collide_philox(const CollideConsts &cc, uint32_t key0, uint32_t key1, uint32_t step,
-                                      double lam, int pid, double &vx, double &vy, double &vz, unsigned &natt,
-                                      unsigned &nsucc, unsigned &nfail) {
+// Statistical collision mode: a Poisson(lambda) number of attempts on each ion, each
+// attempt the reference's scattering formula on Philox draws keyed by (id, step,
+// attempt).  With lambda = Nc / (np * acceptance) the expected number of collisions
+// is Nc.
+// Cumulative Poisson(lambda) probabilities P(K <= k), k < 24, tabulated on the host.
+struct PoissonTable {
+  double cdf[24];
+};
+
+__device__ inline int poisson_attempts(const PoissonTable &pt, uint32_t key0, uint32_t key1, uint32_t step, int pid) {
   u32x4 c0 = {{(uint32_t)pid, step, 0u, 0xC0111DE5u}};
   u32x4 r0 = philox4x32_10(c0, key0, key1);
-  float u = u01(r0.v[0]);
-  double pk = exp(-lam), F = pk;
+  const double u = (double)u01(r0.v[0]);
   int k = 0;
-  while ((double)u > F && k < 24) {
-    ++k;
-    pk *= lam / k;
-    F += pk;
-  }
+  while (k < 24 && u > pt.cdf[k]) ++k;
+  return k;
+}
+
+__device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, uint32_t key1, uint32_t step, int pid,
+                                        int k, double &vx, double &vy, double &vz, unsigned &natt, unsigned &nsucc,
+                                        unsigned &nfail) {
   for (int at = 0; at < k; ++at) {
     u32x4 c1 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0xA77E3B71u}};
     u32x4 c2 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0x5CA77E12u}};
@@ -194,16 +202,35 @@ __device__ inline void collide_philox(const CollideConsts &cc, uint32_t key0, ui
     box_muller(r1.v[2], r1.v[3], g[2], spare);
     double vi[3] = {vx, vy, vz}, vf[3];
     int res = collide_one(cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
-    natt++;
+    natt += 1u;
+    nsucc += res == 1 ? 1u : 0u;
+    nfail += res == 2 ? 1u : 0u;
     if (res == 1) {
       vx = vf[0];
       vy = vf[1];
       vz = vf[2];
-      nsucc++;
-    } else if (res == 2) {
-      nfail++;
     }
   }
+}
+
+__device__ inline void collide_philox(const CollideConsts &cc, const PoissonTable &pt, uint32_t key0, uint32_t key1,
+                                      uint32_t step, int pid, double &vx, double &vy, double &vz, unsigned &natt,
+                                      unsigned &nsucc, unsigned &nfail) {
+  const int k = poisson_attempts(pt, key0, key1, step, pid);
+  collide_attempts(cc, key0, key1, step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+}
+
+// Host: P(K <= k) for K ~ Poisson(lam); the last entry closes the table at 1.
+inline PoissonTable poisson_table(double lam) {
+  PoissonTable t;
+  double pk = exp(-lam), F = pk;
+  for (int k = 0; k < 24; ++k) {
+    t.cdf[k] = F;
+    pk *= lam / (k + 1);
+    F += pk;
+  }
+  t.cdf[23] = 1.0;
+  return t;
 }
 
 // (PetscInt)(np*ran3(&seed)): the int*float product is a float (src/collisions.c:72);

@@ -218,6 +218,9 @@ int ecsprt_hip_dmalloc(void **ptr, long bytes);
 int ecsprt_hip_dfree(void *ptr);
 int ecsprt_hip_h2d(void *dst, const void *src, long bytes);
 int ecsprt_hip_d2h(void *dst, const void *src, long bytes);
+/* Device-to-device copy, timed with HIP events: *ms receives its duration (the bandwidth
+ * probe used to calibrate HBM byte counters). */
+int ecsprt_hip_d2d(void *dst, const void *src, long bytes, double *ms);
 
 /* Philox4x32-10 (Salmon et al., SC'11) on the host, for known-answer tests. */
 void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

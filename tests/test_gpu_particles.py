@@ -199,3 +199,35 @@ def test_device_init_statistics():
     expect = wl.np / 32 * (1 + 0.2 * np.sin(2 * np.pi * 2 * xc / 16))
     assert np.abs(h - expect).max() < 5 * np.sqrt(expect.max())
     ctx.close()
+
+
+@pytest.mark.parametrize("dx,dy,dz", [(0.5, 0.25, 2.0), (0.7, 1.3, 0.9)])
+def test_stages_with_cell_widths(dx, dy, dz):
+    """Power-of-two widths take the multiply-by-reciprocal path, others divide; both
+    must equal the reference's pos/dx arithmetic."""
+    say(f"cell widths {dx},{dy},{dz}")
+    cli = configs.base_cli(12, E0=0.05)
+    cli.update(dx=dx, dy=dy, dz=dz)
+    p = pic.setup_context(**cli)
+    po = oracle.setup_context(**cli)
+    ctx = pic.Context(p, pic.default_options(tile=4, collide_mode=_lib.COLLIDE_OFF))
+    rng = np.random.default_rng(21)
+    n = 6 * 12 ** 3
+    pos = np.column_stack([rng.uniform(0, po.Lx, n), rng.uniform(0, po.Ly, n), rng.uniform(0, po.Lz, n)])
+    vel = rng.normal(size=(n, 3)) * np.array([dx, dy, dz])  # about a cell per step
+    phi = 0.05 * rng.normal(size=(12, 12, 12))
+    ctx.set_particles(pos, vel)
+    ctx.set_potential(phi)
+    for _ in range(2):
+        ctx.push(1.0)
+        ctx.update_positions(1.0)
+        oracle.boris(po, 1.0, pos, vel, phi)
+        oracle.update_positions(1.0, pos, vel)
+        oracle.apply_bc(po, pos)
+    ctx.collect()
+    gpos, gvel = ctx.get_particles()
+    np.testing.assert_array_equal(gpos, pos)
+    np.testing.assert_array_equal(gvel, vel)
+    np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
+    assert rel(ctx.get_moments(), oracle.collect_moments(po, pos, vel)) <= 1e-12
+    ctx.close()
