@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--collide", default="philox", choices=["philox", "off"])
+    ap.add_argument("--mg-pre", type=int, default=0)
+    ap.add_argument("--mg-post", type=int, default=0)
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -127,6 +129,10 @@ def main():
     cmode = _lib.COLLIDE_PHILOX if args.collide == "philox" else _lib.COLLIDE_OFF
     opts = pic.default_options(collide_mode=cmode, ksp_rtol=wl.rtol, timing=1,
                                tile=args.tile, block=args.block, split=args.split)
+    if args.mg_pre:
+        opts.mg_pre = args.mg_pre
+    if args.mg_post:
+        opts.mg_post = args.mg_post
     ctx = pic.Context(p, opts)
     npart = wl.np
     ncell = wl.ncell

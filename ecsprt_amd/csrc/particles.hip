@@ -56,13 +56,35 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
   d.id[i] = id;
 }
 
-// Offset of tile coordinate b relative to a on a periodic axis of n tiles:
-// -1, 0, +1, or 2 for anything further.
-__device__ inline int tile_offset(int b, int a, int n) {
-  if (b == a) return 0;
-  if (b == (a + 1 == n ? 0 : a + 1)) return 1;
-  if (b == (a == 0 ? n - 1 : a - 1)) return -1;
-  return 2;
+// Tile offset d = b - a on a periodic axis of n tiles folded to -1, 0, +1; 2 when
+// the destination is further than one tile.
+__device__ inline int tile_step(int d, int n) {
+  if (d > 1) d -= n;
+  else if (d < -1) d += n;
+  return (d >= -1 && d <= 1) ? d : 2;
+}
+
+// FP64 add into the workgroup's LDS accumulator (ds_add_f64).  Workgroup scope keeps
+// it distinct from the agent-scope global atomics of the overflow path.
+__device__ inline void lds_add(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Deposit of an ion that moved beyond the tile's LDS region (about 0.6% of ions at
+// one cell per step): global FP64 atomics into the overflow moments.  Kept out of
+// line so the LDS path stays free of its address arithmetic.
+__device__ inline void deposit_overflow(double *ovf, const TileGeom &g, int ixl, int iyl, int izl,
+                                              const double w8[8], double vx, double vy, double vz) {
+  for (int c = 0; c < 8; ++c) {
+    const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+    const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
+                      wrap_index(ixl + ax, g.Nx);
+    const double wt = w8[c];
+    unsafeAtomicAdd(&ovf[node * 4 + 0], wt);
+    unsafeAtomicAdd(&ovf[node * 4 + 1], vx * wt);
+    unsafeAtomicAdd(&ovf[node * 4 + 2], vy * wt);
+    unsafeAtomicAdd(&ovf[node * 4 + 3], vz * wt);
+  }
 }
 
 // Place a particle whose destination slot was reserved: into the destination tile
@@ -83,19 +105,19 @@ struct TileWG {
   int t, tx, ty, tz, i0, j0, k0;
   int RA, RA3, RAx, RAy, RAz, H;
   double *sacc;
-  int *s_cnt, *s_base, *s_nbr;
+  int *s_cnt, *s_base, *s_nbr;  // counters and bases: two buffers of 27
+  int par;                       // buffer of the current batch
 };
 
 // Drift + wrap, deposit, and re-binning of one particle per thread.  Every thread
 // of the workgroup calls this in uniform control flow (re-binning uses barriers).
-template <int MODE>
-__device__ inline void particle_tail(const TileArgs &a, const TileWG &w, bool valid, long p, double px, double py,
+template <int MODE, int TT, int HH>
+__device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, long p, double px, double py,
                                      double pz, double vx, double vy, double vz, int pid) {
   const TileGeom &g = a.g;
   const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
   const int tid = threadIdx.x;
   bool lost = false;
-  int ntile = w.t;
   if ((MODE & M_DRIFT) && valid) {
     // UpdatePositions drift (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28).
     px += vx * a.dc.dt;
@@ -115,7 +137,8 @@ __device__ inline void particle_tail(const TileArgs &a, const TileWG &w, bool va
   const double x = cell_coord(px, dx, a.pc.inv[0], a.pc.pow2);
   const double y = cell_coord(py, dy, a.pc.inv[1], a.pc.pow2);
   const double z = cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2);
-  if ((MODE & M_DRIFT) && valid && !lost) ntile = tile_of(g, x, y, z);
+  // Cell and tile of the new position (a coordinate equal to N stays in the last cell).
+  const int ci = min((int)x, g.Nx - 1), cj = min((int)y, g.Ny - 1), ck = min((int)z, g.Nz - 1);
   if ((MODE & M_DEPOSIT) && valid && !lost) {
     // Collect3DFluidMoments body, src/moments.c:154-205.
     int ixl = (int)x, iyl = (int)y, izl = (int)z;
@@ -131,54 +154,58 @@ __device__ inline void particle_tail(const TileArgs &a, const TileWG &w, bool va
     w8[5] = wzl * wyh * wxl;
     w8[6] = wzh * wyl * wxl;
     w8[7] = wzl * wyl * wxl;
-    int la = ixl - (w.i0 - w.H), lb = iyl - (w.j0 - w.H), lc = izl - (w.k0 - w.H);
+    constexpr int RA = TT + 1 + 2 * HH, RA3 = RA * RA * RA;
+    int la = ixl - (w.i0 - HH), lb = iyl - (w.j0 - HH), lc = izl - (w.k0 - HH);
     if (la < 0) la += g.Nx; else if (la > w.RAx - 2) la -= g.Nx;
     if (lb < 0) lb += g.Ny; else if (lb > w.RAy - 2) lb -= g.Ny;
     if (lc < 0) lc += g.Nz; else if (lc > w.RAz - 2) lc -= g.Nz;
     const bool inreg = la >= 0 && la <= w.RAx - 2 && lb >= 0 && lb <= w.RAy - 2 && lc >= 0 && lc <= w.RAz - 2;
+    if (inreg) {
+      const int base = (lc * RA + lb) * RA + la;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
-      const double wt = w8[c];
-      if (inreg) {
-        const int idx = ((lc + az) * w.RA + (lb + ay)) * w.RA + (la + ax);
-        atomicAdd(&w.sacc[idx], wt);
-        atomicAdd(&w.sacc[w.RA3 + idx], vx * wt);
-        atomicAdd(&w.sacc[2 * w.RA3 + idx], vy * wt);
-        atomicAdd(&w.sacc[3 * w.RA3 + idx], vz * wt);
-      } else {
-        const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
-                          wrap_index(ixl + ax, g.Nx);
-        unsafeAtomicAdd(&a.ovf[node * 4 + 0], wt);
-        unsafeAtomicAdd(&a.ovf[node * 4 + 1], vx * wt);
-        unsafeAtomicAdd(&a.ovf[node * 4 + 2], vy * wt);
-        unsafeAtomicAdd(&a.ovf[node * 4 + 3], vz * wt);
+      for (int c = 0; c < 8; ++c) {
+        const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+        const int idx = base + (az * RA + ay) * RA + ax;
+        const double wt = w8[c];
+        lds_add(&w.sacc[idx], wt);
+        lds_add(&w.sacc[RA3 + idx], vx * wt);
+        lds_add(&w.sacc[2 * RA3 + idx], vy * wt);
+        lds_add(&w.sacc[3 * RA3 + idx], vz * wt);
       }
+    } else {
+      deposit_overflow(a.ovf, g, ixl, iyl, izl, w8, vx, vy, vz);
     }
   }
   if (MODE & M_REBIN) {
     // Destination code: neighbour index 0..26, 27 = further than one tile, 28 = none.
     int code = 28;
     if (valid && !lost) {
-      const int nx = ntile % g.ntx, ny = (ntile / g.ntx) % g.nty, nz = ntile / (g.ntx * g.nty);
-      const int ox = tile_offset(nx, w.tx, g.ntx), oy = tile_offset(ny, w.ty, g.nty), oz = tile_offset(nz, w.tz, g.ntz);
+      const int ox = tile_step(ci / TT - w.tx, g.ntx), oy = tile_step(cj / TT - w.ty, g.nty);
+      const int oz = tile_step(ck / TT - w.tz, g.ntz);
       code = (ox < 2 && oy < 2 && oz < 2) ? ((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1) : 27;
     }
-    if (tid < 27) w.s_cnt[tid] = 0;
-    __syncthreads();
+    // Counters and bases are double-buffered across batches: the buffer used by the
+    // previous batch is cleared here, after the barrier every thread passed once it
+    // finished with it, so two barriers per batch suffice.
+    int *cnt = w.s_cnt + 27 * w.par, *bas = w.s_base + 27 * w.par;
     int rank = 0;
-    if (code < 27) rank = atomicAdd(&w.s_cnt[code], 1);
+    if (code < 27) rank = atomicAdd(&cnt[code], 1);
     __syncthreads();
-    if (tid < 27 && w.s_cnt[tid] > 0) w.s_base[tid] = atomicAdd(&a.next[w.s_nbr[tid]], w.s_cnt[tid]);
+    if (tid < 27) {
+      const int n = cnt[tid];
+      if (n > 0) bas[tid] = atomicAdd(&a.next[w.s_nbr[tid]], n);
+    } else if (tid >= 64 && tid < 64 + 27) {
+      w.s_cnt[27 * (w.par ^ 1) + tid - 64] = 0;
+    }
     __syncthreads();
     if (code < 27) {
-      place(a, w.s_nbr[code], w.s_base[code] + rank, px, py, pz, vx, vy, vz, pid);
+      place(a, w.s_nbr[code], bas[code] + rank, px, py, pz, vx, vy, vz, pid);
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
-    __syncthreads();
+    w.par ^= 1;
   } else if ((MODE & M_PUSH) && valid) {
     a.src.vx[p] = vx;
     a.src.vy[p] = vy;
@@ -186,38 +213,40 @@ __device__ inline void particle_tail(const TileArgs &a, const TileWG &w, bool va
   }
 }
 
-template <int MODE, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_tile(TileArgs a) {
+template <int MODE, int TT, int HH>
+__global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ double smem[];
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
-  __shared__ int s_cnt[27], s_base[27], s_nbr[27], s_ndef;
+  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef;
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
   int wi = blockIdx.x;
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so neighbouring tiles share an L2.
   if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
+  constexpr int RP = TT + 4, RP3 = RP * RP * RP;
+  constexpr int RA = TT + 1 + 2 * HH;
   TileWG w;
   w.t = wi / g.split;
   const int s = wi % g.split;
   w.tx = w.t % g.ntx;
   w.ty = (w.t / g.ntx) % g.nty;
   w.tz = w.t / (g.ntx * g.nty);
-  w.i0 = w.tx * g.T;
-  w.j0 = w.ty * g.T;
-  w.k0 = w.tz * g.T;
-  w.H = g.H;
-  w.RA = g.RA;
-  w.RA3 = g.RA * g.RA * g.RA;
-  w.RAx = min(g.T, g.Nx - w.i0) + 1 + 2 * g.H;
-  w.RAy = min(g.T, g.Ny - w.j0) + 1 + 2 * g.H;
-  w.RAz = min(g.T, g.Nz - w.k0) + 1 + 2 * g.H;
+  w.i0 = w.tx * TT;
+  w.j0 = w.ty * TT;
+  w.k0 = w.tz * TT;
+  w.H = HH;
+  w.RA = RA;
+  w.RA3 = RA * RA * RA;
+  w.RAx = min(TT, g.Nx - w.i0) + 1 + 2 * HH;
+  w.RAy = min(TT, g.Ny - w.j0) + 1 + 2 * HH;
+  w.RAz = min(TT, g.Nz - w.k0) + 1 + 2 * HH;
   w.s_cnt = s_cnt;
   w.s_base = s_base;
+  w.par = 0;
   w.s_nbr = s_nbr;
-  const int RP = g.RP, RP3 = RP * RP * RP;
   double *sphi = smem;
   w.sacc = smem + ((MODE & M_PUSH) ? RP3 : 0);
   const int tid = threadIdx.x;
@@ -234,6 +263,7 @@ __global__ void __launch_bounds__(MAXT) k_tile(TileArgs a) {
   if (MODE & M_DEPOSIT) {
     for (int q = tid; q < 4 * w.RA3; q += blockDim.x) w.sacc[q] = 0.0;
   }
+  if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
   if ((MODE & M_REBIN) && tid < 27) {
     const int ox = tid % 3 - 1, oy = (tid / 3) % 3 - 1, oz = tid / 9 - 1;
     s_nbr[tid] = (wrap_index(w.tz + oz, g.ntz) * g.nty + wrap_index(w.ty + oy, g.nty)) * g.ntx +
@@ -281,7 +311,7 @@ __global__ void __launch_bounds__(MAXT) k_tile(TileArgs a) {
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body, src/velocities.c:342-356.
       double E[3];
-      diff_vec3d_lds(sphi, RP, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, cell_coord(px, dx, a.pc.inv[0], a.pc.pow2),
+      diff_vec3d_lds<RP>(sphi, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, cell_coord(px, dx, a.pc.inv[0], a.pc.pow2),
                      cell_coord(py, dy, a.pc.inv[1], a.pc.pow2), cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2), E);
       double vxm = vx + a.pc.Escale[0] * E[0];
       double vym = vy + a.pc.Escale[1] * E[1];
@@ -302,7 +332,7 @@ __global__ void __launch_bounds__(MAXT) k_tile(TileArgs a) {
       const int q = atomicAdd(&s_ndef, 1);
       store_particle(a.def, beg + q, px, py, pz, vx, vy, vz, pid);
     }
-    particle_tail<MODE>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
+    particle_tail<MODE, TT, HH>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
     // Phase 2: the deferred ions, collided then drifted, deposited and re-binned.
@@ -325,7 +355,7 @@ __global__ void __launch_bounds__(MAXT) k_tile(TileArgs a) {
         const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
         collide_attempts(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
       }
-      particle_tail<MODE>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
+      particle_tail<MODE, TT, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
     }
     unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
     if (__lane_id() == 0) {
@@ -640,23 +670,23 @@ static void ensure_keys(ecsprt_hip_ctx *c, long np) {
   c->key_cap = np;
 }
 
-template <int MODE, int MAXT>
+template <int MODE, int TT, int HH>
 static void set_lds_attr() {
   static bool done = false;
   if (!done) {
     // 160 KB of LDS per workgroup on gfx950, less what the kernel declares statically.
     hipFuncAttributes fa;
-    ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, MAXT>));
+    ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, TT, HH>));
     const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
-    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, MAXT>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, TT, HH>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     done = true;
   }
 }
 
 template <int MODE>
 static void set_lds_attrs() {
-  set_lds_attr<MODE, 1024>();
-  set_lds_attr<MODE, 512>();
+  set_lds_attr<MODE, 4, 2>();
+  set_lds_attr<MODE, 8, 2>();
 }
 
 static size_t lds_for(const TileGeom &g, int mode, int) {
@@ -677,8 +707,10 @@ void tiles_init(ecsprt_hip_ctx *c) {
     long n8 = (long)((p.Nx + 7) / 8) * ((p.Ny + 7) / 8) * ((p.Nz + 7) / 8);
     T = n8 >= 512 ? 8 : 4;
   }
+  ECS_CHECK(T == 4 || T == 8, ECSPRT_ERR_ARG, "tile edge must be 4 or 8 cells");
+  ECS_CHECK(c->o.halo <= 0 || c->o.halo == 2, ECSPRT_ERR_ARG, "the deposit halo is 2 cells");
   g.T = T;
-  g.H = c->o.halo > 0 ? c->o.halo : 2;
+  g.H = 2;
   g.ntx = (p.Nx + T - 1) / T;
   g.nty = (p.Ny + T - 1) / T;
   g.ntz = (p.Nz + T - 1) / T;
@@ -693,7 +725,8 @@ void tiles_init(ecsprt_hip_ctx *c) {
   }
   g.split = split;
   int block = c->o.block > 0 ? c->o.block : 1024;
-  ECS_CHECK(block % 64 == 0 && block <= 1024, ECSPRT_ERR_ARG, "block must be a multiple of 64, <= 1024");
+  ECS_CHECK(block % 64 == 0 && block >= 128 && block <= 1024, ECSPRT_ERR_ARG,
+            "block must be a multiple of 64 in [128, 1024]");
   size_t lds = lds_for(g, M_PUSH | M_COLLIDE | M_DEPOSIT, block);
   ECS_CHECK(lds <= 159 * 1024, ECSPRT_ERR_ARG, "tile/halo too large for 160 KB of LDS");
   c->lds_bytes = lds;
@@ -861,10 +894,9 @@ template <int MODE>
 static void launch_tile(ecsprt_hip_ctx *c, const TileArgs &a) {
   const int block = c->o.block > 0 ? c->o.block : 1024;
   const size_t lds = lds_for(c->g, MODE, block);
-  // The launch bound fixes the register budget: 1024 threads leave 128 VGPRs per
-  // lane, 512 threads leave 256.
-  if (block <= 512) k_tile<MODE, 512><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
-  else k_tile<MODE, 1024><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  // Tile edge and halo are compile-time so LDS offsets become immediates.
+  if (c->g.T == 8) k_tile<MODE, 8, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  else k_tile<MODE, 4, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
   ECS_HIP(hipGetLastError());
 }
 

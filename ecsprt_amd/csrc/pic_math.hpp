@@ -16,9 +16,9 @@ namespace ecsprt {
 // periodic potential (global, unwrapped indices).  Branch choices use the global
 // indices exactly as the reference does; the interpolation order keeps the
 // reference's y->wyh, z->wxh, x->wzh weight assignment (vector-math.c:645-651).
-__device__ inline void diff_vec3d_lds(const double *__restrict__ F, int RP, int oi, int oj, int ok,
-                                      int Nx, int Ny, int Nz, double x0, double y0, double z0,
-                                      double f[3]) {
+template <int RP>
+__device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
+                                      double x0, double y0, double z0, double f[3]) {
 #define PL(kk, jj, ii) F[(((kk)-ok) * RP + ((jj)-oj)) * RP + ((ii)-oi)]
   int ixl = (int)x0, ixh = ixl + 1;
   int iyl = (int)y0, iyh = iyl + 1;
