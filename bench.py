@@ -39,6 +39,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision solve check")
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--tile-z", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--collide", default="philox", choices=["philox", "off"])
@@ -128,7 +129,7 @@ def main():
     p = pic.setup_context(**wl.cli)
     cmode = _lib.COLLIDE_PHILOX if args.collide == "philox" else _lib.COLLIDE_OFF
     opts = pic.default_options(collide_mode=cmode, ksp_rtol=wl.rtol, timing=1,
-                               tile=args.tile, block=args.block, split=args.split)
+                               tile=args.tile, tile_z=args.tile_z, block=args.block, split=args.split)
     if args.mg_pre:
         opts.mg_pre = args.mg_pre
     if args.mg_post:

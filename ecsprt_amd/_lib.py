@@ -49,7 +49,7 @@ class Attempt(ctypes.Structure):
 
 
 class Options(ctypes.Structure):
-    _fields_ = [("tile", ctypes.c_int), ("halo", ctypes.c_int), ("block", ctypes.c_int),
+    _fields_ = [("tile", ctypes.c_int), ("tile_z", ctypes.c_int), ("halo", ctypes.c_int), ("block", ctypes.c_int),
                 ("split", ctypes.c_int), ("slack", ctypes.c_double), ("collide_mode", ctypes.c_int),
                 ("seed", ctypes.c_uint64), ("operator_mode", ctypes.c_int), ("initial_guess", ctypes.c_int),
                 ("pc_type", ctypes.c_int), ("solve_precision", ctypes.c_int), ("ksp_restart", ctypes.c_int),

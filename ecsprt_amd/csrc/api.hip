@@ -142,6 +142,7 @@ int ecsprt_hip_default_options(ecsprt_options *o) {
   if (!o) return ECSPRT_ERR_ARG;
   memset(o, 0, sizeof(*o));
   o->tile = 0;
+  o->tile_z = 0;
   o->halo = 2;
   o->block = 1024;
   o->split = 0;

@@ -42,12 +42,11 @@ struct ParticlesF64 {
 // particle arrays; capacity cap[t] leaves room for immigrants.
 struct TileGeom {
   int Nx, Ny, Nz;
-  int T;          // cells per tile edge
+  int Tx, Ty, Tz;     // cells per tile along each axis
   int ntx, nty, ntz;
-  int H;          // deposit halo (cells) of the accumulation region
-  int RP;         // potential region edge: T+4 nodes, origin at tile origin - 1
-  int RA;         // accumulation region edge: T+1+2H nodes, origin at tile origin - H
-  int split;      // workgroups per tile
+  int H;              // deposit halo (cells) of the accumulation region
+  int RAx, RAy, RAz;  // accumulation region: T+1+2H nodes per axis, origin at tile origin - H
+  int split;          // workgroups per tile
 };
 
 struct PushConsts {

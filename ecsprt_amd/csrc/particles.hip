@@ -37,7 +37,7 @@ __device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg
   if (ci >= g.Nx) ci = g.Nx - 1;
   if (cj >= g.Ny) cj = g.Ny - 1;
   if (ck >= g.Nz) ck = g.Nz - 1;
-  return ((ck / g.T) * g.nty + cj / g.T) * g.ntx + ci / g.T;
+  return ((ck / g.Tz) * g.nty + cj / g.Ty) * g.ntx + ci / g.Tx;
 }
 
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
@@ -111,7 +111,7 @@ struct TileWG {
 
 // Drift + wrap, deposit, and re-binning of one particle per thread.  Every thread
 // of the workgroup calls this in uniform control flow (re-binning uses barriers).
-template <int MODE, int TT, int HH>
+template <int MODE, int TX, int TY, int TZ, int HH>
 __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, long p, double px, double py,
                                      double pz, double vx, double vy, double vz, int pid) {
   const TileGeom &g = a.g;
@@ -154,18 +154,18 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     w8[5] = wzl * wyh * wxl;
     w8[6] = wzh * wyl * wxl;
     w8[7] = wzl * wyl * wxl;
-    constexpr int RA = TT + 1 + 2 * HH, RA3 = RA * RA * RA;
+    constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
     int la = ixl - (w.i0 - HH), lb = iyl - (w.j0 - HH), lc = izl - (w.k0 - HH);
     if (la < 0) la += g.Nx; else if (la > w.RAx - 2) la -= g.Nx;
     if (lb < 0) lb += g.Ny; else if (lb > w.RAy - 2) lb -= g.Ny;
     if (lc < 0) lc += g.Nz; else if (lc > w.RAz - 2) lc -= g.Nz;
     const bool inreg = la >= 0 && la <= w.RAx - 2 && lb >= 0 && lb <= w.RAy - 2 && lc >= 0 && lc <= w.RAz - 2;
     if (inreg) {
-      const int base = (lc * RA + lb) * RA + la;
+      const int base = (lc * RAY + lb) * RAX + la;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
-        const int idx = base + (az * RA + ay) * RA + ax;
+        const int idx = base + (az * RAY + ay) * RAX + ax;
         const double wt = w8[c];
         lds_add(&w.sacc[idx], wt);
         lds_add(&w.sacc[RA3 + idx], vx * wt);
@@ -180,8 +180,8 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     // Destination code: neighbour index 0..26, 27 = further than one tile, 28 = none.
     int code = 28;
     if (valid && !lost) {
-      const int ox = tile_step(ci / TT - w.tx, g.ntx), oy = tile_step(cj / TT - w.ty, g.nty);
-      const int oz = tile_step(ck / TT - w.tz, g.ntz);
+      const int ox = tile_step(ci / TX - w.tx, g.ntx), oy = tile_step(cj / TY - w.ty, g.nty);
+      const int oz = tile_step(ck / TZ - w.tz, g.ntz);
       code = (ox < 2 && oy < 2 && oz < 2) ? ((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1) : 27;
     }
     // Counters and bases are double-buffered across batches: the buffer used by the
@@ -213,7 +213,7 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
   }
 }
 
-template <int MODE, int TT, int HH>
+template <int MODE, int TX, int TY, int TZ, int HH>
 __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ double smem[];
@@ -226,23 +226,23 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so neighbouring tiles share an L2.
   if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
-  constexpr int RP = TT + 4, RP3 = RP * RP * RP;
-  constexpr int RA = TT + 1 + 2 * HH;
+  constexpr int RPX = TX + 4, RPY = TY + 4, RPZ = TZ + 4, RP3 = RPX * RPY * RPZ;
+  constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH;
   TileWG w;
   w.t = wi / g.split;
   const int s = wi % g.split;
   w.tx = w.t % g.ntx;
   w.ty = (w.t / g.ntx) % g.nty;
   w.tz = w.t / (g.ntx * g.nty);
-  w.i0 = w.tx * TT;
-  w.j0 = w.ty * TT;
-  w.k0 = w.tz * TT;
+  w.i0 = w.tx * TX;
+  w.j0 = w.ty * TY;
+  w.k0 = w.tz * TZ;
   w.H = HH;
-  w.RA = RA;
-  w.RA3 = RA * RA * RA;
-  w.RAx = min(TT, g.Nx - w.i0) + 1 + 2 * HH;
-  w.RAy = min(TT, g.Ny - w.j0) + 1 + 2 * HH;
-  w.RAz = min(TT, g.Nz - w.k0) + 1 + 2 * HH;
+  w.RA = RAX;
+  w.RA3 = RAX * RAY * RAZ;
+  w.RAx = min(TX, g.Nx - w.i0) + 1 + 2 * HH;
+  w.RAy = min(TY, g.Ny - w.j0) + 1 + 2 * HH;
+  w.RAz = min(TZ, g.Nz - w.k0) + 1 + 2 * HH;
   w.s_cnt = s_cnt;
   w.s_base = s_base;
   w.par = 0;
@@ -253,7 +253,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
 
   if (MODE & M_PUSH) {
     for (int q = tid; q < RP3; q += blockDim.x) {
-      int li = q % RP, lj = (q / RP) % RP, lk = q / (RP * RP);
+      int li = q % RPX, lj = (q / RPX) % RPY, lk = q / (RPX * RPY);
       int gi = wrap_index(w.i0 - 1 + li, g.Nx);
       int gj = wrap_index(w.j0 - 1 + lj, g.Ny);
       int gk = wrap_index(w.k0 - 1 + lk, g.Nz);
@@ -311,7 +311,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body, src/velocities.c:342-356.
       double E[3];
-      diff_vec3d_lds<RP>(sphi, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, cell_coord(px, dx, a.pc.inv[0], a.pc.pow2),
+      diff_vec3d_lds<RPX, RPY>(sphi, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, cell_coord(px, dx, a.pc.inv[0], a.pc.pow2),
                      cell_coord(py, dy, a.pc.inv[1], a.pc.pow2), cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2), E);
       double vxm = vx + a.pc.Escale[0] * E[0];
       double vym = vy + a.pc.Escale[1] * E[1];
@@ -332,7 +332,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       const int q = atomicAdd(&s_ndef, 1);
       store_particle(a.def, beg + q, px, py, pz, vx, vy, vz, pid);
     }
-    particle_tail<MODE, TT, HH>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
+    particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
     // Phase 2: the deferred ions, collided then drifted, deposited and re-binned.
@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
         const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
         collide_attempts(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
       }
-      particle_tail<MODE, TT, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
+      particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
     }
     unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
     if (__lane_id() == 0) {
@@ -404,7 +404,7 @@ __global__ void k_finalize_counts(int ntiles, int *count, int *next, const int *
 __global__ void k_fold(TileGeom g, const double *__restrict__ fold, const double *__restrict__ ovf,
                        const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m) {
   const long nnode = (long)g.Nx * g.Ny * g.Nz;
-  const int RA = g.RA, RA3 = RA * RA * RA;
+  const int RAX = g.RAx, RAY = g.RAy, RA3 = g.RAx * g.RAy * g.RAz;
   for (long node = blockIdx.x * (long)blockDim.x + threadIdx.x; node < nnode; node += (long)gridDim.x * blockDim.x) {
     const int i = node % g.Nx, j = (node / g.Nx) % g.Ny, k = node / ((long)g.Nx * g.Ny);
     const int *tx = tab + offx, *ty = tab + offy, *tz = tab + offz;
@@ -417,7 +417,7 @@ __global__ void k_fold(TileGeom g, const double *__restrict__ fold, const double
       for (int b = 0; b < cy; ++b)
         for (int c = 0; c < cx; ++c) {
           const int t = (pz[2 * a] * g.nty + py[2 * b]) * g.ntx + px[2 * c];
-          const int loc = (pz[2 * a + 1] * RA + py[2 * b + 1]) * RA + px[2 * c + 1];
+          const int loc = (pz[2 * a + 1] * RAY + py[2 * b + 1]) * RAX + px[2 * c + 1];
           for (int s = 0; s < g.split; ++s) {
             const double *r = fold + (size_t)(t * g.split + s) * 4 * RA3 + loc;
             acc[0] += r[0];
@@ -670,29 +670,30 @@ static void ensure_keys(ecsprt_hip_ctx *c, long np) {
   c->key_cap = np;
 }
 
-template <int MODE, int TT, int HH>
+template <int MODE, int TX, int TY, int TZ, int HH>
 static void set_lds_attr() {
   static bool done = false;
   if (!done) {
     // 160 KB of LDS per workgroup on gfx950, less what the kernel declares statically.
     hipFuncAttributes fa;
-    ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, TT, HH>));
+    ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, TX, TY, TZ, HH>));
     const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
-    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, TT, HH>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, TX, TY, TZ, HH>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
     done = true;
   }
 }
 
 template <int MODE>
 static void set_lds_attrs() {
-  set_lds_attr<MODE, 4, 2>();
-  set_lds_attr<MODE, 8, 2>();
+  set_lds_attr<MODE, 4, 4, 4, 2>();
+  set_lds_attr<MODE, 8, 8, 4, 2>();
+  set_lds_attr<MODE, 8, 8, 8, 2>();
 }
 
 static size_t lds_for(const TileGeom &g, int mode, int) {
   size_t b = 0;
-  if (mode & M_PUSH) b += sizeof(double) * g.RP * g.RP * g.RP;
-  if (mode & M_DEPOSIT) b += sizeof(double) * 4 * g.RA * g.RA * g.RA;
+  if (mode & M_PUSH) b += sizeof(double) * (g.Tx + 4) * (g.Ty + 4) * (g.Tz + 4);
+  if (mode & M_DEPOSIT) b += sizeof(double) * 4 * g.RAx * g.RAy * g.RAz;
   return b;
 }
 
@@ -707,15 +708,20 @@ void tiles_init(ecsprt_hip_ctx *c) {
     long n8 = (long)((p.Nx + 7) / 8) * ((p.Ny + 7) / 8) * ((p.Nz + 7) / 8);
     T = n8 >= 512 ? 8 : 4;
   }
-  ECS_CHECK(T == 4 || T == 8, ECSPRT_ERR_ARG, "tile edge must be 4 or 8 cells");
+  const int Tz = c->o.tile_z > 0 ? c->o.tile_z : T;
+  ECS_CHECK((T == 4 && Tz == 4) || (T == 8 && (Tz == 8 || Tz == 4)), ECSPRT_ERR_ARG,
+            "tile shapes: 4x4x4, 8x8x8 or 8x8x4 cells");
   ECS_CHECK(c->o.halo <= 0 || c->o.halo == 2, ECSPRT_ERR_ARG, "the deposit halo is 2 cells");
-  g.T = T;
+  g.Tx = T;
+  g.Ty = T;
+  g.Tz = Tz;
   g.H = 2;
-  g.ntx = (p.Nx + T - 1) / T;
-  g.nty = (p.Ny + T - 1) / T;
-  g.ntz = (p.Nz + T - 1) / T;
-  g.RP = T + 4;
-  g.RA = T + 1 + 2 * g.H;
+  g.ntx = (p.Nx + g.Tx - 1) / g.Tx;
+  g.nty = (p.Ny + g.Ty - 1) / g.Ty;
+  g.ntz = (p.Nz + g.Tz - 1) / g.Tz;
+  g.RAx = g.Tx + 1 + 2 * g.H;
+  g.RAy = g.Ty + 1 + 2 * g.H;
+  g.RAz = g.Tz + 1 + 2 * g.H;
   c->ntiles = g.ntx * g.nty * g.ntz;
   int split = c->o.split;
   if (split <= 0) {
@@ -735,12 +741,13 @@ void tiles_init(ecsprt_hip_ctx *c) {
   std::vector<int> tab;
   const int Ns[3] = {p.Nx, p.Ny, p.Nz};
   const int nts[3] = {g.ntx, g.nty, g.ntz};
+  const int Ts[3] = {g.Tx, g.Ty, g.Tz};
   for (int ax = 0; ax < 3; ++ax) {
     const int N = Ns[ax];
     c->foldtab_off[ax] = (int)tab.size();
     std::vector<int> cnt(N, 0), pairs((size_t)N * FOLD_MAXP * 2, 0);
     for (int ti = 0; ti < nts[ax]; ++ti) {
-      int i0 = ti * T, ext = std::min(T, N - i0), R = ext + 1 + 2 * g.H;
+      int i0 = ti * Ts[ax], ext = std::min(Ts[ax], N - i0), R = ext + 1 + 2 * g.H;
       for (int a = 0; a < R; ++a) {
         int node = wrap_index(i0 - g.H + a, N);
         ECS_CHECK(cnt[node] < FOLD_MAXP, ECSPRT_ERR_ARG, "grid too small for the tile halo");
@@ -756,7 +763,7 @@ void tiles_init(ecsprt_hip_ctx *c) {
   ECS_HIP(hipMemcpy(c->d_foldtab, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice));
 
   const long nreg = (long)c->ntiles * g.split;
-  const size_t fold_bytes = sizeof(double) * 4 * (size_t)g.RA * g.RA * g.RA * nreg;
+  const size_t fold_bytes = sizeof(double) * 4 * (size_t)g.RAx * g.RAy * g.RAz * nreg;
   if (hipMalloc(&c->d_fold, fold_bytes) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "fold buffer"};
   ECS_HIP(hipMalloc(&c->d_ovf, sizeof(double) * 4 * c->nnode));
   ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * 8));
@@ -895,8 +902,10 @@ static void launch_tile(ecsprt_hip_ctx *c, const TileArgs &a) {
   const int block = c->o.block > 0 ? c->o.block : 1024;
   const size_t lds = lds_for(c->g, MODE, block);
   // Tile edge and halo are compile-time so LDS offsets become immediates.
-  if (c->g.T == 8) k_tile<MODE, 8, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
-  else k_tile<MODE, 4, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  const int tx = c->g.Tx, tz = c->g.Tz;
+  if (tx == 8 && tz == 8) k_tile<MODE, 8, 8, 8, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  else if (tx == 8 && tz == 4) k_tile<MODE, 8, 8, 4, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
+  else k_tile<MODE, 4, 4, 4, 2><<<c->ntiles * c->g.split, block, lds, c->stream>>>(a);
   ECS_HIP(hipGetLastError());
 }
 

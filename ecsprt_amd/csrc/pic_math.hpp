@@ -13,13 +13,13 @@ namespace ecsprt {
 
 // Field gather, DifferenceVector3D (src/vector-math.c:596-722), from a potential
 // region staged in LDS.  F holds nodes [oi, oi+RP) x [oj, oj+RP) x [ok, ok+RP) of the
-// periodic potential (global, unwrapped indices).  Branch choices use the global
+// periodic potential (global, unwrapped indices), RPX x RPY nodes per plane.  Branch choices use the global
 // indices exactly as the reference does; the interpolation order keeps the
 // reference's y->wyh, z->wxh, x->wzh weight assignment (vector-math.c:645-651).
-template <int RP>
+template <int RPX, int RPY>
 __device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
                                       double x0, double y0, double z0, double f[3]) {
-#define PL(kk, jj, ii) F[(((kk)-ok) * RP + ((jj)-oj)) * RP + ((ii)-oi)]
+#define PL(kk, jj, ii) F[(((kk)-ok) * RPY + ((jj)-oj)) * RPX + ((ii)-oi)]
   int ixl = (int)x0, ixh = ixl + 1;
   int iyl = (int)y0, iyh = iyl + 1;
   int izl = (int)z0, izh = izl + 1;

@@ -92,7 +92,8 @@ typedef struct {
 #define ECSPRT_SOLVE_MIXED 1 /* FP32 GMRES+GMG inside FP64 iterative refinement */
 
 typedef struct {
-  int      tile;           /* cells per tile edge (0 = 8) */
+  int      tile;           /* cells per tile edge along x and y: 4 or 8 (0 = by grid size) */
+  int      tile_z;         /* cells per tile along z: 4 or 8 (0 = tile) */
   int      halo;           /* deposit halo in cells for the fused step (0 = 2) */
   int      block;          /* threads per tile workgroup (0 = 1024) */
   int      split;          /* workgroups per tile (0 = auto) */

@@ -231,3 +231,26 @@ def test_stages_with_cell_widths(dx, dy, dz):
     np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
     assert rel(ctx.get_moments(), oracle.collect_moments(po, pos, vel)) <= 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("tile,tile_z,block", [(8, 4, 512), (4, 4, 128), (8, 8, 256)])
+def test_fused_step_tile_shapes(tile, tile_z, block):
+    """The fused pass for every tile shape and block size equals the stage sequence."""
+    say(f"fused step tile {tile}x{tile}x{tile_z} block {block}")
+    wl = configs.small(20, 5)
+    ctx, po, pos, vel = make(wl, tile=tile, tile_z=tile_z, block=block, collide_mode=_lib.COLLIDE_OFF,
+                             ksp_rtol=1e-8, operator_mode=_lib.OPERATOR_REBUILD)
+    rng = np.random.default_rng(5)
+    phi = 0.05 * rng.normal(size=(20, 20, 20))
+    ctx.set_particles(pos, vel)
+    for _ in range(3):
+        ctx.set_potential(phi)
+        ctx.step(1.0)
+        oracle.boris(po, 1.0, pos, vel, phi)
+        oracle.update_positions(1.0, pos, vel)
+        oracle.apply_bc(po, pos)
+    gpos, gvel = ctx.get_particles()
+    np.testing.assert_array_equal(gpos, pos)
+    np.testing.assert_array_equal(gvel, vel)
+    assert rel(ctx.get_moments(), oracle.collect_moments(po, pos, vel)) <= 1e-12
+    ctx.close()
