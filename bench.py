@@ -25,6 +25,7 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE = 96  # FP64 SoA: read x,y,z,vx,vy,vz, write them back
+BYTES_PER_PARTICLE_MIXED = 56  # int32 cell + 3 FP32 offsets + 3 FP32 velocities, read and written
 BYTES_PER_NODE = 40      # read phi (8 B), write n, Gx, Gy, Gz (32 B)
 
 
@@ -33,7 +34,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2"],
+                    help="c2 = the c1 workload on the mixed-precision path only")
     ap.add_argument("--ppc", type=int, default=0, help="override ions per cell")
     ap.add_argument("--grid", type=int, default=0, help="override grid edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -54,7 +56,7 @@ def parse():
 
 def workload(args):
     from ecsprt_amd import configs
-    wl = configs.C1 if args.workload == "c1" else configs.C0
+    wl = configs.C0 if args.workload == "c0" else configs.C1
     if args.grid:
         wl.cli.update(Nx=args.grid, Ny=args.grid, Nz=args.grid)
     if args.ppc:
@@ -115,6 +117,104 @@ def copy_probe(gib, reps=5):
     return {"copy_bytes": n, "best_ms": best, "read_plus_write_GBs": 2 * n / (best * 1e-3) / 1e9}
 
 
+def committed_traffic(kernel, workload_name, args):
+    """HBM bytes per launch of `kernel` on this workload, from the calibrated rocprofv3
+    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/hbm_traffic.md.  Only the default
+    launch configuration was profiled; any other gives None."""
+    default = not (args.tile or args.tile_z or args.block or args.split or args.ppc or args.grid or args.mg_pre
+                   or args.mg_post) and args.collide == "philox"
+    path = os.path.join(HERE, "profiles", "hbm_traffic.md")
+    if not default or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic:"):
+            f = dict(kv.split("=", 1) for kv in line.split()[1:])
+            if f.get("kernel") == kernel and f.get("workload") == workload_name:
+                return float(f["bytes_per_launch"])
+    return None
+
+
+def run_timed(p, opts, wl, nsteps, warm, pic):
+    """Initial conditions, t = 0 collect + solve (src/pic.c:211-215), warm-up, then
+    nsteps timed steps.  Returns the per-stage means and the context."""
+    ctx = pic.Context(p, opts)
+    ctx.init_particles(wl.np, 2024, wl.amp, wl.k)
+    ctx.collect()
+    ctx.compute_potential()
+    for _ in range(warm):
+        ctx.step(wl.dt)
+    ctx.synchronize()
+    st_all, iters = [], []
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        ctx.step(wl.dt)
+        st_all.append(ctx.stage_times())
+        iters.append(ctx.solver_stats()[0])
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    st = np.mean(np.array(st_all), axis=0)
+    return ctx, {"seconds": el, "k_ms": float(st[0]), "pre_ms": float(st[1]), "rebin_ms": float(st[2]),
+                 "rhs_ms": float(st[4]), "solve_ms": float(st[5]), "iters": float(np.mean(iters))}
+
+
+def rel_err(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def norms(a, b):
+    """Max-norm and RMS errors of a against b, each relative to b's max-norm / RMS."""
+    d = a - b
+    return (float(np.abs(d).max() / max(np.abs(b).max(), 1e-300)),
+            float(np.sqrt(np.mean(d * d)) / max(np.sqrt(np.mean(b * b)), 1e-300)))
+
+
+def mixed_errors(p, wl, pic, _lib, steps=3):
+    """Errors of the mixed path (FP32 particles and mixed solve) against the FP64 path
+    (which matches the oracle to 1e-12), collisions off so that both follow the same
+    deterministic dynamics.
+
+    per_step: each step starts the mixed context from the FP64 state (particles and
+    potential), so the error is that of one mixed step.  free_running: both contexts
+    advance on their own from the same initial conditions; the reference's gather is
+    discontinuous across cell faces, so the few ions that FP32 rounding puts on the
+    other side of a face diverge and dominate the max-norm, while the RMS error stays
+    at the rounding level."""
+    base = dict(collide_mode=_lib.COLLIDE_OFF, ksp_rtol=wl.rtol)
+    c64 = pic.Context(p, pic.default_options(**base))
+    cmx = pic.Context(p, pic.default_options(**base, particle_precision=_lib.PARTICLES_MIXED,
+                                             solve_precision=_lib.SOLVE_MIXED))
+    for c in (c64, cmx):
+        c.init_particles(wl.np, 2024, wl.amp, wl.k)
+        c.collect()
+        c.compute_potential()
+
+    def compare(step):
+        m64, mmx = c64.get_moments(), cmx.get_moments()
+        n = norms(mmx[..., 0], m64[..., 0])
+        g = [norms(mmx[..., d], m64[..., d]) for d in (1, 2, 3)]
+        f = norms(cmx.get_potential(), c64.get_potential())
+        return {"step": step, "n_max": n[0], "n_rms": n[1], "G_max": max(x[0] for x in g),
+                "G_rms": max(x[1] for x in g), "phi_max": f[0], "phi_rms": f[1]}
+
+    free = []
+    for s in range(steps):
+        c64.step(wl.dt)
+        cmx.step(wl.dt)
+        free.append(compare(s + 1))
+    per = []
+    for s in range(steps):
+        pos, vel = c64.get_particles()
+        cmx.set_particles(pos, vel)
+        cmx.set_potential(c64.get_potential())
+        del pos, vel
+        c64.step(wl.dt)
+        cmx.step(wl.dt)
+        per.append(compare(steps + s + 1))
+    c64.close()
+    cmx.close()
+    return {"per_step": per, "free_running": free}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -128,104 +228,81 @@ def main():
     wl = workload(args)
     p = pic.setup_context(**wl.cli)
     cmode = _lib.COLLIDE_PHILOX if args.collide == "philox" else _lib.COLLIDE_OFF
-    opts = pic.default_options(collide_mode=cmode, ksp_rtol=wl.rtol, timing=1,
-                               tile=args.tile, tile_z=args.tile_z, block=args.block, split=args.split)
+    common = dict(collide_mode=cmode, ksp_rtol=wl.rtol, timing=1, tile=args.tile, tile_z=args.tile_z,
+                  block=args.block, split=args.split)
     if args.mg_pre:
-        opts.mg_pre = args.mg_pre
+        common["mg_pre"] = args.mg_pre
     if args.mg_post:
-        opts.mg_post = args.mg_post
-    ctx = pic.Context(p, opts)
-    npart = wl.np
-    ncell = wl.ncell
-    ctx.init_particles(npart, 2024, wl.amp, wl.k)
-    # t = 0: collect and the first solve (src/pic.c:211-215), which builds the operator.
-    ctx.collect()
-    ctx.compute_potential()
+        common["mg_post"] = args.mg_post
+    mixed_only = args.workload == "c2"
+    if mixed_only:
+        common.update(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
     nsteps = args.profile_steps or args.steps
     warm = 0 if args.profile_steps else args.warmup
-    for _ in range(warm):
-        ctx.step(wl.dt)
-    ctx.synchronize()
-    k_ms, pre_ms, rebin_ms, rhs_ms, solve_ms, iters = [], [], [], [], [], []
-    t0 = time.perf_counter()
-    for _ in range(nsteps):
-        ctx.step(wl.dt)
-        st = ctx.stage_times()
-        k_ms.append(st[0])
-        pre_ms.append(st[1])
-        rebin_ms.append(st[2])
-        rhs_ms.append(st[4])
-        solve_ms.append(st[5])
-        iters.append(ctx.solver_stats()[0])
-    ctx.synchronize()
-    el = time.perf_counter() - t0
+    ctx, r = run_timed(p, pic.default_options(**common), wl, nsteps, warm, pic)
     if args.profile_steps:
-        print(json.dumps({"profile_steps": nsteps, "seconds": el}))
+        print(json.dumps({"profile_steps": nsteps, "seconds": r["seconds"]}))
         ctx.close()
         return
     Nc, Ns, Nf, natt = ctx.collision_stats()
-    ms_step = 1e3 * el / nsteps
-    value = npart * nsteps / el
-    kmean = float(np.mean(k_ms))
-    alg_bytes = BYTES_PER_PARTICLE * npart + BYTES_PER_NODE * ncell
-    achieved = alg_bytes / (kmean * 1e-3) / 1e9
-    particle_ms = kmean + float(np.mean(pre_ms)) + float(np.mean(rebin_ms))
+    ctx.close()
+    npart, ncell = wl.np, wl.ncell
+    bpp = BYTES_PER_PARTICLE_MIXED if mixed_only else BYTES_PER_PARTICLE
+    alg_bytes = bpp * npart + BYTES_PER_NODE * ncell
+    achieved = alg_bytes / (r["k_ms"] * 1e-3) / 1e9
+    particle_ms = r["k_ms"] + r["pre_ms"] + r["rebin_ms"]
+    kernel = "k_tile_mx" if mixed_only else "k_tile"
+    traffic = committed_traffic(kernel, wl.name, args)
     out = {
         "metric": "particle-steps/s (push+deposit+collide); phi-solve ms; mixed vs FP64 @1e-6 err",
-        "value": value,
+        "value": npart * nsteps / r["seconds"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": nsteps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": 1e3 * r["seconds"] / nsteps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 particles, f64 fold, mixed solve" if mixed_only else "f64",
         "data": "synthetic (device Philox initial conditions)",
-        "config": {"workload": wl.name, "grid": [p.Nx, p.Ny, p.Nz], "ions_per_cell": wl.ppc, "np": npart,
-                   "dt": wl.dt, "Omega_i_dt": p.Omega_i * wl.dt, "nu_dt": p.nui * wl.dt, "E0": p.E0,
-                   "kappa_e": p.kappa_e, "ksp_rtol": wl.rtol, "collisions": "Philox statistical mode" if args.collide == "philox" else "off",
+        "config": {"workload": wl.name + ("_mixed" if mixed_only else ""), "grid": [p.Nx, p.Ny, p.Nz],
+                   "ions_per_cell": wl.ppc, "np": npart, "dt": wl.dt, "Omega_i_dt": p.Omega_i * wl.dt,
+                   "nu_dt": p.nui * wl.dt, "E0": p.E0, "kappa_e": p.kappa_e, "ksp_rtol": wl.rtol,
+                   "collisions": "Philox statistical mode" if args.collide == "philox" else "off",
                    "operator": "frozen after the first solve", "initial_guess": "zero each solve"},
         "particle_steps_per_s_particle_stages": npart / (particle_ms * 1e-3),
-        "phi_solve_ms": float(np.mean(solve_ms)) + float(np.mean(rhs_ms)),
-        "ksp_iterations_mean": float(np.mean(iters)),
+        "phi_solve_ms": r["solve_ms"] + r["rhs_ms"],
+        "ksp_iterations_mean": r["iters"],
         "collisions_last_step": {"Nc": Nc, "successes": Ns, "refused": Nf, "attempts": natt},
-        "stage_ms_mean": {"fused_particle_kernel": kmean, "pre": float(np.mean(pre_ms)),
-                          "rebin_fold": float(np.mean(rebin_ms)), "rhs": float(np.mean(rhs_ms)),
-                          "solve": float(np.mean(solve_ms))},
+        "stage_ms_mean": {"fused_particle_kernel": r["k_ms"], "pre": r["pre_ms"], "rebin_fold": r["rebin_ms"],
+                          "rhs": r["rhs_ms"], "solve": r["solve_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_tile<push|collide|drift|deposit|rebin>",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": kernel + "<push|collide|drift|deposit|rebin>",
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
-    if not args.no_mixed:
-        m = ctx.get_moments()
-        phi64 = ctx.get_potential()
-        optm = pic.default_options(collide_mode=_lib.COLLIDE_OFF, ksp_rtol=wl.rtol, timing=1,
-                                   solve_precision=_lib.SOLVE_MIXED)
-        cm = pic.Context(p, optm)
-        cm.set_moments(m)
-        cm.lhs()
-        cm.compute_potential()  # warm
-        cm.compute_potential()
-        st = cm.stage_times()
-        phim = cm.get_potential()
-        # FP64 reference solve on the same moments (the timed context froze its operator at t=0)
-        c64 = pic.Context(p, pic.default_options(collide_mode=_lib.COLLIDE_OFF, ksp_rtol=wl.rtol, timing=1))
-        c64.set_moments(m)
-        c64.compute_potential()
-        c64.compute_potential()
-        st64 = c64.stage_times()
-        phi64b = c64.get_potential()
-        out["mixed_solve"] = {"phi_rel_err_vs_fp64": float(np.abs(phim - phi64b).max() / np.abs(phi64b).max()),
-                              "mixed_solve_ms": float(st[4] + st[5]), "fp64_solve_ms": float(st64[4] + st64[5]),
-                              "speedup": float((st64[4] + st64[5]) / max(st[4] + st[5], 1e-9)),
-                              "iterations": cm.solver_stats()[0], "fp64_iterations": c64.solver_stats()[0]}
-        del phi64
+    if not args.no_mixed and not mixed_only:
+        # The same workload on the mixed-precision path (FP32 particles, mixed solve).
+        mcommon = dict(common, particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
+        cm, rm = run_timed(p, pic.default_options(**mcommon), wl, nsteps, warm, pic)
         cm.close()
-        c64.close()
-    ctx.close()
+        malg = BYTES_PER_PARTICLE_MIXED * npart + BYTES_PER_NODE * ncell
+        mach = malg / (rm["k_ms"] * 1e-3) / 1e9
+        mvalue = npart * nsteps / rm["seconds"]
+        out["mixed"] = {
+            "value": mvalue, "unit": "particle-steps/s", "ms_per_step": 1e3 * rm["seconds"] / nsteps,
+            "speedup_vs_fp64": mvalue / out["value"],
+            "fused_particle_kernel_ms": rm["k_ms"],
+            "particle_kernel_speedup_vs_fp64": r["k_ms"] / rm["k_ms"],
+            "phi_solve_ms": rm["solve_ms"] + rm["rhs_ms"],
+            "solve_speedup_vs_fp64": (r["solve_ms"] + r["rhs_ms"]) / max(rm["solve_ms"] + rm["rhs_ms"], 1e-9),
+            "ksp_iterations_mean": rm["iters"],
+            "roofline": {"bound": "hbm", "achieved": mach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": mach / HBM_PEAK_GBS, "traffic": committed_traffic("k_tile_mx", wl.name, args),
+                         "kernel": "k_tile_mx", "algorithmic_bytes_per_launch": malg},
+            "errors_vs_fp64": mixed_errors(p, wl, pic, _lib),
+        }
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         cb.pop("rc", None)

@@ -23,6 +23,7 @@ COLLIDE_OFF, COLLIDE_PREDRAWN, COLLIDE_PHILOX = 0, 1, 2
 OPERATOR_FROZEN, OPERATOR_REBUILD = 0, 1
 PC_NONE, PC_JACOBI, PC_GMG = 0, 1, 2
 SOLVE_FP64, SOLVE_MIXED = 0, 1
+PARTICLES_FP64, PARTICLES_MIXED = 0, 1
 
 _CLI_DOUBLES = ("dx", "dy", "dz", "x0", "y0", "z0", "x1", "y1", "z1", "B0", "E0", "n0",
                 "qi", "mi", "nui", "vi0x", "vi0y", "vi0z", "viTx", "viTy", "viTz", "Ti",
@@ -55,7 +56,8 @@ class Options(ctypes.Structure):
                 ("pc_type", ctypes.c_int), ("solve_precision", ctypes.c_int), ("ksp_restart", ctypes.c_int),
                 ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_atol", ctypes.c_double),
                 ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("mg_coarse_sweeps", ctypes.c_int),
-                ("inner_rtol", ctypes.c_double), ("timing", ctypes.c_int)]
+                ("inner_rtol", ctypes.c_double), ("timing", ctypes.c_int),
+                ("particle_precision", ctypes.c_int)]
 
 
 _P = ctypes.c_void_p

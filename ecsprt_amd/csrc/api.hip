@@ -162,6 +162,7 @@ int ecsprt_hip_default_options(ecsprt_options *o) {
   o->mg_coarse_sweeps = 50;
   o->inner_rtol = 1e-4;
   o->timing = 0;
+  o->particle_precision = ECSPRT_PARTICLES_FP64;
   return ECSPRT_OK;
 }
 
@@ -176,6 +177,17 @@ int ecsprt_hip_create(const ecsprt_params *p, const ecsprt_options *opt, ecsprt_
     ECS_CHECK(p->Nx >= 3 && p->Ny >= 3 && p->Nz >= 3, ECSPRT_ERR_ARG,
               "grid edges must be >= 3 (the 11-point stencil needs distinct neighbours)");
     ECS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, ECSPRT_ERR_ARG, "cell widths must be positive");
+    ECS_CHECK(c->o.particle_precision == ECSPRT_PARTICLES_FP64 || c->o.particle_precision == ECSPRT_PARTICLES_MIXED,
+              ECSPRT_ERR_ARG, "particle_precision must be ECSPRT_PARTICLES_FP64 or ECSPRT_PARTICLES_MIXED");
+    c->mixed = c->o.particle_precision == ECSPRT_PARTICLES_MIXED;
+    if (c->mixed) {
+      ECS_CHECK(p->Nx <= ecsprt::MX_MAX_NX && p->Ny <= ecsprt::MX_MAX_NY && p->Nz <= ecsprt::MX_MAX_NZ, ECSPRT_ERR_ARG,
+                "mixed-precision particles need Nx, Ny <= 2048 and Nz <= 1024");
+      ECS_CHECK(p->x0 == 0.0 && p->y0 == 0.0 && p->z0 == 0.0, ECSPRT_ERR_ARG,
+                "mixed-precision particles need the grid origin at 0");
+      ECS_CHECK(c->o.collide_mode != ECSPRT_COLLIDE_PREDRAWN, ECSPRT_ERR_ARG,
+                "pre-drawn collision records need FP64 particles");
+    }
     ECS_HIP(hipGetDevice(&c->device));
     ECS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &e : c->ev) ECS_HIP(hipEventCreate(&e));

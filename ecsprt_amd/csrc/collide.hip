@@ -274,7 +274,8 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
 
 // Acceptance estimate for the Philox mode: a dry run of the per-particle Poisson
 // process (velocities are not written) on a stream independent of the real steps.
-__global__ void k_pilot(ParticlesF64 A, const int *base, const int *count, CollideConsts k, PoissonTable pt,
+template <class P>
+__global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k, PoissonTable pt,
                         uint32_t key0, uint32_t key1, uint32_t tag, unsigned long long *ctr) {
   const int t = blockIdx.x;
   const long b = base[t], n = count[t];
@@ -300,8 +301,13 @@ static double pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam) {
   unsigned long long zero[2] = {0, 0};
   ECS_HIP(hipMemcpyAsync(c->d_ctr + 6, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
   CollideConsts k = collide_consts_of(c->p);
-  k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, poisson_table(lam), (uint32_t)c->o.seed,
-                                             (uint32_t)(c->o.seed >> 32), tag, c->d_ctr);
+  const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
+  if (c->mixed)
+    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, k, poisson_table(lam), k0, k1, tag,
+                                               c->d_ctr);
+  else
+    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, poisson_table(lam), k0, k1, tag,
+                                               c->d_ctr);
   ECS_HIP(hipGetLastError());
   unsigned long long h[2];
   ECS_HIP(hipMemcpyAsync(h, c->d_ctr + 6, sizeof(h), hipMemcpyDeviceToHost, c->stream));

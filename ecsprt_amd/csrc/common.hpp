@@ -37,6 +37,25 @@ struct ParticlesF64 {
   int *id = nullptr;
 };
 
+// Mixed-precision particle storage: a packed integer cell index (x in bits 0-10,
+// y in bits 11-21, z in bits 22-31), the FP32 offset within the cell in units of the
+// cell width, and the FP32 velocity.
+struct ParticlesMix {
+  uint32_t *cell = nullptr;
+  float *ox = nullptr, *oy = nullptr, *oz = nullptr;
+  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  int *id = nullptr;
+};
+
+constexpr int MX_MAX_NX = 2048, MX_MAX_NY = 2048, MX_MAX_NZ = 1024;
+
+__host__ __device__ inline uint32_t mx_pack(int i, int j, int k) {
+  return (uint32_t)i | ((uint32_t)j << 11) | ((uint32_t)k << 22);
+}
+__host__ __device__ inline int mx_i(uint32_t c) { return (int)(c & 2047u); }
+__host__ __device__ inline int mx_j(uint32_t c) { return (int)((c >> 11) & 2047u); }
+__host__ __device__ inline int mx_k(uint32_t c) { return (int)(c >> 22); }
+
 // Geometry of the cell tiling.  A tile is a T^3 brick of cells (clipped at the upper
 // grid faces).  Tile t's particles live in slots [base[t], base[t]+count[t]) of the
 // particle arrays; capacity cap[t] leaves room for immigrants.
@@ -92,6 +111,9 @@ struct ecsprt_hip_ctx {
   ecsprt::ParticlesF64 A, B, E;  // current, next, emigrants
   ecsprt::ParticlesF64 S;        // spill for tiles that overflow their capacity
   ecsprt::ParticlesF64 D;        // ions deferred to the collision phase (tile layout of A)
+  // Mixed-precision storage (particle_precision == ECSPRT_PARTICLES_MIXED): same roles.
+  bool mixed = false;
+  ecsprt::ParticlesMix Am, Bm, Em, Sm, Dm;
   long scap = 0;
   // counters: [0] emigrants, [1] spilled, [2] lost, [3] collision attempts,
   // [4] collision successes, [5] collision refusals (accumulated in long long below)

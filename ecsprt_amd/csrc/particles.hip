@@ -7,14 +7,14 @@
 // the potential it gathers from and the moments it deposits into sit in LDS.
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 
 #include "common.hpp"
 #include "philox.hpp"
 #include "pic_math.hpp"
+#include "tiles.hpp"
 
 namespace ecsprt {
-
-enum : int { M_PUSH = 1, M_DRIFT = 2, M_DEPOSIT = 4, M_REBIN = 8, M_COLLIDE = 16 };
 
 struct TileArgs {
   TileGeom g;
@@ -32,19 +32,6 @@ struct TileArgs {
   uint32_t key0, key1, step;
 };
 
-__device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg) {
-  int ci = (int)xg, cj = (int)yg, ck = (int)zg;
-  if (ci >= g.Nx) ci = g.Nx - 1;
-  if (cj >= g.Ny) cj = g.Ny - 1;
-  if (ck >= g.Nz) ck = g.Nz - 1;
-  return ((ck / g.Tz) * g.nty + cj / g.Ty) * g.ntx + ci / g.Tx;
-}
-
-__device__ inline unsigned long long wave_sum(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
-  return v;
-}
-
 __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, double y, double z,
                                       double vx, double vy, double vz, int id) {
   d.x[i] = x;
@@ -54,37 +41,6 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
   d.vy[i] = vy;
   d.vz[i] = vz;
   d.id[i] = id;
-}
-
-// Tile offset d = b - a on a periodic axis of n tiles folded to -1, 0, +1; 2 when
-// the destination is further than one tile.
-__device__ inline int tile_step(int d, int n) {
-  if (d > 1) d -= n;
-  else if (d < -1) d += n;
-  return (d >= -1 && d <= 1) ? d : 2;
-}
-
-// FP64 add into the workgroup's LDS accumulator (ds_add_f64).  Workgroup scope keeps
-// it distinct from the agent-scope global atomics of the overflow path.
-__device__ inline void lds_add(double *p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Deposit of an ion that moved beyond the tile's LDS region (about 0.6% of ions at
-// one cell per step): global FP64 atomics into the overflow moments.  Kept out of
-// line so the LDS path stays free of its address arithmetic.
-__device__ inline void deposit_overflow(double *ovf, const TileGeom &g, int ixl, int iyl, int izl,
-                                              const double w8[8], double vx, double vy, double vz) {
-  for (int c = 0; c < 8; ++c) {
-    const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
-    const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
-                      wrap_index(ixl + ax, g.Nx);
-    const double wt = w8[c];
-    unsafeAtomicAdd(&ovf[node * 4 + 0], wt);
-    unsafeAtomicAdd(&ovf[node * 4 + 1], vx * wt);
-    unsafeAtomicAdd(&ovf[node * 4 + 2], vy * wt);
-    unsafeAtomicAdd(&ovf[node * 4 + 3], vz * wt);
-  }
 }
 
 // Place a particle whose destination slot was reserved: into the destination tile
@@ -401,8 +357,12 @@ __global__ void k_finalize_counts(int ntiles, int *count, int *next, const int *
 
 // Sum of every workgroup region covering a node, in a fixed order, plus the
 // overflow deposits: the moments in FluidNode layout.
-__global__ void k_fold(TileGeom g, const double *__restrict__ fold, const double *__restrict__ ovf,
-                       const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m) {
+// R = double for the FP64 path.  R = long long for the mixed path's fixed-point
+// regions: the integer sum is exact, then scaled to weight (sn) and flux (sg) units.
+template <typename R>
+__global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__restrict__ ovf,
+                       const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m, double sn,
+                       double sg) {
   const long nnode = (long)g.Nx * g.Ny * g.Nz;
   const int RAX = g.RAx, RAY = g.RAy, RA3 = g.RAx * g.RAy * g.RAz;
   for (long node = blockIdx.x * (long)blockDim.x + threadIdx.x; node < nnode; node += (long)gridDim.x * blockDim.x) {
@@ -412,21 +372,26 @@ __global__ void k_fold(TileGeom g, const double *__restrict__ fold, const double
     const int *px = tx + g.Nx + i * FOLD_MAXP * 2;
     const int *py = ty + g.Ny + j * FOLD_MAXP * 2;
     const int *pz = tz + g.Nz + k * FOLD_MAXP * 2;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    R acc[4] = {0, 0, 0, 0};
     for (int a = 0; a < cz; ++a)
       for (int b = 0; b < cy; ++b)
         for (int c = 0; c < cx; ++c) {
           const int t = (pz[2 * a] * g.nty + py[2 * b]) * g.ntx + px[2 * c];
           const int loc = (pz[2 * a + 1] * RAY + py[2 * b + 1]) * RAX + px[2 * c + 1];
           for (int s = 0; s < g.split; ++s) {
-            const double *r = fold + (size_t)(t * g.split + s) * 4 * RA3 + loc;
+            const R *r = fold + (size_t)(t * g.split + s) * 4 * RA3 + loc;
             acc[0] += r[0];
             acc[1] += r[RA3];
             acc[2] += r[2 * RA3];
             acc[3] += r[3 * RA3];
           }
         }
-    for (int d = 0; d < 4; ++d) m[node * 4 + d] = acc[d] + ovf[node * 4 + d];
+    if constexpr (std::is_integral<R>::value) {
+      m[node * 4 + 0] = (double)acc[0] * sn + ovf[node * 4 + 0];
+      for (int d = 1; d < 4; ++d) m[node * 4 + d] = (double)acc[d] * sg + ovf[node * 4 + d];
+    } else {
+      for (int d = 0; d < 4; ++d) m[node * 4 + d] = acc[d] + ovf[node * 4 + d];
+    }
   }
 }
 
@@ -572,60 +537,17 @@ __global__ void k_apply_bc_aos(double *pos, long np, DriftConsts dc) {
 // host side
 // ---------------------------------------------------------------------------
 
-static int grid_for(long n, int block) {
-  long b = (n + block - 1) / block;
-  if (b < 1) b = 1;
-  if (b > 65536) b = 65536;
-  return (int)b;
-}
-
-static void fill_f64(ecsprt_hip_ctx *c, double *p, long n, double v) {
+void fill_f64(ecsprt_hip_ctx *c, double *p, long n, double v) {
   k_fill_f64<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
   ECS_HIP(hipGetLastError());
 }
-static void fill_i32(ecsprt_hip_ctx *c, int *p, long n, int v) {
+void fill_i32(ecsprt_hip_ctx *c, int *p, long n, int v) {
   k_fill_i32<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
   ECS_HIP(hipGetLastError());
 }
-static void zero_ctr(ecsprt_hip_ctx *c) {
+void zero_ctr(ecsprt_hip_ctx *c) {
   k_fill_u64<<<1, 64, 0, c->stream>>>(c->d_ctr, 8);
   ECS_HIP(hipGetLastError());
-}
-
-static DriftConsts drift_consts(const ecsprt_params &p, double dt) {
-  DriftConsts d;
-  d.dt = dt;
-  d.L[0] = p.Lx;
-  d.L[1] = p.Ly;
-  d.L[2] = p.Lz;
-  d.r0[0] = p.x0;
-  d.r0[1] = p.y0;
-  d.r0[2] = p.z0;
-  d.r1[0] = p.x1;
-  d.r1[1] = p.y1;
-  d.r1[2] = p.z1;
-  return d;
-}
-
-// BorisMoverBz3D scalars, src/velocities.c:307-320.
-static PushConsts push_consts(const ecsprt_params &p, double dt) {
-  PushConsts k;
-  double h[3] = {1.0 / p.dx, 1.0 / p.dy, 1.0 / p.dz};
-  double tscale = 0.5 * (p.qi / p.mi) * dt;
-  k.t = tscale * p.B0;
-  k.s = 2.0 * k.t / (1 + k.t * k.t);
-  for (int d = 0; d < 3; ++d) k.Escale[d] = -0.5 * h[d] * tscale;
-  k.dx = p.dx;
-  k.dy = p.dy;
-  k.dz = p.dz;
-  const double d3[3] = {p.dx, p.dy, p.dz};
-  k.pow2 = 1;
-  for (int d = 0; d < 3; ++d) {
-    int e;
-    k.pow2 &= std::frexp(d3[d], &e) == 0.5 ? 1 : 0;
-    k.inv[d] = 1.0 / d3[d];
-  }
-  return k;
 }
 
 static void free_particles(ParticlesF64 &P) {
@@ -781,6 +703,7 @@ void tiles_init(ecsprt_hip_ctx *c) {
   set_lds_attrs<M_DRIFT | M_REBIN>();
   set_lds_attrs<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>();
   set_lds_attrs<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>();
+  if (c->mixed) mx_set_lds_attrs();
 }
 
 void tiles_free(ecsprt_hip_ctx *c) {
@@ -789,6 +712,7 @@ void tiles_free(ecsprt_hip_ctx *c) {
   free_particles(c->E);
   free_particles(c->S);
   free_particles(c->D);
+  for (ParticlesMix *P : {&c->Am, &c->Bm, &c->Em, &c->Sm, &c->Dm}) mx_free(*P);
   (void)hipFree(c->d_fold);
   (void)hipFree(c->d_ovf);
   (void)hipFree(c->d_ctr);
@@ -800,11 +724,6 @@ void tiles_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_foldtab);
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_key);
-}
-
-static void read_ctr(ecsprt_hip_ctx *c) {
-  ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, c->stream));
-  ECS_HIP(hipStreamSynchronize(c->stream));
 }
 
 void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
@@ -835,23 +754,42 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
     total += cap;
   }
   ECS_CHECK(total < 2147483647L, ECSPRT_ERR_ARG, "particle capacity exceeds int32 slots");
-  if (total > c->pcap || !c->A.x) {
-    alloc_particles(c->A, total);
-    alloc_particles(c->B, total);
-    if (c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) alloc_particles(c->D, total);
-    c->pcap = total;
-  }
-  if (!c->E.x || c->np != np) {
-    alloc_particles(c->E, np);
-    c->scap = std::max(np / 64, 65536L);
-    alloc_particles(c->S, c->scap);
+  const bool philox = c->o.collide_mode == ECSPRT_COLLIDE_PHILOX;
+  if (c->mixed) {
+    if (total > c->pcap || !c->Am.cell) {
+      mx_alloc(c->Am, total);
+      mx_alloc(c->Bm, total);
+      if (philox) mx_alloc(c->Dm, total);
+      c->pcap = total;
+    }
+    if (!c->Em.cell || c->np != np) {
+      mx_alloc(c->Em, np);
+      c->scap = std::max(np / 64, 65536L);
+      mx_alloc(c->Sm, c->scap);
+    }
+  } else {
+    if (total > c->pcap || !c->A.x) {
+      alloc_particles(c->A, total);
+      alloc_particles(c->B, total);
+      if (philox) alloc_particles(c->D, total);
+      c->pcap = total;
+    }
+    if (!c->E.x || c->np != np) {
+      alloc_particles(c->E, np);
+      c->scap = std::max(np / 64, 65536L);
+      alloc_particles(c->S, c->scap);
+    }
   }
   ECS_HIP(hipMemcpyAsync(c->d_base, c->h_base.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice, c->stream));
   ECS_HIP(hipMemcpyAsync(c->d_cap, c->h_cap.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice, c->stream));
   ECS_HIP(hipMemcpyAsync(c->d_count, c->d_next, sizeof(int) * c->ntiles, hipMemcpyDeviceToDevice, c->stream));
   fill_i32(c, c->d_next, c->ntiles, 0);
-  k_bin_scatter<<<grid_for(np, 256), 256, 0, c->stream>>>(c->d_stage, np, c->d_key, c->d_base, c->A);
-  ECS_HIP(hipGetLastError());
+  if (c->mixed) {
+    mx_bin_scatter(c, np);
+  } else {
+    k_bin_scatter<<<grid_for(np, 256), 256, 0, c->stream>>>(c->d_stage, np, c->d_key, c->d_base, c->A);
+    ECS_HIP(hipGetLastError());
+  }
   ECS_HIP(hipStreamSynchronize(c->stream));
   // Losing a particle is the reference's Abort (src/boundaries.c:101-109): the
   // context drops its particle state, since ids no longer cover [0, np).
@@ -862,6 +800,10 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
 
 void particles_to_stage(ecsprt_hip_ctx *c) {
   ensure_stage(c, c->np);
+  if (c->mixed) {
+    mx_unbin(c);
+    return;
+  }
   k_unbin<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->d_stage, c->np);
   ECS_HIP(hipGetLastError());
 }
@@ -912,10 +854,10 @@ static void launch_tile(ecsprt_hip_ctx *c, const TileArgs &a) {
 static void rebuild_from_spill(ecsprt_hip_ctx *c, long nspill) {
   // A tile overflowed its capacity: gather everything by id and re-bin with new
   // capacities.
-  ensure_stage(c, c->np);
-  k_unbin<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->d_stage, c->np);
-  ECS_HIP(hipGetLastError());
-  if (nspill > 0) {
+  particles_to_stage(c);
+  if (nspill > 0 && c->mixed) {
+    mx_unbin_list(c, nspill);
+  } else if (nspill > 0) {
     k_unbin_list<<<grid_for(nspill, 256), 256, 0, c->stream>>>(c->S, nspill, c->d_stage, c->np);
     ECS_HIP(hipGetLastError());
   }
@@ -926,12 +868,17 @@ static void rebuild_from_spill(ecsprt_hip_ctx *c, long nspill) {
 static void finish_rebin(ecsprt_hip_ctx *c) {
   const TileGeom &g = c->g;
   const ecsprt_params &p = c->p;
-  k_rebin<<<2048, 256, 0, c->stream>>>(g, c->E, c->B, c->S, c->scap, c->d_base, c->d_cap, c->d_next, c->d_ctr, p.dx,
-                                       p.dy, p.dz);
-  ECS_HIP(hipGetLastError());
+  if (c->mixed) {
+    mx_rebin(c);
+  } else {
+    k_rebin<<<2048, 256, 0, c->stream>>>(g, c->E, c->B, c->S, c->scap, c->d_base, c->d_cap, c->d_next, c->d_ctr,
+                                         p.dx, p.dy, p.dz);
+    ECS_HIP(hipGetLastError());
+  }
   k_finalize_counts<<<(c->ntiles + 255) / 256, 256, 0, c->stream>>>(c->ntiles, c->d_count, c->d_next, c->d_cap);
   ECS_HIP(hipGetLastError());
   std::swap(c->A, c->B);
+  std::swap(c->Am, c->Bm);
 }
 
 static void check_after_rebin(ecsprt_hip_ctx *c) {
@@ -947,6 +894,10 @@ static void check_after_rebin(ecsprt_hip_ctx *c) {
 }
 
 void run_push(ecsprt_hip_ctx *c, double dt) {
+  if (c->mixed) {
+    mx_launch(c, M_PUSH, dt, nullptr, 0, 0, 0);
+    return;
+  }
   TileArgs a = make_args(c);
   a.pc = push_consts(c->p, dt);
   launch_tile<M_PUSH>(c, a);
@@ -954,23 +905,39 @@ void run_push(ecsprt_hip_ctx *c, double dt) {
 
 void run_drift_rebin(ecsprt_hip_ctx *c, double dt) {
   zero_ctr(c);
-  TileArgs a = make_args(c);
-  a.dc = drift_consts(c->p, dt);
-  launch_tile<M_DRIFT | M_REBIN>(c, a);
+  if (c->mixed) {
+    mx_launch(c, M_DRIFT | M_REBIN, dt, nullptr, 0, 0, 0);
+  } else {
+    TileArgs a = make_args(c);
+    a.dc = drift_consts(c->p, dt);
+    launch_tile<M_DRIFT | M_REBIN>(c, a);
+  }
   finish_rebin(c);
   check_after_rebin(c);
 }
 
 static void launch_fold(ecsprt_hip_ctx *c) {
-  k_fold<<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, c->d_foldtab, c->foldtab_off[0],
-                                                           c->foldtab_off[1], c->foldtab_off[2], c->d_moments);
+  if (c->mixed) {
+    const double vmax = mx_vmax(c->p);
+    const double sc[4] = {0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30};
+    k_fold<long long><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(
+        c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_foldtab, c->foldtab_off[0],
+        c->foldtab_off[1], c->foldtab_off[2], c->d_moments, sc[0], sc[1]);
+  } else
+    k_fold<double><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, c->d_foldtab,
+                                                                   c->foldtab_off[0], c->foldtab_off[1],
+                                                                   c->foldtab_off[2], c->d_moments, 1.0, 1.0);
   ECS_HIP(hipGetLastError());
 }
 
 void run_deposit(ecsprt_hip_ctx *c) {
   fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
-  TileArgs a = make_args(c);
-  launch_tile<M_DEPOSIT>(c, a);
+  if (c->mixed) {
+    mx_launch(c, M_DEPOSIT, 0.0, nullptr, 0, 0, 0);
+  } else {
+    TileArgs a = make_args(c);
+    launch_tile<M_DEPOSIT>(c, a);
+  }
   launch_fold(c);
 }
 
@@ -1010,10 +977,12 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
     a.step = (uint32_t)c->step_index;
     a.pt = poisson_table(Nc > 0 ? c->coll_lambda : 0.0);
     if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
-    launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
+    if (c->mixed) mx_launch(c, M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN, dt, &a.pt, a.key0, a.key1, a.step);
+    else launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
   } else {
     if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
-    launch_tile<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
+    if (c->mixed) mx_launch(c, M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN, dt, nullptr, 0, 0, 0);
+    else launch_tile<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
   }
   if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[1], c->stream));
   finish_rebin(c);
@@ -1034,6 +1003,10 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
 
 void run_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
   fill_i32(c, d_counts, c->nnode, 0);
+  if (c->mixed) {
+    mx_cell_counts(c, d_counts);
+    return;
+  }
   k_cell_counts<<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_base, c->d_count, c->p.dx, c->p.dy, c->p.dz,
                                                    d_counts);
   ECS_HIP(hipGetLastError());

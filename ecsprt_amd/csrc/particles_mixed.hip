@@ -1,0 +1,588 @@
+// Mixed-precision particle stages on gfx950.  Each ion is a packed int32 cell index,
+// an FP32 offset within that cell and an FP32 velocity (32 bytes with its id, against
+// 52 for FP64).  The push gathers from an FP32 copy of the potential in LDS, the drift
+// moves the offset and carries whole cells into the integer index (so the periodic
+// wrap is exact), and the deposit computes FP32 cloud-in-cell weights and adds them
+// to 64-bit fixed-point LDS accumulators.  Integer sums are exact and independent of
+// the order in which ions arrive, so the moments are deterministic and carry only the
+// FP32 rounding of each weight (about 1e-9 of a unit weight), not the growth of an
+// FP32 running sum over the ~800 contributions a node receives.
+//
+// The tile structure, re-binning protocol and collision deferral are those of the
+// FP64 tile kernel (particles.hip); only the particle arithmetic differs.
+#include "common.hpp"
+#include "philox.hpp"
+#include "pic_math.hpp"
+#include "tiles.hpp"
+
+namespace ecsprt {
+
+// FP32 push and drift constants (BorisMoverBz3D scalars, src/velocities.c:307-320).
+struct MxConsts {
+  float t, s;
+  float Escale[3];
+  float dtc[3];  // dt / cell width: cells crossed per unit velocity in one step
+  float inv_vmax;  // 1 / vmax: fixed-point flux units are vmax * 2^-30
+};
+
+// Fixed-point deposit: a weight w in [0, 1] is stored as rint(w * 2^30) and a flux
+// contribution v*w as rint(v/vmax * w * 2^30); |v| < vmax keeps both below 2^30.  Ions
+// with a faster component take the FP64 overflow path.
+constexpr float MX_WSCALE = 1073741824.0f;  // 2^30
+
+struct TileArgsMx {
+  TileGeom g;
+  ParticlesMix src, dst, emi, spill, def;
+  long scap;
+  PoissonTable pt;
+  const int *base, *count, *cap;
+  int *next;
+  unsigned long long *ctr;
+  const double *phi;
+  long long *fold;
+  double *ovf;
+  MxConsts k;
+  CollideConsts cc;
+  uint32_t key0, key1, step;
+};
+
+__device__ inline void store_mx(const ParticlesMix &d, long i, uint32_t cell, float ox, float oy, float oz, float vx,
+                                float vy, float vz, int id) {
+  d.cell[i] = cell;
+  d.ox[i] = ox;
+  d.oy[i] = oy;
+  d.oz[i] = oz;
+  d.vx[i] = vx;
+  d.vy[i] = vy;
+  d.vz[i] = vz;
+  d.id[i] = id;
+}
+
+__device__ inline void lds_addq(long long *p, float scaled) {
+  const long long q = (long long)__float2int_rn(scaled);
+  __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), (unsigned long long)q, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Drift along one axis: offset o plus d cells, whole cells moved into the index i and
+// wrapped once over the N periodic cells (ApplyBC, src/boundaries.c:25-26, in cell
+// units).  Returns false for a particle that is still outside after the wrap (or
+// whose displacement is not finite): the reference drops it.
+__device__ inline bool drift_axis(float &o, int &i, float d, int N) {
+  const float u = o + d;
+  const float f = floorf(u);
+  if (!(fabsf(f) < 1048576.0f)) return false;
+  // u - f is below 1; it rounds up to 1 only for u just below an integer.
+  o = fminf(u - f, 0x1.fffffep-1f);
+  int n = i + (int)f;
+  if (n < 0) n += N;
+  if (n >= N) n -= N;
+  i = n;
+  return n >= 0 && n < N;
+}
+
+__device__ inline void place_mx(const TileArgsMx &a, int dt, int slot, uint32_t cell, float ox, float oy, float oz,
+                                float vx, float vy, float vz, int id) {
+  if (slot < a.cap[dt]) {
+    store_mx(a.dst, (long)a.base[dt] + slot, cell, ox, oy, oz, vx, vy, vz, id);
+  } else {
+    const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
+    if ((long)sp < a.scap) store_mx(a.spill, (long)sp, cell, ox, oy, oz, vx, vy, vz, id);
+    else atomicAdd(&a.ctr[2], 1ull);
+  }
+}
+
+struct TileWGMx {
+  int t, tx, ty, tz, i0, j0, k0;
+  int RAx, RAy, RAz;
+  long long *sacc;
+  int *s_cnt, *s_base, *s_nbr;
+  int par;
+};
+
+// Drift + wrap, deposit and re-binning of one ion per thread, in uniform control
+// flow across the workgroup (re-binning uses barriers).
+template <int MODE, int TX, int TY, int TZ, int HH>
+__device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, long p, int ix, int iy, int iz, float ox,
+                               float oy, float oz, float vx, float vy, float vz, int pid) {
+  const TileGeom &g = a.g;
+  const int tid = threadIdx.x;
+  bool lost = false;
+  if ((MODE & M_DRIFT) && valid) {
+    // UpdatePositions (src/positions.c:455-459) with the wrap of src/boundaries.c:19-28.
+    const bool okx = drift_axis(ox, ix, vx * a.k.dtc[0], g.Nx);
+    const bool oky = drift_axis(oy, iy, vy * a.k.dtc[1], g.Ny);
+    const bool okz = drift_axis(oz, iz, vz * a.k.dtc[2], g.Nz);
+    lost = !(okx && oky && okz);
+  }
+  if ((MODE & M_DEPOSIT) && valid && !lost) {
+    // Collect3DFluidMoments weights (src/moments.c:164-184) from the in-cell offsets,
+    // scaled by 2^30 through the z factor (exact: a power of two).
+    const float wxh = ox, wxl = 1.0f - ox;
+    const float wyh = oy, wyl = 1.0f - oy;
+    const float wzh = oz * MX_WSCALE, wzl = (1.0f - oz) * MX_WSCALE;
+    float w8[8];
+    w8[0] = wzh * wyh * wxh;
+    w8[1] = wzl * wyh * wxh;
+    w8[2] = wzh * wyl * wxh;
+    w8[3] = wzl * wyl * wxh;
+    w8[4] = wzh * wyh * wxl;
+    w8[5] = wzl * wyh * wxl;
+    w8[6] = wzh * wyl * wxl;
+    w8[7] = wzl * wyl * wxl;
+    constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
+    int la = ix - (w.i0 - HH), lb = iy - (w.j0 - HH), lc = iz - (w.k0 - HH);
+    if (la < 0) la += g.Nx; else if (la > w.RAx - 2) la -= g.Nx;
+    if (lb < 0) lb += g.Ny; else if (lb > w.RAy - 2) lb -= g.Ny;
+    if (lc < 0) lc += g.Nz; else if (lc > w.RAz - 2) lc -= g.Nz;
+    const bool inreg = la >= 0 && la <= w.RAx - 2 && lb >= 0 && lb <= w.RAy - 2 && lc >= 0 && lc <= w.RAz - 2;
+    const float ux = vx * a.k.inv_vmax, uy = vy * a.k.inv_vmax, uz = vz * a.k.inv_vmax;
+    const bool slow = fabsf(ux) < 1.0f && fabsf(uy) < 1.0f && fabsf(uz) < 1.0f;
+    if (inreg && slow) {
+      const int base = (lc * RAY + lb) * RAX + la;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+        const int idx = base + (az * RAY + ay) * RAX + ax;
+        const float wt = w8[c];
+        lds_addq(&w.sacc[idx], wt);
+        lds_addq(&w.sacc[RA3 + idx], ux * wt);
+        lds_addq(&w.sacc[2 * RA3 + idx], uy * wt);
+        lds_addq(&w.sacc[3 * RA3 + idx], uz * wt);
+      }
+    } else {
+      double wd[8];
+      for (int c = 0; c < 8; ++c) wd[c] = (double)w8[c] * (1.0 / 1073741824.0);
+      deposit_overflow(a.ovf, g, ix, iy, iz, wd, (double)vx, (double)vy, (double)vz);
+    }
+  }
+  if (MODE & M_REBIN) {
+    int code = 28;
+    if (valid && !lost) {
+      const int sx = tile_step(ix / TX - w.tx, g.ntx), sy = tile_step(iy / TY - w.ty, g.nty);
+      const int sz = tile_step(iz / TZ - w.tz, g.ntz);
+      code = (sx < 2 && sy < 2 && sz < 2) ? ((sz + 1) * 3 + (sy + 1)) * 3 + (sx + 1) : 27;
+    }
+    int *cnt = w.s_cnt + 27 * w.par, *bas = w.s_base + 27 * w.par;
+    int rank = 0;
+    if (code < 27) rank = atomicAdd(&cnt[code], 1);
+    __syncthreads();
+    if (tid < 27) {
+      const int n = cnt[tid];
+      if (n > 0) bas[tid] = atomicAdd(&a.next[w.s_nbr[tid]], n);
+    } else if (tid >= 64 && tid < 64 + 27) {
+      w.s_cnt[27 * (w.par ^ 1) + tid - 64] = 0;
+    }
+    __syncthreads();
+    if (code < 27) {
+      place_mx(a, w.s_nbr[code], bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+    } else if (code == 27) {
+      const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
+      store_mx(a.emi, (long)e, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+    }
+    if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
+    w.par ^= 1;
+  } else if ((MODE & M_PUSH) && valid) {
+    a.src.vx[p] = vx;
+    a.src.vy[p] = vy;
+    a.src.vz[p] = vz;
+  }
+}
+
+template <int MODE, int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
+  static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
+  extern __shared__ long long smemq[];
+  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef;
+  const TileGeom g = a.g;
+  const int nwg = gridDim.x;
+  int wi = blockIdx.x;
+  if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
+  constexpr int RPX = TX + 4, RPY = TY + 4, RPZ = TZ + 4, RP3 = RPX * RPY * RPZ;
+  constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
+  TileWGMx w;
+  w.t = wi / g.split;
+  const int s = wi % g.split;
+  w.tx = w.t % g.ntx;
+  w.ty = (w.t / g.ntx) % g.nty;
+  w.tz = w.t / (g.ntx * g.nty);
+  w.i0 = w.tx * TX;
+  w.j0 = w.ty * TY;
+  w.k0 = w.tz * TZ;
+  w.RAx = min(TX, g.Nx - w.i0) + 1 + 2 * HH;
+  w.RAy = min(TY, g.Ny - w.j0) + 1 + 2 * HH;
+  w.RAz = min(TZ, g.Nz - w.k0) + 1 + 2 * HH;
+  w.s_cnt = s_cnt;
+  w.s_base = s_base;
+  w.s_nbr = s_nbr;
+  w.par = 0;
+  // LDS: the accumulators first (8-byte aligned), then the FP32 potential.
+  w.sacc = smemq;
+  float *sphi = reinterpret_cast<float *>(smemq + ((MODE & M_DEPOSIT) ? 4 * RA3 : 0));
+  const int tid = threadIdx.x;
+
+  if (MODE & M_PUSH) {
+    for (int q = tid; q < RP3; q += blockDim.x) {
+      int li = q % RPX, lj = (q / RPX) % RPY, lk = q / (RPX * RPY);
+      int gi = wrap_index(w.i0 - 1 + li, g.Nx);
+      int gj = wrap_index(w.j0 - 1 + lj, g.Ny);
+      int gk = wrap_index(w.k0 - 1 + lk, g.Nz);
+      sphi[q] = (float)a.phi[((long)gk * g.Ny + gj) * g.Nx + gi];
+    }
+  }
+  if (MODE & M_DEPOSIT) {
+    for (int q = tid; q < 4 * RA3; q += blockDim.x) w.sacc[q] = 0;
+  }
+  if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
+  if ((MODE & M_REBIN) && tid < 27) {
+    const int sx = tid % 3 - 1, sy = (tid / 3) % 3 - 1, sz = tid / 9 - 1;
+    s_nbr[tid] = (wrap_index(w.tz + sz, g.ntz) * g.nty + wrap_index(w.ty + sy, g.nty)) * g.ntx +
+                 wrap_index(w.tx + sx, g.ntx);
+  }
+  if (tid == 0) s_ndef = 0;
+  __syncthreads();
+
+  const long cnt = a.count[w.t];
+  const long beg = a.base[w.t] + cnt * s / g.split;
+  const long end = a.base[w.t] + cnt * (s + 1) / g.split;
+  unsigned natt = 0, nsucc = 0, nfail = 0;
+
+  // Phase 1, software-pipelined: the next batch loads while this one computes.
+  uint32_t ncell = 0;
+  float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
+  int nid = 0;
+  if (beg + tid < end) {
+    const long p = beg + tid;
+    ncell = a.src.cell[p];
+    nox = a.src.ox[p];
+    noy = a.src.oy[p];
+    noz = a.src.oz[p];
+    nvx = a.src.vx[p];
+    nvy = a.src.vy[p];
+    nvz = a.src.vz[p];
+    nid = a.src.id[p];
+  }
+  for (long p0 = beg; p0 < end; p0 += blockDim.x) {
+    const long p = p0 + tid;
+    const bool valid = p < end;
+    const uint32_t cell = ncell;
+    float ox = nox, oy = noy, oz = noz, vx = nvx, vy = nvy, vz = nvz;
+    const int pid = nid;
+    {
+      const long pn = p + blockDim.x;
+      if (pn < end) {
+        ncell = a.src.cell[pn];
+        nox = a.src.ox[pn];
+        noy = a.src.oy[pn];
+        noz = a.src.oz[pn];
+        nvx = a.src.vx[pn];
+        nvy = a.src.vy[pn];
+        nvz = a.src.vz[pn];
+        nid = a.src.id[pn];
+      }
+    }
+    const int ix = mx_i(cell), iy = mx_j(cell), iz = mx_k(cell);
+    if ((MODE & M_PUSH) && valid) {
+      // BorisMoverBz3D body (src/velocities.c:342-356) in FP32.
+      float E[3];
+      diff_vec3d_core<float, RPX, RPY>(sphi, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, ix, iy, iz, ox, oy, oz,
+                                       E);
+      const float vxm = vx + a.k.Escale[0] * E[0];
+      const float vym = vy + a.k.Escale[1] * E[1];
+      const float vzm = vz + a.k.Escale[2] * E[2];
+      const float vxt = vxm + vym * a.k.t;
+      const float vyp = vym - vxt * a.k.s;
+      const float vxp = vxt + vyp * a.k.t;
+      vx = vxp;
+      vy = vyp;
+      vz = vzm;
+    }
+    bool defer = false;
+    if ((MODE & M_COLLIDE) && valid && poisson_attempts(a.pt, a.key0, a.key1, a.step, pid) > 0) {
+      defer = true;
+      const int q = atomicAdd(&s_ndef, 1);
+      store_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
+    }
+    tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, ix, iy, iz, ox, oy, oz, vx, vy, vz, pid);
+  }
+  if (MODE & M_COLLIDE) {
+    // Phase 2: deferred ions.  The scattering formula runs in FP64 on the widened
+    // velocity (same Philox draws as the FP64 path), then drift, deposit, re-bin.
+    __syncthreads();
+    const int ndef = s_ndef;
+    for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
+      const int q = q0 + tid;
+      const bool valid = q < ndef;
+      uint32_t cell = 0;
+      float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
+      int pid = 0;
+      if (valid) {
+        const long d = beg + q;
+        cell = a.def.cell[d];
+        ox = a.def.ox[d];
+        oy = a.def.oy[d];
+        oz = a.def.oz[d];
+        pid = a.def.id[d];
+        double dvx = a.def.vx[d], dvy = a.def.vy[d], dvz = a.def.vz[d];
+        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
+        collide_attempts(a.cc, a.key0, a.key1, a.step, pid, k, dvx, dvy, dvz, natt, nsucc, nfail);
+        vx = (float)dvx;
+        vy = (float)dvy;
+        vz = (float)dvz;
+      }
+      tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
+                                    pid);
+    }
+    unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+    if (__lane_id() == 0) {
+      if (s0) atomicAdd(&a.ctr[3], s0);
+      if (s1) atomicAdd(&a.ctr[4], s1);
+      if (s2) atomicAdd(&a.ctr[5], s2);
+    }
+  }
+  if (MODE & M_DEPOSIT) {
+    __syncthreads();
+    long long *out = a.fold + (size_t)wi * 4 * RA3;
+    for (int q = tid; q < 4 * RA3; q += blockDim.x) out[q] = w.sacc[q];
+  }
+}
+
+// Emigrants further than one tile -> destination tiles.
+__global__ void k_rebin_mx(TileGeom g, ParticlesMix emi, ParticlesMix dst, ParticlesMix spill, long scap,
+                           const int *base, const int *cap, int *next, unsigned long long *ctr) {
+  const unsigned long long n = ctr[0];
+  for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
+       e += (unsigned long long)gridDim.x * blockDim.x) {
+    const uint32_t cell = emi.cell[e];
+    const int t = ((mx_k(cell) / g.Tz) * g.nty + mx_j(cell) / g.Ty) * g.ntx + mx_i(cell) / g.Tx;
+    const int slot = atomicAdd(&next[t], 1);
+    if (slot < cap[t]) {
+      store_mx(dst, (long)base[t] + slot, cell, emi.ox[e], emi.oy[e], emi.oz[e], emi.vx[e], emi.vy[e], emi.vz[e],
+               emi.id[e]);
+    } else {
+      const unsigned long long sp = atomicAdd(&ctr[1], 1ull);
+      if ((long)sp < scap)
+        store_mx(spill, (long)sp, cell, emi.ox[e], emi.oy[e], emi.oz[e], emi.vx[e], emi.vy[e], emi.vz[e], emi.id[e]);
+      else atomicAdd(&ctr[2], 1ull);
+    }
+  }
+}
+
+// AoS FP64 staging (pos[np][3], vel[np][3]) -> tile slots.  The cell is (int)(pos/d),
+// as in the binning count, and is kept even when the FP64 remainder rounds up to 1 in
+// FP32: the offset is clamped to the largest float below 1.  The reference's gather
+// is discontinuous across cell faces (its x weight blends along z), so an ion must
+// stay in the cell the FP64 path puts it in.
+__global__ void k_bin_scatter_mx(TileGeom g, const double *stage, long np, const int *key, const int *base,
+                                 ParticlesMix A, double dx, double dy, double dz) {
+  const double *vel = stage + 3 * np;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    const int t = key[2 * p];
+    if (t < 0) continue;
+    const long slot = (long)base[t] + key[2 * p + 1];
+    const double x = stage[p * 3] / dx, y = stage[p * 3 + 1] / dy, z = stage[p * 3 + 2] / dz;
+    const int i = min((int)x, g.Nx - 1), j = min((int)y, g.Ny - 1), k = min((int)z, g.Nz - 1);
+    const float below1 = 0x1.fffffep-1f;
+    store_mx(A, slot, mx_pack(i, j, k), fminf((float)(x - i), below1), fminf((float)(y - j), below1),
+             fminf((float)(z - k), below1), (float)vel[p * 3], (float)vel[p * 3 + 1], (float)vel[p * 3 + 2], (int)p);
+  }
+}
+
+__device__ inline void unstage_mx(const ParticlesMix &A, long s, double *stage, double *vel, double dx, double dy,
+                                  double dz) {
+  const long id = A.id[s];
+  const uint32_t cell = A.cell[s];
+  stage[id * 3] = ((double)mx_i(cell) + (double)A.ox[s]) * dx;
+  stage[id * 3 + 1] = ((double)mx_j(cell) + (double)A.oy[s]) * dy;
+  stage[id * 3 + 2] = ((double)mx_k(cell) + (double)A.oz[s]) * dz;
+  vel[id * 3] = A.vx[s];
+  vel[id * 3 + 1] = A.vy[s];
+  vel[id * 3 + 2] = A.vz[s];
+}
+
+__global__ void k_unbin_mx(ParticlesMix A, const int *base, const int *count, double *stage, long np, double dx,
+                           double dy, double dz) {
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  for (long q = threadIdx.x; q < n; q += blockDim.x) unstage_mx(A, b + q, stage, stage + 3 * np, dx, dy, dz);
+}
+
+__global__ void k_unbin_list_mx(ParticlesMix S, long n, double *stage, long np, double dx, double dy, double dz) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+    unstage_mx(S, q, stage, stage + 3 * np, dx, dy, dz);
+}
+
+__global__ void k_cell_counts_mx(TileGeom g, ParticlesMix A, const int *base, const int *count, int *counts) {
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    const uint32_t cell = A.cell[b + q];
+    atomicAdd(&counts[((long)mx_k(cell) * g.Ny + mx_j(cell)) * g.Nx + mx_i(cell)], 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+// Flux scale of the fixed-point deposit: the power of two at or above 64 times the
+// largest thermal or drift speed in the run.
+double mx_vmax(const ecsprt_params &p) {
+  double v = std::max({p.viT, p.viTx, p.viTy, p.viTz, std::fabs(p.vi0x), std::fabs(p.vi0y), std::fabs(p.vi0z),
+                       p.vnT, std::fabs(p.vn0x), std::fabs(p.vn0y), std::fabs(p.vn0z)});
+  if (!(v > 0.0) || !std::isfinite(v)) v = 1.0;
+  return std::exp2(std::ceil(std::log2(64.0 * v)));
+}
+
+static MxConsts mx_consts(const ecsprt_params &p, double dt) {
+  const PushConsts d = push_consts(p, dt);
+  MxConsts k;
+  k.t = (float)d.t;
+  k.s = (float)d.s;
+  for (int i = 0; i < 3; ++i) k.Escale[i] = (float)d.Escale[i];
+  k.dtc[0] = (float)(dt / p.dx);
+  k.dtc[1] = (float)(dt / p.dy);
+  k.dtc[2] = (float)(dt / p.dz);
+  k.inv_vmax = (float)(1.0 / mx_vmax(p));
+  return k;
+}
+
+void mx_free(ParticlesMix &P) {
+  (void)hipFree(P.cell);
+  (void)hipFree(P.ox);
+  (void)hipFree(P.oy);
+  (void)hipFree(P.oz);
+  (void)hipFree(P.vx);
+  (void)hipFree(P.vy);
+  (void)hipFree(P.vz);
+  (void)hipFree(P.id);
+  P = ParticlesMix{};
+}
+
+void mx_alloc(ParticlesMix &P, long n) {
+  mx_free(P);
+  n = std::max(n, 1L);
+  hipError_t e = hipMalloc(&P.cell, sizeof(uint32_t) * n);
+  float **f[6] = {&P.ox, &P.oy, &P.oz, &P.vx, &P.vy, &P.vz};
+  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMalloc(f[i], sizeof(float) * n);
+  if (e == hipSuccess) e = hipMalloc(&P.id, sizeof(int) * n);
+  if (e != hipSuccess) {
+    mx_free(P);
+    throw Error{ECSPRT_ERR_NOMEM, "particle arrays: " + std::string(hipGetErrorString(e))};
+  }
+}
+
+template <int MODE, int TX, int TY, int TZ, int HH>
+static void set_attr_mx() {
+  static bool done = false;
+  if (!done) {
+    hipFuncAttributes fa;
+    ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile_mx<MODE, TX, TY, TZ, HH>));
+    const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
+    ECS_HIP(hipFuncSetAttribute((const void *)k_tile_mx<MODE, TX, TY, TZ, HH>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    done = true;
+  }
+}
+
+template <int MODE>
+static void set_attrs_mx() {
+  set_attr_mx<MODE, 4, 4, 4, 2>();
+  set_attr_mx<MODE, 8, 8, 4, 2>();
+  set_attr_mx<MODE, 8, 8, 8, 2>();
+}
+
+void mx_set_lds_attrs() {
+  set_attrs_mx<M_PUSH>();
+  set_attrs_mx<M_DEPOSIT>();
+  set_attrs_mx<M_DRIFT | M_REBIN>();
+  set_attrs_mx<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>();
+  set_attrs_mx<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>();
+}
+
+static size_t lds_mx(const TileGeom &g, int mode) {
+  size_t b = 0;
+  if (mode & M_PUSH) b += sizeof(float) * (g.Tx + 4) * (g.Ty + 4) * (g.Tz + 4);
+  if (mode & M_DEPOSIT) b += sizeof(long long) * 4 * g.RAx * g.RAy * g.RAz;
+  return b;
+}
+
+template <int MODE>
+static void launch_mx(ecsprt_hip_ctx *c, const TileArgsMx &a) {
+  const int block = c->o.block > 0 ? c->o.block : 1024;
+  const size_t lds = lds_mx(c->g, MODE);
+  const int nwg = c->ntiles * c->g.split;
+  const int tx = c->g.Tx, tz = c->g.Tz;
+  if (tx == 8 && tz == 8) k_tile_mx<MODE, 8, 8, 8, 2><<<nwg, block, lds, c->stream>>>(a);
+  else if (tx == 8 && tz == 4) k_tile_mx<MODE, 8, 8, 4, 2><<<nwg, block, lds, c->stream>>>(a);
+  else k_tile_mx<MODE, 4, 4, 4, 2><<<nwg, block, lds, c->stream>>>(a);
+  ECS_HIP(hipGetLastError());
+}
+
+void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, uint32_t key0, uint32_t key1,
+               uint32_t step) {
+  TileArgsMx a{};
+  a.g = c->g;
+  a.src = c->Am;
+  a.dst = c->Bm;
+  a.emi = c->Em;
+  a.spill = c->Sm;
+  a.def = c->Dm;
+  a.scap = c->scap;
+  a.base = c->d_base;
+  a.count = c->d_count;
+  a.cap = c->d_cap;
+  a.next = c->d_next;
+  a.ctr = c->d_ctr;
+  a.phi = c->d_phi;
+  a.fold = reinterpret_cast<long long *>(c->d_fold);
+  a.ovf = c->d_ovf;
+  a.k = mx_consts(c->p, dt);
+  if (pt) {
+    a.pt = *pt;
+    a.cc = collide_consts_of(c->p);
+  }
+  a.key0 = key0;
+  a.key1 = key1;
+  a.step = step;
+  switch (mode) {
+    case M_PUSH: launch_mx<M_PUSH>(c, a); break;
+    case M_DEPOSIT: launch_mx<M_DEPOSIT>(c, a); break;
+    case M_DRIFT | M_REBIN: launch_mx<M_DRIFT | M_REBIN>(c, a); break;
+    case M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN: launch_mx<M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a); break;
+    case M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN:
+      launch_mx<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
+      break;
+    default: throw Error{ECSPRT_ERR_ARG, "unsupported tile pass"};
+  }
+}
+
+void mx_rebin(ecsprt_hip_ctx *c) {
+  k_rebin_mx<<<2048, 256, 0, c->stream>>>(c->g, c->Em, c->Bm, c->Sm, c->scap, c->d_base, c->d_cap, c->d_next,
+                                          c->d_ctr);
+  ECS_HIP(hipGetLastError());
+}
+
+void mx_bin_scatter(ecsprt_hip_ctx *c, long np) {
+  k_bin_scatter_mx<<<grid_for(np, 256), 256, 0, c->stream>>>(c->g, c->d_stage, np, c->d_key, c->d_base, c->Am,
+                                                               c->p.dx, c->p.dy, c->p.dz);
+  ECS_HIP(hipGetLastError());
+}
+
+void mx_unbin(ecsprt_hip_ctx *c) {
+  k_unbin_mx<<<c->ntiles, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, c->d_stage, c->np, c->p.dx, c->p.dy,
+                                                c->p.dz);
+  ECS_HIP(hipGetLastError());
+}
+
+void mx_unbin_list(ecsprt_hip_ctx *c, long n) {
+  k_unbin_list_mx<<<grid_for(n, 256), 256, 0, c->stream>>>(c->Sm, n, c->d_stage, c->np, c->p.dx, c->p.dy, c->p.dz);
+  ECS_HIP(hipGetLastError());
+}
+
+void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
+  k_cell_counts_mx<<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_base, c->d_count, d_counts);
+  ECS_HIP(hipGetLastError());
+}
+
+}  // namespace ecsprt

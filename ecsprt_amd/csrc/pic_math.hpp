@@ -13,21 +13,20 @@ namespace ecsprt {
 
 // Field gather, DifferenceVector3D (src/vector-math.c:596-722), from a potential
 // region staged in LDS.  F holds nodes [oi, oi+RP) x [oj, oj+RP) x [ok, ok+RP) of the
-// periodic potential (global, unwrapped indices), RPX x RPY nodes per plane.  Branch choices use the global
-// indices exactly as the reference does; the interpolation order keeps the
-// reference's y->wyh, z->wxh, x->wzh weight assignment (vector-math.c:645-651).
-template <int RPX, int RPY>
-__device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
-                                      double x0, double y0, double z0, double f[3]) {
+// periodic potential (global, unwrapped indices), RPX x RPY nodes per plane.  The
+// particle sits in cell (ixl, iyl, izl) at in-cell weights (wxh, wyh, wzh).  Branch
+// choices use the global indices exactly as the reference does; the interpolation
+// order keeps the reference's y->wyh, z->wxh, x->wzh weight assignment
+// (vector-math.c:645-651).  T is the arithmetic type: double for the FP64 path,
+// float for the mixed-precision path.
+template <typename T, int RPX, int RPY>
+__device__ inline void diff_vec3d_core(const T *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
+                                       int ixl, int iyl, int izl, T wxh, T wyh, T wzh, T f[3]) {
 #define PL(kk, jj, ii) F[(((kk)-ok) * RPY + ((jj)-oj)) * RPX + ((ii)-oi)]
-  int ixl = (int)x0, ixh = ixl + 1;
-  int iyl = (int)y0, iyh = iyl + 1;
-  int izl = (int)z0, izh = izl + 1;
-  double wxh = x0 - (double)ixl;
-  double wyh = y0 - (double)iyl;
-  double wzh = z0 - (double)izl;
-  double hhh, lhh, hlh, llh, hhl, lhl, hll, lll;
-  double whh, whl, wlh, wll, Ewh, Ewl;
+  const int ixh = ixl + 1, iyh = iyl + 1, izh = izl + 1;
+  const T c3 = T(3), c4 = T(4), c1 = T(1);
+  T hhh, lhh, hlh, llh, hhl, lhl, hll, lll;
+  T whh, whl, wlh, wll, Ewh, Ewl;
   // Particles reach this function only from inside their tile, so ixl >= 0 and the
   // reference's forward-difference branch (vector-math.c:625-631) cannot occur.
   hhl = PL(izh, iyh, ixl + 1) - PL(izh, iyh, ixl - 1);
@@ -40,10 +39,10 @@ __device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int 
     hlh = PL(izh, iyl, ixh + 1) - PL(izh, iyl, ixh - 1);
     llh = PL(izl, iyl, ixh + 1) - PL(izl, iyl, ixh - 1);
   } else {
-    hhh = +3.0 * PL(izh, iyh, ixh) - 4.0 * PL(izh, iyh, ixh - 1) + 1.0 * PL(izh, iyh, ixh - 2);
-    lhh = +3.0 * PL(izl, iyh, ixh) - 4.0 * PL(izl, iyh, ixh - 1) + 1.0 * PL(izl, iyh, ixh - 2);
-    hlh = +3.0 * PL(izh, iyl, ixh) - 4.0 * PL(izh, iyl, ixh - 1) + 1.0 * PL(izh, iyl, ixh - 2);
-    llh = +3.0 * PL(izl, iyl, ixh) - 4.0 * PL(izl, iyl, ixh - 1) + 1.0 * PL(izl, iyl, ixh - 2);
+    hhh = +c3 * PL(izh, iyh, ixh) - c4 * PL(izh, iyh, ixh - 1) + c1 * PL(izh, iyh, ixh - 2);
+    lhh = +c3 * PL(izl, iyh, ixh) - c4 * PL(izl, iyh, ixh - 1) + c1 * PL(izl, iyh, ixh - 2);
+    hlh = +c3 * PL(izh, iyl, ixh) - c4 * PL(izh, iyl, ixh - 1) + c1 * PL(izh, iyl, ixh - 2);
+    llh = +c3 * PL(izl, iyl, ixh) - c4 * PL(izl, iyl, ixh - 1) + c1 * PL(izl, iyl, ixh - 2);
   }
   whh = hlh + wyh * (hhh - hlh);
   whl = hll + wyh * (hhl - hll);
@@ -63,10 +62,10 @@ __device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int 
     hhl = PL(izh, iyh + 1, ixl) - PL(izh, iyh - 1, ixl);
     lhl = PL(izl, iyh + 1, ixl) - PL(izl, iyh - 1, ixl);
   } else {
-    hhh = +3.0 * PL(izh, iyh, ixh) - 4.0 * PL(izh, iyh - 1, ixh) + 1.0 * PL(izh, iyh - 2, ixh);
-    lhh = +3.0 * PL(izl, iyh, ixh) - 4.0 * PL(izl, iyh - 1, ixh) + 1.0 * PL(izl, iyh - 2, ixh);
-    hhl = +3.0 * PL(izh, iyh, ixl) - 4.0 * PL(izh, iyh - 1, ixl) + 1.0 * PL(izh, iyh - 2, ixl);
-    lhl = +3.0 * PL(izl, iyh, ixl) - 4.0 * PL(izl, iyh - 1, ixl) + 1.0 * PL(izl, iyh - 2, ixl);
+    hhh = +c3 * PL(izh, iyh, ixh) - c4 * PL(izh, iyh - 1, ixh) + c1 * PL(izh, iyh - 2, ixh);
+    lhh = +c3 * PL(izl, iyh, ixh) - c4 * PL(izl, iyh - 1, ixh) + c1 * PL(izl, iyh - 2, ixh);
+    hhl = +c3 * PL(izh, iyh, ixl) - c4 * PL(izh, iyh - 1, ixl) + c1 * PL(izh, iyh - 2, ixl);
+    lhl = +c3 * PL(izl, iyh, ixl) - c4 * PL(izl, iyh - 1, ixl) + c1 * PL(izl, iyh - 2, ixl);
   }
   whh = hlh + wyh * (hhh - hlh);
   whl = hll + wyh * (hhl - hll);
@@ -86,10 +85,10 @@ __device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int 
     hhl = PL(izh + 1, iyh, ixl) - PL(izh - 1, iyh, ixl);
     hll = PL(izh + 1, iyl, ixl) - PL(izh - 1, iyl, ixl);
   } else {
-    hhh = +3.0 * PL(izh, iyh, ixh) - 4.0 * PL(izh - 1, iyh, ixh) + 1.0 * PL(izh - 2, iyh, ixh);
-    hlh = +3.0 * PL(izh, iyl, ixh) - 4.0 * PL(izh - 1, iyl, ixh) + 1.0 * PL(izh - 2, iyl, ixh);
-    hhl = +3.0 * PL(izh, iyh, ixl) - 4.0 * PL(izh - 1, iyh, ixl) + 1.0 * PL(izh - 2, iyh, ixl);
-    hll = +3.0 * PL(izh, iyl, ixl) - 4.0 * PL(izh - 1, iyl, ixl) + 1.0 * PL(izh - 2, iyl, ixl);
+    hhh = +c3 * PL(izh, iyh, ixh) - c4 * PL(izh - 1, iyh, ixh) + c1 * PL(izh - 2, iyh, ixh);
+    hlh = +c3 * PL(izh, iyl, ixh) - c4 * PL(izh - 1, iyl, ixh) + c1 * PL(izh - 2, iyl, ixh);
+    hhl = +c3 * PL(izh, iyh, ixl) - c4 * PL(izh - 1, iyh, ixl) + c1 * PL(izh - 2, iyh, ixl);
+    hll = +c3 * PL(izh, iyl, ixl) - c4 * PL(izh - 1, iyl, ixl) + c1 * PL(izh - 2, iyl, ixl);
   }
   whh = hlh + wyh * (hhh - hlh);
   whl = hll + wyh * (hhl - hll);
@@ -99,6 +98,16 @@ __device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int 
   Ewl = wll + wxh * (whl - wll);
   f[2] = Ewl + wzh * (Ewh - Ewl);
 #undef PL
+}
+
+// FP64 gather at normalised coordinates (x0, y0, z0) = pos/d: il = (int)x, w = x - il
+// (vector-math.c:607-617).
+template <int RPX, int RPY>
+__device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
+                                      double x0, double y0, double z0, double f[3]) {
+  const int ixl = (int)x0, iyl = (int)y0, izl = (int)z0;
+  diff_vec3d_core<double, RPX, RPY>(F, oi, oj, ok, Nx, Ny, Nz, ixl, iyl, izl, x0 - (double)ixl, y0 - (double)iyl,
+                                    z0 - (double)izl, f);
 }
 
 // Collision constants (src/collisions.c:18-52).

@@ -1,0 +1,128 @@
+// Device and host helpers shared by the FP64 and mixed-precision tile kernels.
+#pragma once
+#include <cmath>
+
+#include "common.hpp"
+#include "pic_math.hpp"
+
+namespace ecsprt {
+
+enum : int { M_PUSH = 1, M_DRIFT = 2, M_DEPOSIT = 4, M_REBIN = 8, M_COLLIDE = 16 };
+
+__device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg) {
+  int ci = (int)xg, cj = (int)yg, ck = (int)zg;
+  if (ci >= g.Nx) ci = g.Nx - 1;
+  if (cj >= g.Ny) cj = g.Ny - 1;
+  if (ck >= g.Nz) ck = g.Nz - 1;
+  return ((ck / g.Tz) * g.nty + cj / g.Ty) * g.ntx + ci / g.Tx;
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  return v;
+}
+
+// Tile offset d = b - a on a periodic axis of n tiles folded to -1, 0, +1; 2 when
+// the destination is further than one tile.
+__device__ inline int tile_step(int d, int n) {
+  if (d > 1) d -= n;
+  else if (d < -1) d += n;
+  return (d >= -1 && d <= 1) ? d : 2;
+}
+
+// FP64 add into the workgroup's LDS accumulator (ds_add_f64).  Workgroup scope keeps
+// it distinct from the agent-scope global atomics of the overflow path.
+__device__ inline void lds_add(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Deposit of an ion that moved beyond the tile's LDS region (about 0.6% of ions at
+// one cell per step): global FP64 atomics into the overflow moments.  Kept out of
+// line so the LDS path stays free of its address arithmetic.
+__device__ inline void deposit_overflow(double *ovf, const TileGeom &g, int ixl, int iyl, int izl,
+                                              const double w8[8], double vx, double vy, double vz) {
+  for (int c = 0; c < 8; ++c) {
+    const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+    const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
+                      wrap_index(ixl + ax, g.Nx);
+    const double wt = w8[c];
+    unsafeAtomicAdd(&ovf[node * 4 + 0], wt);
+    unsafeAtomicAdd(&ovf[node * 4 + 1], vx * wt);
+    unsafeAtomicAdd(&ovf[node * 4 + 2], vy * wt);
+    unsafeAtomicAdd(&ovf[node * 4 + 3], vz * wt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+inline int grid_for(long n, int block) {
+  long b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (int)b;
+}
+
+CollideConsts collide_consts_of(const ecsprt_params &p);
+
+// Mixed-precision storage (particles_mixed.hip).
+void mx_free(ParticlesMix &P);
+void mx_alloc(ParticlesMix &P, long n);
+void mx_set_lds_attrs();
+void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, uint32_t key0, uint32_t key1,
+               uint32_t step);
+void mx_rebin(ecsprt_hip_ctx *c);
+void mx_bin_scatter(ecsprt_hip_ctx *c, long np);
+void mx_unbin(ecsprt_hip_ctx *c);
+void mx_unbin_list(ecsprt_hip_ctx *c, long n);
+void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
+double mx_vmax(const ecsprt_params &p);
+
+// Fill kernels launched on the context's stream (particles.hip).
+void fill_f64(ecsprt_hip_ctx *c, double *p, long n, double v);
+void fill_i32(ecsprt_hip_ctx *c, int *p, long n, int v);
+void zero_ctr(ecsprt_hip_ctx *c);
+
+inline DriftConsts drift_consts(const ecsprt_params &p, double dt) {
+  DriftConsts d;
+  d.dt = dt;
+  d.L[0] = p.Lx;
+  d.L[1] = p.Ly;
+  d.L[2] = p.Lz;
+  d.r0[0] = p.x0;
+  d.r0[1] = p.y0;
+  d.r0[2] = p.z0;
+  d.r1[0] = p.x1;
+  d.r1[1] = p.y1;
+  d.r1[2] = p.z1;
+  return d;
+}
+
+// BorisMoverBz3D scalars, src/velocities.c:307-320.
+inline PushConsts push_consts(const ecsprt_params &p, double dt) {
+  PushConsts k;
+  double h[3] = {1.0 / p.dx, 1.0 / p.dy, 1.0 / p.dz};
+  double tscale = 0.5 * (p.qi / p.mi) * dt;
+  k.t = tscale * p.B0;
+  k.s = 2.0 * k.t / (1 + k.t * k.t);
+  for (int d = 0; d < 3; ++d) k.Escale[d] = -0.5 * h[d] * tscale;
+  k.dx = p.dx;
+  k.dy = p.dy;
+  k.dz = p.dz;
+  const double d3[3] = {p.dx, p.dy, p.dz};
+  k.pow2 = 1;
+  for (int d = 0; d < 3; ++d) {
+    int e;
+    k.pow2 &= std::frexp(d3[d], &e) == 0.5 ? 1 : 0;
+    k.inv[d] = 1.0 / d3[d];
+  }
+  return k;
+}
+
+inline void read_ctr(ecsprt_hip_ctx *c) {
+  ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace ecsprt

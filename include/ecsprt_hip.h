@@ -91,6 +91,12 @@ typedef struct {
 #define ECSPRT_SOLVE_FP64  0
 #define ECSPRT_SOLVE_MIXED 1 /* FP32 GMRES+GMG inside FP64 iterative refinement */
 
+/* Particle storage and arithmetic. */
+#define ECSPRT_PARTICLES_FP64  0 /* FP64 positions and velocities, bit-compatible with the reference */
+#define ECSPRT_PARTICLES_MIXED 1 /* int32 packed cell + FP32 in-cell offset, FP32 velocity;
+                                    FP32 push, drift and deposit, FP64 moment fold.  Grids up
+                                    to 2048 x 2048 x 1024 cells with the origin at 0. */
+
 typedef struct {
   int      tile;           /* cells per tile edge along x and y: 4 or 8 (0 = by grid size) */
   int      tile_z;         /* cells per tile along z: 4 or 8 (0 = tile) */
@@ -112,6 +118,7 @@ typedef struct {
   int      mg_coarse_sweeps;/* sweeps on the coarsest level (0 = 50) */
   double   inner_rtol;     /* mixed precision: inner FP32 solve tolerance (0 = 1e-4) */
   int      timing;         /* 1 = record per-stage HIP events */
+  int      particle_precision; /* ECSPRT_PARTICLES_* */
 } ecsprt_options;
 
 typedef struct ecsprt_hip_ctx ecsprt_hip_ctx;
