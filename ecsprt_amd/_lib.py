@@ -109,6 +109,8 @@ SIGNATURES = {
     "ecsprt_hip_d2d": (ctypes.c_int, [_P, _P, ctypes.c_long, _D]),
     "ecsprt_hip_philox4x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32),
                                         ctypes.POINTER(ctypes.c_uint32)]),
+    "ecsprt_hip_philox2x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
+                                        ctypes.POINTER(ctypes.c_uint32)]),
 }
 
 _lib = None

@@ -554,4 +554,11 @@ void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint
   for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
 
+void ecsprt_hip_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {
+  ecsprt::u32x2 c = {{ctr[0], ctr[1]}};
+  ecsprt::u32x2 r = ecsprt::philox2x32_10(c, key);
+  out[0] = r.v[0];
+  out[1] = r.v[1];
+}
+
 }  // extern "C"

@@ -165,7 +165,7 @@ __global__ void k_att_replay(int mode, long np, const ecsprt_attempt *att, const
       if (mode && a >= cut) break;
       const ecsprt_attempt r = att[a];
       double vf[3];
-      const int res = collide_one(k, r.g, r.u_acc1, r.u_acc2, r.u_z, r.u_phi, v, vf);
+      const int res = collide_one<double>(k, r.g, r.u_acc1, r.u_acc2, r.u_z, r.u_phi, v, vf);
       if (res == 1) {
         v[0] = vf[0];
         v[1] = vf[1];
@@ -274,16 +274,36 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
 
 // Acceptance estimate for the Philox mode: a dry run of the per-particle Poisson
 // process (velocities are not written) on a stream independent of the real steps.
+__device__ inline void pilot_ion(const ParticlesF64 &A, long s, double &vx, double &vy, double &vz, int &id) {
+  vx = A.vx[s];
+  vy = A.vy[s];
+  vz = A.vz[s];
+  id = A.id[s];
+}
+
+__device__ inline void pilot_ion(const ParticlesMix &A, long s, double &vx, double &vy, double &vz, int &id) {
+  const uint32_t *b = A.p + mx_word(s, 0);
+  vx = __uint_as_float(b[64 * MX_VX]);
+  vy = __uint_as_float(b[64 * MX_VY]);
+  vz = __uint_as_float(b[64 * MX_VZ]);
+  id = (int)b[64 * MX_ID];
+}
+
 template <class P>
 __global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k, PoissonTable pt,
                         uint32_t key0, uint32_t key1, uint32_t tag, unsigned long long *ctr) {
+  __shared__ float s_cdf[24];
+  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
+  __syncthreads();
   const int t = blockIdx.x;
   const long b = base[t], n = count[t];
   unsigned natt = 0, nsucc = 0, nfail = 0;
   for (long q = threadIdx.x; q < n; q += blockDim.x) {
     const long s = b + q;
-    double vx = A.vx[s], vy = A.vy[s], vz = A.vz[s];
-    collide_philox(k, pt, key0, key1, tag, A.id[s], vx, vy, vz, natt, nsucc, nfail);
+    double vx, vy, vz;
+    int id;
+    pilot_ion(A, s, vx, vy, vz, id);
+    collide_philox(k, s_cdf, key0, key1, tag, id, vx, vy, vz, natt, nsucc, nfail);
   }
   unsigned long long na = natt, ns = nsucc;
   for (int o = 32; o > 0; o >>= 1) {

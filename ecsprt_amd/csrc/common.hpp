@@ -39,13 +39,17 @@ struct ParticlesF64 {
 
 // Mixed-precision particle storage: a packed integer cell index (x in bits 0-10,
 // y in bits 11-21, z in bits 22-31), the FP32 offset within the cell in units of the
-// cell width, and the FP32 velocity.
+// cell width, the FP32 velocity and the id, in blocks of 64 ions (one wavefront):
+// eight 64-word planes per block.  A wave's fields are whole 256-byte lines and one
+// address per ion reaches all eight fields through immediate offsets.
 struct ParticlesMix {
-  uint32_t *cell = nullptr;
-  float *ox = nullptr, *oy = nullptr, *oz = nullptr;
-  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
-  int *id = nullptr;
+  uint32_t *p = nullptr;
 };
+
+enum : int { MX_CELL = 0, MX_OX = 1, MX_OY = 2, MX_OZ = 3, MX_VX = 4, MX_VY = 5, MX_VZ = 6, MX_ID = 7 };
+
+// Word index of field f of ion i.
+__host__ __device__ inline long mx_word(long i, int f) { return (i >> 6) * 512 + f * 64 + (i & 63); }
 
 constexpr int MX_MAX_NX = 2048, MX_MAX_NY = 2048, MX_MAX_NZ = 1024;
 

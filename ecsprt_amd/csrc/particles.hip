@@ -176,6 +176,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
   __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef;
+  __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
   int wi = blockIdx.x;
@@ -226,6 +227,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
                  wrap_index(w.tx + ox, g.ntx);
   }
   if (tid == 0) s_ndef = 0;
+  if ((MODE & M_COLLIDE) && tid < 24) s_cdf[tid] = a.pt.cdf[tid];
   __syncthreads();
 
   const long cnt = a.count[w.t];
@@ -280,7 +282,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       vz = vzm;
     }
     bool defer = false;
-    if ((MODE & M_COLLIDE) && valid && poisson_attempts(a.pt, a.key0, a.key1, a.step, pid) > 0) {
+    if ((MODE & M_COLLIDE) && valid && poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid) > 0) {
       // About one ion in seven has a collision attempt this step.  It is set aside
       // (post-push state, in this workgroup's own slot range) and finished in
       // phase 2 with every lane busy.
@@ -308,8 +310,8 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
         vy = a.def.vy[d];
         vz = a.def.vz[d];
         pid = a.def.id[d];
-        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
-        collide_attempts(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
       }
       particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
     }
@@ -748,7 +750,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
   long total = 0;
   for (int t = 0; t < c->ntiles; ++t) {
     long cap = cnt[t] + (long)(slack * cnt[t]) + 64;
-    cap = (cap + 15) / 16 * 16;
+    cap = (cap + 63) / 64 * 64;  // tile bases on 64-slot (one wavefront) boundaries
     c->h_base[t] = (int)total;
     c->h_cap[t] = (int)cap;
     total += cap;
@@ -756,13 +758,13 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
   ECS_CHECK(total < 2147483647L, ECSPRT_ERR_ARG, "particle capacity exceeds int32 slots");
   const bool philox = c->o.collide_mode == ECSPRT_COLLIDE_PHILOX;
   if (c->mixed) {
-    if (total > c->pcap || !c->Am.cell) {
+    if (total > c->pcap || !c->Am.p) {
       mx_alloc(c->Am, total);
       mx_alloc(c->Bm, total);
       if (philox) mx_alloc(c->Dm, total);
       c->pcap = total;
     }
-    if (!c->Em.cell || c->np != np) {
+    if (!c->Em.p || c->np != np) {
       mx_alloc(c->Em, np);
       c->scap = std::max(np / 64, 65536L);
       mx_alloc(c->Sm, c->scap);

@@ -46,16 +46,32 @@ struct TileArgsMx {
   uint32_t key0, key1, step;
 };
 
+__device__ inline uint32_t *mx_at(const ParticlesMix &P, long i) { return P.p + (i >> 6) * 512 + (i & 63); }
+
 __device__ inline void store_mx(const ParticlesMix &d, long i, uint32_t cell, float ox, float oy, float oz, float vx,
                                 float vy, float vz, int id) {
-  d.cell[i] = cell;
-  d.ox[i] = ox;
-  d.oy[i] = oy;
-  d.oz[i] = oz;
-  d.vx[i] = vx;
-  d.vy[i] = vy;
-  d.vz[i] = vz;
-  d.id[i] = id;
+  uint32_t *b = mx_at(d, i);
+  b[64 * MX_CELL] = cell;
+  b[64 * MX_OX] = __float_as_uint(ox);
+  b[64 * MX_OY] = __float_as_uint(oy);
+  b[64 * MX_OZ] = __float_as_uint(oz);
+  b[64 * MX_VX] = __float_as_uint(vx);
+  b[64 * MX_VY] = __float_as_uint(vy);
+  b[64 * MX_VZ] = __float_as_uint(vz);
+  b[64 * MX_ID] = (uint32_t)id;
+}
+
+__device__ inline void load_mx(const ParticlesMix &P, long i, uint32_t &cell, float &ox, float &oy, float &oz,
+                               float &vx, float &vy, float &vz, int &id) {
+  const uint32_t *b = mx_at(P, i);
+  cell = b[64 * MX_CELL];
+  ox = __uint_as_float(b[64 * MX_OX]);
+  oy = __uint_as_float(b[64 * MX_OY]);
+  oz = __uint_as_float(b[64 * MX_OZ]);
+  vx = __uint_as_float(b[64 * MX_VX]);
+  vy = __uint_as_float(b[64 * MX_VY]);
+  vz = __uint_as_float(b[64 * MX_VZ]);
+  id = (int)b[64 * MX_ID];
 }
 
 __device__ inline void lds_addq(long long *p, float scaled) {
@@ -183,17 +199,19 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
     w.par ^= 1;
   } else if ((MODE & M_PUSH) && valid) {
-    a.src.vx[p] = vx;
-    a.src.vy[p] = vy;
-    a.src.vz[p] = vz;
+    uint32_t *b = mx_at(a.src, p);
+    b[64 * MX_VX] = __float_as_uint(vx);
+    b[64 * MX_VY] = __float_as_uint(vy);
+    b[64 * MX_VZ] = __float_as_uint(vz);
   }
 }
 
 template <int MODE, int TX, int TY, int TZ, int HH>
-__global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
+__global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ long long smemq[];
   __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef;
+  __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
   int wi = blockIdx.x;
@@ -240,6 +258,7 @@ __global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
                  wrap_index(w.tx + sx, g.ntx);
   }
   if (tid == 0) s_ndef = 0;
+  if ((MODE & M_COLLIDE) && tid < 24) s_cdf[tid] = a.pt.cdf[tid];
   __syncthreads();
 
   const long cnt = a.count[w.t];
@@ -247,40 +266,15 @@ __global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
   const long end = a.base[w.t] + cnt * (s + 1) / g.split;
   unsigned natt = 0, nsucc = 0, nfail = 0;
 
-  // Phase 1, software-pipelined: the next batch loads while this one computes.
-  uint32_t ncell = 0;
-  float nox = 0, noy = 0, noz = 0, nvx = 0, nvy = 0, nvz = 0;
-  int nid = 0;
-  if (beg + tid < end) {
-    const long p = beg + tid;
-    ncell = a.src.cell[p];
-    nox = a.src.ox[p];
-    noy = a.src.oy[p];
-    noz = a.src.oz[p];
-    nvx = a.src.vx[p];
-    nvy = a.src.vy[p];
-    nvz = a.src.vz[p];
-    nid = a.src.id[p];
-  }
+  // Phase 1.  Two workgroups share a CU (64 VGPRs), so one's loads overlap the
+  // other's arithmetic without a software prefetch.
   for (long p0 = beg; p0 < end; p0 += blockDim.x) {
     const long p = p0 + tid;
     const bool valid = p < end;
-    const uint32_t cell = ncell;
-    float ox = nox, oy = noy, oz = noz, vx = nvx, vy = nvy, vz = nvz;
-    const int pid = nid;
-    {
-      const long pn = p + blockDim.x;
-      if (pn < end) {
-        ncell = a.src.cell[pn];
-        nox = a.src.ox[pn];
-        noy = a.src.oy[pn];
-        noz = a.src.oz[pn];
-        nvx = a.src.vx[pn];
-        nvy = a.src.vy[pn];
-        nvz = a.src.vz[pn];
-        nid = a.src.id[pn];
-      }
-    }
+    uint32_t cell = 0;
+    float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
+    int pid = 0;
+    if (valid) load_mx(a.src, p, cell, ox, oy, oz, vx, vy, vz, pid);
     const int ix = mx_i(cell), iy = mx_j(cell), iz = mx_k(cell);
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body (src/velocities.c:342-356) in FP32.
@@ -298,7 +292,7 @@ __global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
       vz = vzm;
     }
     bool defer = false;
-    if ((MODE & M_COLLIDE) && valid && poisson_attempts(a.pt, a.key0, a.key1, a.step, pid) > 0) {
+    if ((MODE & M_COLLIDE) && valid && poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid) > 0) {
       defer = true;
       const int q = atomicAdd(&s_ndef, 1);
       store_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
@@ -306,8 +300,8 @@ __global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
     tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, ix, iy, iz, ox, oy, oz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
-    // Phase 2: deferred ions.  The scattering formula runs in FP64 on the widened
-    // velocity (same Philox draws as the FP64 path), then drift, deposit, re-bin.
+    // Phase 2: deferred ions.  The scattering formula runs in FP32 on the same Philox
+    // draws as the FP64 path, then drift, deposit, re-bin.
     __syncthreads();
     const int ndef = s_ndef;
     for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
@@ -317,18 +311,9 @@ __global__ void __launch_bounds__(1024) k_tile_mx(TileArgsMx a) {
       float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
       int pid = 0;
       if (valid) {
-        const long d = beg + q;
-        cell = a.def.cell[d];
-        ox = a.def.ox[d];
-        oy = a.def.oy[d];
-        oz = a.def.oz[d];
-        pid = a.def.id[d];
-        double dvx = a.def.vx[d], dvy = a.def.vy[d], dvz = a.def.vz[d];
-        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
-        collide_attempts(a.cc, a.key0, a.key1, a.step, pid, k, dvx, dvy, dvz, natt, nsucc, nfail);
-        vx = (float)dvx;
-        vy = (float)dvy;
-        vz = (float)dvz;
+        load_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
+        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
       }
       tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
                                     pid);
@@ -353,16 +338,17 @@ __global__ void k_rebin_mx(TileGeom g, ParticlesMix emi, ParticlesMix dst, Parti
   const unsigned long long n = ctr[0];
   for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
        e += (unsigned long long)gridDim.x * blockDim.x) {
-    const uint32_t cell = emi.cell[e];
+    uint32_t cell;
+    float ox, oy, oz, vx, vy, vz;
+    int id;
+    load_mx(emi, (long)e, cell, ox, oy, oz, vx, vy, vz, id);
     const int t = ((mx_k(cell) / g.Tz) * g.nty + mx_j(cell) / g.Ty) * g.ntx + mx_i(cell) / g.Tx;
     const int slot = atomicAdd(&next[t], 1);
     if (slot < cap[t]) {
-      store_mx(dst, (long)base[t] + slot, cell, emi.ox[e], emi.oy[e], emi.oz[e], emi.vx[e], emi.vy[e], emi.vz[e],
-               emi.id[e]);
+      store_mx(dst, (long)base[t] + slot, cell, ox, oy, oz, vx, vy, vz, id);
     } else {
       const unsigned long long sp = atomicAdd(&ctr[1], 1ull);
-      if ((long)sp < scap)
-        store_mx(spill, (long)sp, cell, emi.ox[e], emi.oy[e], emi.oz[e], emi.vx[e], emi.vy[e], emi.vz[e], emi.id[e]);
+      if ((long)sp < scap) store_mx(spill, (long)sp, cell, ox, oy, oz, vx, vy, vz, id);
       else atomicAdd(&ctr[2], 1ull);
     }
   }
@@ -390,14 +376,17 @@ __global__ void k_bin_scatter_mx(TileGeom g, const double *stage, long np, const
 
 __device__ inline void unstage_mx(const ParticlesMix &A, long s, double *stage, double *vel, double dx, double dy,
                                   double dz) {
-  const long id = A.id[s];
-  const uint32_t cell = A.cell[s];
-  stage[id * 3] = ((double)mx_i(cell) + (double)A.ox[s]) * dx;
-  stage[id * 3 + 1] = ((double)mx_j(cell) + (double)A.oy[s]) * dy;
-  stage[id * 3 + 2] = ((double)mx_k(cell) + (double)A.oz[s]) * dz;
-  vel[id * 3] = A.vx[s];
-  vel[id * 3 + 1] = A.vy[s];
-  vel[id * 3 + 2] = A.vz[s];
+  uint32_t cell;
+  float ox, oy, oz, vx, vy, vz;
+  int pid;
+  load_mx(A, s, cell, ox, oy, oz, vx, vy, vz, pid);
+  const long id = pid;
+  stage[id * 3] = ((double)mx_i(cell) + (double)ox) * dx;
+  stage[id * 3 + 1] = ((double)mx_j(cell) + (double)oy) * dy;
+  stage[id * 3 + 2] = ((double)mx_k(cell) + (double)oz) * dz;
+  vel[id * 3] = vx;
+  vel[id * 3 + 1] = vy;
+  vel[id * 3 + 2] = vz;
 }
 
 __global__ void k_unbin_mx(ParticlesMix A, const int *base, const int *count, double *stage, long np, double dx,
@@ -416,7 +405,7 @@ __global__ void k_cell_counts_mx(TileGeom g, ParticlesMix A, const int *base, co
   const int t = blockIdx.x;
   const long b = base[t], n = count[t];
   for (long q = threadIdx.x; q < n; q += blockDim.x) {
-    const uint32_t cell = A.cell[b + q];
+    const uint32_t cell = mx_at(A, b + q)[64 * MX_CELL];
     atomicAdd(&counts[((long)mx_k(cell) * g.Ny + mx_j(cell)) * g.Nx + mx_i(cell)], 1);
   }
 }
@@ -448,26 +437,16 @@ static MxConsts mx_consts(const ecsprt_params &p, double dt) {
 }
 
 void mx_free(ParticlesMix &P) {
-  (void)hipFree(P.cell);
-  (void)hipFree(P.ox);
-  (void)hipFree(P.oy);
-  (void)hipFree(P.oz);
-  (void)hipFree(P.vx);
-  (void)hipFree(P.vy);
-  (void)hipFree(P.vz);
-  (void)hipFree(P.id);
+  (void)hipFree(P.p);
   P = ParticlesMix{};
 }
 
 void mx_alloc(ParticlesMix &P, long n) {
   mx_free(P);
-  n = std::max(n, 1L);
-  hipError_t e = hipMalloc(&P.cell, sizeof(uint32_t) * n);
-  float **f[6] = {&P.ox, &P.oy, &P.oz, &P.vx, &P.vy, &P.vz};
-  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMalloc(f[i], sizeof(float) * n);
-  if (e == hipSuccess) e = hipMalloc(&P.id, sizeof(int) * n);
+  const long blocks = (std::max(n, 1L) + 63) / 64;
+  hipError_t e = hipMalloc(&P.p, sizeof(uint32_t) * 512 * blocks);
   if (e != hipSuccess) {
-    mx_free(P);
+    P = ParticlesMix{};
     throw Error{ECSPRT_ERR_NOMEM, "particle arrays: " + std::string(hipGetErrorString(e))};
   }
 }

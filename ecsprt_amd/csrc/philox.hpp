@@ -1,6 +1,6 @@
-// Philox4x32-10 counter-based generator (Salmon, Moraes, Dror, Shaw, "Parallel random
-// numbers: as easy as 1, 2, 3", SC'11).  Ten rounds of the 4x32 Philox S-box with the
-// published multipliers and Weyl key increments.
+// Philox4x32-10 and Philox2x32-10 counter-based generators (Salmon, Moraes, Dror,
+// Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11).  Ten rounds of the
+// Philox S-box with the published multipliers and Weyl key increments.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -33,6 +33,25 @@ __host__ __device__ inline u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1
     c = n;
     k0 += W0;
     k1 += W1;
+  }
+  return c;
+}
+
+struct u32x2 {
+  uint32_t v[2];
+};
+
+__host__ __device__ inline u32x2 philox2x32_10(u32x2 c, uint32_t k) {
+  const uint32_t M = 0xD256D193u, W = 0x9E3779B9u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi, lo;
+    philox_mulhilo(M, c.v[0], hi, lo);
+    u32x2 n;
+    n.v[0] = hi ^ k ^ c.v[1];
+    n.v[1] = lo;
+    c = n;
+    k += W;
   }
   return c;
 }

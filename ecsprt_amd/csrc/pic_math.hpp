@@ -116,54 +116,82 @@ struct CollideConsts {
   double vnT, mi, mn, M, vrm, viT;
 };
 
+// Arithmetic of the scattering formula by precision.  FP64 is the reference's
+// (sqrt, division, cos/sin of 2*pi*u); FP32 is the statistical mixed path's, on the
+// hardware square root, reciprocal and sin/cos of revolutions (v_sqrt_f32, v_rcp_f32,
+// v_sin_f32, v_cos_f32; about 1 ulp), which the FP32 draws cannot resolve anyway.
+__device__ inline double c_sqrt(double x) { return sqrt(x); }
+__device__ inline float c_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ inline double c_div(double a, double b) { return a / b; }
+__device__ inline float c_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline double c_cos2pi(double u) { return cos(2 * 3.14159265358979323846 * u); }
+__device__ inline float c_cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
+__device__ inline double c_sin2pi(double u) { return sin(2 * 3.14159265358979323846 * u); }
+__device__ inline float c_sin2pi(float u) { return __builtin_amdgcn_sinf(u); }
+
 // One attempt of the ScatterElastic loop body (src/collisions.c:77-154) on ion
 // velocity vi.  Returns 0 rejected by the acceptance draw, 1 collided (vf written),
-// 2 refused because |vf|/viT > 10 (collisions.c:141).
-__device__ inline int collide_one(const CollideConsts &k, const float g[3], float u_acc1, float u_acc2,
-                                  float u_z, float u_phi, const double vi[3], double vf[3]) {
-  double vn[3], vr[3], vc[3], u2[3];
-  double tmp = 0.0;
+// 2 refused because |vf|/viT > 10 (collisions.c:141).  T = double follows the
+// reference's arithmetic; T = float is the mixed-precision path's.
+template <typename T>
+__device__ inline int collide_one(const CollideConsts &k, const float g[3], float u_acc1, float u_acc2, float u_z,
+                                  float u_phi, const T vi[3], T vf[3]) {
+  const T vnT = (T)k.vnT, mi = (T)k.mi, mn = (T)k.mn, M = (T)k.M, vrm = (T)k.vrm, viT = (T)k.viT;
+  T vn[3], vr[3], vc[3], u2[3];
+  T tmp = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    vn[d] = g[d] * k.vnT + k.vn0[d];
+    vn[d] = g[d] * vnT + (T)k.vn0[d];
     vr[d] = vi[d] - vn[d];
     tmp += vr[d] * vr[d];
   }
-  double vrr = sqrt(tmp);
+  T vrr = c_sqrt(tmp);
   float acc = u_acc1 * u_acc2;
-  if (!(acc < vrr * vrr / (k.vrm * k.vrm))) return 0;
-  tmp = 0.0;
+  if (!(acc < c_div(vrr * vrr, vrm * vrm))) return 0;
+  tmp = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    vc[d] = (vi[d] * k.mi + vn[d] * k.mn) / k.M;
+    vc[d] = c_div(vi[d] * mi + vn[d] * mn, M);
     tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
   }
-  double vcr = sqrt(tmp);
-  double u0z = 2.0 * u_z - 1.0;
-  double uperp = sqrt(1.0 - u0z * u0z);
-  double uphi = 2 * 3.14159265358979323846 * u_phi;
-  double u0x = uperp * cos(uphi);
-  double u0y = uperp * sin(uphi);
-  double costht = vr[2] / vrr;
-  double sintht = sqrt(1.0 - costht * costht);
-  double cosphi = vr[0] / (vrr * sintht);
-  double sinphi = vr[1] / (vrr * sintht);
-  double u1x = u0z * sintht + u0x * costht;
-  double u1y = u0y;
-  double u1z = u0z * costht - u0x * sintht;
+  T vcr = c_sqrt(tmp);
+  T u0z = T(2) * (T)u_z - T(1);
+  T uperp = c_sqrt(T(1) - u0z * u0z);
+  T u0x = uperp * c_cos2pi((T)u_phi);
+  T u0y = uperp * c_sin2pi((T)u_phi);
+  T costht, sintht, cosphi, sinphi;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    costht = vr[2] / vrr;
+    sintht = sqrt(T(1) - costht * costht);
+    cosphi = vr[0] / (vrr * sintht);
+    sinphi = vr[1] / (vrr * sintht);
+  } else {
+    // In FP32 cos(theta) rounds to 1 once |vr_perp| < 3.5e-4 |vr| (about one attempt
+    // in 3e7), where the reference's vr_x / (vrr sin(theta)) is 0/0.  The azimuth is
+    // taken from vr_perp itself, with a fixed direction when it vanishes.
+    const T vperp = c_sqrt(vr[0] * vr[0] + vr[1] * vr[1]);
+    const T ivrr = c_div(T(1), vrr), ivp = c_div(T(1), vperp);
+    costht = vr[2] * ivrr;
+    sintht = vperp * ivrr;
+    cosphi = vperp > T(0) ? vr[0] * ivp : T(1);
+    sinphi = vperp > T(0) ? vr[1] * ivp : T(0);
+  }
+  T u1x = u0z * sintht + u0x * costht;
+  T u1y = u0y;
+  T u1z = u0z * costht - u0x * sintht;
   u2[0] = u1x * cosphi - u1y * sinphi;
   u2[1] = u1x * sinphi + u1y * cosphi;
   u2[2] = u1z;
-  tmp = 0.0;
-  double w[3];
+  tmp = 0;
+  T w[3];
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     w[d] = vc[d] + vcr * u2[d];
     tmp += w[d] * w[d];
   }
-  double vfr = sqrt(tmp);
-  double ratio = vfr / k.viT;
-  if (ratio > 10) return 2;
+  T vfr = c_sqrt(tmp);
+  T ratio = c_div(vfr, viT);
+  if (ratio > T(10)) return 2;
   vf[0] = w[0];
   vf[1] = w[1];
   vf[2] = w[2];
@@ -171,35 +199,38 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
 }
 
 // Box-Muller pair from two 32-bit draws (statistical collision mode only).
+// Hardware log2, square root and sin/cos of revolutions (statistical mode only).
 __device__ inline void box_muller(uint32_t a, uint32_t b, float &g0, float &g1) {
-  float u1 = u01(a), u2 = u01(b);
-  float r = sqrtf(-2.0f * logf(u1));
-  float s, c;
-  sincosf(6.283185307179586f * u2, &s, &c);
-  g0 = r * c;
-  g1 = r * s;
+  const float u1 = u01(a), u2 = u01(b);
+  const float r = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u1));  // -2 ln u1
+  g0 = r * __builtin_amdgcn_cosf(u2);
+  g1 = r * __builtin_amdgcn_sinf(u2);
 }
 
 // Statistical collision mode: a Poisson(lambda) number of attempts on each ion, each
 // attempt the reference's scattering formula on Philox draws keyed by (id, step,
 // attempt).  With lambda = Nc / (np * acceptance) the expected number of collisions
 // is Nc.
-// Cumulative Poisson(lambda) probabilities P(K <= k), k < 24, tabulated on the host.
+// Cumulative Poisson(lambda) probabilities P(K <= k), k < 24, tabulated on the host
+// in FP32 (the draw it is compared with is a 24-bit uniform).  Kernels stage it in LDS.
 struct PoissonTable {
-  double cdf[24];
+  float cdf[24];
 };
 
-__device__ inline int poisson_attempts(const PoissonTable &pt, uint32_t key0, uint32_t key1, uint32_t step, int pid) {
-  u32x4 c0 = {{(uint32_t)pid, step, 0u, 0xC0111DE5u}};
-  u32x4 r0 = philox4x32_10(c0, key0, key1);
-  const double u = (double)u01(r0.v[0]);
+// The number of attempts of ion pid at this step: one Philox2x32-10 draw, counter
+// (pid, step), key mixed from both seed words.
+__device__ inline int poisson_attempts(const float *cdf, uint32_t key0, uint32_t key1, uint32_t step, int pid) {
+  const u32x2 c0 = {{(uint32_t)pid, step}};
+  const u32x2 r0 = philox2x32_10(c0, key0 ^ (key1 * 0x9E3779B9u));
+  const float u = u01(r0.v[0]);
   int k = 0;
-  while (k < 24 && u > pt.cdf[k]) ++k;
+  while (k < 24 && u > cdf[k]) ++k;
   return k;
 }
 
+template <typename T>
 __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, uint32_t key1, uint32_t step, int pid,
-                                        int k, double &vx, double &vy, double &vz, unsigned &natt, unsigned &nsucc,
+                                        int k, T &vx, T &vy, T &vz, unsigned &natt, unsigned &nsucc,
                                         unsigned &nfail) {
   for (int at = 0; at < k; ++at) {
     u32x4 c1 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0xA77E3B71u}};
@@ -209,8 +240,8 @@ __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, 
     float g[3], spare;
     box_muller(r1.v[0], r1.v[1], g[0], g[1]);
     box_muller(r1.v[2], r1.v[3], g[2], spare);
-    double vi[3] = {vx, vy, vz}, vf[3];
-    int res = collide_one(cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
+    T vi[3] = {vx, vy, vz}, vf[3];
+    int res = collide_one<T>(cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
     natt += 1u;
     nsucc += res == 1 ? 1u : 0u;
     nfail += res == 2 ? 1u : 0u;
@@ -222,11 +253,11 @@ __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, 
   }
 }
 
-__device__ inline void collide_philox(const CollideConsts &cc, const PoissonTable &pt, uint32_t key0, uint32_t key1,
+__device__ inline void collide_philox(const CollideConsts &cc, const float *cdf, uint32_t key0, uint32_t key1,
                                       uint32_t step, int pid, double &vx, double &vy, double &vz, unsigned &natt,
                                       unsigned &nsucc, unsigned &nfail) {
-  const int k = poisson_attempts(pt, key0, key1, step, pid);
-  collide_attempts(cc, key0, key1, step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+  const int k = poisson_attempts(cdf, key0, key1, step, pid);
+  collide_attempts<double>(cc, key0, key1, step, pid, k, vx, vy, vz, natt, nsucc, nfail);
 }
 
 // Host: P(K <= k) for K ~ Poisson(lam); the last entry closes the table at 1.
@@ -234,11 +265,11 @@ inline PoissonTable poisson_table(double lam) {
   PoissonTable t;
   double pk = exp(-lam), F = pk;
   for (int k = 0; k < 24; ++k) {
-    t.cdf[k] = F;
+    t.cdf[k] = (float)F;
     pk *= lam / (k + 1);
     F += pk;
   }
-  t.cdf[23] = 1.0;
+  t.cdf[23] = 1.0f;
   return t;
 }
 

@@ -39,12 +39,18 @@ __device__ inline void lds_add(double *p, double v) {
 // Deposit of an ion that moved beyond the tile's LDS region (about 0.6% of ions at
 // one cell per step): global FP64 atomics into the overflow moments.  Kept out of
 // line so the LDS path stays free of its address arithmetic.
+// Periodic index for i within one period of [0, n) (no integer division).
+__device__ inline int wrap_near(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
+
 __device__ inline void deposit_overflow(double *ovf, const TileGeom &g, int ixl, int iyl, int izl,
                                               const double w8[8], double vx, double vy, double vz) {
+  const int xs[2] = {wrap_near(ixl, g.Nx), wrap_near(ixl + 1, g.Nx)};
+  const int ys[2] = {wrap_near(iyl, g.Ny), wrap_near(iyl + 1, g.Ny)};
+  const int zs[2] = {wrap_near(izl, g.Nz), wrap_near(izl + 1, g.Nz)};
+#pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
-    const long node = ((long)wrap_index(izl + az, g.Nz) * g.Ny + wrap_index(iyl + ay, g.Ny)) * g.Nx +
-                      wrap_index(ixl + ax, g.Nx);
+    const long node = ((long)zs[az] * g.Ny + ys[ay]) * g.Nx + xs[ax];
     const double wt = w8[c];
     unsafeAtomicAdd(&ovf[node * 4 + 0], wt);
     unsafeAtomicAdd(&ovf[node * 4 + 1], vx * wt);

@@ -245,3 +245,10 @@ def philox4x32_10(ctr, key):
     o = (ctypes.c_uint32 * 4)()
     _lib.load().ecsprt_hip_philox4x32_10(c, k, o)
     return list(o)
+
+
+def philox2x32_10(ctr, key):
+    c = (ctypes.c_uint32 * 2)(*ctr)
+    o = (ctypes.c_uint32 * 2)()
+    _lib.load().ecsprt_hip_philox2x32_10(c, key, o)
+    return list(o)

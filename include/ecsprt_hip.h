@@ -230,8 +230,10 @@ int ecsprt_hip_d2h(void *dst, const void *src, long bytes);
  * probe used to calibrate HBM byte counters). */
 int ecsprt_hip_d2d(void *dst, const void *src, long bytes, double *ms);
 
-/* Philox4x32-10 (Salmon et al., SC'11) on the host, for known-answer tests. */
+/* Philox4x32-10 and Philox2x32-10 (Salmon et al., SC'11) on the host, for known-answer
+ * tests of the generators the statistical collision mode draws from. */
 void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void ecsprt_hip_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);
 
 #ifdef __cplusplus
 }

@@ -46,6 +46,13 @@ def test_philox_known_answers():
         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
 
 
+def test_philox2x32_known_answers():
+    """Random123 known-answer vectors for Philox2x32-10."""
+    assert pic.philox2x32_10([0, 0], 0) == [0xFF1DAE59, 0x6CD10DF2]
+    assert pic.philox2x32_10([0xFFFFFFFF] * 2, 0xFFFFFFFF) == [0x2C3F628B, 0xAB4FD7AD]
+    assert pic.philox2x32_10([0x243F6A88, 0x85A308D3], 0x13198A2E) == [0xDD7CE038, 0xF62A4C12]
+
+
 @pytest.mark.parametrize("wl", [configs.C0, configs.C1, configs.small(12)])
 def test_setup_context_matches_oracle(wl):
     a = pic.setup_context(**wl.cli)
