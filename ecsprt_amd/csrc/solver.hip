@@ -8,9 +8,10 @@
 // coarse spacing from the full-weighting restriction of the density.  An axis is
 // coarsened when its coupling is within a factor 2 of the strongest one, so the
 // (1+kappa_e^2) z-anisotropy is removed by z-semicoarsening before full
-// coarsening; red-black point Gauss-Seidel smooths; the coarsest level is solved by
-// one workgroup in LDS.  All reductions are two-stage with a fixed order, so every
-// solve is bitwise reproducible.
+// coarsening.  Red-black Gauss-Seidel smooths, one LDS-tiled launch per sweep.  The
+// levels small enough for one workgroup's LDS (the coarse end of the V-cycle) run in
+// a single launch that ends in a dense inverse of the coarsest operator.  Smoothing
+// and all reductions have a fixed order, so every solve is bitwise reproducible.
 #include <algorithm>
 #include <cmath>
 
@@ -27,21 +28,11 @@ struct StencilConsts {
   double sxx, syy, szz, sod;
 };
 
-// 11 coefficients of row (i,j,k) in the reference order (lhs.c:409-423).
+// 11 coefficients of a row in the reference order (lhs.c:409-423) from the density at
+// the node (nijk) and its six axis neighbours (m/p: minus/plus along the axis).
 template <typename T>
-__device__ inline void row_coeffs(const T *__restrict__ dn, const StencilConsts &s, int i, int j, int k, T c[11]) {
-  const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
-  const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
-  const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
-  const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
-  const long row = (long)Ny * Nx;
-  const T nijk = dn[k * row + (long)j * Nx + i];
-  const T nmjk = dn[k * row + (long)j * Nx + im];
-  const T npjk = dn[k * row + (long)j * Nx + ip];
-  const T nimk = dn[k * row + (long)jm * Nx + i];
-  const T nipk = dn[k * row + (long)jp * Nx + i];
-  const T nijm = dn[km * row + (long)j * Nx + i];
-  const T nijp = dn[kp * row + (long)j * Nx + i];
+__host__ __device__ inline void coeffs7(const StencilConsts &s, T nijk, T nmjk, T npjk, T nimk, T nipk, T nijm, T nijp,
+                                        T c[11]) {
   const T sxx = (T)s.sxx, syy = (T)s.syy, szz = (T)s.szz, sod = (T)s.sod;
   c[6] = sod * (nipk - npjk);
   c[7] = sod * (npjk - nimk);
@@ -56,9 +47,28 @@ __device__ inline void row_coeffs(const T *__restrict__ dn, const StencilConsts 
   c[10] = -(sxx * (npjk + 2 * nijk + nmjk) + syy * (nipk + 2 * nijk + nimk) + szz * (nijp + 2 * nijk + nijm));
 }
 
+// 11 coefficients of row (i,j,k) of the periodic grid.
+template <typename T>
+__host__ __device__ inline void row_coeffs(const T *__restrict__ dn, const StencilConsts &s, int i, int j, int k,
+                                           T c[11]) {
+  const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
+  const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
+  const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
+  const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
+  const long row = (long)Ny * Nx;
+  const T nijk = dn[k * row + (long)j * Nx + i];
+  const T nmjk = dn[k * row + (long)j * Nx + im];
+  const T npjk = dn[k * row + (long)j * Nx + ip];
+  const T nimk = dn[k * row + (long)jm * Nx + i];
+  const T nipk = dn[k * row + (long)jp * Nx + i];
+  const T nijm = dn[km * row + (long)j * Nx + i];
+  const T nijp = dn[kp * row + (long)j * Nx + i];
+  coeffs7(s, nijk, nmjk, npjk, nimk, nipk, nijm, nijp, c);
+}
+
 // Off-diagonal part of row (i,j,k) applied to x.
 template <typename T>
-__device__ inline T offdiag(const T *__restrict__ x, const StencilConsts &s, int i, int j, int k, const T c[11]) {
+__host__ __device__ inline T offdiag(const T *__restrict__ x, const StencilConsts &s, int i, int j, int k, const T c[11]) {
   const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
   const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
   const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
@@ -71,53 +81,126 @@ __device__ inline T offdiag(const T *__restrict__ x, const StencilConsts &s, int
          c[8] * x[zc + yp + im] + c[9] * x[zc + ym + im];
 }
 
+// Node-wise kernels run on a 3D grid of 64 x 4 thread blocks, one thread per node:
+// block (x, y) tiles an xy plane, blockIdx.z is the plane.
+struct Launch3 {
+  dim3 grid, block;
+};
+static Launch3 launch3(int Nx, int Ny, int Nz) {
+  return {dim3((Nx + 63) / 64, (Ny + 3) / 4, Nz), dim3(64, 4, 1)};
+}
+#define NODE_IJK(NX, NY)                                              \
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;                \
+  const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z; \
+  if (i >= (NX) || j >= (NY)) return;                                 \
+  const long q = ((long)k * (NY) + j) * (NX) + i;
+
 template <typename T>
 __global__ void k_apply(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ x, T *__restrict__ y) {
-  const long n = (long)s.Nx * s.Ny * s.Nz;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / ((long)s.Nx * s.Ny);
-    T c[11];
-    row_coeffs(dn, s, i, j, k, c);
-    y[q] = offdiag(x, s, i, j, k, c) + c[10] * x[q];
-  }
+  NODE_IJK(s.Nx, s.Ny)
+  T c[11];
+  row_coeffs(dn, s, i, j, k, c);
+  y[q] = offdiag(x, s, i, j, k, c) + c[10] * x[q];
 }
 
 // r = b - A x
 template <typename T>
 __global__ void k_residual(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ x, const T *__restrict__ b,
                            T *__restrict__ r) {
-  const long n = (long)s.Nx * s.Ny * s.Nz;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / ((long)s.Nx * s.Ny);
-    T c[11];
-    row_coeffs(dn, s, i, j, k, c);
-    r[q] = b[q] - (offdiag(x, s, i, j, k, c) + c[10] * x[q]);
+  NODE_IJK(s.Nx, s.Ny)
+  T c[11];
+  row_coeffs(dn, s, i, j, k, c);
+  r[q] = b[q] - (offdiag(x, s, i, j, k, c) + c[10] * x[q]);
+}
+
+// Periodic index for a within one period of [0, n).
+__host__ __device__ inline int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// 1/x from the hardware reciprocal refined by Newton steps (two in FP64, one in
+// FP32): within an ulp of the quotient, for the smoother's diagonal scaling.
+__device__ inline double smooth_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+__device__ inline float smooth_rcp(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
+// Gauss-Seidel value of LDS node p (a box of HX x HY nodes per plane) from the
+// density sd and the neighbour values xs; `old` where the diagonal vanishes.
+template <typename T, int HX, int HY>
+__device__ inline T relax_lds(const StencilConsts &s, const T *sd, const T *xs, int p, T bq, T old) {
+  constexpr int DY = HX, DZ = HX * HY;
+  T c[11];
+  coeffs7(s, sd[p], sd[p - 1], sd[p + 1], sd[p - DY], sd[p + DY], sd[p - DZ], sd[p + DZ], c);
+  if (c[10] == T(0)) return old;
+  const T off = c[0] * xs[p + 1] + c[1] * xs[p - 1] + c[2] * xs[p + DY] + c[3] * xs[p - DY] + c[4] * xs[p + DZ] +
+                c[5] * xs[p - DZ] + c[6] * xs[p + DY + 1] + c[7] * xs[p - DY + 1] + c[8] * xs[p + DY - 1] +
+                c[9] * xs[p - DY - 1];
+  return (bq - off) * smooth_rcp(c[10]);
+}
+
+// One red-black Gauss-Seidel sweep (colour `first`, then the other; colour of node
+// (i,j,k) = (i+j+k)&1) from xin (zero when null) into xout, on tiles of TX x TY x TZ
+// nodes.  Density and potential are staged in LDS with a halo of 2.  The first colour
+// is updated on the tile plus a halo of 1 into a second LDS array, so the second
+// colour's update on the tile reads LDS only.  The 11-point stencil's xy-diagonal
+// couplings join nodes of one colour; a colour's update reads those neighbours from
+// before its half-sweep, so the result does not depend on thread order.
+template <typename T, int TX, int TY, int TZ>
+__global__ void __launch_bounds__(512) k_sweep(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ xin,
+                                              const T *__restrict__ b, T *__restrict__ xout, int first) {
+  constexpr int HX = TX + 4, HY = TY + 4, HZ = TZ + 4, H3 = HX * HY * HZ;
+  constexpr int RX = TX + 2, RY = TY + 2, RZ = TZ + 2, R3 = RX * RY * RZ;
+  __shared__ T sx[H3], sn[H3], sd[H3];
+  const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
+  const int ntx = (Nx + TX - 1) / TX, nty = (Ny + TY - 1) / TY;
+  const int t = blockIdx.x;
+  const int i0 = (t % ntx) * TX, j0 = ((t / ntx) % nty) * TY, k0 = (t / (ntx * nty)) * TZ;
+  const int ex = min(TX, Nx - i0), ey = min(TY, Ny - j0), ez = min(TZ, Nz - k0);
+  for (int q = threadIdx.x; q < H3; q += blockDim.x) {
+    const int li = q % HX - 2, lj = (q / HX) % HY - 2, lk = q / (HX * HY) - 2;
+    if (li > ex + 1 || lj > ey + 1 || lk > ez + 1) continue;
+    const long g = ((long)wrap1(k0 + lk, Nz) * Ny + wrap1(j0 + lj, Ny)) * Nx + wrap1(i0 + li, Nx);
+    sd[q] = dn[g];
+    sx[q] = xin ? xin[g] : T(0);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < R3; q += blockDim.x) {
+    const int li = q % RX - 1, lj = (q / RX) % RY - 1, lk = q / (RX * RY) - 1;
+    if (li > ex || lj > ey || lk > ez) continue;
+    const int gi = wrap1(i0 + li, Nx), gj = wrap1(j0 + lj, Ny), gk = wrap1(k0 + lk, Nz);
+    const int p = ((lk + 2) * HY + (lj + 2)) * HX + (li + 2);
+    T v = sx[p];
+    if (((gi + gj + gk) & 1) == first)
+      v = relax_lds<T, HX, HY>(s, sd, sx, p, b[((long)gk * Ny + gj) * Nx + gi], v);
+    sn[p] = v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < TX * TY * TZ; q += blockDim.x) {
+    const int li = q % TX, lj = (q / TX) % TY, lk = q / (TX * TY);
+    if (li >= ex || lj >= ey || lk >= ez) continue;
+    const int gi = i0 + li, gj = j0 + lj, gk = k0 + lk;
+    const int p = ((lk + 2) * HY + (lj + 2)) * HX + (li + 2);
+    const long g = ((long)gk * Ny + gj) * Nx + gi;
+    T v = sn[p];
+    if (((gi + gj + gk) & 1) != first) v = relax_lds<T, HX, HY>(s, sd, sn, p, b[g], v);
+    xout[g] = v;
   }
 }
 
-// One colour of red-black Gauss-Seidel: nodes with (i+j+k)&1 == colour.
-template <typename T>
-__global__ void k_smooth_rb(StencilConsts s, const T *__restrict__ dn, T *__restrict__ x, const T *__restrict__ b,
-                            int colour) {
-  const long n = (long)s.Nx * s.Ny * s.Nz;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / ((long)s.Nx * s.Ny);
-    if (((i + j + k) & 1) != colour) continue;
-    T c[11];
-    row_coeffs(dn, s, i, j, k, c);
-    if (c[10] != (T)0) x[q] = (b[q] - offdiag(x, s, i, j, k, c)) / c[10];
-  }
-}
+constexpr int SW_TX = 16, SW_TY = 8, SW_TZ = 4;
+constexpr int DENSE_MAX = 1024;                // largest coarsest level solved by a dense inverse
+constexpr size_t TAIL_LDS_MAX = 140 * 1024;    // LDS of the single-workgroup tail
 
 template <typename T>
 __global__ void k_jacobi(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ r, T *__restrict__ z) {
-  const long n = (long)s.Nx * s.Ny * s.Nz;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / ((long)s.Nx * s.Ny);
-    T c[11];
-    row_coeffs(dn, s, i, j, k, c);
-    z[q] = c[10] != (T)0 ? r[q] / c[10] : (T)0;
-  }
+  NODE_IJK(s.Nx, s.Ny)
+  T c[11];
+  row_coeffs(dn, s, i, j, k, c);
+  z[q] = c[10] != (T)0 ? r[q] / c[10] : (T)0;
 }
 
 struct CoarsenMap {
@@ -127,91 +210,173 @@ struct CoarsenMap {
 
 // Full weighting [1/4 1/2 1/4] along the coarsened axes, injection along the others.
 template <typename T>
-__global__ void k_restrict(CoarsenMap m, const T *__restrict__ f, T *__restrict__ cvec) {
-  const long n = (long)m.Nxc * m.Nyc * m.Nzc;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int I = q % m.Nxc, J = (q / m.Nxc) % m.Nyc, K = q / ((long)m.Nxc * m.Nyc);
-    const int fi = m.cx ? 2 * I : I, fj = m.cy ? 2 * J : J, fk = m.cz ? 2 * K : K;
-    double acc = 0.0;
-    for (int c = -m.cz; c <= m.cz; ++c) {
-      const double wz = m.cz ? (c == 0 ? 0.5 : 0.25) : 1.0;
-      const int kk = (fk + c + m.Nzf) % m.Nzf;
-      for (int b = -m.cy; b <= m.cy; ++b) {
-        const double wy = m.cy ? (b == 0 ? 0.5 : 0.25) : 1.0;
-        const int jj = (fj + b + m.Nyf) % m.Nyf;
-        for (int a = -m.cx; a <= m.cx; ++a) {
-          const double wx = m.cx ? (a == 0 ? 0.5 : 0.25) : 1.0;
-          const int ii = (fi + a + m.Nxf) % m.Nxf;
-          acc += wz * wy * wx * (double)f[((long)kk * m.Nyf + jj) * m.Nxf + ii];
-        }
+__device__ inline T restrict_point(const CoarsenMap &m, const T *f, int I, int J, int K) {
+  const int fi = m.cx ? 2 * I : I, fj = m.cy ? 2 * J : J, fk = m.cz ? 2 * K : K;
+  double acc = 0.0;
+  for (int c = -m.cz; c <= m.cz; ++c) {
+    const double wz = m.cz ? (c == 0 ? 0.5 : 0.25) : 1.0;
+    const int kk = wrap1(fk + c, m.Nzf);
+    for (int b = -m.cy; b <= m.cy; ++b) {
+      const double wy = m.cy ? (b == 0 ? 0.5 : 0.25) : 1.0;
+      const int jj = wrap1(fj + b, m.Nyf);
+      for (int a = -m.cx; a <= m.cx; ++a) {
+        const double wx = m.cx ? (a == 0 ? 0.5 : 0.25) : 1.0;
+        acc += wz * wy * wx * (double)f[((long)kk * m.Nyf + jj) * m.Nxf + wrap1(fi + a, m.Nxf)];
       }
     }
-    cvec[q] = (T)acc;
   }
+  return (T)acc;
 }
 
-// x_f += P x_c, linear interpolation along the coarsened axes.
+// Linear interpolation along the coarsened axes of the coarse vector at fine node (i,j,k).
+template <typename T>
+__device__ inline T prolong_point(const CoarsenMap &m, const T *xc, int i, int j, int k) {
+  int ia[2], ja[2], ka[2];
+  T wa[2], wb[2], wc[2];
+  int na = 1, nb = 1, nc = 1;
+  if (m.cx && (i & 1)) { ia[0] = i >> 1; ia[1] = wrap1((i + 1) >> 1, m.Nxc); wa[0] = wa[1] = (T)0.5; na = 2; }
+  else { ia[0] = m.cx ? i >> 1 : i; wa[0] = (T)1; }
+  if (m.cy && (j & 1)) { ja[0] = j >> 1; ja[1] = wrap1((j + 1) >> 1, m.Nyc); wb[0] = wb[1] = (T)0.5; nb = 2; }
+  else { ja[0] = m.cy ? j >> 1 : j; wb[0] = (T)1; }
+  if (m.cz && (k & 1)) { ka[0] = k >> 1; ka[1] = wrap1((k + 1) >> 1, m.Nzc); wc[0] = wc[1] = (T)0.5; nc = 2; }
+  else { ka[0] = m.cz ? k >> 1 : k; wc[0] = (T)1; }
+  T acc = 0;
+  for (int c = 0; c < nc; ++c)
+    for (int b = 0; b < nb; ++b)
+      for (int a = 0; a < na; ++a) acc += wc[c] * wb[b] * wa[a] * xc[((long)ka[c] * m.Nyc + ja[b]) * m.Nxc + ia[a]];
+  return acc;
+}
+
+template <typename T>
+__global__ void k_restrict(CoarsenMap m, const T *__restrict__ f, T *__restrict__ cvec) {
+  NODE_IJK(m.Nxc, m.Nyc)
+  cvec[q] = restrict_point(m, f, i, j, k);
+}
+
+// x_f += P x_c
 template <typename T>
 __global__ void k_prolong_add(CoarsenMap m, const T *__restrict__ xc, T *__restrict__ xf) {
-  const long n = (long)m.Nxf * m.Nyf * m.Nzf;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int i = q % m.Nxf, j = (q / m.Nxf) % m.Nyf, k = q / ((long)m.Nxf * m.Nyf);
-    int ia[2], ja[2], ka[2];
-    T wa[2], wb[2], wc[2];
-    int na = 1, nb = 1, nc = 1;
-    if (m.cx) {
-      if (i & 1) { ia[0] = i >> 1; ia[1] = ((i + 1) >> 1) % m.Nxc; wa[0] = wa[1] = (T)0.5; na = 2; }
-      else { ia[0] = i >> 1; wa[0] = (T)1; }
-    } else { ia[0] = i; wa[0] = (T)1; }
-    if (m.cy) {
-      if (j & 1) { ja[0] = j >> 1; ja[1] = ((j + 1) >> 1) % m.Nyc; wb[0] = wb[1] = (T)0.5; nb = 2; }
-      else { ja[0] = j >> 1; wb[0] = (T)1; }
-    } else { ja[0] = j; wb[0] = (T)1; }
-    if (m.cz) {
-      if (k & 1) { ka[0] = k >> 1; ka[1] = ((k + 1) >> 1) % m.Nzc; wc[0] = wc[1] = (T)0.5; nc = 2; }
-      else { ka[0] = k >> 1; wc[0] = (T)1; }
-    } else { ka[0] = k; wc[0] = (T)1; }
-    T acc = 0;
-    for (int c = 0; c < nc; ++c)
-      for (int b = 0; b < nb; ++b)
-        for (int a = 0; a < na; ++a)
-          acc += wc[c] * wb[b] * wa[a] * xc[((long)ka[c] * m.Nyc + ja[b]) * m.Nxc + ia[a]];
-    xf[q] += acc;
-  }
+  NODE_IJK(m.Nxf, m.Nyf)
+  xf[q] += prolong_point(m, xc, i, j, k);
 }
 
-// Coarsest level: zero guess, `sweeps` red-black sweeps and mean removal, all in
-// LDS by one workgroup (n <= 4096).
+// ---- coarse end of the V-cycle in one workgroup -----------------------------------
+// Levels whose density, two potential buffers, right-hand side and residual fit in
+// LDS run in one launch: smoothing, residual, restriction and prolongation are
+// workgroup-wide passes between barriers, and the coarsest level is solved with a
+// dense inverse of (A + alpha 1 1^T), exact for the mean-free right-hand sides the
+// cycle produces (A's rows and columns sum to zero), followed by mean removal.
+constexpr int TAIL_MAX = 8;
+constexpr int TAIL_THREADS = 1024;
+
 template <typename T>
-__global__ void __launch_bounds__(1024) k_coarse_solve(StencilConsts s, const T *__restrict__ dn,
-                                                       const T *__restrict__ b, T *__restrict__ x, int sweeps) {
-  __shared__ T sx[4096];
-  __shared__ double red[1024];
+struct TailArgs {
+  int nl;  // tail levels, the last being the coarsest
+  int pre, post, nc;
+  StencilConsts s[TAIL_MAX];
+  CoarsenMap m[TAIL_MAX];  // level l -> l + 1
+  int off[TAIL_MAX];       // LDS block of level l: dn, x, x2, b, r (n_l elements each)
+  const T *dn[TAIL_MAX];
+  const T *ainv;           // nc x nc, row-major
+  const T *bin;            // right-hand side of the first tail level
+  T *xout;                 // its solution
+};
+
+// One workgroup-wide red-black sweep on LDS arrays: X -> X2 (first colour) -> X.
+template <typename T>
+__device__ inline void tail_sweep(const StencilConsts &s, const T *D, T *X, T *X2, const T *B, int first, bool zero) {
   const int n = s.Nx * s.Ny * s.Nz;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) sx[q] = 0;
-  __syncthreads();
-  for (int it = 0; it < sweeps; ++it) {
-    for (int colour = 0; colour < 2; ++colour) {
-      for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / (s.Nx * s.Ny);
-        if (((i + j + k) & 1) != colour) continue;
-        T c[11];
-        row_coeffs(dn, s, i, j, k, c);
-        if (c[10] != (T)0) sx[q] = (b[q] - offdiag(sx, s, i, j, k, c)) / c[10];
-      }
-      __syncthreads();
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / (s.Nx * s.Ny);
+    T v = zero ? T(0) : X[q];
+    if (((i + j + k) & 1) == first) {
+      T c[11];
+      row_coeffs(D, s, i, j, k, c);
+      if (c[10] != T(0)) v = (B[q] - (zero ? T(0) : offdiag(X, s, i, j, k, c))) / c[10];
     }
+    X2[q] = v;
   }
-  double part = 0.0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) part += (double)sx[q];
-  red[threadIdx.x] = part;
   __syncthreads();
-  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / (s.Nx * s.Ny);
+    T v = X2[q];
+    if (((i + j + k) & 1) != first) {
+      T c[11];
+      row_coeffs(D, s, i, j, k, c);
+      if (c[10] != T(0)) v = (B[q] - offdiag(X2, s, i, j, k, c)) / c[10];
+    }
+    X[q] = v;
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TAIL_THREADS) k_tail(TailArgs<T> a) {
+  extern __shared__ double tail_lds[];
+  T *sm = reinterpret_cast<T *>(tail_lds);
+  __shared__ double red[TAIL_THREADS];
+  for (int l = 0; l < a.nl; ++l) {
+    const int n = a.s[l].Nx * a.s[l].Ny * a.s[l].Nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) sm[a.off[l] + q] = a.dn[l][q];
+  }
+  {
+    const int n = a.s[0].Nx * a.s[0].Ny * a.s[0].Nz;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) sm[a.off[0] + 3 * n + q] = a.bin[q];
+  }
+  __syncthreads();
+  for (int l = 0; l + 1 < a.nl; ++l) {
+    const StencilConsts &s = a.s[l];
+    const int n = s.Nx * s.Ny * s.Nz;
+    T *D = sm + a.off[l], *X = D + n, *X2 = X + n, *B = X2 + n, *R = B + n;
+    for (int it = 0; it < a.pre; ++it) tail_sweep(s, D, X, X2, B, 0, it == 0);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      const int i = q % s.Nx, j = (q / s.Nx) % s.Ny, k = q / (s.Nx * s.Ny);
+      T c[11];
+      row_coeffs(D, s, i, j, k, c);
+      R[q] = B[q] - (offdiag(X, s, i, j, k, c) + c[10] * X[q]);
+    }
+    __syncthreads();
+    const CoarsenMap &m = a.m[l];
+    const int nc = m.Nxc * m.Nyc * m.Nzc;
+    T *Bc = sm + a.off[l + 1] + 3 * nc;
+    for (int q = threadIdx.x; q < nc; q += blockDim.x)
+      Bc[q] = restrict_point(m, R, q % m.Nxc, (q / m.Nxc) % m.Nyc, q / (m.Nxc * m.Nyc));
     __syncthreads();
   }
-  const T mean = (T)(red[0] / n);
-  for (int q = threadIdx.x; q < n; q += blockDim.x) x[q] = sx[q] - mean;
+  {
+    const int l = a.nl - 1, n = a.nc;
+    T *X = sm + a.off[l] + n, *B = X + 2 * n;
+    double part = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      T acc = 0;
+      const T *row = a.ainv + (long)q * n;
+      for (int c = 0; c < n; ++c) acc += row[c] * B[c];
+      X[q] = acc;
+      part += (double)acc;
+    }
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    const T mean = (T)(red[0] / n);
+    for (int q = threadIdx.x; q < n; q += blockDim.x) X[q] -= mean;
+    __syncthreads();
+  }
+  for (int l = a.nl - 2; l >= 0; --l) {
+    const StencilConsts &s = a.s[l];
+    const int n = s.Nx * s.Ny * s.Nz;
+    T *D = sm + a.off[l], *X = D + n, *X2 = X + n, *B = X2 + n;
+    const CoarsenMap &m = a.m[l];
+    const T *Xc = sm + a.off[l + 1] + m.Nxc * m.Nyc * m.Nzc;
+    for (int q = threadIdx.x; q < n; q += blockDim.x)
+      X[q] += prolong_point(m, Xc, q % s.Nx, (q / s.Nx) % s.Ny, q / (s.Nx * s.Ny));
+    __syncthreads();
+    for (int it = 0; it < a.post; ++it) tail_sweep(s, D, X, X2, B, 1, false);
+  }
+  const int n0 = a.s[0].Nx * a.s[0].Ny * a.s[0].Nz;
+  for (int q = threadIdx.x; q < n0; q += blockDim.x) a.xout[q] = sm[a.off[0] + n0 + q];
 }
 
 // ---- vector kernels ------------------------------------------------------------
@@ -332,9 +497,8 @@ struct RhsConsts {
 };
 
 __global__ void k_rhs(RhsConsts c, const double *__restrict__ m, double *__restrict__ b) {
-  const long n = (long)c.Nx * c.Ny * c.Nz;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const int i = q % c.Nx, j = (q / c.Nx) % c.Ny, k = q / ((long)c.Nx * c.Ny);
+  {
+    NODE_IJK(c.Nx, c.Ny)
     const int ip = i + 1 == c.Nx ? 0 : i + 1, im = i == 0 ? c.Nx - 1 : i - 1;
     const int jp = j + 1 == c.Ny ? 0 : j + 1, jm = j == 0 ? c.Ny - 1 : j - 1;
     const int kp = k + 1 == c.Nz ? 0 : k + 1, km = k == 0 ? c.Nz - 1 : k - 1;
@@ -371,12 +535,17 @@ struct Level {
   long n = 0;
   double dx = 1, dy = 1, dz = 1;
   CoarsenMap to_coarse{};  // map from this level to the next
-  T *dn = nullptr, *x = nullptr, *b = nullptr, *r = nullptr;
+  T *dn = nullptr, *x = nullptr, *x2 = nullptr, *b = nullptr, *r = nullptr;
 };
 
 template <typename T>
 struct MG {
   std::vector<Level<T>> L;
+  T *tmp0 = nullptr;  // second level-0 buffer of the ping-pong sweeps
+  T *ainv = nullptr;  // dense inverse of the coarsest operator (+ alpha 1 1^T), or null
+  int tail = 0;       // first level of the single-workgroup tail (== L.size(): none)
+  size_t tail_lds = 0;
+  TailArgs<T> targs{};
 };
 
 template <typename T>
@@ -430,6 +599,7 @@ static void mg_create(ecsprt_hip_ctx *c, MG<T> &mg) {
     L.dz = dz;
     ECS_HIP(hipMalloc(&L.dn, sizeof(T) * L.n));
     ECS_HIP(hipMalloc(&L.x, sizeof(T) * L.n));
+    ECS_HIP(hipMalloc(&L.x2, sizeof(T) * L.n));
     ECS_HIP(hipMalloc(&L.b, sizeof(T) * L.n));
     ECS_HIP(hipMalloc(&L.r, sizeof(T) * L.n));
     // Coarsening rule: strong axes (coupling >= half the strongest) that halve cleanly.
@@ -454,6 +624,38 @@ static void mg_create(ecsprt_hip_ctx *c, MG<T> &mg) {
     if (cf[1]) dy *= 2;
     if (cf[2]) dz *= 2;
   }
+  ECS_HIP(hipMalloc(&mg.tmp0, sizeof(T) * mg.L[0].n));
+  // The tail: the coarsest levels whose five LDS arrays fit in one workgroup, when
+  // the coarsest is small enough for a dense inverse.
+  const int nl = (int)mg.L.size();
+  mg.tail = nl;
+  if (mg.L.back().n <= DENSE_MAX) {
+    size_t bytes = 0;
+    int t = nl;
+    while (t > 0 && nl - (t - 1) <= TAIL_MAX) {
+      const size_t add = 5 * sizeof(T) * (size_t)mg.L[t - 1].n;
+      if (bytes + add > TAIL_LDS_MAX) break;
+      bytes += add;
+      --t;
+    }
+    if (t < nl) {
+      mg.tail = t;
+      mg.tail_lds = bytes;
+      TailArgs<T> &a = mg.targs;
+      a.nl = nl - t;
+      a.nc = (int)mg.L.back().n;
+      size_t off = 0;
+      for (int l = t; l < nl; ++l) {
+        a.s[l - t] = mg.L[l].s;
+        a.m[l - t] = mg.L[l].to_coarse;
+        a.off[l - t] = (int)off;
+        a.dn[l - t] = mg.L[l].dn;
+        off += 5 * (size_t)mg.L[l].n;
+      }
+      ECS_HIP(hipFuncSetAttribute((const void *)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)TAIL_LDS_MAX));
+    }
+  }
 }
 
 template <typename T>
@@ -461,10 +663,85 @@ static void mg_free(MG<T> &mg) {
   for (auto &L : mg.L) {
     (void)hipFree(L.dn);
     (void)hipFree(L.x);
+    (void)hipFree(L.x2);
     (void)hipFree(L.b);
     (void)hipFree(L.r);
   }
-  mg.L.clear();
+  (void)hipFree(mg.tmp0);
+  (void)hipFree(mg.ainv);
+  mg = MG<T>{};
+}
+
+// Dense inverse of the coarsest operator plus alpha 1 1^T (Gauss-Jordan with partial
+// pivoting in FP64 on the host), built whenever the operator is.
+template <typename T>
+static void mg_dense_coarse(ecsprt_hip_ctx *c, MG<T> &mg) {
+  Level<T> &L = mg.L.back();
+  const int n = (int)L.n;
+  if (n > DENSE_MAX) return;
+  std::vector<T> dnh(n);
+  ECS_HIP(hipMemcpyAsync(dnh.data(), L.dn, sizeof(T) * n, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipStreamSynchronize(c->stream));
+  const StencilConsts &s = L.s;
+  std::vector<double> A((size_t)n * n, 0.0), Inv((size_t)n * n, 0.0);
+  double diag = 0.0;
+  for (int k = 0; k < s.Nz; ++k)
+    for (int j = 0; j < s.Ny; ++j)
+      for (int i = 0; i < s.Nx; ++i) {
+        T cf[11];
+        row_coeffs(dnh.data(), s, i, j, k, cf);
+        const int ip = (i + 1) % s.Nx, im = (i + s.Nx - 1) % s.Nx, jp = (j + 1) % s.Ny, jm = (j + s.Ny - 1) % s.Ny;
+        const int kp = (k + 1) % s.Nz, km = (k + s.Nz - 1) % s.Nz;
+        auto id = [&](int a, int b2, int d) { return ((long)d * s.Ny + b2) * s.Nx + a; };
+        const long q = id(i, j, k);
+        double *row = &A[(size_t)q * n];
+        row[id(ip, j, k)] += cf[0];
+        row[id(im, j, k)] += cf[1];
+        row[id(i, jp, k)] += cf[2];
+        row[id(i, jm, k)] += cf[3];
+        row[id(i, j, kp)] += cf[4];
+        row[id(i, j, km)] += cf[5];
+        row[id(ip, jp, k)] += cf[6];
+        row[id(ip, jm, k)] += cf[7];
+        row[id(im, jp, k)] += cf[8];
+        row[id(im, jm, k)] += cf[9];
+        row[q] += cf[10];
+        diag += std::fabs((double)cf[10]);
+      }
+  const double alpha = diag > 0 ? diag / n / n : 1.0;
+  for (auto &v : A) v += alpha;
+  for (int q = 0; q < n; ++q) Inv[(size_t)q * n + q] = 1.0;
+  for (int col = 0; col < n; ++col) {
+    int piv = col;
+    for (int r = col + 1; r < n; ++r)
+      if (std::fabs(A[(size_t)r * n + col]) > std::fabs(A[(size_t)piv * n + col])) piv = r;
+    ECS_CHECK(A[(size_t)piv * n + col] != 0.0, ECSPRT_ERR_NOT_CONVERGED, "singular coarse operator");
+    if (piv != col)
+      for (int q = 0; q < n; ++q) {
+        std::swap(A[(size_t)piv * n + q], A[(size_t)col * n + q]);
+        std::swap(Inv[(size_t)piv * n + q], Inv[(size_t)col * n + q]);
+      }
+    const double d = 1.0 / A[(size_t)col * n + col];
+    for (int q = 0; q < n; ++q) {
+      A[(size_t)col * n + q] *= d;
+      Inv[(size_t)col * n + q] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == col) continue;
+      const double f = A[(size_t)r * n + col];
+      if (f == 0.0) continue;
+      for (int q = 0; q < n; ++q) {
+        A[(size_t)r * n + q] -= f * A[(size_t)col * n + q];
+        Inv[(size_t)r * n + q] -= f * Inv[(size_t)col * n + q];
+      }
+    }
+  }
+  std::vector<T> h((size_t)n * n);
+  for (size_t q = 0; q < h.size(); ++q) h[q] = (T)Inv[q];
+  if (!mg.ainv) ECS_HIP(hipMalloc(&mg.ainv, sizeof(T) * h.size()));
+  ECS_HIP(hipMemcpyAsync(mg.ainv, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
+  ECS_HIP(hipStreamSynchronize(c->stream));
+  mg.targs.ainv = mg.ainv;
 }
 
 // Level-0 density from the moments, restricted down the hierarchy.
@@ -474,12 +751,22 @@ static void mg_set_density(ecsprt_hip_ctx *c, MG<T> &mg) {
   k_density<T><<<gsz(mg.L[0].n), 256, 0, st>>>(c->d_moments, mg.L[0].dn, mg.L[0].n);
   ECS_HIP(hipGetLastError());
   for (size_t l = 0; l + 1 < mg.L.size(); ++l) {
-    k_restrict<T><<<gsz(mg.L[l + 1].n), 256, 0, st>>>(mg.L[l].to_coarse, mg.L[l].dn, mg.L[l + 1].dn);
+    const Launch3 g3 = launch3(mg.L[l + 1].s.Nx, mg.L[l + 1].s.Ny, mg.L[l + 1].s.Nz);
+    k_restrict<T><<<g3.grid, g3.block, 0, st>>>(mg.L[l].to_coarse, mg.L[l].dn, mg.L[l + 1].dn);
     ECS_HIP(hipGetLastError());
   }
+  mg_dense_coarse(c, mg);
 }
 
-// One V-cycle: z = M^-1 r (r, z on level 0).
+template <typename T>
+static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first) {
+  const int nt = ((L.s.Nx + SW_TX - 1) / SW_TX) * ((L.s.Ny + SW_TY - 1) / SW_TY) * ((L.s.Nz + SW_TZ - 1) / SW_TZ);
+  k_sweep<T, SW_TX, SW_TY, SW_TZ><<<nt, 512, 0, st>>>(L.s, L.dn, xin, b, xout, first);
+}
+
+// One V-cycle: z = M^-1 r (r, z on level 0).  Each level ping-pongs between two
+// buffers, starting on the one that leaves the last post-smoothing sweep in the
+// level's home buffer (z on level 0).
 template <typename T>
 static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
   hipStream_t st = c->stream;
@@ -487,39 +774,50 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
   const int post = c->o.mg_post > 0 ? c->o.mg_post : 2;
   const int csw = c->o.mg_coarse_sweeps > 0 ? c->o.mg_coarse_sweeps : 50;
   const int nl = (int)mg.L.size();
-  const T *b = r;
-  T *x = z;
-  for (int l = 0; l < nl; ++l) {
+  const int lt = mg.tail;
+  std::vector<T *> cur(nl, nullptr);
+  auto home = [&](int l) { return l == 0 ? z : mg.L[l].x; };
+  auto tmp = [&](int l) { return l == 0 ? mg.tmp0 : mg.L[l].x2; };
+  auto rhs = [&](int l) { return l == 0 ? r : (const T *)mg.L[l].b; };
+  for (int l = 0; l < std::min(lt, nl); ++l) {
     Level<T> &L = mg.L[l];
-    if (l > 0) {
-      b = L.b;
-      x = L.x;
+    const int total = l == nl - 1 ? csw : pre + post;
+    const int nsw = l == nl - 1 ? csw : pre;
+    const T *in = nullptr;
+    for (int k = 1; k <= nsw; ++k) {
+      T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
+      launch_sweep(st, L, in, rhs(l), out, 0);
+      in = out;
     }
-    if (l == nl - 1) {
-      if (L.n <= 4096) {
-        k_coarse_solve<T><<<1, 1024, 0, st>>>(L.s, L.dn, b, x, csw);
-      } else {
-        k_fill<T><<<gsz(L.n), 256, 0, st>>>(x, L.n, (T)0);
-        for (int it = 0; it < csw; ++it)
-          for (int col = 0; col < 2; ++col) k_smooth_rb<T><<<gsz(L.n), 256, 0, st>>>(L.s, L.dn, x, b, col);
-      }
-      ECS_HIP(hipGetLastError());
-      break;
-    }
-    k_fill<T><<<gsz(L.n), 256, 0, st>>>(x, L.n, (T)0);
-    for (int it = 0; it < pre; ++it)
-      for (int col = 0; col < 2; ++col) k_smooth_rb<T><<<gsz(L.n), 256, 0, st>>>(L.s, L.dn, x, b, col);
-    k_residual<T><<<gsz(L.n), 256, 0, st>>>(L.s, L.dn, x, b, L.r);
-    k_restrict<T><<<gsz(mg.L[l + 1].n), 256, 0, st>>>(L.to_coarse, L.r, mg.L[l + 1].b);
+    cur[l] = const_cast<T *>(in);
+    if (l == nl - 1) break;  // coarsest level without a tail: smoothed to convergence
+    const Launch3 gf = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
+    const Launch3 gc = launch3(mg.L[l + 1].s.Nx, mg.L[l + 1].s.Ny, mg.L[l + 1].s.Nz);
+    k_residual<T><<<gf.grid, gf.block, 0, st>>>(L.s, L.dn, in, rhs(l), L.r);
+    k_restrict<T><<<gc.grid, gc.block, 0, st>>>(L.to_coarse, L.r, mg.L[l + 1].b);
     ECS_HIP(hipGetLastError());
   }
-  for (int l = nl - 2; l >= 0; --l) {
+  if (lt < nl) {
+    TailArgs<T> a = mg.targs;
+    a.pre = pre;
+    a.post = post;
+    a.bin = rhs(lt);
+    a.xout = home(lt);
+    k_tail<T><<<1, TAIL_THREADS, mg.tail_lds, st>>>(a);
+    ECS_HIP(hipGetLastError());
+    cur[lt] = a.xout;
+  }
+  for (int l = std::min(lt, nl - 1) - 1; l >= 0; --l) {
     Level<T> &L = mg.L[l];
-    T *xl = l == 0 ? z : L.x;
-    const T *bl = l == 0 ? r : L.b;
-    k_prolong_add<T><<<gsz(L.n), 256, 0, st>>>(L.to_coarse, mg.L[l + 1].x, xl);
-    for (int it = 0; it < post; ++it)
-      for (int col = 1; col >= 0; --col) k_smooth_rb<T><<<gsz(L.n), 256, 0, st>>>(L.s, L.dn, xl, bl, col);
+    const Launch3 gf = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
+    k_prolong_add<T><<<gf.grid, gf.block, 0, st>>>(L.to_coarse, cur[l + 1], cur[l]);
+    const T *in = cur[l];
+    const int total = pre + post;
+    for (int k = pre + 1; k <= total; ++k) {
+      T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
+      launch_sweep(st, L, in, rhs(l), out, 1);
+      in = out;
+    }
     ECS_HIP(hipGetLastError());
   }
 }
@@ -531,7 +829,8 @@ static void precond(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
   if (pc == ECSPRT_PC_NONE) {
     ECS_HIP(hipMemcpyAsync(z, r, sizeof(T) * L.n, hipMemcpyDeviceToDevice, c->stream));
   } else if (pc == ECSPRT_PC_JACOBI) {
-    k_jacobi<T><<<gsz(L.n), 256, 0, c->stream>>>(L.s, L.dn, r, z);
+    const Launch3 g3 = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
+    k_jacobi<T><<<g3.grid, g3.block, 0, c->stream>>>(L.s, L.dn, r, z);
     ECS_HIP(hipGetLastError());
   } else {
     mg_vcycle(c, mg, r, z);
@@ -650,12 +949,13 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
   Level<T> &L0 = mg.L[0];
   const long n = L0.n;
   const int m = g.m;
+  const Launch3 g0 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1);
   int it = 0;
   *converged = false;
   double beta = 0.0;
   while (true) {
-    k_residual<T><<<gsz(n), 256, 0, st>>>(L0.s, L0.dn, x, b, g.r);
+    k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
     ECS_HIP(hipGetLastError());
     dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
     beta = std::sqrt(fetch1(c, g.dh, g.hpin));
@@ -673,7 +973,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
     bool done = false;
     for (; j < m && it < maxit; ++j, ++it) {
       precond<T>(c, mg, g.V[j], g.Z[j]);
-      k_apply<T><<<gsz(n), 256, 0, st>>>(L0.s, L0.dn, g.Z[j], g.w);
+      k_apply<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, g.Z[j], g.w);
       ECS_HIP(hipGetLastError());
       // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2.
       dots<T>(c, g, j + 1, g.dV, g.w, g.dh, 0);
@@ -727,7 +1027,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
     ECS_HIP(hipStreamSynchronize(st));
     (void)done;
     if (it >= maxit) {
-      k_residual<T><<<gsz(n), 256, 0, st>>>(L0.s, L0.dn, x, b, g.r);
+      k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
       dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
       rnorm = std::sqrt(fetch1(c, g.dh, g.hpin));
       *converged = rnorm <= tol;
@@ -795,7 +1095,8 @@ void solver_rhs(ecsprt_hip_ctx *c, double *b) {
   k.hxx = 1.0 / (p.dx * p.dx);
   k.hyy = 1.0 / (p.dy * p.dy);
   k.hzz = 1.0 / (p.dz * p.dz);
-  k_rhs<<<gsz(c->nnode), 256, 0, c->stream>>>(k, c->d_moments, b);
+  const Launch3 g3 = launch3(p.Nx, p.Ny, p.Nz);
+  k_rhs<<<g3.grid, g3.block, 0, c->stream>>>(k, c->d_moments, b);
   ECS_HIP(hipGetLastError());
   solver_remove_mean(c, b);
 }
@@ -803,7 +1104,8 @@ void solver_rhs(ecsprt_hip_ctx *c, double *b) {
 void solver_matmult(ecsprt_hip_ctx *c, const double *x, double *y) {
   ECS_CHECK(c->operator_built, ECSPRT_ERR_STATE, "operator not built (call ecsprt_hip_lhs)");
   Level<double> &L = c->solver->mg64.L[0];
-  k_apply<double><<<gsz(L.n), 256, 0, c->stream>>>(L.s, L.dn, x, y);
+  const Launch3 g3 = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
+  k_apply<double><<<g3.grid, g3.block, 0, c->stream>>>(L.s, L.dn, x, y);
   ECS_HIP(hipGetLastError());
 }
 
@@ -840,7 +1142,8 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
     const double inner = c->o.inner_rtol > 0 ? c->o.inner_rtol : 1e-4;
     Level<double> &L0 = s->mg64.L[0];
     for (int outer = 0; outer < 100; ++outer) {
-      k_residual<double><<<gsz(n), 256, 0, c->stream>>>(L0.s, L0.dn, x, b, s->r64);
+      const Launch3 g3 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
+      k_residual<double><<<g3.grid, g3.block, 0, c->stream>>>(L0.s, L0.dn, x, b, s->r64);
       ECS_HIP(hipGetLastError());
       dot_single<double>(c, s->g64.partial, s->r64, s->r64, n, s->g64.dh);
       rnorm = std::sqrt(fetch1(c, s->g64.dh, s->g64.hpin));
