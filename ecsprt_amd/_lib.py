@@ -111,6 +111,14 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_uint32)]),
     "ecsprt_hip_philox2x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
                                         ctypes.POINTER(ctypes.c_uint32)]),
+    "ecsprt_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(Options), ctypes.c_int, _I,
+                                               ctypes.c_int, ctypes.POINTER(_P)]),
+    "ecsprt_hip_batch_destroy": (ctypes.c_int, [_P]),
+    "ecsprt_hip_batch_solve": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "ecsprt_hip_batch_stats": (ctypes.c_int, [_P, _I, _D]),
+    "ecsprt_hip_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ecsprt_hip_lognormal_moments": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                                    ctypes.c_double, ctypes.c_double, ctypes.c_double, _P]),
 }
 
 _lib = None

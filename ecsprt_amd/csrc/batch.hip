@@ -1,0 +1,275 @@
+// Batched standalone potential solves.  Independent periodic grids of different
+// shapes are solved concurrently: a fixed pool of slots, each a solver context with
+// its own HIP stream and host thread (the Krylov loop decides convergence on the
+// host), takes problems round-robin.  Every slot carves its workspace from one
+// device arena sized for the largest problem, so the batch allocates nothing while
+// it runs and the slots never serialise on hipMalloc/hipFree.
+//
+// Also the device generator of the synthetic density fields: log-normal node values
+// smoothed by a periodic Gaussian and normalised to mean 1, with flux = scale * n.
+#include <algorithm>
+#include <cmath>
+#include <thread>
+
+#include "common.hpp"
+#include "philox.hpp"
+
+namespace ecsprt {
+
+// n = exp(sigma g - sigma^2 / 2), g ~ N(0, 1) per node: Box-Muller (cosine branch)
+// on two 53-bit uniforms from Philox4x32-10, counter (node, node >> 32, 0x106E, 0).
+__global__ void k_lognormal(double *out, long n, double sigma, uint32_t k0, uint32_t k1) {
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const u32x4 c = {{(uint32_t)q, (uint32_t)(q >> 32), 0x106Eu, 0u}};
+    const u32x4 r = philox4x32_10(c, k0, k1);
+    const double u1 = ((double)(r.v[0] >> 5) * 67108864.0 + (double)(r.v[1] >> 6) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = ((double)(r.v[2] >> 5) * 67108864.0 + (double)(r.v[3] >> 6)) * (1.0 / 9007199254740992.0);
+    const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    out[q] = exp(sigma * g - 0.5 * sigma * sigma);
+  }
+}
+
+// Periodic convolution along one axis with weights w[0..2R] (offsets -R..R).
+__global__ void k_smooth_axis(const double *__restrict__ a, double *__restrict__ b, int Nx, int Ny, int Nz, int axis,
+                              const double *__restrict__ w, int R) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
+  if (i >= Nx || j >= Ny) return;
+  const int N = axis == 0 ? Nx : (axis == 1 ? Ny : Nz);
+  const int at = axis == 0 ? i : (axis == 1 ? j : k);
+  const long stride = axis == 0 ? 1 : (axis == 1 ? (long)Nx : (long)Nx * Ny);
+  const long base = ((long)k * Ny + j) * Nx + i - (long)at * stride;
+  double acc = 0.0;
+  for (int d = -R; d <= R; ++d) {
+    int t = (at + d) % N;
+    if (t < 0) t += N;
+    acc += w[d + R] * a[base + (long)t * stride];
+  }
+  b[((long)k * Ny + j) * Nx + i] = acc;
+}
+
+__global__ void k_block_sums(const double *__restrict__ a, long n, double *__restrict__ partial) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) acc += a[q];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// FluidNode moments (n, G = fs n per component) with n normalised to mean 1; the sum
+// of the node values is the fixed-order sum of the block partial sums.
+__global__ void k_moments_from(const double *__restrict__ a, long n, const double *__restrict__ partial, int nb,
+                               double fs, double *__restrict__ m) {
+  double sum = 0.0;
+  for (int b = 0; b < nb; ++b) sum += partial[b];
+  const double inv = (double)n / sum;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    const double v = a[q] * inv;
+    m[q * 4] = v;
+    m[q * 4 + 1] = fs * v;
+    m[q * 4 + 2] = fs * v;
+    m[q * 4 + 3] = fs * v;
+  }
+}
+
+}  // namespace ecsprt
+
+using ecsprt::Error;
+
+struct ecsprt_hip_batch {
+  ecsprt_params p;
+  ecsprt_options o;
+  int device = 0;
+  std::vector<int> dims;
+  std::vector<ecsprt_hip_ctx *> slots;
+  std::vector<ecsprt::Arena> arenas;
+  std::vector<int> iters;
+  std::vector<double> rnorm;
+  std::string last_error;
+};
+
+namespace {
+
+void free_slot(ecsprt_hip_ctx *c, ecsprt::Arena &a) {
+  if (c && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  if (a.dev) (void)hipFree(a.dev);
+  if (a.host) (void)hipHostFree(a.host);
+  a = ecsprt::Arena{};
+}
+
+}  // namespace
+
+extern "C" {
+
+int ecsprt_hip_lognormal_moments(int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth,
+                                 double flux_scale, double *moments) {
+  if (Nx < 1 || Ny < 1 || Nz < 1 || !moments || !(smooth >= 0)) return ECSPRT_ERR_ARG;
+  try {
+    const long n = (long)Nx * Ny * Nz;
+    hipStream_t st;
+    ECS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    double *a = nullptr, *b = nullptr, *w = nullptr, *partial = nullptr;
+    const int R = smooth > 0 ? (int)std::ceil(4.0 * smooth) : 0;
+    std::vector<double> hw(2 * R + 1);
+    double ws = 0.0;
+    for (int d = -R; d <= R; ++d) ws += hw[d + R] = smooth > 0 ? std::exp(-0.5 * d * d / (smooth * smooth)) : 1.0;
+    for (auto &x : hw) x /= ws;
+    const int nb = 256;
+    hipError_t e = hipMalloc(&a, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc(&b, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc(&w, sizeof(double) * hw.size());
+    if (e == hipSuccess) e = hipMalloc(&partial, sizeof(double) * nb);
+    if (e == hipSuccess) e = hipMemcpyAsync(w, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+      const int g = (int)std::min<long>((n + 255) / 256, 65536);
+      ecsprt::k_lognormal<<<g, 256, 0, st>>>(a, n, sigma, (uint32_t)seed, (uint32_t)(seed >> 32));
+      const dim3 g3((Nx + 63) / 64, (Ny + 3) / 4, Nz), b3(64, 4, 1);
+      double *src = a, *dst = b;
+      for (int ax = 0; ax < 3 && R > 0; ++ax) {
+        ecsprt::k_smooth_axis<<<g3, b3, 0, st>>>(src, dst, Nx, Ny, Nz, ax, w, R);
+        std::swap(src, dst);
+      }
+      ecsprt::k_block_sums<<<nb, 256, 0, st>>>(src, n, partial);
+      ecsprt::k_moments_from<<<g, 256, 0, st>>>(src, n, partial, nb, flux_scale, moments);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(w);
+    (void)hipFree(partial);
+    (void)hipStreamDestroy(st);
+    return e == hipSuccess ? ECSPRT_OK : ECSPRT_ERR_HIP;
+  } catch (const Error &e) {
+    return e.code;
+  }
+}
+
+int ecsprt_hip_batch_create(const ecsprt_params *p, const ecsprt_options *o, int nprob, const int *dims, int nslots,
+                            ecsprt_hip_batch **out) {
+  if (!p || !o || !dims || !out || nprob <= 0 || nslots <= 0) return ECSPRT_ERR_ARG;
+  *out = nullptr;
+  auto *b = new ecsprt_hip_batch();
+  b->p = *p;
+  b->o = *o;
+  b->dims.assign(dims, dims + 3 * nprob);
+  long nmax = 0;
+  for (int q = 0; q < nprob; ++q) {
+    if (dims[3 * q] < 3 || dims[3 * q + 1] < 3 || dims[3 * q + 2] < 3) {
+      delete b;
+      return ECSPRT_ERR_ARG;
+    }
+    nmax = std::max(nmax, (long)dims[3 * q] * dims[3 * q + 1] * dims[3 * q + 2]);
+  }
+  nslots = std::min(nslots, nprob);
+  try {
+    ECS_HIP(hipGetDevice(&b->device));
+    b->arenas.resize(nslots);
+    b->slots.assign(nslots, nullptr);
+    const size_t dev_bytes = ecsprt::solver_arena_bytes(*o, nmax), host_bytes = 1 << 16;
+    for (int s = 0; s < nslots; ++s) {
+      auto *c = new ecsprt_hip_ctx();
+      b->slots[s] = c;
+      c->o = *o;
+      c->o.timing = 0;
+      c->device = b->device;
+      c->arena = &b->arenas[s];
+      ECS_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+      if (hipMalloc(&b->arenas[s].dev, dev_bytes) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "batch arena"};
+      b->arenas[s].dev_cap = dev_bytes;
+      ECS_HIP(hipHostMalloc((void **)&b->arenas[s].host, host_bytes, hipHostMallocDefault));
+      b->arenas[s].host_cap = host_bytes;
+    }
+  } catch (const Error &e) {
+    for (size_t s = 0; s < b->slots.size(); ++s) free_slot(b->slots[s], b->arenas[s]);
+    delete b;
+    return e.code;
+  }
+  b->iters.assign(nprob, 0);
+  b->rnorm.assign(nprob, 0.0);
+  *out = b;
+  return ECSPRT_OK;
+}
+
+int ecsprt_hip_batch_destroy(ecsprt_hip_batch *b) {
+  if (!b) return ECSPRT_OK;
+  for (size_t s = 0; s < b->slots.size(); ++s) free_slot(b->slots[s], b->arenas[s]);
+  delete b;
+  return ECSPRT_OK;
+}
+
+int ecsprt_hip_batch_solve(ecsprt_hip_batch *b, double *const *moments, double *const *phi) {
+  if (!b || !moments || !phi) return ECSPRT_ERR_ARG;
+  const int nprob = (int)b->iters.size(), nslots = (int)b->slots.size();
+  std::vector<int> rc(nslots, ECSPRT_OK);
+  std::vector<std::string> msg(nslots);
+  auto work = [&](int s) {
+    ecsprt_hip_ctx *c = b->slots[s];
+    try {
+      ECS_HIP(hipSetDevice(b->device));
+      for (int q = s; q < nprob && rc[s] == ECSPRT_OK; q += nslots) {
+        ecsprt_params &p = c->p;
+        p = b->p;
+        p.Nx = b->dims[3 * q];
+        p.Ny = b->dims[3 * q + 1];
+        p.Nz = b->dims[3 * q + 2];
+        p.Lx = p.Nx * p.dx;
+        p.Ly = p.Ny * p.dy;
+        p.Lz = p.Nz * p.dz;
+        p.x1 = p.x0 + p.Lx;
+        p.y1 = p.y0 + p.Ly;
+        p.z1 = p.z0 + p.Lz;
+        c->nnode = (long)p.Nx * p.Ny * p.Nz;
+        c->d_moments = moments[q];
+        c->d_phi = phi[q];
+        c->operator_built = false;
+        c->arena->reset();
+        const int r = ecsprt::solver_standalone(c);
+        b->iters[q] = c->last_iters;
+        b->rnorm[q] = c->last_rnorm;
+        if (r != ECSPRT_OK) {
+          rc[s] = r;
+          msg[s] = "problem " + std::to_string(q) + " (" + std::to_string(p.Nx) + "x" + std::to_string(p.Ny) + "x" +
+                   std::to_string(p.Nz) + "): KSP did not converge, ||r|| = " + std::to_string(c->last_rnorm) +
+                   " after " + std::to_string(c->last_iters) + " iterations";
+        }
+      }
+    } catch (const Error &e) {
+      rc[s] = e.code;
+      msg[s] = e.msg;
+    } catch (const std::exception &e) {
+      rc[s] = ECSPRT_ERR_STATE;
+      msg[s] = e.what();
+    }
+    c->d_moments = nullptr;
+    c->d_phi = nullptr;
+  };
+  std::vector<std::thread> th;
+  for (int s = 1; s < nslots; ++s) th.emplace_back(work, s);
+  work(0);
+  for (auto &t : th) t.join();
+  for (int s = 0; s < nslots; ++s)
+    if (rc[s] != ECSPRT_OK) {
+      b->last_error = msg[s];
+      return rc[s];
+    }
+  return ECSPRT_OK;
+}
+
+int ecsprt_hip_batch_stats(const ecsprt_hip_batch *b, int *iters, double *rnorm) {
+  if (!b) return ECSPRT_ERR_ARG;
+  for (size_t q = 0; q < b->iters.size(); ++q) {
+    if (iters) iters[q] = b->iters[q];
+    if (rnorm) rnorm[q] = b->rnorm[q];
+  }
+  return ECSPRT_OK;
+}
+
+const char *ecsprt_hip_batch_last_error(const ecsprt_hip_batch *b) { return b ? b->last_error.c_str() : ""; }
+
+}  // extern "C"

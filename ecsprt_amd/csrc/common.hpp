@@ -91,6 +91,15 @@ struct DriftConsts {
   double L[3], r0[3], r1[3];
 };
 
+// Bump allocators for solver workspace.  A context with an arena (the slots of a
+// batch) carves its per-problem buffers from one device and one pinned-host block,
+// so a batch allocates nothing between problems; reset() releases everything.
+struct Arena {
+  char *dev = nullptr, *host = nullptr;
+  size_t dev_cap = 0, dev_used = 0, host_cap = 0, host_used = 0;
+  void reset() { dev_used = host_used = 0; }
+};
+
 // Per-axis fold table: for global node i, up to FOLD_MAXP (tile index, local index) pairs.
 constexpr int FOLD_MAXP = 4;
 
@@ -151,6 +160,7 @@ struct ecsprt_hip_ctx {
 
   // solver
   ecsprt::Solver *solver = nullptr;
+  ecsprt::Arena *arena = nullptr;  // null: solver buffers come from hipMalloc
   bool operator_built = false;
   int last_iters = 0;
   double last_rnorm = 0.0, last_bnorm = 0.0;
@@ -190,5 +200,7 @@ void solver_matmult(ecsprt_hip_ctx *c, const double *x, double *y);
 void solver_pcapply(ecsprt_hip_ctx *c, const double *r, double *z);
 void solver_remove_mean(ecsprt_hip_ctx *c, double *v);
 int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x);
+size_t solver_arena_bytes(const ecsprt_options &o, long n);
+int solver_standalone(ecsprt_hip_ctx *c);
 
 }  // namespace ecsprt

@@ -14,6 +14,7 @@
 // and all reductions have a fixed order, so every solve is bitwise reproducible.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -529,6 +530,36 @@ static int gsz(long n) {
   return (int)std::max(1L, std::min(b, 8192L));
 }
 
+// Solver workspace: from the context's arena when it has one, else hipMalloc.
+template <typename P>
+static void sv_alloc(ecsprt_hip_ctx *c, P **p, size_t bytes) {
+  if (Arena *a = c->arena) {
+    const size_t off = (a->dev_used + 255) & ~(size_t)255;
+    ECS_CHECK(off + bytes <= a->dev_cap, ECSPRT_ERR_NOMEM, "solver arena exhausted");
+    *p = reinterpret_cast<P *>(a->dev + off);
+    a->dev_used = off + bytes;
+    return;
+  }
+  ECS_HIP(hipMalloc(p, bytes));
+}
+template <typename P>
+static void sv_host_alloc(ecsprt_hip_ctx *c, P **p, size_t bytes) {
+  if (Arena *a = c->arena) {
+    const size_t off = (a->host_used + 63) & ~(size_t)63;
+    ECS_CHECK(off + bytes <= a->host_cap, ECSPRT_ERR_NOMEM, "solver host arena exhausted");
+    *p = reinterpret_cast<P *>(a->host + off);
+    a->host_used = off + bytes;
+    return;
+  }
+  ECS_HIP(hipHostMalloc((void **)p, bytes, hipHostMallocDefault));
+}
+static void sv_free(ecsprt_hip_ctx *c, void *p) {
+  if (!c->arena && p) (void)hipFree(p);
+}
+static void sv_host_free(ecsprt_hip_ctx *c, void *p) {
+  if (!c->arena && p) (void)hipHostFree(p);
+}
+
 template <typename T>
 struct Level {
   StencilConsts s;
@@ -597,20 +628,26 @@ static void mg_create(ecsprt_hip_ctx *c, MG<T> &mg) {
     L.dx = dx;
     L.dy = dy;
     L.dz = dz;
-    ECS_HIP(hipMalloc(&L.dn, sizeof(T) * L.n));
-    ECS_HIP(hipMalloc(&L.x, sizeof(T) * L.n));
-    ECS_HIP(hipMalloc(&L.x2, sizeof(T) * L.n));
-    ECS_HIP(hipMalloc(&L.b, sizeof(T) * L.n));
-    ECS_HIP(hipMalloc(&L.r, sizeof(T) * L.n));
-    // Coarsening rule: strong axes (coupling >= half the strongest) that halve cleanly.
+    sv_alloc(c, &L.dn, sizeof(T) * L.n);
+    sv_alloc(c, &L.x, sizeof(T) * L.n);
+    sv_alloc(c, &L.x2, sizeof(T) * L.n);
+    sv_alloc(c, &L.b, sizeof(T) * L.n);
+    sv_alloc(c, &L.r, sizeof(T) * L.n);
+    // Coarsening rule: among the axes that halve cleanly (even, at least 4 nodes),
+    // those whose coupling is within a factor 2 of the strongest of them.
     const double a[3] = {1.0 / (dx * dx), 1.0 / (dy * dy), k2 / (dz * dz)};
     const int N[3] = {Nx, Ny, Nz};
-    const double amax = std::max(a[0], std::max(a[1], a[2]));
+    bool can[3];
+    double amax = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      can[d] = N[d] % 2 == 0 && N[d] >= 4;
+      if (can[d]) amax = std::max(amax, a[d]);
+    }
     int cf[3] = {0, 0, 0};
     int any = 0;
     for (int d = 0; d < 3; ++d)
-      if (a[d] >= 0.5 * amax && N[d] % 2 == 0 && N[d] >= 4) { cf[d] = 1; any = 1; }
-    if (!any || L.n <= 512) {
+      if (can[d] && a[d] >= 0.5 * amax) { cf[d] = 1; any = 1; }
+    if (!any || L.n <= 128) {
       mg.L.push_back(L);
       break;
     }
@@ -624,7 +661,7 @@ static void mg_create(ecsprt_hip_ctx *c, MG<T> &mg) {
     if (cf[1]) dy *= 2;
     if (cf[2]) dz *= 2;
   }
-  ECS_HIP(hipMalloc(&mg.tmp0, sizeof(T) * mg.L[0].n));
+  sv_alloc(c, &mg.tmp0, sizeof(T) * mg.L[0].n);
   // The tail: the coarsest levels whose five LDS arrays fit in one workgroup, when
   // the coarsest is small enough for a dense inverse.
   const int nl = (int)mg.L.size();
@@ -659,16 +696,16 @@ static void mg_create(ecsprt_hip_ctx *c, MG<T> &mg) {
 }
 
 template <typename T>
-static void mg_free(MG<T> &mg) {
+static void mg_free(ecsprt_hip_ctx *c, MG<T> &mg) {
   for (auto &L : mg.L) {
-    (void)hipFree(L.dn);
-    (void)hipFree(L.x);
-    (void)hipFree(L.x2);
-    (void)hipFree(L.b);
-    (void)hipFree(L.r);
+    sv_free(c, L.dn);
+    sv_free(c, L.x);
+    sv_free(c, L.x2);
+    sv_free(c, L.b);
+    sv_free(c, L.r);
   }
-  (void)hipFree(mg.tmp0);
-  (void)hipFree(mg.ainv);
+  sv_free(c, mg.tmp0);
+  sv_free(c, mg.ainv);
   mg = MG<T>{};
 }
 
@@ -738,7 +775,7 @@ static void mg_dense_coarse(ecsprt_hip_ctx *c, MG<T> &mg) {
   }
   std::vector<T> h((size_t)n * n);
   for (size_t q = 0; q < h.size(); ++q) h[q] = (T)Inv[q];
-  if (!mg.ainv) ECS_HIP(hipMalloc(&mg.ainv, sizeof(T) * h.size()));
+  if (!mg.ainv) sv_alloc(c, &mg.ainv, sizeof(T) * h.size());
   ECS_HIP(hipMemcpyAsync(mg.ainv, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c->stream));
   ECS_HIP(hipStreamSynchronize(c->stream));
   mg.targs.ainv = mg.ainv;
@@ -757,6 +794,9 @@ static void mg_set_density(ecsprt_hip_ctx *c, MG<T> &mg) {
   }
   mg_dense_coarse(c, mg);
 }
+
+template <typename T>
+static void remove_mean_dev(ecsprt_hip_ctx *c, double *partial, double *sumdst, T *v, long n);
 
 template <typename T>
 static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first) {
@@ -790,7 +830,11 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
       in = out;
     }
     cur[l] = const_cast<T *>(in);
-    if (l == nl - 1) break;  // coarsest level without a tail: smoothed to convergence
+    if (l == nl - 1) {  // coarsest level without a tail: smoothed, then made mean-free
+      double *partial = std::is_same<T, double>::value ? c->solver->g64.partial : c->solver->g32.partial;
+      remove_mean_dev<T>(c, partial, c->solver->sum, cur[l], L.n);
+      break;
+    }
     const Launch3 gf = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
     const Launch3 gc = launch3(mg.L[l + 1].s.Nx, mg.L[l + 1].s.Ny, mg.L[l + 1].s.Nz);
     k_residual<T><<<gf.grid, gf.block, 0, st>>>(L.s, L.dn, in, rhs(l), L.r);
@@ -837,42 +881,45 @@ static void precond(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
   }
 }
 
+// GMRES(m) workspace.  m = 0 keeps only the reduction buffers (the FP64 side of the
+// mixed-precision solve needs no Krylov basis).
 template <typename T>
-static void gmres_create(Gmres<T> &g, int m, long n) {
+static void gmres_create(ecsprt_hip_ctx *c, Gmres<T> &g, int m, long n) {
   g.m = m;
   g.n = n;
-  g.V.resize(m + 1);
-  g.Z.resize(m);
-  for (int q = 0; q <= m; ++q) ECS_HIP(hipMalloc(&g.V[q], sizeof(T) * n));
-  for (int q = 0; q < m; ++q) ECS_HIP(hipMalloc(&g.Z[q], sizeof(T) * n));
-  ECS_HIP(hipMalloc(&g.dV, sizeof(T *) * (m + 1)));
-  ECS_HIP(hipMalloc(&g.dZ, sizeof(T *) * m));
-  ECS_HIP(hipMemcpy(g.dV, g.V.data(), sizeof(T *) * (m + 1), hipMemcpyHostToDevice));
-  ECS_HIP(hipMemcpy(g.dZ, g.Z.data(), sizeof(T *) * m, hipMemcpyHostToDevice));
-  ECS_HIP(hipMalloc(&g.w, sizeof(T) * n));
-  ECS_HIP(hipMalloc(&g.r, sizeof(T) * n));
-  ECS_HIP(hipMalloc(&g.partial, sizeof(double) * RED_BLOCKS * (m + 2)));
-  ECS_HIP(hipMalloc(&g.dh, sizeof(double) * 2 * (m + 2)));
-  ECS_HIP(hipMalloc(&g.dy, sizeof(double) * (m + 1)));
-  ECS_HIP(hipHostMalloc((void **)&g.hpin, sizeof(double) * 2 * (m + 2), hipHostMallocDefault));
-  ECS_HIP(hipHostMalloc((void **)&g.ypin, sizeof(double) * (m + 1), hipHostMallocDefault));
+  if (m > 0) {
+    g.V.resize(m + 1);
+    g.Z.resize(m);
+    for (int q = 0; q <= m; ++q) sv_alloc(c, &g.V[q], sizeof(T) * n);
+    for (int q = 0; q < m; ++q) sv_alloc(c, &g.Z[q], sizeof(T) * n);
+    sv_alloc(c, &g.dV, sizeof(T *) * (m + 1));
+    sv_alloc(c, &g.dZ, sizeof(T *) * m);
+    ECS_HIP(hipMemcpyAsync(g.dV, g.V.data(), sizeof(T *) * (m + 1), hipMemcpyHostToDevice, c->stream));
+    ECS_HIP(hipMemcpyAsync(g.dZ, g.Z.data(), sizeof(T *) * m, hipMemcpyHostToDevice, c->stream));
+    ECS_HIP(hipStreamSynchronize(c->stream));
+    sv_alloc(c, &g.w, sizeof(T) * n);
+    sv_alloc(c, &g.r, sizeof(T) * n);
+  }
+  sv_alloc(c, &g.partial, sizeof(double) * RED_BLOCKS * (m + 2));
+  sv_alloc(c, &g.dh, sizeof(double) * 2 * (m + 2));
+  sv_alloc(c, &g.dy, sizeof(double) * (m + 1));
+  sv_host_alloc(c, &g.hpin, sizeof(double) * 2 * (m + 2));
+  sv_host_alloc(c, &g.ypin, sizeof(double) * (m + 1));
 }
 
 template <typename T>
-static void gmres_free(Gmres<T> &g) {
-  for (T *v : g.V) (void)hipFree(v);
-  for (T *v : g.Z) (void)hipFree(v);
-  g.V.clear();
-  g.Z.clear();
-  (void)hipFree(g.dV);
-  (void)hipFree(g.dZ);
-  (void)hipFree(g.w);
-  (void)hipFree(g.r);
-  (void)hipFree(g.partial);
-  (void)hipFree(g.dh);
-  (void)hipFree(g.dy);
-  if (g.hpin) (void)hipHostFree(g.hpin);
-  if (g.ypin) (void)hipHostFree(g.ypin);
+static void gmres_free(ecsprt_hip_ctx *c, Gmres<T> &g) {
+  for (T *v : g.V) sv_free(c, v);
+  for (T *v : g.Z) sv_free(c, v);
+  sv_free(c, g.dV);
+  sv_free(c, g.dZ);
+  sv_free(c, g.w);
+  sv_free(c, g.r);
+  sv_free(c, g.partial);
+  sv_free(c, g.dh);
+  sv_free(c, g.dy);
+  sv_host_free(c, g.hpin);
+  sv_host_free(c, g.ypin);
   g = Gmres<T>{};
 }
 
@@ -941,10 +988,12 @@ static double fetch1(ecsprt_hip_ctx *c, const double *d, double *pin) {
 
 // Right-preconditioned GMRES(m) with classical Gram-Schmidt and one
 // re-orthogonalisation; convergence on the true residual ||b - A x|| <= tol.
-// Returns iterations; rnorm receives the final true residual norm.
+// Returns iterations; rnorm receives the final true residual norm.  single_cycle: one
+// restart cycle stopped by the Arnoldi residual estimate (the inner solve of the
+// mixed-precision refinement, whose FP32 true residual has a rounding floor).
 template <typename T>
 static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, double tol, int maxit,
-                 double &rnorm, bool *converged) {
+                 double &rnorm, bool *converged, bool single_cycle = false) {
   hipStream_t st = c->stream;
   Level<T> &L0 = mg.L[0];
   const long n = L0.n;
@@ -1025,7 +1074,11 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
     if (j > 0) k_add_multi<T><<<gsz(n), 256, 0, st>>>(j, g.dZ, g.dy, x, n, 1.0);
     ECS_HIP(hipGetLastError());
     ECS_HIP(hipStreamSynchronize(st));
-    (void)done;
+    if (single_cycle) {
+      rnorm = std::fabs(gv[j]);
+      *converged = done;
+      break;
+    }
     if (it >= maxit) {
       k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
       dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
@@ -1043,14 +1096,15 @@ void solver_create(ecsprt_hip_ctx *c) {
   c->solver = s;
   mg_create<double>(c, s->mg64);
   const int m = c->o.ksp_restart > 0 ? c->o.ksp_restart : 30;
-  gmres_create<double>(s->g64, m, c->nnode);
-  ECS_HIP(hipMalloc(&s->r64, sizeof(double) * c->nnode));
-  ECS_HIP(hipMalloc(&s->sum, sizeof(double) * 4));
-  if (c->o.solve_precision == ECSPRT_SOLVE_MIXED) {
+  const bool mixed = c->o.solve_precision == ECSPRT_SOLVE_MIXED;
+  gmres_create<double>(c, s->g64, mixed ? 0 : m, c->nnode);
+  sv_alloc(c, &s->r64, sizeof(double) * c->nnode);
+  sv_alloc(c, &s->sum, sizeof(double) * 4);
+  if (mixed) {
     mg_create<float>(c, s->mg32);
-    gmres_create<float>(s->g32, m, c->nnode);
-    ECS_HIP(hipMalloc(&s->b32, sizeof(float) * c->nnode));
-    ECS_HIP(hipMalloc(&s->x32, sizeof(float) * c->nnode));
+    gmres_create<float>(c, s->g32, m, c->nnode);
+    sv_alloc(c, &s->b32, sizeof(float) * c->nnode);
+    sv_alloc(c, &s->x32, sizeof(float) * c->nnode);
     s->have32 = true;
   }
 }
@@ -1058,14 +1112,14 @@ void solver_create(ecsprt_hip_ctx *c) {
 void solver_destroy(ecsprt_hip_ctx *c) {
   Solver *s = c->solver;
   if (!s) return;
-  mg_free(s->mg64);
-  mg_free(s->mg32);
-  gmres_free(s->g64);
-  gmres_free(s->g32);
-  (void)hipFree(s->r64);
-  (void)hipFree(s->sum);
-  (void)hipFree(s->b32);
-  (void)hipFree(s->x32);
+  mg_free(c, s->mg64);
+  mg_free(c, s->mg32);
+  gmres_free(c, s->g64);
+  gmres_free(c, s->g32);
+  sv_free(c, s->r64);
+  sv_free(c, s->sum);
+  sv_free(c, s->b32);
+  sv_free(c, s->x32);
   delete s;
   c->solver = nullptr;
 }
@@ -1118,6 +1172,43 @@ void solver_remove_mean(ecsprt_hip_ctx *c, double *v) {
   remove_mean_dev<double>(c, c->solver->g64.partial, c->solver->sum, v, c->nnode);
 }
 
+// Upper bound of the solver workspace of an n-node grid (batch arenas are sized with
+// it): level buffers sum to at most 2n nodes per array, GMRES keeps 2m+3 vectors.
+size_t solver_arena_bytes(const ecsprt_options &o, long n) {
+  const int m = o.ksp_restart > 0 ? o.ksp_restart : 30;
+  const bool mixed = o.solve_precision == ECSPRT_SOLVE_MIXED;
+  auto mg = [&](size_t t) { return t * (11 * (size_t)n + (size_t)DENSE_MAX * DENSE_MAX); };
+  auto gm = [&](size_t t, int mm) {
+    return (mm > 0 ? t * (size_t)(2 * mm + 3) * n + 16 * (size_t)(2 * mm + 1) : 0) + 8 * (size_t)RED_BLOCKS * (mm + 2) +
+           8 * (size_t)(3 * mm + 5);
+  };
+  size_t b = mg(8) + gm(8, mixed ? 0 : m) + 8 * (size_t)n + 32 + 8 * (size_t)n;  // + r64, sum, rhs
+  if (mixed) b += mg(4) + gm(4, m) + 8 * (size_t)n;
+  return b + 256 * 512;  // alignment of every buffer
+}
+
+// One standalone solve on a context without particles: operator from c->d_moments,
+// the RHS of ComputeFullRHS3D, solution into c->d_phi.  The solver is built and
+// released around the solve (from the arena when the context has one).
+int solver_standalone(ecsprt_hip_ctx *c) {
+  double *rhs = nullptr;
+  sv_alloc(c, &rhs, sizeof(double) * c->nnode);
+  solver_create(c);
+  int rc = ECSPRT_OK;
+  try {
+    solver_build_operator(c);
+    solver_rhs(c, rhs);
+    rc = solver_solve(c, rhs, c->d_phi);
+  } catch (...) {
+    solver_destroy(c);
+    sv_free(c, rhs);
+    throw;
+  }
+  solver_destroy(c);
+  sv_free(c, rhs);
+  return rc;
+}
+
 // KSPSolve: A x = b.  Returns ECSPRT_OK or ECSPRT_ERR_NOT_CONVERGED.
 int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
   ECS_CHECK(c->operator_built, ECSPRT_ERR_STATE, "operator not built (call ecsprt_hip_lhs)");
@@ -1157,7 +1248,7 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
       k_fill<float><<<gsz(n), 256, 0, c->stream>>>(s->x32, n, 0.0f);
       double rn32 = 0.0;
       bool c32 = false;
-      iters += gmres<float>(c, s->mg32, s->g32, s->b32, s->x32, inner, maxit - iters, rn32, &c32);
+      iters += gmres<float>(c, s->mg32, s->g32, s->b32, s->x32, inner, maxit - iters, rn32, &c32, true);
       // x += rnorm * e, accumulated in FP64.
       k_convert<float, double><<<gsz(n), 256, 0, c->stream>>>(s->x32, s->r64, n, rnorm);
       k_axpy_from<double><<<gsz(n), 256, 0, c->stream>>>(s->r64, x, n, 1.0);

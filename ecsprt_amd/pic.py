@@ -252,3 +252,50 @@ def philox2x32_10(ctr, key):
     o = (ctypes.c_uint32 * 2)()
     _lib.load().ecsprt_hip_philox2x32_10(c, key, o)
     return list(o)
+
+
+def lognormal_moments(dims, seed, sigma=0.3, smooth=3.0, flux_scale=1.0):
+    """Device FluidNode moments of a smoothed log-normal density (mean 1) with flux =
+    flux_scale * n.  Returns the DeviceVector (4 doubles per node)."""
+    Nx, Ny, Nz = dims
+    dv = DeviceVector(4 * Nx * Ny * Nz)
+    _check(_lib.load().ecsprt_hip_lognormal_moments(Nx, Ny, Nz, seed, sigma, smooth, flux_scale, dv.ptr))
+    return dv
+
+
+class Batch:
+    """Batched standalone potential solves (one per grid in `dims`) on `slots`
+    concurrent streams; the physical parameters and solver options are shared."""
+
+    def __init__(self, params, options, dims, slots=8):
+        L = _lib.load()
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        flat = (ctypes.c_int * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        self.ptr = ctypes.c_void_p()
+        rc = L.ecsprt_hip_batch_create(ctypes.byref(params), ctypes.byref(options), len(self.dims), flat, slots,
+                                       ctypes.byref(self.ptr))
+        _check(rc)
+
+    def solve(self, moments, phi):
+        """moments, phi: sequences of DeviceVector (or device pointers) per problem."""
+        L = _lib.load()
+        n = len(self.dims)
+        mp = (ctypes.c_void_p * n)(*[getattr(m, "ptr", m) for m in moments])
+        pp = (ctypes.c_void_p * n)(*[getattr(f, "ptr", f) for f in phi])
+        rc = L.ecsprt_hip_batch_solve(self.ptr, mp, pp)
+        if rc != _lib.OK:
+            msg = L.ecsprt_hip_batch_last_error(self.ptr)
+            raise HipError(rc, msg.decode() if msg else "")
+
+    def stats(self):
+        n = len(self.dims)
+        it = np.zeros(n, dtype=np.int32)
+        rn = np.zeros(n)
+        _check(_lib.load().ecsprt_hip_batch_stats(self.ptr, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                  _dp(rn)))
+        return it, rn
+
+    def close(self):
+        if self.ptr:
+            _lib.load().ecsprt_hip_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()

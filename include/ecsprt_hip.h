@@ -235,6 +235,31 @@ int ecsprt_hip_d2d(void *dst, const void *src, long bytes, double *ms);
 void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void ecsprt_hip_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);
 
+/* ---- Batched standalone potential solves -------------------------------------------
+ * No counterpart entry point in the reference, which solves one field per KSPSolve
+ * (src/potential.c:11); each problem of a batch is that solve: the operator of
+ * ComputeFullLHS3D (src/lhs.c:337-505) from the problem's density, the RHS of
+ * ComputeFullRHS3D (src/rhs.c:259-373) with the mean removed, and the configured
+ * Krylov/multigrid solve (ksp_rtol etc. from the options).  Problems share the physical
+ * parameters of `p` and differ in grid shape (dims = Nx, Ny, Nz per problem). */
+typedef struct ecsprt_hip_batch ecsprt_hip_batch;
+int ecsprt_hip_batch_create(const ecsprt_params *p, const ecsprt_options *o, int nprob, const int *dims,
+                            int nslots, ecsprt_hip_batch **out);
+int ecsprt_hip_batch_destroy(ecsprt_hip_batch *b);
+/* moments[q]: device FluidNode [Nz][Ny][Nx][4] of problem q; phi[q]: device [Nz][Ny][Nx] that
+ * receives its potential.  Problems run concurrently on the batch's slots (streams). */
+int ecsprt_hip_batch_solve(ecsprt_hip_batch *b, double *const *moments, double *const *phi);
+/* Per-problem Krylov iterations and final residual norms of the last batch_solve. */
+int ecsprt_hip_batch_stats(const ecsprt_hip_batch *b, int *iters, double *rnorm);
+const char *ecsprt_hip_batch_last_error(const ecsprt_hip_batch *b);
+
+/* Synthetic density field on the device: n = exp(sigma g - sigma^2/2), g ~ N(0,1) per node
+ * (Philox4x32-10 keyed by seed), smoothed by a periodic Gaussian of `smooth` cells (radius
+ * ceil(4 smooth)), normalised to mean 1; written as FluidNode moments (n, s n, s n, s n) with
+ * s = flux_scale, as the reference's density input sets the flux (src/file-io.c:51-59). */
+int ecsprt_hip_lognormal_moments(int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth,
+                                 double flux_scale, double *moments);
+
 #ifdef __cplusplus
 }
 #endif
