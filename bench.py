@@ -34,8 +34,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3"],
-                    help="c2 = the c1 workload on the mixed-precision path only; c3 = batched solves")
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"],
+                    help="c2 = the c1 workload on the mixed-precision path only; c3 = batched solves; "
+                         "c4 = ensemble of 16 runs")
     ap.add_argument("--problems", type=int, default=256, help="c3: number of fields")
     ap.add_argument("--slots", type=int, default=8, help="c3: concurrent solver slots")
     ap.add_argument("--ppc", type=int, default=0, help="override ions per cell")
@@ -270,6 +271,83 @@ def bench_c3(args, pic, _lib):
     print(json.dumps(out))
 
 
+def c4_runs(n=16, seed=4):
+    """The ensemble: per run a cubic grid edge from {48, 64, 80, 96}, 16-64 ions per cell,
+    Omega_i dt in [0.02, 0.1], nu dt in [0.005, 0.05], seed r + 1."""
+    rng = np.random.default_rng(seed)
+    return [dict(N=int(rng.choice([48, 64, 80, 96])), ppc=int(rng.integers(16, 65)), B0=float(rng.uniform(0.02, 0.1)),
+                 nui=float(rng.uniform(0.005, 0.05)), seed=r + 1) for r in range(n)]
+
+
+def bench_c4(args, pic, _lib):
+    """16 independent PIC runs of uneven size, mixed precision (particles and solve, rtol
+    1e-10 as in the c1 workload), stepped concurrently (one stream and host thread per run)."""
+    from ecsprt_amd import configs
+    runs = c4_runs()
+    mixed = dict(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
+
+    def make(r, extra, collide=_lib.COLLIDE_PHILOX):
+        cli = configs.base_cli(r["N"], B0=r["B0"], nui=r["nui"], E0=0.05)
+        c = pic.Context(pic.setup_context(**cli), pic.default_options(ksp_rtol=1e-10, collide_mode=collide, **extra))
+        c.init_particles(r["ppc"] * r["N"] ** 3, r["seed"], 1e-3, 2)
+        c.collect()
+        c.compute_potential()
+        return c
+
+    ctxs = [make(r, mixed) for r in runs]
+    nps = [r["ppc"] * r["N"] ** 3 for r in runs]
+    nsteps = args.profile_steps or args.steps
+    for _ in range(0 if args.profile_steps else args.warmup):
+        pic.ensemble_step(ctxs, 1.0)
+    for c in ctxs:
+        c.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        pic.ensemble_step(ctxs, 1.0)
+    for c in ctxs:
+        c.synchronize()
+    el = time.perf_counter() - t0
+    if args.profile_steps:
+        print(json.dumps({"profile_steps": nsteps, "seconds": el}))
+        return
+    # the same runs stepped one after another, for the concurrency gain
+    t0 = time.perf_counter()
+    for _ in range(max(1, nsteps // 4)):
+        for c in ctxs:
+            c.step(1.0)
+    for c in ctxs:
+        c.synchronize()
+    el_seq = (time.perf_counter() - t0) / max(1, nsteps // 4)
+    for c in ctxs:
+        c.close()
+    # per-run error of one mixed step against the FP64 path from the same state
+    errs = []
+    for r in runs:
+        a = make(r, {}, _lib.COLLIDE_OFF)
+        b = make(r, mixed, _lib.COLLIDE_OFF)
+        a.step(1.0)
+        b.step(1.0)
+        ma, mb = a.get_moments(), b.get_moments()
+        fa, fb = a.get_potential(), b.get_potential()
+        errs.append({"N": r["N"], "ppc": r["ppc"], "n_rel_max": rel_err(mb[..., 0], ma[..., 0]),
+                     "G_rel_max": max(rel_err(mb[..., d], ma[..., d]) for d in (1, 2, 3)),
+                     "phi_rel_max": rel_err(fb, fa)})
+        a.close()
+        b.close()
+    total = sum(nps)
+    out = {
+        "metric": "aggregate particle-steps/s (c4: 16-run mixed-precision ensemble)",
+        "value": total * nsteps / el, "unit": "particle-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / nsteps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 particles, f64 fold, mixed solve", "data": "synthetic (device Philox initial conditions)",
+        "config": {"workload": "c4_ensemble", "runs": runs, "particles_total": total,
+                   "concurrency": "one stream and host thread per run"},
+        "sequential_ms_per_step": 1e3 * el_seq, "concurrency_speedup": el_seq / (el / nsteps),
+        "errors_vs_fp64_one_step": errs,
+    }
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -282,6 +360,9 @@ def main():
         return
     if args.workload == "c3":
         bench_c3(args, pic, _lib)
+        return
+    if args.workload == "c4":
+        bench_c4(args, pic, _lib)
         return
     wl = workload(args)
     p = pic.setup_context(**wl.cli)

@@ -272,4 +272,31 @@ int ecsprt_hip_batch_stats(const ecsprt_hip_batch *b, int *iters, double *rnorm)
 
 const char *ecsprt_hip_batch_last_error(const ecsprt_hip_batch *b) { return b ? b->last_error.c_str() : ""; }
 
+// One timestep of every context of an ensemble of independent runs (each created by
+// ecsprt_hip_create on this device, each with its own stream): the steps run
+// concurrently, one host thread per context, because each step's particle-count check,
+// collision-rate controller and Krylov loop read results back on the host.
+int ecsprt_hip_ensemble_step(ecsprt_hip_ctx *const *ctxs, int n, double dt) {
+  if (!ctxs || n <= 0) return ECSPRT_ERR_ARG;
+  for (int q = 0; q < n; ++q)
+    if (!ctxs[q]) return ECSPRT_ERR_ARG;
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) return ECSPRT_ERR_HIP;
+  std::vector<int> rc(n, ECSPRT_OK);
+  auto work = [&](int q) {
+    if (hipSetDevice(device) != hipSuccess) {
+      rc[q] = ECSPRT_ERR_HIP;
+      return;
+    }
+    rc[q] = ecsprt_hip_step(ctxs[q], dt);
+  };
+  std::vector<std::thread> th;
+  for (int q = 1; q < n; ++q) th.emplace_back(work, q);
+  work(0);
+  for (auto &t : th) t.join();
+  for (int q = 0; q < n; ++q)
+    if (rc[q] != ECSPRT_OK) return rc[q];
+  return ECSPRT_OK;
+}
+
 }  // extern "C"

@@ -263,6 +263,19 @@ def lognormal_moments(dims, seed, sigma=0.3, smooth=3.0, flux_scale=1.0):
     return dv
 
 
+def ensemble_step(contexts, dt):
+    """One timestep of every context, run concurrently (ecsprt_hip_ensemble_step)."""
+    n = len(contexts)
+    arr = (ctypes.c_void_p * n)(*[c.ptr.value for c in contexts])
+    rc = _lib.load().ecsprt_hip_ensemble_step(arr, n, dt)
+    if rc != _lib.OK:
+        for c in contexts:
+            msg = _lib.load().ecsprt_hip_last_error(c.ptr)
+            if msg:
+                raise HipError(rc, msg.decode())
+        raise HipError(rc, "")
+
+
 class Batch:
     """Batched standalone potential solves (one per grid in `dims`) on `slots`
     concurrent streams; the physical parameters and solver options are shared."""

@@ -253,6 +253,13 @@ int ecsprt_hip_batch_solve(ecsprt_hip_batch *b, double *const *moments, double *
 int ecsprt_hip_batch_stats(const ecsprt_hip_batch *b, int *iters, double *rnorm);
 const char *ecsprt_hip_batch_last_error(const ecsprt_hip_batch *b);
 
+/* ---- Ensembles of independent runs --------------------------------------------------
+ * One ecsprt_hip_step (the timestep of src/pic.c:246-258) on each of n contexts, run
+ * concurrently (one host thread per context, each context on its own stream).  No
+ * counterpart in the reference, which runs one simulation per process.  Returns the first
+ * non-zero code; each context's message is in its ecsprt_hip_last_error. */
+int ecsprt_hip_ensemble_step(ecsprt_hip_ctx *const *ctxs, int n, double dt);
+
 /* Synthetic density field on the device: n = exp(sigma g - sigma^2/2), g ~ N(0,1) per node
  * (Philox4x32-10 keyed by seed), smoothed by a periodic Gaussian of `smooth` cells (radius
  * ceil(4 smooth)), normalised to mean 1; written as FluidNode moments (n, s n, s n, s n) with

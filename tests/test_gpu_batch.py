@@ -135,3 +135,34 @@ def test_batch_rejects_bad_dims():
     p = c3_params()
     with pytest.raises(pic.HipError):
         pic.Batch(p, pic.default_options(), [(8, 8, 2)], slots=1)
+
+
+def test_ensemble_step_matches_sequential_steps():
+    """Concurrent ensemble steps of uneven runs equal the same runs stepped one by one."""
+    say("ensemble vs sequential")
+    runs = [(12, 6, _lib.PARTICLES_MIXED, 0.05), (16, 4, _lib.PARTICLES_FP64, 0.08), (8, 10, _lib.PARTICLES_MIXED, 0.02)]
+
+    def make(N, ppc, prec, B0, seed):
+        cli = configs.base_cli(N, B0=B0, nui=0.02)
+        o = pic.default_options(particle_precision=prec, ksp_rtol=1e-10, collide_mode=_lib.COLLIDE_PHILOX)
+        c = pic.Context(pic.setup_context(**cli), o)
+        c.init_particles(ppc * N ** 3, seed, 0.05, 1)
+        c.collect()
+        c.compute_potential()
+        return c
+
+    ens = [make(*r, seed=q + 1) for q, r in enumerate(runs)]
+    seq = [make(*r, seed=q + 1) for q, r in enumerate(runs)]
+    for _ in range(2):
+        pic.ensemble_step(ens, 1.0)
+        for c in seq:
+            c.step(1.0)
+    for a, b in zip(ens, seq):
+        pa, va = a.get_particles()
+        pb, vb = b.get_particles()
+        np.testing.assert_allclose(pa, pb, rtol=0, atol=1e-11)
+        np.testing.assert_allclose(va, vb, rtol=0, atol=1e-11)
+        np.testing.assert_allclose(a.get_moments(), b.get_moments(), rtol=1e-11, atol=1e-11)
+        assert a.collision_stats()[1] > 0
+    for c in ens + seq:
+        c.close()
