@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--collide", default="philox", choices=["philox", "off"])
     ap.add_argument("--mg-pre", type=int, default=0)
     ap.add_argument("--mg-post", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick the tile shape and block size of each particle path with ecsprt_amd.autotune")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many GiB (HBM counter calibration)")
     ap.add_argument("--profile-steps", type=int, default=0,
@@ -120,19 +122,27 @@ def copy_probe(gib, reps=5):
     return {"copy_bytes": n, "best_ms": best, "read_plus_write_GBs": 2 * n / (best * 1e-3) / 1e9}
 
 
-def committed_traffic(kernel, workload_name, args):
-    """HBM bytes per launch of `kernel` on this workload, from the calibrated rocprofv3
-    FETCH_SIZE/WRITE_SIZE runs summarised in profiles/hbm_traffic.md.  Only the default
-    launch configuration was profiled; any other gives None."""
-    default = not (args.tile or args.tile_z or args.block or args.split or args.ppc or args.grid or args.mg_pre
-                   or args.mg_post) and args.collide == "philox"
+def launch_of(p, o):
+    """Effective tile shape and block size of the particle pass (0 = the library's
+    choice: 8-cell tiles once a grid holds at least 512 of them, else 4)."""
+    n8 = ((p.Nx + 7) // 8) * ((p.Ny + 7) // 8) * ((p.Nz + 7) // 8)
+    t = o.get("tile") or (8 if n8 >= 512 else 4)
+    return f"{t}x{t}x{o.get('tile_z') or t}/{o.get('block') or 1024}"
+
+
+def committed_traffic(kernel, workload_name, launch, args):
+    """HBM bytes per launch of `kernel` on this workload and launch shape, from the
+    calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/hbm_traffic.md;
+    None for a configuration that was not profiled."""
+    if args.ppc or args.grid or args.mg_pre or args.mg_post or args.split or args.collide != "philox":
+        return None
     path = os.path.join(HERE, "profiles", "hbm_traffic.md")
-    if not default or not os.path.exists(path):
+    if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic:"):
             f = dict(kv.split("=", 1) for kv in line.split()[1:])
-            if f.get("kernel") == kernel and f.get("workload") == workload_name:
+            if f.get("kernel") == kernel and f.get("workload") == workload_name and f.get("launch") == launch:
                 return float(f["bytes_per_launch"])
     return None
 
@@ -376,6 +386,18 @@ def main():
     mixed_only = args.workload == "c2"
     if mixed_only:
         common.update(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
+    tuned = {}
+    if args.autotune:
+        from ecsprt_amd import autotune
+        for name, extra in (("fp64", {}), ("mixed", dict(particle_precision=_lib.PARTICLES_MIXED,
+                                                          solve_precision=_lib.SOLVE_MIXED))):
+            if (name == "fp64" and mixed_only) or (name == "mixed" and args.no_mixed and not mixed_only):
+                continue
+            o, table = autotune.tune_particles(p, pic.default_options(**dict(common, **extra)), wl.np, amp=wl.amp,
+                                               k=wl.k, dt=wl.dt)
+            tuned[name] = {"choice": [o.tile, o.tile_z, o.block], "kernel_ms": autotune.describe(table)}
+        key = "mixed" if mixed_only else "fp64"
+        common.update(tile=tuned[key]["choice"][0], tile_z=tuned[key]["choice"][1], block=tuned[key]["choice"][2])
     nsteps = args.profile_steps or args.steps
     warm = 0 if args.profile_steps else args.warmup
     ctx, r = run_timed(p, pic.default_options(**common), wl, nsteps, warm, pic)
@@ -391,7 +413,7 @@ def main():
     achieved = alg_bytes / (r["k_ms"] * 1e-3) / 1e9
     particle_ms = r["k_ms"] + r["pre_ms"] + r["rebin_ms"]
     kernel = "k_tile_mx" if mixed_only else "k_tile"
-    traffic = committed_traffic(kernel, wl.name, args)
+    traffic = committed_traffic(kernel, wl.name, launch_of(p, common), args)
     out = {
         "metric": "particle-steps/s (push+deposit+collide); phi-solve ms; mixed vs FP64 @1e-6 err",
         "value": npart * nsteps / r["seconds"],
@@ -418,12 +440,15 @@ def main():
                           "rhs": r["rhs_ms"], "solve": r["solve_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel + "<push|collide|drift|deposit|rebin>",
+                     "kernel": kernel + "<push|collide|drift|deposit|rebin>", "launch": launch_of(p, common),
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not args.no_mixed and not mixed_only:
         # The same workload on the mixed-precision path (FP32 particles, mixed solve).
         mcommon = dict(common, particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
+        if "mixed" in tuned:
+            mcommon.update(tile=tuned["mixed"]["choice"][0], tile_z=tuned["mixed"]["choice"][1],
+                           block=tuned["mixed"]["choice"][2])
         cm, rm = run_timed(p, pic.default_options(**mcommon), wl, nsteps, warm, pic)
         cm.close()
         malg = BYTES_PER_PARTICLE_MIXED * npart + BYTES_PER_NODE * ncell
@@ -438,10 +463,13 @@ def main():
             "solve_speedup_vs_fp64": (r["solve_ms"] + r["rhs_ms"]) / max(rm["solve_ms"] + rm["rhs_ms"], 1e-9),
             "ksp_iterations_mean": rm["iters"],
             "roofline": {"bound": "hbm", "achieved": mach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": mach / HBM_PEAK_GBS, "traffic": committed_traffic("k_tile_mx", wl.name, args),
-                         "kernel": "k_tile_mx", "algorithmic_bytes_per_launch": malg},
+                         "frac": mach / HBM_PEAK_GBS,
+                         "traffic": committed_traffic("k_tile_mx", wl.name, launch_of(p, mcommon), args),
+                         "kernel": "k_tile_mx", "launch": launch_of(p, mcommon), "algorithmic_bytes_per_launch": malg},
             "errors_vs_fp64": mixed_errors(p, wl, pic, _lib),
         }
+    if tuned:
+        out["autotune"] = tuned
     if not args.no_cpu:
         cb = cpu_baseline(wl)
         cb.pop("rc", None)

@@ -26,6 +26,7 @@ struct TileArgs {
   unsigned long long *ctr;
   const double *phi;
   double *fold, *ovf;
+  int *late, *late_cnt;  // per-tile lists of ions whose deposit waits for k_late
   PushConsts pc;
   DriftConsts dc;
   CollideConsts cc;
@@ -45,15 +46,43 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
 
 // Place a particle whose destination slot was reserved: into the destination tile
 // when there is room, else into the spill list (the host re-bins after the step).
-__device__ inline void place(const TileArgs &a, int dt, int slot, double x, double y, double z, double vx,
-                             double vy, double vz, int id) {
+// Returns the ion's slot in the destination arrays, or -1 when it went to the spill list.
+__device__ inline long place(const TileArgs &a, int dt, int slot, double x, double y, double z, double vx, double vy,
+                             double vz, int id) {
   if (slot < a.cap[dt]) {
-    store_particle(a.dst, (long)a.base[dt] + slot, x, y, z, vx, vy, vz, id);
-  } else {
-    const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
-    if ((long)sp < a.scap) store_particle(a.spill, (long)sp, x, y, z, vx, vy, vz, id);
-    else atomicAdd(&a.ctr[2], 1ull);
+    const long s = (long)a.base[dt] + slot;
+    store_particle(a.dst, s, x, y, z, vx, vy, vz, id);
+    return s;
   }
+  const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
+  if ((long)sp < a.scap) store_particle(a.spill, (long)sp, x, y, z, vx, vy, vz, id);
+  else atomicAdd(&a.ctr[2], 1ull);
+  return -1;
+}
+
+// Deposit of one ion with global FP64 atomics (Collect3DFluidMoments weights).
+__device__ inline void deposit_direct(const TileArgs &a, double px, double py, double pz, double vx, double vy,
+                                      double vz) {
+  const double x = cell_coord(px, a.pc.dx, a.pc.inv[0], a.pc.pow2);
+  const double y = cell_coord(py, a.pc.dy, a.pc.inv[1], a.pc.pow2);
+  const double z = cell_coord(pz, a.pc.dz, a.pc.inv[2], a.pc.pow2);
+  const int ixl = (int)x, iyl = (int)y, izl = (int)z;
+  const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+  const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+  const double wzh = z - (double)izl, wzl = 1.0 - wzh;
+  const double w8[8] = {wzh * wyh * wxh, wzl * wyh * wxh, wzh * wyl * wxh, wzl * wyl * wxh,
+                        wzh * wyh * wxl, wzl * wyh * wxl, wzh * wyl * wxl, wzl * wyl * wxl};
+  deposit_overflow(a.ovf, a.g, ixl, iyl, izl, w8, vx, vy, vz);
+}
+
+// An ion that left its source tile's deposit region is, once placed, inside its
+// destination tile: list its slot there (one atomic) for k_late, or deposit it now
+// when the tile's list is full.
+__device__ inline void defer_deposit(const TileArgs &a, int dt, long s, double px, double py, double pz, double vx,
+                                     double vy, double vz) {
+  const int li = atomicAdd(&a.late_cnt[dt], 1);
+  if (li < (a.cap[dt] >> 2)) a.late[(a.base[dt] >> 2) + li] = (int)s;
+  else deposit_direct(a, px, py, pz, vx, vy, vz);
 }
 
 // Per-workgroup tile constants and LDS pointers used by the particle tail.
@@ -73,7 +102,7 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
   const TileGeom &g = a.g;
   const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
   const int tid = threadIdx.x;
-  bool lost = false;
+  bool lost = false, late = false;
   if ((MODE & M_DRIFT) && valid) {
     // UpdatePositions drift (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28).
     px += vx * a.dc.dt;
@@ -128,6 +157,8 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
         lds_add(&w.sacc[2 * RA3 + idx], vy * wt);
         lds_add(&w.sacc[3 * RA3 + idx], vz * wt);
       }
+    } else if (MODE & M_REBIN) {
+      late = true;
     } else {
       deposit_overflow(a.ovf, g, ixl, iyl, izl, w8, vx, vy, vz);
     }
@@ -155,10 +186,15 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     }
     __syncthreads();
     if (code < 27) {
-      place(a, w.s_nbr[code], bas[code] + rank, px, py, pz, vx, vy, vz, pid);
+      const long s = place(a, w.s_nbr[code], bas[code] + rank, px, py, pz, vx, vy, vz, pid);
+      if (late) {
+        if (s >= 0) defer_deposit(a, w.s_nbr[code], s, px, py, pz, vx, vy, vz);
+        else deposit_direct(a, px, py, pz, vx, vy, vz);
+      }
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
+      if (late) deposit_direct(a, px, py, pz, vx, vy, vz);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
     w.par ^= 1;
@@ -175,7 +211,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   extern __shared__ double smem[];
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
-  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef;
+  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef, s_n3;
   __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
@@ -226,7 +262,10 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     s_nbr[tid] = (wrap_index(w.tz + oz, g.ntz) * g.nty + wrap_index(w.ty + oy, g.nty)) * g.ntx +
                  wrap_index(w.tx + ox, g.ntx);
   }
-  if (tid == 0) s_ndef = 0;
+  if (tid == 0) {
+    s_ndef = 0;
+    s_n3 = 0;
+  }
   if ((MODE & M_COLLIDE) && tid < 24) s_cdf[tid] = a.pt.cdf[tid];
   __syncthreads();
 
@@ -293,12 +332,16 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
-    // Phase 2: the deferred ions, collided then drifted, deposited and re-binned.
+    // Phase 2: the deferred ions, first attempt each (every lane busy), then drifted,
+    // deposited and re-binned.  Ions with more attempts are parked after the deferred
+    // ones in this workgroup's slots of D and finish in phase 3.
     __syncthreads();
     const int ndef = s_ndef;
+    const long room = (end - beg) - ndef;
     for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
       const int q = q0 + tid;
       const bool valid = q < ndef;
+      bool done = valid;
       double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
       int pid = 0;
       if (valid) {
@@ -311,7 +354,38 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
         vz = a.def.vz[d];
         pid = a.def.id[d];
         const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
-        collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+        collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
+        if (k > 1) {
+          const int r = atomicAdd(&s_n3, 1);
+          if (r < room) {
+            store_particle(a.def, beg + ndef + r, px, py, pz, vx, vy, vz, pid);
+            done = false;
+          } else {
+            collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
+          }
+        }
+      }
+      particle_tail<MODE, TX, TY, TZ, HH>(a, w, done, 0, px, py, pz, vx, vy, vz, pid);
+    }
+    // Phase 3: the remaining attempts of the parked ions.
+    __syncthreads();
+    const int n3 = (int)min((long)s_n3, room);
+    for (int q0 = 0; q0 < n3; q0 += blockDim.x) {
+      const int q = q0 + tid;
+      const bool valid = q < n3;
+      double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
+      int pid = 0;
+      if (valid) {
+        const long d = beg + ndef + q;
+        px = a.def.x[d];
+        py = a.def.y[d];
+        pz = a.def.z[d];
+        vx = a.def.vx[d];
+        vy = a.def.vy[d];
+        vz = a.def.vz[d];
+        pid = a.def.id[d];
+        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
       particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
     }
@@ -326,6 +400,63 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     __syncthreads();
     double *out = a.fold + (size_t)wi * 4 * w.RA3;
     for (int q = tid; q < 4 * w.RA3; q += blockDim.x) out[q] = w.sacc[q];
+  }
+}
+
+// Deposit of the ions listed for tile t: those that arrived from beyond their source
+// tile's deposit region.  They now lie in tile t, so FP64 LDS accumulators over its
+// (T+1)^3 nodes take them, and the region is added into the overflow moments with one
+// global atomic per non-zero node and moment.  Resets the tile's list.
+template <int TX, int TY, int TZ>
+__global__ void __launch_bounds__(256) k_late(TileGeom g, ParticlesF64 P, const int *__restrict__ late,
+                                             int *__restrict__ late_cnt, const int *__restrict__ base,
+                                             const int *__restrict__ cap, PushConsts pc, double *__restrict__ ovf) {
+  constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
+  __shared__ double acc[4 * N3];
+  const int t = blockIdx.x;
+  const int n = min(late_cnt[t], cap[t] >> 2);
+  __syncthreads();
+  if (threadIdx.x == 0) late_cnt[t] = 0;
+  if (n == 0) return;
+  const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
+  for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0.0;
+  __syncthreads();
+  const int *list = late + (base[t] >> 2);
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const long s = list[q];
+    const double px = P.x[s], py = P.y[s], pz = P.z[s], vx = P.vx[s], vy = P.vy[s], vz = P.vz[s];
+    const double x = cell_coord(px, pc.dx, pc.inv[0], pc.pow2);
+    const double y = cell_coord(py, pc.dy, pc.inv[1], pc.pow2);
+    const double z = cell_coord(pz, pc.dz, pc.inv[2], pc.pow2);
+    const int ixl = (int)x, iyl = (int)y, izl = (int)z;
+    const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+    const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+    const double wzh = z - (double)izl, wzl = 1.0 - wzh;
+    const double w8[8] = {wzh * wyh * wxh, wzl * wyh * wxh, wzh * wyl * wxh, wzl * wyl * wxh,
+                          wzh * wyh * wxl, wzl * wyh * wxl, wzh * wyl * wxl, wzl * wyl * wxl};
+    const int la = ixl - i0, lb = iyl - j0, lc = izl - k0;
+    if (la < 0 || la >= TX || lb < 0 || lb >= TY || lc < 0 || lc >= TZ) {
+      deposit_overflow(ovf, g, ixl, iyl, izl, w8, vx, vy, vz);  // a coordinate on the upper face
+      continue;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+      const int idx = ((lc + az) * NY + (lb + ay)) * NX + (la + ax);
+      lds_add(&acc[idx], w8[c]);
+      lds_add(&acc[N3 + idx], vx * w8[c]);
+      lds_add(&acc[2 * N3 + idx], vy * w8[c]);
+      lds_add(&acc[3 * N3 + idx], vz * w8[c]);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int li = q % NX, lj = (q / NX) % NY, lk = q / (NX * NY);
+    const long node = ((long)wrap_near(k0 + lk, g.Nz) * g.Ny + wrap_near(j0 + lj, g.Ny)) * g.Nx + wrap_near(i0 + li, g.Nx);
+    for (int d = 0; d < 4; ++d) {
+      const double v = acc[d * N3 + q];
+      if (v != 0.0) unsafeAtomicAdd(&ovf[node * 4 + d], v);
+    }
   }
 }
 
@@ -363,7 +494,7 @@ __global__ void k_finalize_counts(int ntiles, int *count, int *next, const int *
 // regions: the integer sum is exact, then scaled to weight (sn) and flux (sg) units.
 template <typename R>
 __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__restrict__ ovf,
-                       const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m, double sn,
+                       const long long *__restrict__ ovfq, const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m, double sn,
                        double sg) {
   const long nnode = (long)g.Nx * g.Ny * g.Nz;
   const int RAX = g.RAx, RAY = g.RAy, RA3 = g.RAx * g.RAy * g.RAz;
@@ -389,6 +520,7 @@ __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__r
           }
         }
     if constexpr (std::is_integral<R>::value) {
+      for (int d = 0; d < 4; ++d) acc[d] += ovfq[node * 4 + d];
       m[node * 4 + 0] = (double)acc[0] * sn + ovf[node * 4 + 0];
       for (int d = 1; d < 4; ++d) m[node * 4 + d] = (double)acc[d] * sg + ovf[node * 4 + d];
     } else {
@@ -547,6 +679,13 @@ void fill_i32(ecsprt_hip_ctx *c, int *p, long n, int v) {
   k_fill_i32<<<grid_for(n, 256), 256, 0, c->stream>>>(p, n, v);
   ECS_HIP(hipGetLastError());
 }
+
+// Overflow moments start each deposit at zero: FP64 atomics, and on the mixed path
+// the 64-bit fixed-point sums of listed ions (all-zero bits in both).
+static void clear_overflow(ecsprt_hip_ctx *c) {
+  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  if (c->d_ovfq) fill_f64(c, reinterpret_cast<double *>(c->d_ovfq), 4 * c->nnode, 0.0);
+}
 void zero_ctr(ecsprt_hip_ctx *c) {
   k_fill_u64<<<1, 64, 0, c->stream>>>(c->d_ctr, 8);
   ECS_HIP(hipGetLastError());
@@ -696,9 +835,12 @@ void tiles_init(ecsprt_hip_ctx *c) {
   ECS_HIP(hipMalloc(&c->d_cap, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_count, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_next, sizeof(int) * c->ntiles));
+  ECS_HIP(hipMalloc(&c->d_late_cnt, sizeof(int) * c->ntiles));
+  fill_i32(c, c->d_late_cnt, c->ntiles, 0);
+  if (c->mixed) ECS_HIP(hipMalloc(&c->d_ovfq, sizeof(long long) * 4 * c->nnode));
   fill_i32(c, c->d_count, c->ntiles, 0);
   fill_i32(c, c->d_next, c->ntiles, 0);
-  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  clear_overflow(c);
   zero_ctr(c);
   set_lds_attrs<M_PUSH>();
   set_lds_attrs<M_DEPOSIT>();
@@ -723,6 +865,9 @@ void tiles_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_cap);
   (void)hipFree(c->d_count);
   (void)hipFree(c->d_next);
+  (void)hipFree(c->d_late_cnt);
+  (void)hipFree(c->d_late);
+  (void)hipFree(c->d_ovfq);
   (void)hipFree(c->d_foldtab);
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_key);
@@ -763,6 +908,9 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
       mx_alloc(c->Bm, total);
       if (philox) mx_alloc(c->Dm, total);
       c->pcap = total;
+      (void)hipFree(c->d_late);
+      c->d_late = nullptr;
+      ECS_HIP(hipMalloc(&c->d_late, sizeof(int) * (total / 4)));
     }
     if (!c->Em.p || c->np != np) {
       mx_alloc(c->Em, np);
@@ -775,6 +923,9 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
       alloc_particles(c->B, total);
       if (philox) alloc_particles(c->D, total);
       c->pcap = total;
+      (void)hipFree(c->d_late);
+      c->d_late = nullptr;
+      ECS_HIP(hipMalloc(&c->d_late, sizeof(int) * (total / 4)));
     }
     if (!c->E.x || c->np != np) {
       alloc_particles(c->E, np);
@@ -837,6 +988,8 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
   a.phi = c->d_phi;
   a.fold = c->d_fold;
   a.ovf = c->d_ovf;
+  a.late = c->d_late;
+  a.late_cnt = c->d_late_cnt;
   a.pc = push_consts(c->p, 0.0);
   return a;
 }
@@ -918,22 +1071,42 @@ void run_drift_rebin(ecsprt_hip_ctx *c, double dt) {
   check_after_rebin(c);
 }
 
+// Deposits of the ions listed by the fused pass (the re-binned arrays are now c->A).
+static void launch_late(ecsprt_hip_ctx *c) {
+  if (c->mixed) {
+    mx_late(c);
+    return;
+  }
+  const PushConsts pc = push_consts(c->p, 0.0);
+  const int tx = c->g.Tx, tz = c->g.Tz;
+  if (tx == 8 && tz == 8)
+    k_late<8, 8, 8><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_late, c->d_late_cnt, c->d_base, c->d_cap, pc,
+                                                      c->d_ovf);
+  else if (tx == 8 && tz == 4)
+    k_late<8, 8, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_late, c->d_late_cnt, c->d_base, c->d_cap, pc,
+                                                      c->d_ovf);
+  else
+    k_late<4, 4, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_late, c->d_late_cnt, c->d_base, c->d_cap, pc,
+                                                      c->d_ovf);
+  ECS_HIP(hipGetLastError());
+}
+
 static void launch_fold(ecsprt_hip_ctx *c) {
   if (c->mixed) {
     const double vmax = mx_vmax(c->p);
     const double sc[4] = {0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30};
     k_fold<long long><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(
-        c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_foldtab, c->foldtab_off[0],
+        c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_ovfq, c->d_foldtab, c->foldtab_off[0],
         c->foldtab_off[1], c->foldtab_off[2], c->d_moments, sc[0], sc[1]);
   } else
-    k_fold<double><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, c->d_foldtab,
+    k_fold<double><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, nullptr, c->d_foldtab,
                                                                    c->foldtab_off[0], c->foldtab_off[1],
                                                                    c->foldtab_off[2], c->d_moments, 1.0, 1.0);
   ECS_HIP(hipGetLastError());
 }
 
 void run_deposit(ecsprt_hip_ctx *c) {
-  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  clear_overflow(c);
   if (c->mixed) {
     mx_launch(c, M_DEPOSIT, 0.0, nullptr, 0, 0, 0);
   } else {
@@ -963,7 +1136,7 @@ CollideConsts collide_consts_of(const ecsprt_params &p) { return collide_consts(
 
 void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
   zero_ctr(c);
-  fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
+  clear_overflow(c);
   TileArgs a = make_args(c);
   a.pc = push_consts(c->p, dt);
   a.dc = drift_consts(c->p, dt);
@@ -988,6 +1161,7 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
   }
   if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[1], c->stream));
   finish_rebin(c);
+  launch_late(c);
   launch_fold(c);
   if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[2], c->stream));
   check_after_rebin(c);

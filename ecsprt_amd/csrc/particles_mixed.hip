@@ -41,6 +41,7 @@ struct TileArgsMx {
   const double *phi;
   long long *fold;
   double *ovf;
+  int *late, *late_cnt;  // per-tile lists of ions whose deposit waits for k_late_mx
   MxConsts k;
   CollideConsts cc;
   uint32_t key0, key1, step;
@@ -97,15 +98,54 @@ __device__ inline bool drift_axis(float &o, int &i, float d, int N) {
   return n >= 0 && n < N;
 }
 
-__device__ inline void place_mx(const TileArgsMx &a, int dt, int slot, uint32_t cell, float ox, float oy, float oz,
+// Stores an ion in tile dt; returns its slot, or -1 when the tile is full and the ion
+// went to the spill list.
+__device__ inline long place_mx(const TileArgsMx &a, int dt, int slot, uint32_t cell, float ox, float oy, float oz,
                                 float vx, float vy, float vz, int id) {
   if (slot < a.cap[dt]) {
-    store_mx(a.dst, (long)a.base[dt] + slot, cell, ox, oy, oz, vx, vy, vz, id);
-  } else {
-    const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
-    if ((long)sp < a.scap) store_mx(a.spill, (long)sp, cell, ox, oy, oz, vx, vy, vz, id);
-    else atomicAdd(&a.ctr[2], 1ull);
+    const long s = (long)a.base[dt] + slot;
+    store_mx(a.dst, s, cell, ox, oy, oz, vx, vy, vz, id);
+    return s;
   }
+  const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
+  if ((long)sp < a.scap) store_mx(a.spill, (long)sp, cell, ox, oy, oz, vx, vy, vz, id);
+  else atomicAdd(&a.ctr[2], 1ull);
+  return -1;
+}
+
+// Collect3DFluidMoments weights (src/moments.c:164-184) from the in-cell offsets,
+// scaled by 2^30 through the z factor (exact: a power of two).
+__device__ inline void weights_mx(float ox, float oy, float oz, float w8[8]) {
+  const float wxh = ox, wxl = 1.0f - ox;
+  const float wyh = oy, wyl = 1.0f - oy;
+  const float wzh = oz * MX_WSCALE, wzl = (1.0f - oz) * MX_WSCALE;
+  w8[0] = wzh * wyh * wxh;
+  w8[1] = wzl * wyh * wxh;
+  w8[2] = wzh * wyl * wxh;
+  w8[3] = wzl * wyl * wxh;
+  w8[4] = wzh * wyh * wxl;
+  w8[5] = wzl * wyh * wxl;
+  w8[6] = wzh * wyl * wxl;
+  w8[7] = wzl * wyl * wxl;
+}
+
+// Deposit through FP64 global atomics on the overflow moments.
+__device__ inline void deposit_direct_mx(const TileArgsMx &a, int ix, int iy, int iz, float ox, float oy, float oz,
+                                         float vx, float vy, float vz) {
+  float w8[8];
+  weights_mx(ox, oy, oz, w8);
+  double wd[8];
+  for (int c = 0; c < 8; ++c) wd[c] = (double)w8[c] * (1.0 / 1073741824.0);
+  deposit_overflow(a.ovf, a.g, ix, iy, iz, wd, (double)vx, (double)vy, (double)vz);
+}
+
+// An ion beyond its source tile's deposit region, now stored at slot s of tile dt:
+// list it for k_late_mx (one atomic), or deposit it directly when the list is full.
+__device__ inline void defer_deposit_mx(const TileArgsMx &a, int dt, long s, int ix, int iy, int iz, float ox,
+                                        float oy, float oz, float vx, float vy, float vz) {
+  const int li = atomicAdd(&a.late_cnt[dt], 1);
+  if (li < (a.cap[dt] >> 2)) a.late[(a.base[dt] >> 2) + li] = (int)s;
+  else deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
 }
 
 struct TileWGMx {
@@ -123,7 +163,7 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
                                float oy, float oz, float vx, float vy, float vz, int pid) {
   const TileGeom &g = a.g;
   const int tid = threadIdx.x;
-  bool lost = false;
+  bool lost = false, late = false;
   if ((MODE & M_DRIFT) && valid) {
     // UpdatePositions (src/positions.c:455-459) with the wrap of src/boundaries.c:19-28.
     const bool okx = drift_axis(ox, ix, vx * a.k.dtc[0], g.Nx);
@@ -132,20 +172,8 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     lost = !(okx && oky && okz);
   }
   if ((MODE & M_DEPOSIT) && valid && !lost) {
-    // Collect3DFluidMoments weights (src/moments.c:164-184) from the in-cell offsets,
-    // scaled by 2^30 through the z factor (exact: a power of two).
-    const float wxh = ox, wxl = 1.0f - ox;
-    const float wyh = oy, wyl = 1.0f - oy;
-    const float wzh = oz * MX_WSCALE, wzl = (1.0f - oz) * MX_WSCALE;
     float w8[8];
-    w8[0] = wzh * wyh * wxh;
-    w8[1] = wzl * wyh * wxh;
-    w8[2] = wzh * wyl * wxh;
-    w8[3] = wzl * wyl * wxh;
-    w8[4] = wzh * wyh * wxl;
-    w8[5] = wzl * wyh * wxl;
-    w8[6] = wzh * wyl * wxl;
-    w8[7] = wzl * wyl * wxl;
+    weights_mx(ox, oy, oz, w8);
     constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
     int la = ix - (w.i0 - HH), lb = iy - (w.j0 - HH), lc = iz - (w.k0 - HH);
     if (la < 0) la += g.Nx; else if (la > w.RAx - 2) la -= g.Nx;
@@ -166,10 +194,10 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
         lds_addq(&w.sacc[2 * RA3 + idx], uy * wt);
         lds_addq(&w.sacc[3 * RA3 + idx], uz * wt);
       }
+    } else if ((MODE & M_REBIN) && slow) {
+      late = true;  // deposited by k_late_mx once re-binned into its new tile
     } else {
-      double wd[8];
-      for (int c = 0; c < 8; ++c) wd[c] = (double)w8[c] * (1.0 / 1073741824.0);
-      deposit_overflow(a.ovf, g, ix, iy, iz, wd, (double)vx, (double)vy, (double)vz);
+      deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
     }
   }
   if (MODE & M_REBIN) {
@@ -191,10 +219,16 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     }
     __syncthreads();
     if (code < 27) {
-      place_mx(a, w.s_nbr[code], bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+      const int dt = w.s_nbr[code];
+      const long s = place_mx(a, dt, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+      if (late) {
+        if (s >= 0) defer_deposit_mx(a, dt, s, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+        else deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+      }
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_mx(a.emi, (long)e, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+      if (late) deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
     w.par ^= 1;
@@ -210,7 +244,7 @@ template <int MODE, int TX, int TY, int TZ, int HH>
 __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ long long smemq[];
-  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef;
+  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef, s_n3;
   __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
@@ -257,7 +291,10 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
     s_nbr[tid] = (wrap_index(w.tz + sz, g.ntz) * g.nty + wrap_index(w.ty + sy, g.nty)) * g.ntx +
                  wrap_index(w.tx + sx, g.ntx);
   }
-  if (tid == 0) s_ndef = 0;
+  if (tid == 0) {
+    s_ndef = 0;
+    s_n3 = 0;
+  }
   if ((MODE & M_COLLIDE) && tid < 24) s_cdf[tid] = a.pt.cdf[tid];
   __syncthreads();
 
@@ -300,20 +337,50 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
     tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, ix, iy, iz, ox, oy, oz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
-    // Phase 2: deferred ions.  The scattering formula runs in FP32 on the same Philox
-    // draws as the FP64 path, then drift, deposit, re-bin.
+    // Phase 2: deferred ions, first attempt each (every lane busy).  The scattering
+    // formula runs in FP32 on the same Philox draws as the FP64 path.  Ions with more
+    // attempts (about 7% of them) are parked after the deferred ones in this
+    // workgroup's slots of D and finish in phase 3, so no wave loops for a few lanes.
     __syncthreads();
     const int ndef = s_ndef;
+    const long room = (end - beg) - ndef;
     for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
       const int q = q0 + tid;
       const bool valid = q < ndef;
+      bool done = valid;
       uint32_t cell = 0;
       float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
       int pid = 0;
       if (valid) {
         load_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
         const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
-        collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+        collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
+        if (k > 1) {
+          const int r = atomicAdd(&s_n3, 1);
+          if (r < room) {
+            store_mx(a.def, beg + ndef + r, cell, ox, oy, oz, vx, vy, vz, pid);
+            done = false;
+          } else {
+            collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
+          }
+        }
+      }
+      tail_mx<MODE, TX, TY, TZ, HH>(a, w, done, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
+                                    pid);
+    }
+    // Phase 3: the remaining attempts of the parked ions.
+    __syncthreads();
+    const int n3 = (int)min((long)s_n3, room);
+    for (int q0 = 0; q0 < n3; q0 += blockDim.x) {
+      const int q = q0 + tid;
+      const bool valid = q < n3;
+      uint32_t cell = 0;
+      float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
+      int pid = 0;
+      if (valid) {
+        load_mx(a.def, beg + ndef + q, cell, ox, oy, oz, vx, vy, vz, pid);
+        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
       tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
                                     pid);
@@ -333,6 +400,64 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
 }
 
 // Emigrants further than one tile -> destination tiles.
+// Deposit of the ions listed for tile t (they arrived from beyond their source tile's
+// deposit region and now lie in tile t): fixed-point LDS sums over the tile's
+// (T+1)^3 nodes, added exactly into the 64-bit overflow moments.  Resets the list.
+template <int TX, int TY, int TZ>
+__global__ void __launch_bounds__(256) k_late_mx(TileGeom g, ParticlesMix P, const int *__restrict__ late,
+                                                int *__restrict__ late_cnt, const int *__restrict__ base,
+                                                const int *__restrict__ cap, float inv_vmax,
+                                                long long *__restrict__ ovfq, double *__restrict__ ovf) {
+  constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
+  __shared__ long long acc[4 * N3];
+  const int t = blockIdx.x;
+  const int n = min(late_cnt[t], cap[t] >> 2);
+  __syncthreads();
+  if (threadIdx.x == 0) late_cnt[t] = 0;
+  if (n == 0) return;
+  const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
+  for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0;
+  __syncthreads();
+  const int *list = late + (base[t] >> 2);
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    uint32_t cell;
+    float ox, oy, oz, vx, vy, vz;
+    int id;
+    load_mx(P, list[q], cell, ox, oy, oz, vx, vy, vz, id);
+    float w8[8];
+    weights_mx(ox, oy, oz, w8);
+    const float ux = vx * inv_vmax, uy = vy * inv_vmax, uz = vz * inv_vmax;
+    const int la = mx_i(cell) - i0, lb = mx_j(cell) - j0, lc = mx_k(cell) - k0;
+    if (la < 0 || la >= TX || lb < 0 || lb >= TY || lc < 0 || lc >= TZ) {  // not expected: listed ions lie in t
+      double wd[8];
+      for (int c = 0; c < 8; ++c) wd[c] = (double)w8[c] * (1.0 / 1073741824.0);
+      deposit_overflow(ovf, g, mx_i(cell), mx_j(cell), mx_k(cell), wd, (double)vx, (double)vy, (double)vz);
+      continue;
+    }
+    const int base0 = (lc * NY + lb) * NX + la;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
+      const int idx = base0 + (az * NY + ay) * NX + ax;
+      const float wt = w8[c];
+      lds_addq(&acc[idx], wt);
+      lds_addq(&acc[N3 + idx], ux * wt);
+      lds_addq(&acc[2 * N3 + idx], uy * wt);
+      lds_addq(&acc[3 * N3 + idx], uz * wt);
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < N3; q += blockDim.x) {
+    const int li = q % NX, lj = (q / NX) % NY, lk = q / (NX * NY);
+    const long node = ((long)wrap_near(k0 + lk, g.Nz) * g.Ny + wrap_near(j0 + lj, g.Ny)) * g.Nx + wrap_near(i0 + li, g.Nx);
+    for (int d = 0; d < 4; ++d) {
+      const long long v = acc[d * N3 + q];
+      if (v != 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&ovfq[node * 4 + d]), (unsigned long long)v);
+    }
+  }
+}
+
 __global__ void k_rebin_mx(TileGeom g, ParticlesMix emi, ParticlesMix dst, ParticlesMix spill, long scap,
                            const int *base, const int *cap, int *next, unsigned long long *ctr) {
   const unsigned long long n = ctr[0];
@@ -516,6 +641,8 @@ void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, u
   a.phi = c->d_phi;
   a.fold = reinterpret_cast<long long *>(c->d_fold);
   a.ovf = c->d_ovf;
+  a.late = c->d_late;
+  a.late_cnt = c->d_late_cnt;
   a.k = mx_consts(c->p, dt);
   if (pt) {
     a.pt = *pt;
@@ -561,6 +688,21 @@ void mx_unbin_list(ecsprt_hip_ctx *c, long n) {
 
 void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
   k_cell_counts_mx<<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_base, c->d_count, d_counts);
+  ECS_HIP(hipGetLastError());
+}
+
+void mx_late(ecsprt_hip_ctx *c) {
+  const float iv = (float)(1.0 / mx_vmax(c->p));
+  const int tx = c->g.Tx, tz = c->g.Tz;
+  if (tx == 8 && tz == 8)
+    k_late_mx<8, 8, 8><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_late, c->d_late_cnt, c->d_base, c->d_cap,
+                                                         iv, c->d_ovfq, c->d_ovf);
+  else if (tx == 8 && tz == 4)
+    k_late_mx<8, 8, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_late, c->d_late_cnt, c->d_base, c->d_cap,
+                                                         iv, c->d_ovfq, c->d_ovf);
+  else
+    k_late_mx<4, 4, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_late, c->d_late_cnt, c->d_base, c->d_cap,
+                                                         iv, c->d_ovfq, c->d_ovf);
   ECS_HIP(hipGetLastError());
 }
 

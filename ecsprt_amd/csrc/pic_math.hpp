@@ -228,11 +228,12 @@ __device__ inline int poisson_attempts(const float *cdf, uint32_t key0, uint32_t
   return k;
 }
 
+// Attempts first..k-1 of ion pid at this step, in order.
 template <typename T>
 __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, uint32_t key1, uint32_t step, int pid,
-                                        int k, T &vx, T &vy, T &vz, unsigned &natt, unsigned &nsucc,
+                                        int first, int k, T &vx, T &vy, T &vz, unsigned &natt, unsigned &nsucc,
                                         unsigned &nfail) {
-  for (int at = 0; at < k; ++at) {
+  for (int at = first; at < k; ++at) {
     u32x4 c1 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0xA77E3B71u}};
     u32x4 c2 = {{(uint32_t)pid, step, (uint32_t)(at + 1), 0x5CA77E12u}};
     u32x4 r1 = philox4x32_10(c1, key0, key1);
@@ -257,7 +258,7 @@ __device__ inline void collide_philox(const CollideConsts &cc, const float *cdf,
                                       uint32_t step, int pid, double &vx, double &vy, double &vz, unsigned &natt,
                                       unsigned &nsucc, unsigned &nfail) {
   const int k = poisson_attempts(cdf, key0, key1, step, pid);
-  collide_attempts<double>(cc, key0, key1, step, pid, k, vx, vy, vz, natt, nsucc, nfail);
+  collide_attempts<double>(cc, key0, key1, step, pid, 0, k, vx, vy, vz, natt, nsucc, nfail);
 }
 
 // Host: P(K <= k) for K ~ Poisson(lam); the last entry closes the table at 1.

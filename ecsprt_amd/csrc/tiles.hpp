@@ -83,6 +83,7 @@ void mx_bin_scatter(ecsprt_hip_ctx *c, long np);
 void mx_unbin(ecsprt_hip_ctx *c);
 void mx_unbin_list(ecsprt_hip_ctx *c, long n);
 void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
+void mx_late(ecsprt_hip_ctx *c);
 double mx_vmax(const ecsprt_params &p);
 
 // Fill kernels launched on the context's stream (particles.hip).

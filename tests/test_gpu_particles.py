@@ -254,3 +254,45 @@ def test_fused_step_tile_shapes(tile, tile_z, block):
     np.testing.assert_array_equal(gvel, vel)
     assert rel(ctx.get_moments(), oracle.collect_moments(po, pos, vel)) <= 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("prec,tile,coll", [(_lib.PARTICLES_FP64, 8, False), (_lib.PARTICLES_FP64, 4, True),
+                                            (_lib.PARTICLES_MIXED, 8, True), (_lib.PARTICLES_MIXED, 4, False)])
+def test_fused_deposit_of_far_movers(prec, tile, coll):
+    """Ions that move 3-6 cells in a step leave their source tile's deposit region
+    (tile plus two cells); their deposit is deferred to their destination tile.  After
+    each fused step the moments equal the oracle's deposit of the returned ions (FP64:
+    1e-12; mixed: 2e-6 of the largest moment, FP32 weights), with tight tile slack so
+    that spills mix with the deferred deposits."""
+    say(f"far movers prec={prec} tile={tile} collisions={coll}")
+    N, ppc = 16, 8
+    cli = configs.base_cli(N, nui=0.1 if coll else 0.0, B0=0.02)
+    po = oracle.setup_context(**cli)
+    o = pic.default_options(particle_precision=prec, tile=tile, slack=0.02, ksp_rtol=1e-8,
+                            operator_mode=_lib.OPERATOR_REBUILD,
+                            collide_mode=_lib.COLLIDE_PHILOX if coll else _lib.COLLIDE_OFF)
+    ctx = pic.Context(pic.setup_context(**cli), o)
+    rng = np.random.default_rng(12)
+    n = ppc * N ** 3
+    pos = rng.uniform(0, N, (n, 3))
+    vel = rng.normal(size=(n, 3))
+    far = rng.random(n) < 0.3
+    u = rng.normal(size=(far.sum(), 3))
+    vel[far] = rng.uniform(3.0, 6.0, (far.sum(), 1)) * u / np.linalg.norm(u, axis=1, keepdims=True)
+    ctx.set_particles(pos, vel)
+    ctx.set_potential(np.zeros((N, N, N)))
+    tol = 1e-12 if prec == _lib.PARTICLES_FP64 else 2e-6
+    for step in range(3):
+        ctx.step(1.0)
+        gpos, gvel = ctx.get_particles()
+        om = oracle.collect_moments(po, gpos, gvel)
+        gm = ctx.get_moments()
+        e = rel(gm, om)
+        say(f"  step {step} moment error {e:.2e}")
+        assert e <= tol
+        # FP32 weights of one ion sum to 1 within a few FP32 roundings
+        assert gm[..., 0].sum() == pytest.approx(n, rel=1e-10 if prec == _lib.PARTICLES_FP64 else 1e-6)
+    assert ctx.num_particles == n
+    if coll:
+        assert ctx.collision_stats()[1] > 0
+    ctx.close()
