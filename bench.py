@@ -34,11 +34,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3", "c4"],
-                    help="c2 = the c1 workload on the mixed-precision path only; c3 = batched solves; "
-                         "c4 = ensemble of 16 runs")
-    ap.add_argument("--problems", type=int, default=256, help="c3: number of fields")
-    ap.add_argument("--slots", type=int, default=8, help="c3: concurrent solver slots")
+    ap.add_argument("--workload", default="fp64", choices=["cpu32", "fp64", "mixed", "batch", "ensemble"],
+                    help="cpu32 = 32^3 x 64 ions/cell FP64; fp64 = 128^3 x 100 ions/cell FP64 (with a mixed "
+                         "leg); mixed = the 128^3 workload on the mixed-precision path only; batch = 256 "
+                         "standalone solves; ensemble = 16 independent runs")
+    ap.add_argument("--problems", type=int, default=256, help="batch: number of fields")
+    ap.add_argument("--slots", type=int, default=8, help="batch: concurrent solver slots")
     ap.add_argument("--ppc", type=int, default=0, help="override ions per cell")
     ap.add_argument("--grid", type=int, default=0, help="override grid edge")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -61,7 +62,7 @@ def parse():
 
 def workload(args):
     from ecsprt_amd import configs
-    wl = configs.C0 if args.workload == "c0" else configs.C1
+    wl = configs.CPU32 if args.workload == "cpu32" else configs.PROD128
     if args.grid:
         wl.cli.update(Nx=args.grid, Ny=args.grid, Nz=args.grid)
     if args.ppc:
@@ -228,7 +229,7 @@ def mixed_errors(p, wl, pic, _lib, steps=3):
     return {"per_step": per, "free_running": free}
 
 
-def bench_c3(args, pic, _lib):
+def bench_batch(args, pic, _lib):
     """256 standalone solves of smoothed log-normal density fields, grid edges drawn from
     {32, 48, 64, 96, 128, 192} per axis (seed 3), mixed precision to an FP64 residual of
     1e-8 relative; the FP64 batch on the same fields is the reference."""
@@ -264,11 +265,11 @@ def bench_c3(args, pic, _lib):
         m.free()
     ms, fs = res["mixed"]["seconds"], res["fp64"]["seconds"]
     out = {
-        "metric": "batched potential solves/s (c3: 256 fields, mixed precision, 1e-8 FP64 residual)",
+        "metric": "batched potential solves/s (256 fields, mixed precision, 1e-8 FP64 residual)",
         "value": nprob / ms, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": 1e3 * ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 inner / f64 refinement", "data": "synthetic (device log-normal fields)",
-        "config": {"workload": "c3_batched_solves", "problems": nprob, "points_total": int(npts.sum()),
+        "config": {"workload": "batched_solves_256_fields", "problems": nprob, "points_total": int(npts.sum()),
                    "points_max": int(npts.max()), "slots": args.slots, "ksp_rtol": 1e-8,
                    "density": "exp(0.3 g - 0.045), Gaussian smoothing sigma 3 cells, mean 1; flux = n"},
         "points_per_s": float(npts.sum() / ms),
@@ -281,7 +282,7 @@ def bench_c3(args, pic, _lib):
     print(json.dumps(out))
 
 
-def c4_runs(n=16, seed=4):
+def ensemble_runs(n=16, seed=4):
     """The ensemble: per run a cubic grid edge from {48, 64, 80, 96}, 16-64 ions per cell,
     Omega_i dt in [0.02, 0.1], nu dt in [0.005, 0.05], seed r + 1."""
     rng = np.random.default_rng(seed)
@@ -289,11 +290,11 @@ def c4_runs(n=16, seed=4):
                  nui=float(rng.uniform(0.005, 0.05)), seed=r + 1) for r in range(n)]
 
 
-def bench_c4(args, pic, _lib):
+def bench_ensemble(args, pic, _lib):
     """16 independent PIC runs of uneven size, mixed precision (particles and solve, rtol
-    1e-10 as in the c1 workload), stepped concurrently (one stream and host thread per run)."""
+    1e-10 as in the 128^3 workload), stepped concurrently (one stream and host thread per run)."""
     from ecsprt_amd import configs
-    runs = c4_runs()
+    runs = ensemble_runs()
     mixed = dict(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
 
     def make(r, extra, collide=_lib.COLLIDE_PHILOX):
@@ -346,11 +347,11 @@ def bench_c4(args, pic, _lib):
         b.close()
     total = sum(nps)
     out = {
-        "metric": "aggregate particle-steps/s (c4: 16-run mixed-precision ensemble)",
+        "metric": "aggregate particle-steps/s (16-run mixed-precision ensemble)",
         "value": total * nsteps / el, "unit": "particle-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
         "ms_per_step": 1e3 * el / nsteps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 particles, f64 fold, mixed solve", "data": "synthetic (device Philox initial conditions)",
-        "config": {"workload": "c4_ensemble", "runs": runs, "particles_total": total,
+        "config": {"workload": "ensemble_16_runs", "runs": runs, "particles_total": total,
                    "concurrency": "one stream and host thread per run"},
         "sequential_ms_per_step": 1e3 * el_seq, "concurrency_speedup": el_seq / (el / nsteps),
         "errors_vs_fp64_one_step": errs,
@@ -368,11 +369,11 @@ def main():
     if args.copy_probe:
         print(json.dumps(copy_probe(args.copy_probe)))
         return
-    if args.workload == "c3":
-        bench_c3(args, pic, _lib)
+    if args.workload == "batch":
+        bench_batch(args, pic, _lib)
         return
-    if args.workload == "c4":
-        bench_c4(args, pic, _lib)
+    if args.workload == "ensemble":
+        bench_ensemble(args, pic, _lib)
         return
     wl = workload(args)
     p = pic.setup_context(**wl.cli)
@@ -383,7 +384,7 @@ def main():
         common["mg_pre"] = args.mg_pre
     if args.mg_post:
         common["mg_post"] = args.mg_post
-    mixed_only = args.workload == "c2"
+    mixed_only = args.workload == "mixed"
     if mixed_only:
         common.update(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
     tuned = {}

@@ -76,11 +76,11 @@ if __name__ == "__main__":
     from . import configs
 
     ap = argparse.ArgumentParser(description="time the particle-pass candidates on a workload")
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--workload", default="fp64", choices=["cpu32", "fp64"])
     ap.add_argument("--mixed", action="store_true")
     ap.add_argument("--steps", type=int, default=3)
     a = ap.parse_args()
-    wl = configs.C0 if a.workload == "c0" else configs.C1
+    wl = configs.CPU32 if a.workload == "cpu32" else configs.PROD128
     p = pic.setup_context(**wl.cli)
     opts = pic.default_options(ksp_rtol=wl.rtol)
     if a.mixed:

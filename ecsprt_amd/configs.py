@@ -59,12 +59,12 @@ def base_cli(N, B0=0.05, kappa_e=4.0, nui=0.01, Tn_over_Ti=1.0, E0=0.0, viT=1.0)
 
 # Plain FP64 hybrid PIC, CPU-runnable: 32^3, 64 ions/cell, Omega_i*dt = 0.05,
 # nu*dt = 0.01 with a cold neutral background (Tn = Ti/100, vnT = 0.1*|vT|).
-C0 = Workload("c0_fp64_32cube", base_cli(32, nui=0.01, Tn_over_Ti=0.01), ppc=64, dt=1.0, steps=10,
+CPU32 = Workload("fp64_32cube_64ppc", base_cli(32, nui=0.01, Tn_over_Ti=0.01), ppc=64, dt=1.0, steps=10,
               rtol=1e-8)
 
 # Plain FP64 at production size: 128^3, 100 ions/cell, E0 along x, k=4 density
 # perturbation of 1e-3, nu*dt = 0.02 with Tn = Ti (the reference's default).
-C1 = Workload("c1_fp64_128cube", base_cli(128, nui=0.02, Tn_over_Ti=1.0, E0=0.05), ppc=100, dt=1.0,
+PROD128 = Workload("fp64_128cube_100ppc", base_cli(128, nui=0.02, Tn_over_Ti=1.0, E0=0.05), ppc=100, dt=1.0,
               steps=200, rtol=1e-10, amp=1e-3, k=4)
 
 

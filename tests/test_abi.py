@@ -53,7 +53,7 @@ def test_philox2x32_known_answers():
     assert pic.philox2x32_10([0x243F6A88, 0x85A308D3], 0x13198A2E) == [0xDD7CE038, 0xF62A4C12]
 
 
-@pytest.mark.parametrize("wl", [configs.C0, configs.C1, configs.small(12)])
+@pytest.mark.parametrize("wl", [configs.CPU32, configs.PROD128, configs.small(12)])
 def test_setup_context_matches_oracle(wl):
     a = pic.setup_context(**wl.cli)
     b = oracle.setup_context(**wl.cli)
@@ -72,9 +72,9 @@ def test_collision_count_matches_oracle(Tn_over_Ti, nui, np_):
 
 def test_workload_collision_counts():
     """The counts the workloads run with (recorded in the design notes)."""
-    p0 = oracle.setup_context(**configs.C0.cli)
-    p1 = oracle.setup_context(**configs.C1.cli)
-    n0 = oracle.collision_count(p0, configs.C0.np, configs.C0.dt)
-    n1 = oracle.collision_count(p1, configs.C1.np, configs.C1.dt)
-    assert 0.3 * configs.C0.np < n0 < 0.5 * configs.C0.np
-    assert 0.05 * configs.C1.np < n1 < 0.1 * configs.C1.np
+    p0 = oracle.setup_context(**configs.CPU32.cli)
+    p1 = oracle.setup_context(**configs.PROD128.cli)
+    n0 = oracle.collision_count(p0, configs.CPU32.np, configs.CPU32.dt)
+    n1 = oracle.collision_count(p1, configs.PROD128.np, configs.PROD128.dt)
+    assert 0.3 * configs.CPU32.np < n0 < 0.5 * configs.CPU32.np
+    assert 0.05 * configs.PROD128.np < n1 < 0.1 * configs.PROD128.np
