@@ -123,12 +123,16 @@ def copy_probe(gib, reps=5):
     return {"copy_bytes": n, "best_ms": best, "read_plus_write_GBs": 2 * n / (best * 1e-3) / 1e9}
 
 
-def launch_of(p, o):
-    """Effective tile shape and block size of the particle pass (0 = the library's
-    choice: 8-cell tiles once a grid holds at least 512 of them, else 4)."""
-    n8 = ((p.Nx + 7) // 8) * ((p.Ny + 7) // 8) * ((p.Nz + 7) // 8)
+def launch_of(p, o, mixed=False):
+    """Effective tile shape and block size of the particle pass.  With none given the
+    library chooses: 8-cell tiles once a grid holds at least 512 of them (else 4), as
+    8x8x4 cells with 512 threads for FP64 particles and 8x8x8 with 1024 for mixed."""
+    n8 = ((p.Nx + 7) // 8) * ((p.Nz + 7) // 8) * ((p.Ny + 7) // 8)
+    auto = not (o.get("tile") or o.get("tile_z") or o.get("block"))
     t = o.get("tile") or (8 if n8 >= 512 else 4)
-    return f"{t}x{t}x{o.get('tile_z') or t}/{o.get('block') or 1024}"
+    f64_auto = auto and t == 8 and not mixed
+    tz = o.get("tile_z") or (4 if f64_auto else t)
+    return f"{t}x{t}x{tz}/{o.get('block') or (512 if f64_auto else 1024)}"
 
 
 def committed_traffic(kernel, workload_name, launch, args):
@@ -414,7 +418,7 @@ def main():
     achieved = alg_bytes / (r["k_ms"] * 1e-3) / 1e9
     particle_ms = r["k_ms"] + r["pre_ms"] + r["rebin_ms"]
     kernel = "k_tile_mx" if mixed_only else "k_tile"
-    traffic = committed_traffic(kernel, wl.name, launch_of(p, common), args)
+    traffic = committed_traffic(kernel, wl.name, launch_of(p, common, mixed_only), args)
     out = {
         "metric": "particle-steps/s (push+deposit+collide); phi-solve ms; mixed vs FP64 @1e-6 err",
         "value": npart * nsteps / r["seconds"],
@@ -441,7 +445,7 @@ def main():
                           "rhs": r["rhs_ms"], "solve": r["solve_ms"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": kernel + "<push|collide|drift|deposit|rebin>", "launch": launch_of(p, common),
+                     "kernel": kernel + "<push|collide|drift|deposit|rebin>", "launch": launch_of(p, common, mixed_only),
                      "algorithmic_bytes_per_launch": alg_bytes},
     }
     if not args.no_mixed and not mixed_only:
@@ -465,8 +469,9 @@ def main():
             "ksp_iterations_mean": rm["iters"],
             "roofline": {"bound": "hbm", "achieved": mach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": mach / HBM_PEAK_GBS,
-                         "traffic": committed_traffic("k_tile_mx", wl.name, launch_of(p, mcommon), args),
-                         "kernel": "k_tile_mx", "launch": launch_of(p, mcommon), "algorithmic_bytes_per_launch": malg},
+                         "traffic": committed_traffic("k_tile_mx", wl.name, launch_of(p, mcommon, True), args),
+                         "kernel": "k_tile_mx", "launch": launch_of(p, mcommon, True),
+                         "algorithmic_bytes_per_launch": malg},
             "errors_vs_fp64": mixed_errors(p, wl, pic, _lib),
         }
     if tuned:

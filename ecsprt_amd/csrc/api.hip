@@ -144,7 +144,7 @@ int ecsprt_hip_default_options(ecsprt_options *o) {
   o->tile = 0;
   o->tile_z = 0;
   o->halo = 2;
-  o->block = 1024;
+  o->block = 0;
   o->split = 0;
   o->slack = 0.25;
   o->collide_mode = ECSPRT_COLLIDE_PHILOX;

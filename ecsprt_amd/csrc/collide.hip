@@ -146,7 +146,7 @@ __global__ void k_att_sort(long np, const int *cnt, const long long *off, int *l
 __global__ void k_slot_of_id(ParticlesF64 A, const int *base, const int *count, int *slot_of_id) {
   const int t = blockIdx.x;
   const long b = base[t], n = count[t];
-  for (long q = threadIdx.x; q < n; q += blockDim.x) slot_of_id[A.id[b + q]] = (int)(b + q);
+  for (long q = threadIdx.x; q < n; q += blockDim.x) slot_of_id[A.id(b + q)] = (int)(b + q);
 }
 
 // mode 0: record outcomes; mode 1: apply attempts below the cut.
@@ -158,7 +158,7 @@ __global__ void k_att_replay(int mode, long np, const ecsprt_attempt *att, const
     const int n = cnt[ip];
     if (n == 0) continue;
     const int s = slot_of_id[ip];
-    double v[3] = {A.vx[s], A.vy[s], A.vz[s]};
+    double v[3] = {A.vx(s), A.vy(s), A.vz(s)};
     const int *L = list + off[ip];
     for (int q = 0; q < n; ++q) {
       const int a = L[q];
@@ -177,9 +177,9 @@ __global__ void k_att_replay(int mode, long np, const ecsprt_attempt *att, const
       }
     }
     if (mode) {
-      A.vx[s] = v[0];
-      A.vy[s] = v[1];
-      A.vz[s] = v[2];
+      A.vx(s) = v[0];
+      A.vy(s) = v[1];
+      A.vz(s) = v[2];
     }
   }
 }
@@ -275,10 +275,10 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
 // Acceptance estimate for the Philox mode: a dry run of the per-particle Poisson
 // process (velocities are not written) on a stream independent of the real steps.
 __device__ inline void pilot_ion(const ParticlesF64 &A, long s, double &vx, double &vy, double &vz, int &id) {
-  vx = A.vx[s];
-  vy = A.vy[s];
-  vz = A.vz[s];
-  id = A.id[s];
+  vx = A.vx(s);
+  vy = A.vy(s);
+  vz = A.vz(s);
+  id = A.id(s);
 }
 
 __device__ inline void pilot_ion(const ParticlesMix &A, long s, double &vx, double &vy, double &vz, int &id) {

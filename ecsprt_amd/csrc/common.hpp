@@ -30,11 +30,25 @@ struct Error {
     if (!(cond)) throw ::ecsprt::Error{(code), (msg)};    \
   } while (0)
 
-// Structure-of-arrays particle storage (FP64).
+// FP64 particle storage in blocks of 64 ions (one wavefront): eight 512-byte planes per
+// block, x, y, z, vx, vy, vz as double, then the 64 int32 ids (the last 256 bytes of the
+// seventh plane and the eighth plane are unused).  A wave's field is one contiguous
+// 512-byte run, and one address per ion reaches every field through immediate offsets,
+// so a kernel carries one pointer per particle array instead of seven.
 struct ParticlesF64 {
-  double *x = nullptr, *y = nullptr, *z = nullptr;
-  double *vx = nullptr, *vy = nullptr, *vz = nullptr;
-  int *id = nullptr;
+  double *p = nullptr;
+  static constexpr long BLOCK = 512;  // doubles per 64-ion block
+  __host__ __device__ static long at(long i) { return (i >> 6) * BLOCK + (i & 63); }
+  __host__ __device__ double &x(long i) const { return p[at(i)]; }
+  __host__ __device__ double &y(long i) const { return p[at(i) + 64]; }
+  __host__ __device__ double &z(long i) const { return p[at(i) + 128]; }
+  __host__ __device__ double &vx(long i) const { return p[at(i) + 192]; }
+  __host__ __device__ double &vy(long i) const { return p[at(i) + 256]; }
+  __host__ __device__ double &vz(long i) const { return p[at(i) + 320]; }
+  __host__ __device__ int &id(long i) const {
+    return reinterpret_cast<int *>(p + (i >> 6) * BLOCK + 384)[i & 63];
+  }
+  static size_t bytes(long n) { return (size_t)((n + 63) / 64) * BLOCK * sizeof(double); }
 };
 
 // Mixed-precision particle storage: a packed integer cell index (x in bits 0-10,
@@ -146,6 +160,7 @@ struct ecsprt_hip_ctx {
   int *d_foldtab = nullptr;     // 3 axes x (count, pairs)
   int foldtab_off[3] = {0, 0, 0};
   size_t lds_bytes = 0;
+  int block = 1024;  // threads per tile workgroup in effect
 
   // staging (host<->device AoS)
   double *d_stage = nullptr;

@@ -35,13 +35,13 @@ struct TileArgs {
 
 __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, double y, double z,
                                       double vx, double vy, double vz, int id) {
-  d.x[i] = x;
-  d.y[i] = y;
-  d.z[i] = z;
-  d.vx[i] = vx;
-  d.vy[i] = vy;
-  d.vz[i] = vz;
-  d.id[i] = id;
+  d.x(i) = x;
+  d.y(i) = y;
+  d.z(i) = z;
+  d.vx(i) = vx;
+  d.vy(i) = vy;
+  d.vz(i) = vz;
+  d.id(i) = id;
 }
 
 // Place a particle whose destination slot was reserved: into the destination tile
@@ -175,8 +175,7 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     // previous batch is cleared here, after the barrier every thread passed once it
     // finished with it, so two barriers per batch suffice.
     int *cnt = w.s_cnt + 27 * w.par, *bas = w.s_base + 27 * w.par;
-    int rank = 0;
-    if (code < 27) rank = atomicAdd(&cnt[code], 1);
+    const int rank = rebin_rank(cnt, code);
     __syncthreads();
     if (tid < 27) {
       const int n = cnt[tid];
@@ -199,9 +198,9 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
     w.par ^= 1;
   } else if ((MODE & M_PUSH) && valid) {
-    a.src.vx[p] = vx;
-    a.src.vy[p] = vy;
-    a.src.vz[p] = vz;
+    a.src.vx(p) = vx;
+    a.src.vy(p) = vy;
+    a.src.vz(p) = vz;
   }
 }
 
@@ -219,7 +218,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
   // contiguous run of tiles so neighbouring tiles share an L2.
   if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
-  constexpr int RPX = TX + 4, RPY = TY + 4, RPZ = TZ + 4, RP3 = RPX * RPY * RPZ;
+  constexpr int NEX = TX + 1, NEY = TY + 1, NE3 = NEX * NEY * (TZ + 1);
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH;
   TileWG w;
   w.t = wi / g.split;
@@ -240,17 +239,21 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   w.s_base = s_base;
   w.par = 0;
   w.s_nbr = s_nbr;
-  double *sphi = smem;
-  w.sacc = smem + ((MODE & M_PUSH) ? RP3 : 0);
+  // LDS: the gather's difference numerators at the tile's (T+1)^3 corner nodes, three
+  // components, then the deposit region.
+  double *sE = smem;
+  w.sacc = smem + ((MODE & M_PUSH) ? 3 * NE3 : 0);
   const int tid = threadIdx.x;
+  const PhiGlobal gphi{a.phi, g.Nx, g.Ny, g.Nz};
 
   if (MODE & M_PUSH) {
-    for (int q = tid; q < RP3; q += blockDim.x) {
-      int li = q % RPX, lj = (q / RPX) % RPY, lk = q / (RPX * RPY);
-      int gi = wrap_index(w.i0 - 1 + li, g.Nx);
-      int gj = wrap_index(w.j0 - 1 + lj, g.Ny);
-      int gk = wrap_index(w.k0 - 1 + lk, g.Nz);
-      sphi[q] = a.phi[((long)gk * g.Ny + gj) * g.Nx + gi];
+    for (int q = tid; q < NE3; q += blockDim.x) {
+      const int li = q % NEX, lj = (q / NEX) % NEY, lk = q / (NEX * NEY);
+      double d[3];
+      node_diff<double>(gphi, g.Nx, g.Ny, g.Nz, w.i0 + li, w.j0 + lj, w.k0 + lk, d);
+      sE[q] = d[0];
+      sE[NE3 + q] = d[1];
+      sE[2 * NE3 + q] = d[2];
     }
   }
   if (MODE & M_DEPOSIT) {
@@ -280,13 +283,13 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   int nid = 0;
   if (beg + tid < end) {
     const long p = beg + tid;
-    nxp = a.src.x[p];
-    nyp = a.src.y[p];
-    nzp = a.src.z[p];
-    nvx = a.src.vx[p];
-    nvy = a.src.vy[p];
-    nvz = a.src.vz[p];
-    nid = a.src.id[p];
+    nxp = a.src.x(p);
+    nyp = a.src.y(p);
+    nzp = a.src.z(p);
+    nvx = a.src.vx(p);
+    nvy = a.src.vy(p);
+    nvz = a.src.vz(p);
+    nid = a.src.id(p);
   }
   for (long p0 = beg; p0 < end; p0 += blockDim.x) {
     const long p = p0 + tid;
@@ -296,20 +299,30 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     {
       const long pn = p + blockDim.x;
       if (pn < end) {
-        nxp = a.src.x[pn];
-        nyp = a.src.y[pn];
-        nzp = a.src.z[pn];
-        nvx = a.src.vx[pn];
-        nvy = a.src.vy[pn];
-        nvz = a.src.vz[pn];
-        nid = a.src.id[pn];
+        nxp = a.src.x(pn);
+        nyp = a.src.y(pn);
+        nzp = a.src.z(pn);
+        nvx = a.src.vx(pn);
+        nvy = a.src.vy(pn);
+        nvz = a.src.vz(pn);
+        nid = a.src.id(pn);
       }
     }
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body, src/velocities.c:342-356.
+      // DifferenceVector3D at x = pos/d: il = (int)x, w = x - il (vector-math.c:607-617).
       double E[3];
-      diff_vec3d_lds<RPX, RPY>(sphi, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, cell_coord(px, dx, a.pc.inv[0], a.pc.pow2),
-                     cell_coord(py, dy, a.pc.inv[1], a.pc.pow2), cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2), E);
+      const double x0 = cell_coord(px, dx, a.pc.inv[0], a.pc.pow2);
+      const double y0 = cell_coord(py, dy, a.pc.inv[1], a.pc.pow2);
+      const double z0 = cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2);
+      const int ixl = (int)x0, iyl = (int)y0, izl = (int)z0;
+      const int la = ixl - w.i0, lb = iyl - w.j0, lc = izl - w.k0;
+      if ((unsigned)la < TX && (unsigned)lb < TY && (unsigned)lc < TZ && ixl < g.Nx && iyl < g.Ny && izl < g.Nz)
+        gather_nodes<double, NEX, NEY, NE3>(sE, (lc * NEY + lb) * NEX + la, x0 - (double)ixl, y0 - (double)iyl,
+                                            z0 - (double)izl, E);
+      else  // a coordinate that rounded to N: the node differences differ by corner role
+        diff_vec3d_core<double>(gphi, g.Nx, g.Ny, g.Nz, ixl, iyl, izl, x0 - (double)ixl, y0 - (double)iyl,
+                                z0 - (double)izl, E);
       double vxm = vx + a.pc.Escale[0] * E[0];
       double vym = vy + a.pc.Escale[1] * E[1];
       double vzm = vz + a.pc.Escale[2] * E[2];
@@ -346,13 +359,13 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       int pid = 0;
       if (valid) {
         const long d = beg + q;
-        px = a.def.x[d];
-        py = a.def.y[d];
-        pz = a.def.z[d];
-        vx = a.def.vx[d];
-        vy = a.def.vy[d];
-        vz = a.def.vz[d];
-        pid = a.def.id[d];
+        px = a.def.x(d);
+        py = a.def.y(d);
+        pz = a.def.z(d);
+        vx = a.def.vx(d);
+        vy = a.def.vy(d);
+        vz = a.def.vz(d);
+        pid = a.def.id(d);
         const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
         collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
         if (k > 1) {
@@ -377,13 +390,13 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       int pid = 0;
       if (valid) {
         const long d = beg + ndef + q;
-        px = a.def.x[d];
-        py = a.def.y[d];
-        pz = a.def.z[d];
-        vx = a.def.vx[d];
-        vy = a.def.vy[d];
-        vz = a.def.vz[d];
-        pid = a.def.id[d];
+        px = a.def.x(d);
+        py = a.def.y(d);
+        pz = a.def.z(d);
+        vx = a.def.vx(d);
+        vy = a.def.vy(d);
+        vz = a.def.vz(d);
+        pid = a.def.id(d);
         const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
         collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
@@ -424,7 +437,7 @@ __global__ void __launch_bounds__(256) k_late(TileGeom g, ParticlesF64 P, const 
   const int *list = late + (base[t] >> 2);
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
     const long s = list[q];
-    const double px = P.x[s], py = P.y[s], pz = P.z[s], vx = P.vx[s], vy = P.vy[s], vz = P.vz[s];
+    const double px = P.x(s), py = P.y(s), pz = P.z(s), vx = P.vx(s), vy = P.vy(s), vz = P.vz(s);
     const double x = cell_coord(px, pc.dx, pc.inv[0], pc.pow2);
     const double y = cell_coord(py, pc.dy, pc.inv[1], pc.pow2);
     const double z = cell_coord(pz, pc.dz, pc.inv[2], pc.pow2);
@@ -467,14 +480,14 @@ __global__ void k_rebin(TileGeom g, ParticlesF64 emi, ParticlesF64 dst, Particle
   const unsigned long long n = ctr[0];
   for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
        e += (unsigned long long)gridDim.x * blockDim.x) {
-    double x = emi.x[e], y = emi.y[e], z = emi.z[e];
+    double x = emi.x(e), y = emi.y(e), z = emi.z(e);
     int t = tile_of(g, x / dx, y / dy, z / dz);
     int slot = atomicAdd(&next[t], 1);
     if (slot < cap[t]) {
-      store_particle(dst, (long)base[t] + slot, x, y, z, emi.vx[e], emi.vy[e], emi.vz[e], emi.id[e]);
+      store_particle(dst, (long)base[t] + slot, x, y, z, emi.vx(e), emi.vy(e), emi.vz(e), emi.id(e));
     } else {
       unsigned long long sp = atomicAdd(&ctr[1], 1ull);
-      if ((long)sp < scap) store_particle(spill, (long)sp, x, y, z, emi.vx[e], emi.vy[e], emi.vz[e], emi.id[e]);
+      if ((long)sp < scap) store_particle(spill, (long)sp, x, y, z, emi.vx(e), emi.vy(e), emi.vz(e), emi.id(e));
       else atomicAdd(&ctr[2], 1ull);
     }
   }
@@ -573,26 +586,26 @@ __global__ void k_unbin(ParticlesF64 A, const int *base, const int *count, doubl
   double *vel = stage + 3 * np;
   for (long q = threadIdx.x; q < n; q += blockDim.x) {
     long s = b + q;
-    long id = A.id[s];
-    stage[id * 3] = A.x[s];
-    stage[id * 3 + 1] = A.y[s];
-    stage[id * 3 + 2] = A.z[s];
-    vel[id * 3] = A.vx[s];
-    vel[id * 3 + 1] = A.vy[s];
-    vel[id * 3 + 2] = A.vz[s];
+    long id = A.id(s);
+    stage[id * 3] = A.x(s);
+    stage[id * 3 + 1] = A.y(s);
+    stage[id * 3 + 2] = A.z(s);
+    vel[id * 3] = A.vx(s);
+    vel[id * 3 + 1] = A.vy(s);
+    vel[id * 3 + 2] = A.vz(s);
   }
 }
 
 __global__ void k_unbin_list(ParticlesF64 S, long n, double *stage, long np) {
   double *vel = stage + 3 * np;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    long id = S.id[q];
-    stage[id * 3] = S.x[q];
-    stage[id * 3 + 1] = S.y[q];
-    stage[id * 3 + 2] = S.z[q];
-    vel[id * 3] = S.vx[q];
-    vel[id * 3 + 1] = S.vy[q];
-    vel[id * 3 + 2] = S.vz[q];
+    long id = S.id(q);
+    stage[id * 3] = S.x(q);
+    stage[id * 3 + 1] = S.y(q);
+    stage[id * 3 + 2] = S.z(q);
+    vel[id * 3] = S.vx(q);
+    vel[id * 3 + 1] = S.vy(q);
+    vel[id * 3 + 2] = S.vz(q);
   }
 }
 
@@ -649,9 +662,9 @@ __global__ void k_cell_counts(TileGeom g, ParticlesF64 A, const int *base, const
   const long b = base[t], n = count[t];
   for (long q = threadIdx.x; q < n; q += blockDim.x) {
     long s = b + q;
-    int i = wrap_index((int)(A.x[s] / dx), g.Nx);
-    int j = wrap_index((int)(A.y[s] / dy), g.Ny);
-    int k = wrap_index((int)(A.z[s] / dz), g.Nz);
+    int i = wrap_index((int)(A.x(s) / dx), g.Nx);
+    int j = wrap_index((int)(A.y(s) / dy), g.Ny);
+    int k = wrap_index((int)(A.z(s) / dz), g.Nz);
     atomicAdd(&counts[((long)k * g.Ny + j) * g.Nx + i], 1);
   }
 }
@@ -692,23 +705,14 @@ void zero_ctr(ecsprt_hip_ctx *c) {
 }
 
 static void free_particles(ParticlesF64 &P) {
-  (void)hipFree(P.x);
-  (void)hipFree(P.y);
-  (void)hipFree(P.z);
-  (void)hipFree(P.vx);
-  (void)hipFree(P.vy);
-  (void)hipFree(P.vz);
-  (void)hipFree(P.id);
+  (void)hipFree(P.p);
   P = ParticlesF64{};
 }
 
 static void alloc_particles(ParticlesF64 &P, long n) {
   free_particles(P);
   n = std::max(n, 1L);
-  hipError_t e = hipSuccess;
-  double **f[6] = {&P.x, &P.y, &P.z, &P.vx, &P.vy, &P.vz};
-  for (int i = 0; i < 6 && e == hipSuccess; ++i) e = hipMalloc(f[i], sizeof(double) * n);
-  if (e == hipSuccess) e = hipMalloc(&P.id, sizeof(int) * n);
+  hipError_t e = hipMalloc(&P.p, ParticlesF64::bytes(n));
   if (e != hipSuccess) {
     free_particles(P);
     throw Error{ECSPRT_ERR_NOMEM, "particle arrays: " + std::string(hipGetErrorString(e))};
@@ -755,7 +759,7 @@ static void set_lds_attrs() {
 
 static size_t lds_for(const TileGeom &g, int mode, int) {
   size_t b = 0;
-  if (mode & M_PUSH) b += sizeof(double) * (g.Tx + 4) * (g.Ty + 4) * (g.Tz + 4);
+  if (mode & M_PUSH) b += sizeof(double) * 3 * (g.Tx + 1) * (g.Ty + 1) * (g.Tz + 1);
   if (mode & M_DEPOSIT) b += sizeof(double) * 4 * g.RAx * g.RAy * g.RAz;
   return b;
 }
@@ -771,7 +775,11 @@ void tiles_init(ecsprt_hip_ctx *c) {
     long n8 = (long)((p.Nx + 7) / 8) * ((p.Ny + 7) / 8) * ((p.Nz + 7) / 8);
     T = n8 >= 512 ? 8 : 4;
   }
-  const int Tz = c->o.tile_z > 0 ? c->o.tile_z : T;
+  // Measured on the 128^3 x 100 ions-per-cell workload (ecsprt_amd.autotune): FP64
+  // tiles run fastest as 8x8x4 cells with 512 threads (two workgroups per CU fit in
+  // LDS), mixed tiles as 8x8x8 with 1024 threads.
+  const bool auto_shape = c->o.tile <= 0 && c->o.tile_z <= 0 && c->o.block <= 0;
+  const int Tz = c->o.tile_z > 0 ? c->o.tile_z : ((auto_shape && T == 8 && !c->mixed) ? 4 : T);
   ECS_CHECK((T == 4 && Tz == 4) || (T == 8 && (Tz == 8 || Tz == 4)), ECSPRT_ERR_ARG,
             "tile shapes: 4x4x4, 8x8x8 or 8x8x4 cells");
   ECS_CHECK(c->o.halo <= 0 || c->o.halo == 2, ECSPRT_ERR_ARG, "the deposit halo is 2 cells");
@@ -793,7 +801,8 @@ void tiles_init(ecsprt_hip_ctx *c) {
     if (split < 1) split = 1;
   }
   g.split = split;
-  int block = c->o.block > 0 ? c->o.block : 1024;
+  int block = c->o.block > 0 ? c->o.block : ((auto_shape && T == 8 && !c->mixed) ? 512 : 1024);
+  c->block = block;
   ECS_CHECK(block % 64 == 0 && block >= 128 && block <= 1024, ECSPRT_ERR_ARG,
             "block must be a multiple of 64 in [128, 1024]");
   size_t lds = lds_for(g, M_PUSH | M_COLLIDE | M_DEPOSIT, block);
@@ -918,7 +927,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
       mx_alloc(c->Sm, c->scap);
     }
   } else {
-    if (total > c->pcap || !c->A.x) {
+    if (total > c->pcap || !c->A.p) {
       alloc_particles(c->A, total);
       alloc_particles(c->B, total);
       if (philox) alloc_particles(c->D, total);
@@ -927,7 +936,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
       c->d_late = nullptr;
       ECS_HIP(hipMalloc(&c->d_late, sizeof(int) * (total / 4)));
     }
-    if (!c->E.x || c->np != np) {
+    if (!c->E.p || c->np != np) {
       alloc_particles(c->E, np);
       c->scap = std::max(np / 64, 65536L);
       alloc_particles(c->S, c->scap);
@@ -996,7 +1005,7 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
 
 template <int MODE>
 static void launch_tile(ecsprt_hip_ctx *c, const TileArgs &a) {
-  const int block = c->o.block > 0 ? c->o.block : 1024;
+  const int block = c->block;
   const size_t lds = lds_for(c->g, MODE, block);
   // Tile edge and halo are compile-time so LDS offsets become immediates.
   const int tx = c->g.Tx, tz = c->g.Tz;

@@ -25,7 +25,7 @@ struct MxConsts {
   float inv_vmax;  // 1 / vmax: fixed-point flux units are vmax * 2^-30
 };
 
-// Fixed-point deposit: a weight w in [0, 1] is stored as rint(w * 2^30) and a flux
+// Fixed-point deposit: a weight w in [0, 1] is stored as trunc(w * 2^30) and a flux
 // contribution v*w as rint(v/vmax * w * 2^30); |v| < vmax keeps both below 2^30.  Ions
 // with a faster component take the FP64 overflow path.
 constexpr float MX_WSCALE = 1073741824.0f;  // 2^30
@@ -75,6 +75,14 @@ __device__ inline void load_mx(const ParticlesMix &P, long i, uint32_t &cell, fl
   id = (int)b[64 * MX_ID];
 }
 
+// Fixed-point LDS accumulation; the conversion rule is fixed, so sums stay exact and
+// independent of order.  Weights are non-negative: truncated (one instruction, a bias
+// below 2^-31 per weight) and zero-extended.  Fluxes are signed: rounded to nearest, so
+// their sums carry no bias toward zero.
+__device__ inline void lds_addw(long long *p, float scaled) {
+  const unsigned long long q = (unsigned long long)__float2uint_rz(scaled);
+  __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ inline void lds_addq(long long *p, float scaled) {
   const long long q = (long long)__float2int_rn(scaled);
   __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), (unsigned long long)q, __ATOMIC_RELAXED,
@@ -189,7 +197,7 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
         const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
         const int idx = base + (az * RAY + ay) * RAX + ax;
         const float wt = w8[c];
-        lds_addq(&w.sacc[idx], wt);
+        lds_addw(&w.sacc[idx], wt);
         lds_addq(&w.sacc[RA3 + idx], ux * wt);
         lds_addq(&w.sacc[2 * RA3 + idx], uy * wt);
         lds_addq(&w.sacc[3 * RA3 + idx], uz * wt);
@@ -208,8 +216,7 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
       code = (sx < 2 && sy < 2 && sz < 2) ? ((sz + 1) * 3 + (sy + 1)) * 3 + (sx + 1) : 27;
     }
     int *cnt = w.s_cnt + 27 * w.par, *bas = w.s_base + 27 * w.par;
-    int rank = 0;
-    if (code < 27) rank = atomicAdd(&cnt[code], 1);
+    const int rank = rebin_rank(cnt, code);
     __syncthreads();
     if (tid < 27) {
       const int n = cnt[tid];
@@ -250,7 +257,7 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   const int nwg = gridDim.x;
   int wi = blockIdx.x;
   if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
-  constexpr int RPX = TX + 4, RPY = TY + 4, RPZ = TZ + 4, RP3 = RPX * RPY * RPZ;
+  constexpr int NEX = TX + 1, NEY = TY + 1, NE3 = NEX * NEY * (TZ + 1);
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
   TileWGMx w;
   w.t = wi / g.split;
@@ -268,18 +275,21 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   w.s_base = s_base;
   w.s_nbr = s_nbr;
   w.par = 0;
-  // LDS: the accumulators first (8-byte aligned), then the FP32 potential.
+  // LDS: the accumulators first (8-byte aligned), then the gather's difference
+  // numerators at the tile's (T+1)^3 corner nodes (FP64 differences rounded to FP32).
   w.sacc = smemq;
-  float *sphi = reinterpret_cast<float *>(smemq + ((MODE & M_DEPOSIT) ? 4 * RA3 : 0));
+  float *sE = reinterpret_cast<float *>(smemq + ((MODE & M_DEPOSIT) ? 4 * RA3 : 0));
   const int tid = threadIdx.x;
 
   if (MODE & M_PUSH) {
-    for (int q = tid; q < RP3; q += blockDim.x) {
-      int li = q % RPX, lj = (q / RPX) % RPY, lk = q / (RPX * RPY);
-      int gi = wrap_index(w.i0 - 1 + li, g.Nx);
-      int gj = wrap_index(w.j0 - 1 + lj, g.Ny);
-      int gk = wrap_index(w.k0 - 1 + lk, g.Nz);
-      sphi[q] = (float)a.phi[((long)gk * g.Ny + gj) * g.Nx + gi];
+    const PhiGlobal gphi{a.phi, g.Nx, g.Ny, g.Nz};
+    for (int q = tid; q < NE3; q += blockDim.x) {
+      const int li = q % NEX, lj = (q / NEX) % NEY, lk = q / (NEX * NEY);
+      float d[3];
+      node_diff<float>(gphi, g.Nx, g.Ny, g.Nz, w.i0 + li, w.j0 + lj, w.k0 + lk, d);
+      sE[q] = d[0];
+      sE[NE3 + q] = d[1];
+      sE[2 * NE3 + q] = d[2];
     }
   }
   if (MODE & M_DEPOSIT) {
@@ -315,9 +325,10 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
     const int ix = mx_i(cell), iy = mx_j(cell), iz = mx_k(cell);
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body (src/velocities.c:342-356) in FP32.
+      // Cells of a tile's ions lie in the tile and below N, so every corner node is in
+      // the staged (T+1)^3 block.
       float E[3];
-      diff_vec3d_core<float, RPX, RPY>(sphi, w.i0 - 1, w.j0 - 1, w.k0 - 1, g.Nx, g.Ny, g.Nz, ix, iy, iz, ox, oy, oz,
-                                       E);
+      gather_nodes<float, NEX, NEY, NE3>(sE, ((iz - w.k0) * NEY + (iy - w.j0)) * NEX + (ix - w.i0), ox, oy, oz, E);
       const float vxm = vx + a.k.Escale[0] * E[0];
       const float vym = vy + a.k.Escale[1] * E[1];
       const float vzm = vz + a.k.Escale[2] * E[2];
@@ -440,7 +451,7 @@ __global__ void __launch_bounds__(256) k_late_mx(TileGeom g, ParticlesMix P, con
       const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
       const int idx = base0 + (az * NY + ay) * NX + ax;
       const float wt = w8[c];
-      lds_addq(&acc[idx], wt);
+      lds_addw(&acc[idx], wt);
       lds_addq(&acc[N3 + idx], ux * wt);
       lds_addq(&acc[2 * N3 + idx], uy * wt);
       lds_addq(&acc[3 * N3 + idx], uz * wt);
@@ -606,14 +617,14 @@ void mx_set_lds_attrs() {
 
 static size_t lds_mx(const TileGeom &g, int mode) {
   size_t b = 0;
-  if (mode & M_PUSH) b += sizeof(float) * (g.Tx + 4) * (g.Ty + 4) * (g.Tz + 4);
+  if (mode & M_PUSH) b += sizeof(float) * 3 * (g.Tx + 1) * (g.Ty + 1) * (g.Tz + 1);
   if (mode & M_DEPOSIT) b += sizeof(long long) * 4 * g.RAx * g.RAy * g.RAz;
   return b;
 }
 
 template <int MODE>
 static void launch_mx(ecsprt_hip_ctx *c, const TileArgsMx &a) {
-  const int block = c->o.block > 0 ? c->o.block : 1024;
+  const int block = c->block;
   const size_t lds = lds_mx(c->g, MODE);
   const int nwg = c->ntiles * c->g.split;
   const int tx = c->g.Tx, tz = c->g.Tz;

@@ -11,38 +11,41 @@
 
 namespace ecsprt {
 
-// Field gather, DifferenceVector3D (src/vector-math.c:596-722), from a potential
-// region staged in LDS.  F holds nodes [oi, oi+RP) x [oj, oj+RP) x [ok, ok+RP) of the
-// periodic potential (global, unwrapped indices), RPX x RPY nodes per plane.  The
+__host__ __device__ inline int wrap_index(int i, int n) {
+  int r = i % n;
+  return r < 0 ? r + n : r;
+}
+
+// Field gather, DifferenceVector3D (src/vector-math.c:596-722).  F(k, j, i) returns the
+// potential at global node (i, j, k) (unwrapped indices; the loader wraps them).  The
 // particle sits in cell (ixl, iyl, izl) at in-cell weights (wxh, wyh, wzh).  Branch
-// choices use the global indices exactly as the reference does; the interpolation
-// order keeps the reference's y->wyh, z->wxh, x->wzh weight assignment
-// (vector-math.c:645-651).  T is the arithmetic type: double for the FP64 path,
-// float for the mixed-precision path.
-template <typename T, int RPX, int RPY>
-__device__ inline void diff_vec3d_core(const T *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
-                                       int ixl, int iyl, int izl, T wxh, T wyh, T wzh, T f[3]) {
-#define PL(kk, jj, ii) F[(((kk)-ok) * RPY + ((jj)-oj)) * RPX + ((ii)-oi)]
+// choices use the global indices exactly as the reference does; the interpolation order
+// keeps the reference's y->wyh, z->wxh, x->wzh weight assignment (vector-math.c:645-651).
+// T is the arithmetic type.  The tile kernels use this form only for an ion whose
+// cell index reached N by rounding; all others go through gather_nodes below.
+template <typename T, typename L>
+__device__ inline void diff_vec3d_core(const L &F, int Nx, int Ny, int Nz, int ixl, int iyl, int izl, T wxh, T wyh,
+                                       T wzh, T f[3]) {
   const int ixh = ixl + 1, iyh = iyl + 1, izh = izl + 1;
   const T c3 = T(3), c4 = T(4), c1 = T(1);
   T hhh, lhh, hlh, llh, hhl, lhl, hll, lll;
   T whh, whl, wlh, wll, Ewh, Ewl;
-  // Particles reach this function only from inside their tile, so ixl >= 0 and the
-  // reference's forward-difference branch (vector-math.c:625-631) cannot occur.
-  hhl = PL(izh, iyh, ixl + 1) - PL(izh, iyh, ixl - 1);
-  lhl = PL(izl, iyh, ixl + 1) - PL(izl, iyh, ixl - 1);
-  hll = PL(izh, iyl, ixl + 1) - PL(izh, iyl, ixl - 1);
-  lll = PL(izl, iyl, ixl + 1) - PL(izl, iyl, ixl - 1);
+  // ixl >= 0 for every ion in the domain, so the reference's forward-difference branch
+  // (vector-math.c:625-631) cannot occur.
+  hhl = F(izh, iyh, ixl + 1) - F(izh, iyh, ixl - 1);
+  lhl = F(izl, iyh, ixl + 1) - F(izl, iyh, ixl - 1);
+  hll = F(izh, iyl, ixl + 1) - F(izh, iyl, ixl - 1);
+  lll = F(izl, iyl, ixl + 1) - F(izl, iyl, ixl - 1);
   if (ixh < Nx) {
-    hhh = PL(izh, iyh, ixh + 1) - PL(izh, iyh, ixh - 1);
-    lhh = PL(izl, iyh, ixh + 1) - PL(izl, iyh, ixh - 1);
-    hlh = PL(izh, iyl, ixh + 1) - PL(izh, iyl, ixh - 1);
-    llh = PL(izl, iyl, ixh + 1) - PL(izl, iyl, ixh - 1);
+    hhh = F(izh, iyh, ixh + 1) - F(izh, iyh, ixh - 1);
+    lhh = F(izl, iyh, ixh + 1) - F(izl, iyh, ixh - 1);
+    hlh = F(izh, iyl, ixh + 1) - F(izh, iyl, ixh - 1);
+    llh = F(izl, iyl, ixh + 1) - F(izl, iyl, ixh - 1);
   } else {
-    hhh = +c3 * PL(izh, iyh, ixh) - c4 * PL(izh, iyh, ixh - 1) + c1 * PL(izh, iyh, ixh - 2);
-    lhh = +c3 * PL(izl, iyh, ixh) - c4 * PL(izl, iyh, ixh - 1) + c1 * PL(izl, iyh, ixh - 2);
-    hlh = +c3 * PL(izh, iyl, ixh) - c4 * PL(izh, iyl, ixh - 1) + c1 * PL(izh, iyl, ixh - 2);
-    llh = +c3 * PL(izl, iyl, ixh) - c4 * PL(izl, iyl, ixh - 1) + c1 * PL(izl, iyl, ixh - 2);
+    hhh = +c3 * F(izh, iyh, ixh) - c4 * F(izh, iyh, ixh - 1) + c1 * F(izh, iyh, ixh - 2);
+    lhh = +c3 * F(izl, iyh, ixh) - c4 * F(izl, iyh, ixh - 1) + c1 * F(izl, iyh, ixh - 2);
+    hlh = +c3 * F(izh, iyl, ixh) - c4 * F(izh, iyl, ixh - 1) + c1 * F(izh, iyl, ixh - 2);
+    llh = +c3 * F(izl, iyl, ixh) - c4 * F(izl, iyl, ixh - 1) + c1 * F(izl, iyl, ixh - 2);
   }
   whh = hlh + wyh * (hhh - hlh);
   whl = hll + wyh * (hhl - hll);
@@ -52,20 +55,20 @@ __device__ inline void diff_vec3d_core(const T *__restrict__ F, int oi, int oj, 
   Ewl = wll + wxh * (whl - wll);
   f[0] = Ewl + wzh * (Ewh - Ewl);
 
-  hlh = PL(izh, iyl + 1, ixh) - PL(izh, iyl - 1, ixh);
-  llh = PL(izl, iyl + 1, ixh) - PL(izl, iyl - 1, ixh);
-  hll = PL(izh, iyl + 1, ixl) - PL(izh, iyl - 1, ixl);
-  lll = PL(izl, iyl + 1, ixl) - PL(izl, iyl - 1, ixl);
+  hlh = F(izh, iyl + 1, ixh) - F(izh, iyl - 1, ixh);
+  llh = F(izl, iyl + 1, ixh) - F(izl, iyl - 1, ixh);
+  hll = F(izh, iyl + 1, ixl) - F(izh, iyl - 1, ixl);
+  lll = F(izl, iyl + 1, ixl) - F(izl, iyl - 1, ixl);
   if (iyh < Ny) {
-    hhh = PL(izh, iyh + 1, ixh) - PL(izh, iyh - 1, ixh);
-    lhh = PL(izl, iyh + 1, ixh) - PL(izl, iyh - 1, ixh);
-    hhl = PL(izh, iyh + 1, ixl) - PL(izh, iyh - 1, ixl);
-    lhl = PL(izl, iyh + 1, ixl) - PL(izl, iyh - 1, ixl);
+    hhh = F(izh, iyh + 1, ixh) - F(izh, iyh - 1, ixh);
+    lhh = F(izl, iyh + 1, ixh) - F(izl, iyh - 1, ixh);
+    hhl = F(izh, iyh + 1, ixl) - F(izh, iyh - 1, ixl);
+    lhl = F(izl, iyh + 1, ixl) - F(izl, iyh - 1, ixl);
   } else {
-    hhh = +c3 * PL(izh, iyh, ixh) - c4 * PL(izh, iyh - 1, ixh) + c1 * PL(izh, iyh - 2, ixh);
-    lhh = +c3 * PL(izl, iyh, ixh) - c4 * PL(izl, iyh - 1, ixh) + c1 * PL(izl, iyh - 2, ixh);
-    hhl = +c3 * PL(izh, iyh, ixl) - c4 * PL(izh, iyh - 1, ixl) + c1 * PL(izh, iyh - 2, ixl);
-    lhl = +c3 * PL(izl, iyh, ixl) - c4 * PL(izl, iyh - 1, ixl) + c1 * PL(izl, iyh - 2, ixl);
+    hhh = +c3 * F(izh, iyh, ixh) - c4 * F(izh, iyh - 1, ixh) + c1 * F(izh, iyh - 2, ixh);
+    lhh = +c3 * F(izl, iyh, ixh) - c4 * F(izl, iyh - 1, ixh) + c1 * F(izl, iyh - 2, ixh);
+    hhl = +c3 * F(izh, iyh, ixl) - c4 * F(izh, iyh - 1, ixl) + c1 * F(izh, iyh - 2, ixl);
+    lhl = +c3 * F(izl, iyh, ixl) - c4 * F(izl, iyh - 1, ixl) + c1 * F(izl, iyh - 2, ixl);
   }
   whh = hlh + wyh * (hhh - hlh);
   whl = hll + wyh * (hhl - hll);
@@ -75,20 +78,20 @@ __device__ inline void diff_vec3d_core(const T *__restrict__ F, int oi, int oj, 
   Ewl = wll + wxh * (whl - wll);
   f[1] = Ewl + wzh * (Ewh - Ewl);
 
-  lhh = PL(izl + 1, iyh, ixh) - PL(izl - 1, iyh, ixh);
-  llh = PL(izl + 1, iyl, ixh) - PL(izl - 1, iyl, ixh);
-  lhl = PL(izl + 1, iyh, ixl) - PL(izl - 1, iyh, ixl);
-  lll = PL(izl + 1, iyl, ixl) - PL(izl - 1, iyl, ixl);
+  lhh = F(izl + 1, iyh, ixh) - F(izl - 1, iyh, ixh);
+  llh = F(izl + 1, iyl, ixh) - F(izl - 1, iyl, ixh);
+  lhl = F(izl + 1, iyh, ixl) - F(izl - 1, iyh, ixl);
+  lll = F(izl + 1, iyl, ixl) - F(izl - 1, iyl, ixl);
   if (izh < Nz) {
-    hhh = PL(izh + 1, iyh, ixh) - PL(izh - 1, iyh, ixh);
-    hlh = PL(izh + 1, iyl, ixh) - PL(izh - 1, iyl, ixh);
-    hhl = PL(izh + 1, iyh, ixl) - PL(izh - 1, iyh, ixl);
-    hll = PL(izh + 1, iyl, ixl) - PL(izh - 1, iyl, ixl);
+    hhh = F(izh + 1, iyh, ixh) - F(izh - 1, iyh, ixh);
+    hlh = F(izh + 1, iyl, ixh) - F(izh - 1, iyl, ixh);
+    hhl = F(izh + 1, iyh, ixl) - F(izh - 1, iyh, ixl);
+    hll = F(izh + 1, iyl, ixl) - F(izh - 1, iyl, ixl);
   } else {
-    hhh = +c3 * PL(izh, iyh, ixh) - c4 * PL(izh - 1, iyh, ixh) + c1 * PL(izh - 2, iyh, ixh);
-    hlh = +c3 * PL(izh, iyl, ixh) - c4 * PL(izh - 1, iyl, ixh) + c1 * PL(izh - 2, iyl, ixh);
-    hhl = +c3 * PL(izh, iyh, ixl) - c4 * PL(izh - 1, iyh, ixl) + c1 * PL(izh - 2, iyh, ixl);
-    hll = +c3 * PL(izh, iyl, ixl) - c4 * PL(izh - 1, iyl, ixl) + c1 * PL(izh - 2, iyl, ixl);
+    hhh = +c3 * F(izh, iyh, ixh) - c4 * F(izh - 1, iyh, ixh) + c1 * F(izh - 2, iyh, ixh);
+    hlh = +c3 * F(izh, iyl, ixh) - c4 * F(izh - 1, iyl, ixh) + c1 * F(izh - 2, iyl, ixh);
+    hhl = +c3 * F(izh, iyh, ixl) - c4 * F(izh - 1, iyh, ixl) + c1 * F(izh - 2, iyh, ixl);
+    hll = +c3 * F(izh, iyl, ixl) - c4 * F(izh - 1, iyl, ixl) + c1 * F(izh - 2, iyl, ixl);
   }
   whh = hlh + wyh * (hhh - hlh);
   whl = hll + wyh * (hhl - hll);
@@ -97,17 +100,49 @@ __device__ inline void diff_vec3d_core(const T *__restrict__ F, int oi, int oj, 
   Ewh = wlh + wxh * (whh - wlh);
   Ewl = wll + wxh * (whl - wll);
   f[2] = Ewl + wzh * (Ewh - Ewl);
-#undef PL
 }
 
-// FP64 gather at normalised coordinates (x0, y0, z0) = pos/d: il = (int)x, w = x - il
-// (vector-math.c:607-617).
-template <int RPX, int RPY>
-__device__ inline void diff_vec3d_lds(const double *__restrict__ F, int oi, int oj, int ok, int Nx, int Ny, int Nz,
-                                      double x0, double y0, double z0, double f[3]) {
-  const int ixl = (int)x0, iyl = (int)y0, izl = (int)z0;
-  diff_vec3d_core<double, RPX, RPY>(F, oi, oj, ok, Nx, Ny, Nz, ixl, iyl, izl, x0 - (double)ixl, y0 - (double)iyl,
-                                    z0 - (double)izl, f);
+// Potential loader over the global periodic array (any unwrapped index).
+struct PhiGlobal {
+  const double *phi;
+  int Nx, Ny, Nz;
+  __device__ double operator()(int k, int j, int i) const {
+    return phi[((long)wrap_index(k, Nz) * Ny + wrap_index(j, Ny)) * Nx + wrap_index(i, Nx)];
+  }
+};
+
+// The difference numerators DifferenceVector3D takes at one grid node, per component:
+// F[i+1]-F[i-1] for a node below N (the reference's central branches at both the low
+// and the high corner, vector-math.c:619-637), the backward 3F[i]-4F[i-1]+F[i-2] for a
+// node index equal to N (its high-corner branch, :638-644).  These depend on the node
+// alone, so a tile computes them once for its (T+1)^3 corner nodes and every ion reads
+// eight of them.  Same expressions, same order: FP64 results are bit-identical.
+template <typename T, typename L>
+__device__ inline void node_diff(const L &F, int Nx, int Ny, int Nz, int i, int j, int k, T d[3]) {
+  const T c3 = T(3), c4 = T(4), c1 = T(1);
+  d[0] = (i < Nx) ? T(F(k, j, i + 1) - F(k, j, i - 1)) : T(+c3 * F(k, j, i) - c4 * F(k, j, i - 1) + c1 * F(k, j, i - 2));
+  d[1] = (j < Ny) ? T(F(k, j + 1, i) - F(k, j - 1, i)) : T(+c3 * F(k, j, i) - c4 * F(k, j - 1, i) + c1 * F(k, j - 2, i));
+  d[2] = (k < Nz) ? T(F(k + 1, j, i) - F(k - 1, j, i)) : T(+c3 * F(k, j, i) - c4 * F(k - 1, j, i) + c1 * F(k - 2, j, i));
+}
+
+// The gather from per-node differences: E holds NX x NY x NZ nodes per component
+// (component c at E + c*NE3), base is the index of the ion's low corner.  Corner names
+// are (z, y, x) low/high as in the reference; interpolation as vector-math.c:645-651.
+template <typename T, int NX, int NY, int NE3>
+__device__ inline void gather_nodes(const T *__restrict__ E, int base, T wxh, T wyh, T wzh, T f[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T *e = E + c * NE3 + base;
+    const T lll = e[0], llh = e[1], lhl = e[NX], lhh = e[NX + 1];
+    const T hll = e[NX * NY], hlh = e[NX * NY + 1], hhl = e[NX * NY + NX], hhh = e[NX * NY + NX + 1];
+    const T whh = hlh + wyh * (hhh - hlh);
+    const T whl = hll + wyh * (hhl - hll);
+    const T wlh = llh + wyh * (lhh - llh);
+    const T wll = lll + wyh * (lhl - lll);
+    const T Ewh = wlh + wxh * (whh - wlh);
+    const T Ewl = wll + wxh * (whl - wll);
+    f[c] = Ewl + wzh * (Ewh - Ewl);
+  }
 }
 
 // Collision constants (src/collisions.c:18-52).
@@ -283,9 +318,5 @@ __host__ __device__ inline long pick_ip(long np, float u) {
   return ip;
 }
 
-__host__ __device__ inline int wrap_index(int i, int n) {
-  int r = i % n;
-  return r < 0 ? r + n : r;
-}
 
 }  // namespace ecsprt

@@ -40,6 +40,25 @@ __device__ inline void lds_add(double *p, double v) {
 // one cell per step): global FP64 atomics into the overflow moments.  Kept out of
 // line so the LDS path stays free of its address arithmetic.
 // Periodic index for i within one period of [0, n) (no integer division).
+// Rank of this ion among the workgroup's ions bound for destination code (0..26; codes
+// 27 and 28 take none), from the LDS counters cnt.  Most ions stay in their tile (code
+// 13): a wave reserves their ranks with one atomic and numbers them by lane, where 64
+// atomics on one LDS address would be serialised.  Called by every lane of the wave.
+__device__ inline int rebin_rank(int *cnt, int code) {
+  const unsigned long long stay = __ballot(code == 13);
+  int rank = 0;
+  if (code < 27 && code != 13) rank = atomicAdd(&cnt[code], 1);
+  if (stay) {
+    const int below =
+        (int)__builtin_amdgcn_mbcnt_hi((unsigned)(stay >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)stay, 0u));
+    int b = 0;
+    if (code == 13 && below == 0) b = atomicAdd(&cnt[13], (int)__popcll(stay));
+    b = __builtin_amdgcn_readlane(b, __ffsll((long long)stay) - 1);
+    if (code == 13) rank = b + below;
+  }
+  return rank;
+}
+
 __device__ inline int wrap_near(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 
 __device__ inline void deposit_overflow(double *ovf, const TileGeom &g, int ixl, int iyl, int izl,
