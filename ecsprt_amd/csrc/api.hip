@@ -163,6 +163,7 @@ int ecsprt_hip_default_options(ecsprt_options *o) {
   o->inner_rtol = 1e-4;
   o->timing = 0;
   o->particle_precision = ECSPRT_PARTICLES_FP64;
+  o->mg_sweep = 0;
   return ECSPRT_OK;
 }
 
@@ -180,6 +181,7 @@ int ecsprt_hip_create(const ecsprt_params *p, const ecsprt_options *opt, ecsprt_
     ECS_CHECK(c->o.particle_precision == ECSPRT_PARTICLES_FP64 || c->o.particle_precision == ECSPRT_PARTICLES_MIXED,
               ECSPRT_ERR_ARG, "particle_precision must be ECSPRT_PARTICLES_FP64 or ECSPRT_PARTICLES_MIXED");
     c->mixed = c->o.particle_precision == ECSPRT_PARTICLES_MIXED;
+    ECS_CHECK(c->o.mg_sweep >= 0 && c->o.mg_sweep <= 2, ECSPRT_ERR_ARG, "mg_sweep must be 0, 1 or 2");
     if (c->mixed) {
       ECS_CHECK(p->Nx <= ecsprt::MX_MAX_NX && p->Ny <= ecsprt::MX_MAX_NY && p->Nz <= ecsprt::MX_MAX_NZ, ECSPRT_ERR_ARG,
                 "mixed-precision particles need Nx, Ny <= 2048 and Nz <= 1024");

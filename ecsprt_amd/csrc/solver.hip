@@ -192,7 +192,122 @@ __global__ void __launch_bounds__(512) k_sweep(StencilConsts s, const T *__restr
   }
 }
 
-constexpr int SW_TX = 16, SW_TY = 8, SW_TZ = 4;
+// Gauss-Seidel value of a node from three planes of density (d0 below, d1 at, d2 above
+// the node; row stride DX, node index pd) and of potential (x0, x1, x2; row stride XX,
+// node index px); `old` where the diagonal vanishes.  Same arithmetic as relax_lds.
+template <typename T, int DX, int XX>
+__device__ inline T relax_planes(const StencilConsts &s, const T *d0, const T *d1, const T *d2, int pd, const T *x0,
+                                 const T *x1, const T *x2, int px, T bq, T old) {
+  T c[11];
+  coeffs7(s, d1[pd], d1[pd - 1], d1[pd + 1], d1[pd - DX], d1[pd + DX], d0[pd], d2[pd], c);
+  if (c[10] == T(0)) return old;
+  const T off = c[0] * x1[px + 1] + c[1] * x1[px - 1] + c[2] * x1[px + XX] + c[3] * x1[px - XX] + c[4] * x2[px] +
+                c[5] * x0[px] + c[6] * x1[px + XX + 1] + c[7] * x1[px - XX + 1] + c[8] * x1[px + XX - 1] +
+                c[9] * x1[px - XX - 1];
+  return (bq - off) * smooth_rcp(c[10]);
+}
+
+// One red-black Gauss-Seidel sweep (colour `first`, then the other; colour of node
+// (i,j,k) = (i+j+k)&1) from xin (zero when null) into xout, marching along z.  A
+// workgroup owns a TX x TY column of nodes over KZ planes.  Planes of the old potential
+// and of the density enter LDS rings one at a time with an xy halo of 2 (the next
+// plane's loads are in flight while the current one is relaxed).  The first colour is
+// relaxed on the plane below the newest, over the tile plus a halo of 1, into a third
+// ring; the second colour on the plane below that, over the tile, from the third ring.
+// The first colour reads old values everywhere (the 11-point stencil's xy diagonals
+// join nodes of one colour), the second reads the first colour's new values and its
+// own colour's old ones, so the result does not depend on thread order.  Rings of 4, 5
+// and 4 planes keep every plane until no thread can still read it, with two barriers
+// per plane.
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ xin,
+                                                const T *__restrict__ b, T *__restrict__ xout, int first, int KZ) {
+  constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;  // potential and density planes
+  constexpr int RX = TX + 2, RY = TY + 2, RP = RX * RY;  // first-colour planes
+  constexpr int NXO = 4, ND = 5, NSN = 4;
+  constexpr int LPT = (HP + 511) / 512;
+  __shared__ T sxo[NXO * HP], sd[ND * HP], ssn[NSN * RP];
+  const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
+  const int ntx = (Nx + TX - 1) / TX, nty = (Ny + TY - 1) / TY;
+  const int t = blockIdx.x % (ntx * nty), kc = blockIdx.x / (ntx * nty);
+  const int i0 = (t % ntx) * TX, j0 = (t / ntx) * TY, k0 = kc * KZ;
+  const int ex = min(TX, Nx - i0), ey = min(TY, Ny - j0), ez = min(KZ, Nz - k0);
+  const int tid = threadIdx.x;
+  const long plane = (long)Nx * Ny;
+  int lq[LPT];
+  long goff[LPT];
+#pragma unroll
+  for (int u = 0; u < LPT; ++u) {
+    const int q = tid + u * 512;
+    const int li = q % HX - 2, lj = q / HX - 2;
+    lq[u] = (q < HP && li <= ex + 1 && lj <= ey + 1) ? q : -1;
+    goff[u] = (long)wrap1(j0 + lj, Ny) * Nx + wrap1(i0 + li, Nx);
+  }
+  T rx[LPT], rd[LPT];
+  const int Lbeg = k0 - 2, Lend = k0 + ez + 1;
+  auto fetch = [&](int L) {
+    const long zo = (long)wrap1(L, Nz) * plane;
+#pragma unroll
+    for (int u = 0; u < LPT; ++u)
+      if (lq[u] >= 0) {
+        rd[u] = dn[zo + goff[u]];
+        rx[u] = xin ? xin[zo + goff[u]] : T(0);
+      }
+  };
+  fetch(Lbeg);
+  for (int L = Lbeg; L <= Lend; ++L) {
+    const int r = L - Lbeg;
+    {
+      T *xo = sxo + (r % NXO) * HP, *dd = sd + (r % ND) * HP;
+#pragma unroll
+      for (int u = 0; u < LPT; ++u)
+        if (lq[u] >= 0) {
+          xo[lq[u]] = rx[u];
+          dd[lq[u]] = rd[u];
+        }
+    }
+    __syncthreads();
+    if (L < Lend) fetch(L + 1);
+    if (r >= 2) {  // first colour on plane L-1
+      const int r1 = r - 1;
+      const T *x0 = sxo + ((r1 - 1) % NXO) * HP, *x1 = sxo + (r1 % NXO) * HP, *x2 = sxo + ((r1 + 1) % NXO) * HP;
+      const T *d0 = sd + ((r1 - 1) % ND) * HP, *d1 = sd + (r1 % ND) * HP, *d2 = sd + ((r1 + 1) % ND) * HP;
+      T *sn = ssn + (r1 % NSN) * RP;
+      const int gk = wrap1(L - 1, Nz);
+      for (int q = tid; q < RP; q += 512) {
+        const int li = q % RX - 1, lj = q / RX - 1;
+        if (li > ex || lj > ey) continue;
+        const int p = (lj + 2) * HX + (li + 2);
+        const int gi = wrap1(i0 + li, Nx), gj = wrap1(j0 + lj, Ny);
+        T v = x1[p];
+        if (((gi + gj + gk) & 1) == first)
+          v = relax_planes<T, HX, HX>(s, d0, d1, d2, p, x0, x1, x2, p, b[gk * plane + (long)gj * Nx + gi], v);
+        sn[q] = v;
+      }
+    }
+    __syncthreads();
+    if (r >= 4) {  // second colour on plane L-2, written out
+      const int r2 = r - 2;
+      const T *s0 = ssn + ((r2 - 1) % NSN) * RP, *s1 = ssn + (r2 % NSN) * RP, *s2 = ssn + ((r2 + 1) % NSN) * RP;
+      const T *d0 = sd + ((r2 - 1) % ND) * HP, *d1 = sd + (r2 % ND) * HP, *d2 = sd + ((r2 + 1) % ND) * HP;
+      const int gk = L - 2;
+      for (int q = tid; q < TX * TY; q += 512) {
+        const int li = q % TX, lj = q / TX;
+        if (li >= ex || lj >= ey) continue;
+        const int pd = (lj + 2) * HX + (li + 2), ps = (lj + 1) * RX + (li + 1);
+        const int gi = i0 + li, gj = j0 + lj;
+        const long g = gk * plane + (long)gj * Nx + gi;
+        T v = s1[ps];
+        if (((gi + gj + gk) & 1) != first) v = relax_planes<T, HX, RX>(s, d0, d1, d2, pd, s0, s1, s2, ps, b[g], v);
+        xout[g] = v;
+      }
+    }
+  }
+}
+
+constexpr int SW_TX = 16, SW_TY = 8, SW_TZ = 4;  // box of the one-shot sweep
+constexpr int SZ_TX = 32, SZ_TY = 16;            // xy tile of the marching sweep
+constexpr long SZ_MIN_POINTS = 1L << 21;         // levels at least this large march
 constexpr int DENSE_MAX = 1024;                // largest coarsest level solved by a dense inverse
 constexpr size_t TAIL_LDS_MAX = 140 * 1024;    // LDS of the single-workgroup tail
 
@@ -798,10 +913,28 @@ static void mg_set_density(ecsprt_hip_ctx *c, MG<T> &mg) {
 template <typename T>
 static void remove_mean_dev(ecsprt_hip_ctx *c, double *partial, double *sumdst, T *v, long n);
 
+// z-chunk of the marching sweep: about 512 workgroups (two per CU), at least 6 planes
+// per chunk so the z halo of 4 planes stays a modest overhead.
+static int sweep_kz(int Nx, int Ny, int Nz) {
+  const int nxy = ((Nx + SZ_TX - 1) / SZ_TX) * ((Ny + SZ_TY - 1) / SZ_TY);
+  const int chunks = std::max(1, 512 / nxy);
+  return std::min(Nz, std::max(6, (Nz + chunks - 1) / chunks));
+}
+
+// The marching sweep loads each node about 2.1 times at 128^3 (the box sweep 3.75
+// times) but runs its planes in sequence; it is faster only where the level has
+// enough points to keep every CU streaming (measured: 33 against 43 us at 2^21 FP64
+// points, slower from 2^20 points down).
 template <typename T>
-static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first) {
-  const int nt = ((L.s.Nx + SW_TX - 1) / SW_TX) * ((L.s.Ny + SW_TY - 1) / SW_TY) * ((L.s.Nz + SW_TZ - 1) / SW_TZ);
-  k_sweep<T, SW_TX, SW_TY, SW_TZ><<<nt, 512, 0, st>>>(L.s, L.dn, xin, b, xout, first);
+static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first, int mode) {
+  if (mode == 2 || (mode == 0 && (long)L.s.Nx * L.s.Ny * L.s.Nz >= SZ_MIN_POINTS)) {
+    const int kz = sweep_kz(L.s.Nx, L.s.Ny, L.s.Nz);
+    const int nt = ((L.s.Nx + SZ_TX - 1) / SZ_TX) * ((L.s.Ny + SZ_TY - 1) / SZ_TY) * ((L.s.Nz + kz - 1) / kz);
+    k_sweep_z<T, SZ_TX, SZ_TY><<<nt, 512, 0, st>>>(L.s, L.dn, xin, b, xout, first, kz);
+  } else {
+    const int nt = ((L.s.Nx + SW_TX - 1) / SW_TX) * ((L.s.Ny + SW_TY - 1) / SW_TY) * ((L.s.Nz + SW_TZ - 1) / SW_TZ);
+    k_sweep<T, SW_TX, SW_TY, SW_TZ><<<nt, 512, 0, st>>>(L.s, L.dn, xin, b, xout, first);
+  }
 }
 
 // One V-cycle: z = M^-1 r (r, z on level 0).  Each level ping-pongs between two
@@ -826,7 +959,7 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
     const T *in = nullptr;
     for (int k = 1; k <= nsw; ++k) {
       T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
-      launch_sweep(st, L, in, rhs(l), out, 0);
+      launch_sweep(st, L, in, rhs(l), out, 0, c->o.mg_sweep);
       in = out;
     }
     cur[l] = const_cast<T *>(in);
@@ -859,7 +992,7 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
     const int total = pre + post;
     for (int k = pre + 1; k <= total; ++k) {
       T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
-      launch_sweep(st, L, in, rhs(l), out, 1);
+      launch_sweep(st, L, in, rhs(l), out, 1, c->o.mg_sweep);
       in = out;
     }
     ECS_HIP(hipGetLastError());

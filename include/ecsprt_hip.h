@@ -119,6 +119,8 @@ typedef struct {
   double   inner_rtol;     /* mixed precision: inner FP32 solve tolerance (0 = 1e-4) */
   int      timing;         /* 1 = record per-stage HIP events */
   int      particle_precision; /* ECSPRT_PARTICLES_* */
+  int      mg_sweep;       /* smoother kernels: 0 = by level size, 1 = box sweeps, 2 = marching sweeps
+                              (identical arithmetic; a choice of speed only) */
 } ecsprt_options;
 
 typedef struct ecsprt_hip_ctx ecsprt_hip_ctx;
