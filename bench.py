@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--split", type=int, default=0)
     ap.add_argument("--collide", default="philox", choices=["philox", "off"])
     ap.add_argument("--mg-pre", type=int, default=0)
+    ap.add_argument("--inner-rtol", type=float, default=0.0, help="mixed solve: inner FP32 tolerance (0 = library's)")
     ap.add_argument("--mg-post", type=int, default=0)
     ap.add_argument("--autotune", action="store_true",
                     help="pick the tile shape and block size of each particle path with ecsprt_amd.autotune")
@@ -386,6 +387,8 @@ def main():
                   block=args.block, split=args.split)
     if args.mg_pre:
         common["mg_pre"] = args.mg_pre
+    if args.inner_rtol:
+        common["inner_rtol"] = args.inner_rtol
     if args.mg_post:
         common["mg_post"] = args.mg_post
     mixed_only = args.workload == "mixed"

@@ -160,7 +160,7 @@ int ecsprt_hip_default_options(ecsprt_options *o) {
   o->mg_pre = 2;
   o->mg_post = 2;
   o->mg_coarse_sweeps = 50;
-  o->inner_rtol = 1e-4;
+  o->inner_rtol = 0.0;
   o->timing = 0;
   o->particle_precision = ECSPRT_PARTICLES_FP64;
   o->mg_sweep = 0;

@@ -554,6 +554,52 @@ __global__ void k_sub_multi(int m, T *const *__restrict__ V, const double *__res
   }
 }
 
+// Fused Gram-Schmidt step for m <= 8 basis vectors: w -= sum_q h[q] V[q] (as
+// k_sub_multi), then per block either the products V[q].w (norm == 0; partial[b*m+q]) or
+// w.w (norm == 1; partial[b]) of the updated, stored w, reduced as k_dot_multi does.
+// One pass over V and w instead of two.
+template <typename T>
+__global__ void __launch_bounds__(RED_THREADS) k_sub_dot_multi(int m, T *const *__restrict__ V,
+                                                               const double *__restrict__ h, T *__restrict__ w, long n,
+                                                               double *__restrict__ partial, int norm) {
+  __shared__ double red[RED_THREADS];
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double hq[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) hq[q] = q < m ? h[q] : 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double v[8];
+    double a = (double)w[i];
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < m) {
+        v[q] = (double)V[q][i];
+        a -= hq[q] * v[q];
+      }
+    const T wt = (T)a;
+    w[i] = wt;
+    const double wn = (double)wt;
+    if (norm) {
+      acc[0] += wn * wn;
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < m) acc[q] += v[q] * wn;
+    }
+  }
+  const int mq = norm ? 1 : m;
+  for (int q = 0; q < mq; ++q) {
+    red[threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(long)blockIdx.x * mq + q] = red[0];
+    __syncthreads();
+  }
+}
+
 // x += sum_q y[q] Z[q]
 template <typename T>
 __global__ void k_add_multi(int m, T *const *__restrict__ Z, const double *__restrict__ y, T *__restrict__ x, long n,
@@ -1157,12 +1203,21 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       precond<T>(c, mg, g.V[j], g.Z[j]);
       k_apply<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, g.Z[j], g.w);
       ECS_HIP(hipGetLastError());
-      // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2.
+      // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2; |w|^2.  Up to 8 basis
+      // vectors the subtractions carry the following products in the same pass.
       dots<T>(c, g, j + 1, g.dV, g.w, g.dh, 0);
-      k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
-      dots<T>(c, g, j + 1, g.dV, g.w, g.dh + (m + 2), 0);
-      k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n);
-      dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
+      if (j + 1 <= 8) {
+        const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
+        k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh, g.w, n, g.partial, 0);
+        k_reduce_partials<<<1, 256, 0, st>>>(j + 1, nb, g.partial, g.dh + (m + 2), 0);
+        k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n, g.partial, 1);
+        k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (j + 1), 0);
+      } else {
+        k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
+        dots<T>(c, g, j + 1, g.dV, g.w, g.dh + (m + 2), 0);
+        k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n);
+        dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
+      }
       ECS_HIP(hipGetLastError());
       ECS_HIP(hipMemcpyAsync(g.hpin, g.dh, sizeof(double) * 2 * (m + 2), hipMemcpyDeviceToHost, st));
       ECS_HIP(hipStreamSynchronize(st));
@@ -1363,7 +1418,15 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
   int iters = 0;
   if (c->o.solve_precision == ECSPRT_SOLVE_MIXED && s->have32) {
     // FP64 iterative refinement around an FP32 GMRES+GMG inner solve.
-    const double inner = c->o.inner_rtol > 0 ? c->o.inner_rtol : 1e-4;
+    // Inner tolerance: the outer target split evenly over the fewest refinement cycles
+    // that each ask for at most 1e-5 (FP32 resolves that comfortably): 1e-5 twice for
+    // rtol 1e-10, 1e-4 twice for 1e-8.
+    double inner = c->o.inner_rtol;
+    if (inner <= 0) {
+      const double lr = -std::log10(std::max(rtol, 1e-15));
+      const int cycles = std::max(1, (int)std::ceil(lr / 5.0 - 1e-9));
+      inner = std::pow(10.0, -lr / cycles);
+    }
     Level<double> &L0 = s->mg64.L[0];
     for (int outer = 0; outer < 100; ++outer) {
       const Launch3 g3 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);

@@ -116,7 +116,8 @@ typedef struct {
   double   ksp_atol;       /* (0 = 1e-50) */
   int      mg_pre, mg_post;/* smoothing sweeps (0 = 2 each) */
   int      mg_coarse_sweeps;/* sweeps on the coarsest level (0 = 50) */
-  double   inner_rtol;     /* mixed precision: inner FP32 solve tolerance (0 = 1e-4) */
+  double   inner_rtol;     /* mixed precision: inner FP32 solve tolerance (0 = ksp_rtol^(1/k), the fewest
+                              k refinement cycles of at most 1e-5 each) */
   int      timing;         /* 1 = record per-stage HIP events */
   int      particle_precision; /* ECSPRT_PARTICLES_* */
   int      mg_sweep;       /* smoother kernels: 0 = by level size, 1 = box sweeps, 2 = marching sweeps
