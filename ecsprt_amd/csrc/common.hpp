@@ -153,8 +153,7 @@ struct ecsprt_hip_ctx {
   double *d_moments = nullptr;  // [Nz][Ny][Nx][4]
   double *d_fold = nullptr;     // per workgroup accumulation regions
   double *d_ovf = nullptr;      // [Nz][Ny][Nx][4] overflow deposits (global atomics)
-  long long *d_ovfq = nullptr;  // mixed path: [Nz][Ny][Nx][4] fixed-point deposits of listed ions
-  int *d_late = nullptr;        // per-tile lists of ions whose deposit is done by k_late (cap/4 each)
+  void *d_late = nullptr;       // per-tile lists of ions whose deposit is done by k_late (cap/4 entries each)
   int *d_late_cnt = nullptr;    // list lengths, per tile
   double *d_rhs = nullptr;      // [Nz][Ny][Nx]
   int *d_foldtab = nullptr;     // 3 axes x (count, pairs)

@@ -26,7 +26,8 @@ struct TileArgs {
   unsigned long long *ctr;
   const double *phi;
   double *fold, *ovf;
-  int *late, *late_cnt;  // per-tile lists of ions whose deposit waits for k_late
+  double *late;   // per-tile lists of ions whose deposit waits for k_late: x, y, z, vx, vy, vz each
+  int *late_cnt;  // list lengths
   PushConsts pc;
   DriftConsts dc;
   CollideConsts cc;
@@ -46,18 +47,15 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
 
 // Place a particle whose destination slot was reserved: into the destination tile
 // when there is room, else into the spill list (the host re-bins after the step).
-// Returns the ion's slot in the destination arrays, or -1 when it went to the spill list.
-__device__ inline long place(const TileArgs &a, int dt, int slot, double x, double y, double z, double vx, double vy,
+__device__ inline void place(const TileArgs &a, int dt, int slot, double x, double y, double z, double vx, double vy,
                              double vz, int id) {
   if (slot < a.cap[dt]) {
-    const long s = (long)a.base[dt] + slot;
-    store_particle(a.dst, s, x, y, z, vx, vy, vz, id);
-    return s;
+    store_particle(a.dst, (long)a.base[dt] + slot, x, y, z, vx, vy, vz, id);
+    return;
   }
   const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
   if ((long)sp < a.scap) store_particle(a.spill, (long)sp, x, y, z, vx, vy, vz, id);
   else atomicAdd(&a.ctr[2], 1ull);
-  return -1;
 }
 
 // Deposit of one ion with global FP64 atomics (Collect3DFluidMoments weights).
@@ -75,14 +73,20 @@ __device__ inline void deposit_direct(const TileArgs &a, double px, double py, d
   deposit_overflow(a.ovf, a.g, ixl, iyl, izl, w8, vx, vy, vz);
 }
 
-// An ion that left its source tile's deposit region is, once placed, inside its
-// destination tile: list its slot there (one atomic) for k_late, or deposit it now
-// when the tile's list is full.
-__device__ inline void defer_deposit(const TileArgs &a, int dt, long s, double px, double py, double pz, double vx,
+// An ion that left its source tile's deposit region lies in its destination tile dt:
+// append its position and velocity to dt's list (one atomic, 48 contiguous bytes) for
+// k_late, or deposit it now when the tile's list is full.
+__device__ inline void defer_deposit(const TileArgs &a, int dt, double px, double py, double pz, double vx,
                                      double vy, double vz) {
   const int li = atomicAdd(&a.late_cnt[dt], 1);
-  if (li < (a.cap[dt] >> 2)) a.late[(a.base[dt] >> 2) + li] = (int)s;
-  else deposit_direct(a, px, py, pz, vx, vy, vz);
+  if (li < (a.cap[dt] >> 2)) {
+    double2 *e = reinterpret_cast<double2 *>(a.late + 6 * ((long)(a.base[dt] >> 2) + li));
+    e[0] = make_double2(px, py);
+    e[1] = make_double2(pz, vx);
+    e[2] = make_double2(vy, vz);
+  } else {
+    deposit_direct(a, px, py, pz, vx, vy, vz);
+  }
 }
 
 // Per-workgroup tile constants and LDS pointers used by the particle tail.
@@ -185,11 +189,8 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     }
     __syncthreads();
     if (code < 27) {
-      const long s = place(a, w.s_nbr[code], bas[code] + rank, px, py, pz, vx, vy, vz, pid);
-      if (late) {
-        if (s >= 0) defer_deposit(a, w.s_nbr[code], s, px, py, pz, vx, vy, vz);
-        else deposit_direct(a, px, py, pz, vx, vy, vz);
-      }
+      place(a, w.s_nbr[code], bas[code] + rank, px, py, pz, vx, vy, vz, pid);
+      if (late) defer_deposit(a, w.s_nbr[code], px, py, pz, vx, vy, vz);
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
@@ -418,12 +419,15 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
 
 // Deposit of the ions listed for tile t: those that arrived from beyond their source
 // tile's deposit region.  They now lie in tile t, so FP64 LDS accumulators over its
-// (T+1)^3 nodes take them, and the region is added into the overflow moments with one
-// global atomic per non-zero node and moment.  Resets the tile's list.
-template <int TX, int TY, int TZ>
-__global__ void __launch_bounds__(256) k_late(TileGeom g, ParticlesF64 P, const int *__restrict__ late,
+// (T+1)^3 nodes take them, and the non-zero nodes are added into the tile's own slot of
+// the fold buffer (its deposit region contains those nodes; no other workgroup touches
+// the slot now), so k_fold sums them with everything else.  Resets the tile's list.
+template <int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(256) k_late(TileGeom g, const double *__restrict__ late,
                                              int *__restrict__ late_cnt, const int *__restrict__ base,
-                                             const int *__restrict__ cap, PushConsts pc, double *__restrict__ ovf) {
+                                             const int *__restrict__ cap, PushConsts pc, double *__restrict__ ovf,
+                                             double *__restrict__ fold) {
+  constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RA3 = RAX * RAY * (TZ + 1 + 2 * HH);
   constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
   __shared__ double acc[4 * N3];
   const int t = blockIdx.x;
@@ -434,10 +438,10 @@ __global__ void __launch_bounds__(256) k_late(TileGeom g, ParticlesF64 P, const 
   const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
   for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0.0;
   __syncthreads();
-  const int *list = late + (base[t] >> 2);
+  const double2 *list = reinterpret_cast<const double2 *>(late + 6 * (long)(base[t] >> 2));
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    const long s = list[q];
-    const double px = P.x(s), py = P.y(s), pz = P.z(s), vx = P.vx(s), vy = P.vy(s), vz = P.vz(s);
+    const double2 e0 = list[3 * q], e1 = list[3 * q + 1], e2 = list[3 * q + 2];
+    const double px = e0.x, py = e0.y, pz = e1.x, vx = e1.y, vy = e2.x, vz = e2.y;
     const double x = cell_coord(px, pc.dx, pc.inv[0], pc.pow2);
     const double y = cell_coord(py, pc.dy, pc.inv[1], pc.pow2);
     const double z = cell_coord(pz, pc.dz, pc.inv[2], pc.pow2);
@@ -463,12 +467,13 @@ __global__ void __launch_bounds__(256) k_late(TileGeom g, ParticlesF64 P, const 
     }
   }
   __syncthreads();
+  double *slot = fold + (size_t)t * g.split * 4 * RA3;
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int li = q % NX, lj = (q / NX) % NY, lk = q / (NX * NY);
-    const long node = ((long)wrap_near(k0 + lk, g.Nz) * g.Ny + wrap_near(j0 + lj, g.Ny)) * g.Nx + wrap_near(i0 + li, g.Nx);
+    const int r = ((lk + HH) * RAY + (lj + HH)) * RAX + (li + HH);
     for (int d = 0; d < 4; ++d) {
       const double v = acc[d * N3 + q];
-      if (v != 0.0) unsafeAtomicAdd(&ovf[node * 4 + d], v);
+      if (v != 0.0) slot[d * RA3 + r] += v;
     }
   }
 }
@@ -507,7 +512,7 @@ __global__ void k_finalize_counts(int ntiles, int *count, int *next, const int *
 // regions: the integer sum is exact, then scaled to weight (sn) and flux (sg) units.
 template <typename R>
 __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__restrict__ ovf,
-                       const long long *__restrict__ ovfq, const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m, double sn,
+                       const int *__restrict__ tab, int offx, int offy, int offz, double *__restrict__ m, double sn,
                        double sg) {
   const long nnode = (long)g.Nx * g.Ny * g.Nz;
   const int RAX = g.RAx, RAY = g.RAy, RA3 = g.RAx * g.RAy * g.RAz;
@@ -533,7 +538,6 @@ __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__r
           }
         }
     if constexpr (std::is_integral<R>::value) {
-      for (int d = 0; d < 4; ++d) acc[d] += ovfq[node * 4 + d];
       m[node * 4 + 0] = (double)acc[0] * sn + ovf[node * 4 + 0];
       for (int d = 1; d < 4; ++d) m[node * 4 + d] = (double)acc[d] * sg + ovf[node * 4 + d];
     } else {
@@ -693,11 +697,10 @@ void fill_i32(ecsprt_hip_ctx *c, int *p, long n, int v) {
   ECS_HIP(hipGetLastError());
 }
 
-// Overflow moments start each deposit at zero: FP64 atomics, and on the mixed path
-// the 64-bit fixed-point sums of listed ions (all-zero bits in both).
+// The overflow moments (FP64 atomics of ions deposited outside the tile regions) start
+// each deposit at zero.
 static void clear_overflow(ecsprt_hip_ctx *c) {
   fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
-  if (c->d_ovfq) fill_f64(c, reinterpret_cast<double *>(c->d_ovfq), 4 * c->nnode, 0.0);
 }
 void zero_ctr(ecsprt_hip_ctx *c) {
   k_fill_u64<<<1, 64, 0, c->stream>>>(c->d_ctr, 8);
@@ -846,7 +849,6 @@ void tiles_init(ecsprt_hip_ctx *c) {
   ECS_HIP(hipMalloc(&c->d_next, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_late_cnt, sizeof(int) * c->ntiles));
   fill_i32(c, c->d_late_cnt, c->ntiles, 0);
-  if (c->mixed) ECS_HIP(hipMalloc(&c->d_ovfq, sizeof(long long) * 4 * c->nnode));
   fill_i32(c, c->d_count, c->ntiles, 0);
   fill_i32(c, c->d_next, c->ntiles, 0);
   clear_overflow(c);
@@ -876,11 +878,14 @@ void tiles_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_next);
   (void)hipFree(c->d_late_cnt);
   (void)hipFree(c->d_late);
-  (void)hipFree(c->d_ovfq);
   (void)hipFree(c->d_foldtab);
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_key);
 }
+
+// Bytes per entry of the late-deposit lists: x, y, z, vx, vy, vz (FP64), or the packed
+// cell, offsets and velocities padded to eight words (mixed).
+static size_t late_entry_bytes(const ecsprt_hip_ctx *c) { return c->mixed ? 32 : 48; }
 
 void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
   ECS_CHECK(np > 0 && np < 2147483647L, ECSPRT_ERR_ARG, "np must be in [1, 2^31-1)");
@@ -919,7 +924,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
       c->pcap = total;
       (void)hipFree(c->d_late);
       c->d_late = nullptr;
-      ECS_HIP(hipMalloc(&c->d_late, sizeof(int) * (total / 4)));
+      ECS_HIP(hipMalloc(&c->d_late, late_entry_bytes(c) * (total / 4)));
     }
     if (!c->Em.p || c->np != np) {
       mx_alloc(c->Em, np);
@@ -934,7 +939,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
       c->pcap = total;
       (void)hipFree(c->d_late);
       c->d_late = nullptr;
-      ECS_HIP(hipMalloc(&c->d_late, sizeof(int) * (total / 4)));
+      ECS_HIP(hipMalloc(&c->d_late, late_entry_bytes(c) * (total / 4)));
     }
     if (!c->E.p || c->np != np) {
       alloc_particles(c->E, np);
@@ -997,7 +1002,7 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
   a.phi = c->d_phi;
   a.fold = c->d_fold;
   a.ovf = c->d_ovf;
-  a.late = c->d_late;
+  a.late = reinterpret_cast<double *>(c->d_late);
   a.late_cnt = c->d_late_cnt;
   a.pc = push_consts(c->p, 0.0);
   return a;
@@ -1087,16 +1092,17 @@ static void launch_late(ecsprt_hip_ctx *c) {
     return;
   }
   const PushConsts pc = push_consts(c->p, 0.0);
+  const double *late = reinterpret_cast<const double *>(c->d_late);
   const int tx = c->g.Tx, tz = c->g.Tz;
   if (tx == 8 && tz == 8)
-    k_late<8, 8, 8><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_late, c->d_late_cnt, c->d_base, c->d_cap, pc,
-                                                      c->d_ovf);
+    k_late<8, 8, 8, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base, c->d_cap,
+                                                         pc, c->d_ovf, c->d_fold);
   else if (tx == 8 && tz == 4)
-    k_late<8, 8, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_late, c->d_late_cnt, c->d_base, c->d_cap, pc,
-                                                      c->d_ovf);
+    k_late<8, 8, 4, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base, c->d_cap,
+                                                         pc, c->d_ovf, c->d_fold);
   else
-    k_late<4, 4, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->A, c->d_late, c->d_late_cnt, c->d_base, c->d_cap, pc,
-                                                      c->d_ovf);
+    k_late<4, 4, 4, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base, c->d_cap,
+                                                         pc, c->d_ovf, c->d_fold);
   ECS_HIP(hipGetLastError());
 }
 
@@ -1105,10 +1111,10 @@ static void launch_fold(ecsprt_hip_ctx *c) {
     const double vmax = mx_vmax(c->p);
     const double sc[4] = {0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30};
     k_fold<long long><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(
-        c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_ovfq, c->d_foldtab, c->foldtab_off[0],
+        c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_foldtab, c->foldtab_off[0],
         c->foldtab_off[1], c->foldtab_off[2], c->d_moments, sc[0], sc[1]);
   } else
-    k_fold<double><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, nullptr, c->d_foldtab,
+    k_fold<double><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, c->d_foldtab,
                                                                    c->foldtab_off[0], c->foldtab_off[1],
                                                                    c->foldtab_off[2], c->d_moments, 1.0, 1.0);
   ECS_HIP(hipGetLastError());

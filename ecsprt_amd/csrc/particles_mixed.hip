@@ -41,7 +41,8 @@ struct TileArgsMx {
   const double *phi;
   long long *fold;
   double *ovf;
-  int *late, *late_cnt;  // per-tile lists of ions whose deposit waits for k_late_mx
+  uint4 *late;    // per-tile lists of ions whose deposit waits for k_late_mx: two uint4 per ion
+  int *late_cnt;  // list lengths
   MxConsts k;
   CollideConsts cc;
   uint32_t key0, key1, step;
@@ -106,19 +107,16 @@ __device__ inline bool drift_axis(float &o, int &i, float d, int N) {
   return n >= 0 && n < N;
 }
 
-// Stores an ion in tile dt; returns its slot, or -1 when the tile is full and the ion
-// went to the spill list.
-__device__ inline long place_mx(const TileArgsMx &a, int dt, int slot, uint32_t cell, float ox, float oy, float oz,
+// Stores an ion in tile dt, or in the spill list when the tile is full.
+__device__ inline void place_mx(const TileArgsMx &a, int dt, int slot, uint32_t cell, float ox, float oy, float oz,
                                 float vx, float vy, float vz, int id) {
   if (slot < a.cap[dt]) {
-    const long s = (long)a.base[dt] + slot;
-    store_mx(a.dst, s, cell, ox, oy, oz, vx, vy, vz, id);
-    return s;
+    store_mx(a.dst, (long)a.base[dt] + slot, cell, ox, oy, oz, vx, vy, vz, id);
+    return;
   }
   const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
   if ((long)sp < a.scap) store_mx(a.spill, (long)sp, cell, ox, oy, oz, vx, vy, vz, id);
   else atomicAdd(&a.ctr[2], 1ull);
-  return -1;
 }
 
 // Collect3DFluidMoments weights (src/moments.c:164-184) from the in-cell offsets,
@@ -147,13 +145,19 @@ __device__ inline void deposit_direct_mx(const TileArgsMx &a, int ix, int iy, in
   deposit_overflow(a.ovf, a.g, ix, iy, iz, wd, (double)vx, (double)vy, (double)vz);
 }
 
-// An ion beyond its source tile's deposit region, now stored at slot s of tile dt:
-// list it for k_late_mx (one atomic), or deposit it directly when the list is full.
-__device__ inline void defer_deposit_mx(const TileArgsMx &a, int dt, long s, int ix, int iy, int iz, float ox,
-                                        float oy, float oz, float vx, float vy, float vz) {
+// An ion beyond its source tile's deposit region lies in its destination tile dt:
+// append cell, offsets and velocity to dt's list (one atomic, 32 contiguous bytes) for
+// k_late_mx, or deposit it directly when the list is full.
+__device__ inline void defer_deposit_mx(const TileArgsMx &a, int dt, int ix, int iy, int iz, float ox, float oy,
+                                        float oz, float vx, float vy, float vz) {
   const int li = atomicAdd(&a.late_cnt[dt], 1);
-  if (li < (a.cap[dt] >> 2)) a.late[(a.base[dt] >> 2) + li] = (int)s;
-  else deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+  if (li < (a.cap[dt] >> 2)) {
+    uint4 *e = a.late + 2 * ((long)(a.base[dt] >> 2) + li);
+    e[0] = make_uint4(mx_pack(ix, iy, iz), __float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz));
+    e[1] = make_uint4(__float_as_uint(vx), __float_as_uint(vy), __float_as_uint(vz), 0u);
+  } else {
+    deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+  }
 }
 
 struct TileWGMx {
@@ -227,11 +231,8 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     __syncthreads();
     if (code < 27) {
       const int dt = w.s_nbr[code];
-      const long s = place_mx(a, dt, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
-      if (late) {
-        if (s >= 0) defer_deposit_mx(a, dt, s, ix, iy, iz, ox, oy, oz, vx, vy, vz);
-        else deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
-      }
+      place_mx(a, dt, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+      if (late) defer_deposit_mx(a, dt, ix, iy, iz, ox, oy, oz, vx, vy, vz);
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_mx(a.emi, (long)e, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
@@ -413,12 +414,15 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
 // Emigrants further than one tile -> destination tiles.
 // Deposit of the ions listed for tile t (they arrived from beyond their source tile's
 // deposit region and now lie in tile t): fixed-point LDS sums over the tile's
-// (T+1)^3 nodes, added exactly into the 64-bit overflow moments.  Resets the list.
-template <int TX, int TY, int TZ>
-__global__ void __launch_bounds__(256) k_late_mx(TileGeom g, ParticlesMix P, const int *__restrict__ late,
+// (T+1)^3 nodes, added exactly into the tile's own slot of the fold buffer (its deposit
+// region contains those nodes; no other workgroup touches the slot now).  Resets the
+// list.
+template <int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(256) k_late_mx(TileGeom g, const uint4 *__restrict__ late,
                                                 int *__restrict__ late_cnt, const int *__restrict__ base,
                                                 const int *__restrict__ cap, float inv_vmax,
-                                                long long *__restrict__ ovfq, double *__restrict__ ovf) {
+                                                long long *__restrict__ fold, double *__restrict__ ovf) {
+  constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RA3 = RAX * RAY * (TZ + 1 + 2 * HH);
   constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
   __shared__ long long acc[4 * N3];
   const int t = blockIdx.x;
@@ -429,12 +433,12 @@ __global__ void __launch_bounds__(256) k_late_mx(TileGeom g, ParticlesMix P, con
   const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
   for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0;
   __syncthreads();
-  const int *list = late + (base[t] >> 2);
+  const uint4 *list = late + 2 * (long)(base[t] >> 2);
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    uint32_t cell;
-    float ox, oy, oz, vx, vy, vz;
-    int id;
-    load_mx(P, list[q], cell, ox, oy, oz, vx, vy, vz, id);
+    const uint4 e0 = list[2 * q], e1 = list[2 * q + 1];
+    const uint32_t cell = e0.x;
+    const float ox = __uint_as_float(e0.y), oy = __uint_as_float(e0.z), oz = __uint_as_float(e0.w);
+    const float vx = __uint_as_float(e1.x), vy = __uint_as_float(e1.y), vz = __uint_as_float(e1.z);
     float w8[8];
     weights_mx(ox, oy, oz, w8);
     const float ux = vx * inv_vmax, uy = vy * inv_vmax, uz = vz * inv_vmax;
@@ -458,13 +462,13 @@ __global__ void __launch_bounds__(256) k_late_mx(TileGeom g, ParticlesMix P, con
     }
   }
   __syncthreads();
+  long long *slot = fold + (size_t)t * g.split * 4 * RA3;
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int li = q % NX, lj = (q / NX) % NY, lk = q / (NX * NY);
-    const long node = ((long)wrap_near(k0 + lk, g.Nz) * g.Ny + wrap_near(j0 + lj, g.Ny)) * g.Nx + wrap_near(i0 + li, g.Nx);
+    const int r = ((lk + HH) * RAY + (lj + HH)) * RAX + (li + HH);
     for (int d = 0; d < 4; ++d) {
       const long long v = acc[d * N3 + q];
-      if (v != 0)
-        atomicAdd(reinterpret_cast<unsigned long long *>(&ovfq[node * 4 + d]), (unsigned long long)v);
+      if (v != 0) slot[d * RA3 + r] += v;
     }
   }
 }
@@ -652,7 +656,7 @@ void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, u
   a.phi = c->d_phi;
   a.fold = reinterpret_cast<long long *>(c->d_fold);
   a.ovf = c->d_ovf;
-  a.late = c->d_late;
+  a.late = reinterpret_cast<uint4 *>(c->d_late);
   a.late_cnt = c->d_late_cnt;
   a.k = mx_consts(c->p, dt);
   if (pt) {
@@ -704,16 +708,18 @@ void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
 
 void mx_late(ecsprt_hip_ctx *c) {
   const float iv = (float)(1.0 / mx_vmax(c->p));
+  long long *fq = reinterpret_cast<long long *>(c->d_fold);
+  const uint4 *late = reinterpret_cast<const uint4 *>(c->d_late);
   const int tx = c->g.Tx, tz = c->g.Tz;
   if (tx == 8 && tz == 8)
-    k_late_mx<8, 8, 8><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_late, c->d_late_cnt, c->d_base, c->d_cap,
-                                                         iv, c->d_ovfq, c->d_ovf);
+    k_late_mx<8, 8, 8, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base,
+                                                            c->d_cap, iv, fq, c->d_ovf);
   else if (tx == 8 && tz == 4)
-    k_late_mx<8, 8, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_late, c->d_late_cnt, c->d_base, c->d_cap,
-                                                         iv, c->d_ovfq, c->d_ovf);
+    k_late_mx<8, 8, 4, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base,
+                                                            c->d_cap, iv, fq, c->d_ovf);
   else
-    k_late_mx<4, 4, 4><<<c->ntiles, 256, 0, c->stream>>>(c->g, c->Am, c->d_late, c->d_late_cnt, c->d_base, c->d_cap,
-                                                         iv, c->d_ovfq, c->d_ovf);
+    k_late_mx<4, 4, 4, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base,
+                                                            c->d_cap, iv, fq, c->d_ovf);
   ECS_HIP(hipGetLastError());
 }
 
