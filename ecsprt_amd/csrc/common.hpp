@@ -160,6 +160,7 @@ struct ecsprt_hip_ctx {
   int foldtab_off[3] = {0, 0, 0};
   size_t lds_bytes = 0;
   int block = 1024;  // threads per tile workgroup in effect
+  int fx_bits = 40;  // FP64 path: fixed-point bits of the deposit regions (set at binning)
 
   // staging (host<->device AoS)
   double *d_stage = nullptr;

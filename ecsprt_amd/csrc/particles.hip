@@ -25,7 +25,9 @@ struct TileArgs {
   int *next;
   unsigned long long *ctr;
   const double *phi;
-  double *fold, *ovf;
+  long long *fold;  // per-workgroup deposit regions, fixed point (weights at fx_w, fluxes at fx_w / vmax)
+  double *ovf;
+  double fx_w, inv_vmax;
   double *late;   // per-tile lists of ions whose deposit waits for k_late: x, y, z, vx, vy, vz each
   int *late_cnt;  // list lengths
   PushConsts pc;
@@ -93,7 +95,7 @@ __device__ inline void defer_deposit(const TileArgs &a, int dt, double px, doubl
 struct TileWG {
   int t, tx, ty, tz, i0, j0, k0;
   int RA, RA3, RAx, RAy, RAz, H;
-  double *sacc;
+  long long *sacc;
   int *s_cnt, *s_base, *s_nbr;  // counters and bases: two buffers of 27
   int par;                       // buffer of the current batch
 };
@@ -149,19 +151,25 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     if (lb < 0) lb += g.Ny; else if (lb > w.RAy - 2) lb -= g.Ny;
     if (lc < 0) lc += g.Nz; else if (lc > w.RAz - 2) lc -= g.Nz;
     const bool inreg = la >= 0 && la <= w.RAx - 2 && lb >= 0 && lb <= w.RAy - 2 && lc >= 0 && lc <= w.RAz - 2;
-    if (inreg) {
+    // Weights and fluxes go into LDS as exact 64-bit fixed point: weight w as
+    // rint(w * fx_w), flux v*w as rint(v/vmax * w * fx_w) (vmax and fx_w are powers of
+    // two, so the scalings are exact).  Ions with a component at or above vmax keep FP64
+    // global atomics.
+    const double ux = vx * a.inv_vmax, uy = vy * a.inv_vmax, uz = vz * a.inv_vmax;
+    const bool slow = fabs(ux) < 1.0 && fabs(uy) < 1.0 && fabs(uz) < 1.0;
+    if (inreg && slow) {
       const int base = (lc * RAY + lb) * RAX + la;
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
         const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
         const int idx = base + (az * RAY + ay) * RAX + ax;
-        const double wt = w8[c];
-        lds_add(&w.sacc[idx], wt);
-        lds_add(&w.sacc[RA3 + idx], vx * wt);
-        lds_add(&w.sacc[2 * RA3 + idx], vy * wt);
-        lds_add(&w.sacc[3 * RA3 + idx], vz * wt);
+        const double wt = w8[c] * a.fx_w;
+        lds_addfx(&w.sacc[idx], wt);
+        lds_addfx(&w.sacc[RA3 + idx], ux * wt);
+        lds_addfx(&w.sacc[2 * RA3 + idx], uy * wt);
+        lds_addfx(&w.sacc[3 * RA3 + idx], uz * wt);
       }
-    } else if (MODE & M_REBIN) {
+    } else if ((MODE & M_REBIN) && slow) {
       late = true;
     } else {
       deposit_overflow(a.ovf, g, ixl, iyl, izl, w8, vx, vy, vz);
@@ -243,7 +251,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   // LDS: the gather's difference numerators at the tile's (T+1)^3 corner nodes, three
   // components, then the deposit region.
   double *sE = smem;
-  w.sacc = smem + ((MODE & M_PUSH) ? 3 * NE3 : 0);
+  w.sacc = reinterpret_cast<long long *>(smem + ((MODE & M_PUSH) ? 3 * NE3 : 0));
   const int tid = threadIdx.x;
   const PhiGlobal gphi{a.phi, g.Nx, g.Ny, g.Nz};
 
@@ -258,7 +266,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     }
   }
   if (MODE & M_DEPOSIT) {
-    for (int q = tid; q < 4 * w.RA3; q += blockDim.x) w.sacc[q] = 0.0;
+    for (int q = tid; q < 4 * w.RA3; q += blockDim.x) w.sacc[q] = 0;
   }
   if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
   if ((MODE & M_REBIN) && tid < 27) {
@@ -412,31 +420,32 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   }
   if (MODE & M_DEPOSIT) {
     __syncthreads();
-    double *out = a.fold + (size_t)wi * 4 * w.RA3;
+    long long *out = a.fold + (size_t)wi * 4 * w.RA3;
     for (int q = tid; q < 4 * w.RA3; q += blockDim.x) out[q] = w.sacc[q];
   }
 }
 
 // Deposit of the ions listed for tile t: those that arrived from beyond their source
-// tile's deposit region.  They now lie in tile t, so FP64 LDS accumulators over its
-// (T+1)^3 nodes take them, and the non-zero nodes are added into the tile's own slot of
-// the fold buffer (its deposit region contains those nodes; no other workgroup touches
-// the slot now), so k_fold sums them with everything else.  Resets the tile's list.
+// tile's deposit region.  They now lie in tile t, so fixed-point LDS accumulators over
+// its (T+1)^3 nodes take them (the tile kernel's units), and the non-zero nodes are
+// added into the tile's own slot of the fold buffer (its deposit region contains those
+// nodes; no other workgroup touches the slot now), so k_fold sums them with everything
+// else.  Resets the tile's list.
 template <int TX, int TY, int TZ, int HH>
 __global__ void __launch_bounds__(256) k_late(TileGeom g, const double *__restrict__ late,
                                              int *__restrict__ late_cnt, const int *__restrict__ base,
                                              const int *__restrict__ cap, PushConsts pc, double *__restrict__ ovf,
-                                             double *__restrict__ fold) {
+                                             long long *__restrict__ fold, double fx_w, double inv_vmax) {
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RA3 = RAX * RAY * (TZ + 1 + 2 * HH);
   constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
-  __shared__ double acc[4 * N3];
+  __shared__ long long acc[4 * N3];
   const int t = blockIdx.x;
   const int n = min(late_cnt[t], cap[t] >> 2);
   __syncthreads();
   if (threadIdx.x == 0) late_cnt[t] = 0;
   if (n == 0) return;
   const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
-  for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0.0;
+  for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0;
   __syncthreads();
   const double2 *list = reinterpret_cast<const double2 *>(late + 6 * (long)(base[t] >> 2));
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
@@ -456,24 +465,26 @@ __global__ void __launch_bounds__(256) k_late(TileGeom g, const double *__restri
       deposit_overflow(ovf, g, ixl, iyl, izl, w8, vx, vy, vz);  // a coordinate on the upper face
       continue;
     }
+    const double ux = vx * inv_vmax, uy = vy * inv_vmax, uz = vz * inv_vmax;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int ax = (c < 4) ? 1 : 0, ay = (c & 2) ? 0 : 1, az = (c & 1) ? 0 : 1;
       const int idx = ((lc + az) * NY + (lb + ay)) * NX + (la + ax);
-      lds_add(&acc[idx], w8[c]);
-      lds_add(&acc[N3 + idx], vx * w8[c]);
-      lds_add(&acc[2 * N3 + idx], vy * w8[c]);
-      lds_add(&acc[3 * N3 + idx], vz * w8[c]);
+      const double wt = w8[c] * fx_w;
+      lds_addfx(&acc[idx], wt);
+      lds_addfx(&acc[N3 + idx], ux * wt);
+      lds_addfx(&acc[2 * N3 + idx], uy * wt);
+      lds_addfx(&acc[3 * N3 + idx], uz * wt);
     }
   }
   __syncthreads();
-  double *slot = fold + (size_t)t * g.split * 4 * RA3;
+  long long *slot = fold + (size_t)t * g.split * 4 * RA3;
   for (int q = threadIdx.x; q < N3; q += blockDim.x) {
     const int li = q % NX, lj = (q / NX) % NY, lk = q / (NX * NY);
     const int r = ((lk + HH) * RAY + (lj + HH)) * RAX + (li + HH);
     for (int d = 0; d < 4; ++d) {
-      const double v = acc[d * N3 + q];
-      if (v != 0.0) slot[d * RA3 + r] += v;
+      const long long v = acc[d * N3 + q];
+      if (v != 0) slot[d * RA3 + r] += v;
     }
   }
 }
@@ -523,7 +534,9 @@ __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__r
     const int *px = tx + g.Nx + i * FOLD_MAXP * 2;
     const int *py = ty + g.Ny + j * FOLD_MAXP * 2;
     const int *pz = tz + g.Nz + k * FOLD_MAXP * 2;
-    R acc[4] = {0, 0, 0, 0};
+    // Integer regions are summed in FP64: each slot value converts exactly (mixed) or to
+    // within 2^-53 (FP64 path, up to 2^62), in a fixed order.
+    double acc[4] = {0, 0, 0, 0};
     for (int a = 0; a < cz; ++a)
       for (int b = 0; b < cy; ++b)
         for (int c = 0; c < cx; ++c) {
@@ -531,18 +544,14 @@ __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__r
           const int loc = (pz[2 * a + 1] * RAY + py[2 * b + 1]) * RAX + px[2 * c + 1];
           for (int s = 0; s < g.split; ++s) {
             const R *r = fold + (size_t)(t * g.split + s) * 4 * RA3 + loc;
-            acc[0] += r[0];
-            acc[1] += r[RA3];
-            acc[2] += r[2 * RA3];
-            acc[3] += r[3 * RA3];
+            acc[0] += (double)r[0];
+            acc[1] += (double)r[RA3];
+            acc[2] += (double)r[2 * RA3];
+            acc[3] += (double)r[3 * RA3];
           }
         }
-    if constexpr (std::is_integral<R>::value) {
-      m[node * 4 + 0] = (double)acc[0] * sn + ovf[node * 4 + 0];
-      for (int d = 1; d < 4; ++d) m[node * 4 + d] = (double)acc[d] * sg + ovf[node * 4 + d];
-    } else {
-      for (int d = 0; d < 4; ++d) m[node * 4 + d] = acc[d] + ovf[node * 4 + d];
-    }
+    m[node * 4 + 0] = acc[0] * sn + ovf[node * 4 + 0];
+    for (int d = 1; d < 4; ++d) m[node * 4 + d] = acc[d] * sg + ovf[node * 4 + d];
   }
 }
 
@@ -915,6 +924,10 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
     total += cap;
   }
   ECS_CHECK(total < 2147483647L, ECSPRT_ERR_ARG, "particle capacity exceeds int32 slots");
+  {
+    const int maxcap = *std::max_element(c->h_cap.begin(), c->h_cap.end());
+    c->fx_bits = std::min(50, 62 - (int)std::ceil(std::log2(1.25 * maxcap + 1.0)));
+  }
   const bool philox = c->o.collide_mode == ECSPRT_COLLIDE_PHILOX;
   if (c->mixed) {
     if (total > c->pcap || !c->Am.p) {
@@ -1000,8 +1013,10 @@ static TileArgs make_args(ecsprt_hip_ctx *c) {
   a.next = c->d_next;
   a.ctr = c->d_ctr;
   a.phi = c->d_phi;
-  a.fold = c->d_fold;
+  a.fold = reinterpret_cast<long long *>(c->d_fold);
   a.ovf = c->d_ovf;
+  a.fx_w = std::ldexp(1.0, fx_bits(c));
+  a.inv_vmax = 1.0 / mx_vmax(c->p);
   a.late = reinterpret_cast<double *>(c->d_late);
   a.late_cnt = c->d_late_cnt;
   a.pc = push_consts(c->p, 0.0);
@@ -1093,30 +1108,32 @@ static void launch_late(ecsprt_hip_ctx *c) {
   }
   const PushConsts pc = push_consts(c->p, 0.0);
   const double *late = reinterpret_cast<const double *>(c->d_late);
+  long long *fold = reinterpret_cast<long long *>(c->d_fold);
+  const double fw = std::ldexp(1.0, fx_bits(c)), iv = 1.0 / mx_vmax(c->p);
   const int tx = c->g.Tx, tz = c->g.Tz;
   if (tx == 8 && tz == 8)
     k_late<8, 8, 8, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base, c->d_cap,
-                                                         pc, c->d_ovf, c->d_fold);
+                                                         pc, c->d_ovf, fold, fw, iv);
   else if (tx == 8 && tz == 4)
     k_late<8, 8, 4, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base, c->d_cap,
-                                                         pc, c->d_ovf, c->d_fold);
+                                                         pc, c->d_ovf, fold, fw, iv);
   else
     k_late<4, 4, 4, 2><<<c->ntiles, 256, 0, c->stream>>>(c->g, late, c->d_late_cnt, c->d_base, c->d_cap,
-                                                         pc, c->d_ovf, c->d_fold);
+                                                         pc, c->d_ovf, fold, fw, iv);
   ECS_HIP(hipGetLastError());
 }
 
+// Fixed-point bits of the deposit regions: 2^30 on the mixed path; on the FP64 path the
+// finest scale at which no region can overflow: a workgroup's region sums at most
+// 1.25 cap ions (its tile's ions and the tile's late list), each below 2^bits, so
+// bits = 62 - ceil(log2(1.25 max cap)), at most 50 (the rounding add needs |v| < 2^51).
+int fx_bits(const ecsprt_hip_ctx *c) { return c->mixed ? 30 : c->fx_bits; }
+
 static void launch_fold(ecsprt_hip_ctx *c) {
-  if (c->mixed) {
-    const double vmax = mx_vmax(c->p);
-    const double sc[4] = {0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30, vmax * 0x1p-30};
-    k_fold<long long><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(
-        c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_foldtab, c->foldtab_off[0],
-        c->foldtab_off[1], c->foldtab_off[2], c->d_moments, sc[0], sc[1]);
-  } else
-    k_fold<double><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(c->g, c->d_fold, c->d_ovf, c->d_foldtab,
-                                                                   c->foldtab_off[0], c->foldtab_off[1],
-                                                                   c->foldtab_off[2], c->d_moments, 1.0, 1.0);
+  const double sn = std::ldexp(1.0, -fx_bits(c)), sg = mx_vmax(c->p) * sn;
+  k_fold<long long><<<grid_for(c->nnode, 256), 256, 0, c->stream>>>(
+      c->g, reinterpret_cast<const long long *>(c->d_fold), c->d_ovf, c->d_foldtab, c->foldtab_off[0],
+      c->foldtab_off[1], c->foldtab_off[2], c->d_moments, sn, sg);
   ECS_HIP(hipGetLastError());
 }
 

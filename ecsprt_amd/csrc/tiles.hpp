@@ -36,6 +36,19 @@ __device__ inline void lds_add(double *p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Exact fixed-point accumulation of an FP64 value already scaled to integer units
+// (|v| < 2^51): v + 1.5*2^52 rounds v to the nearest integer into the low mantissa bits,
+// whose difference from the bit pattern of 1.5*2^52 is that integer as a signed 64-bit
+// value; one integer LDS atomic adds it.  The sum is exact and independent of order.
+__device__ inline long long fx_round(double scaled) {
+  const double y = scaled + 6755399441055744.0;
+  return __double_as_longlong(y) - 0x4338000000000000LL;
+}
+__device__ inline void lds_addfx(long long *p, double scaled) {
+  __hip_atomic_fetch_add(reinterpret_cast<unsigned long long *>(p), (unsigned long long)fx_round(scaled),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Deposit of an ion that moved beyond the tile's LDS region (about 0.6% of ions at
 // one cell per step): global FP64 atomics into the overflow moments.  Kept out of
 // line so the LDS path stays free of its address arithmetic.
@@ -104,6 +117,7 @@ void mx_unbin_list(ecsprt_hip_ctx *c, long n);
 void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
 void mx_late(ecsprt_hip_ctx *c);
 double mx_vmax(const ecsprt_params &p);
+int fx_bits(const ecsprt_hip_ctx *c);
 
 // Fill kernels launched on the context's stream (particles.hip).
 void fill_f64(ecsprt_hip_ctx *c, double *p, long n, double v);
