@@ -74,6 +74,7 @@ SIGNATURES = {
     "ecsprt_hip_set_particles": (ctypes.c_int, [_P, ctypes.c_long, _D, _D]),
     "ecsprt_hip_get_particles": (ctypes.c_int, [_P, _D, _D]),
     "ecsprt_hip_init_particles": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]),
+    "ecsprt_hip_init_reference": (ctypes.c_int, [_P, ctypes.c_long]),
     "ecsprt_hip_num_particles": (ctypes.c_long, [_P]),
     "ecsprt_hip_cell_counts": (ctypes.c_int, [_P, _I]),
     "ecsprt_hip_push": (ctypes.c_int, [ctypes.c_double, _P]),

@@ -258,6 +258,11 @@ int ecsprt_hip_init_particles(ecsprt_hip_ctx *c, long np, uint64_t seed, double 
   return guarded(c, [&] { ecsprt::particles_init_device(c, np, seed, amp, k); });
 }
 
+int ecsprt_hip_init_reference(ecsprt_hip_ctx *c, long np) {
+  if (!c || np <= 0) return ECSPRT_ERR_ARG;
+  return guarded(c, [&] { ecsprt::particles_init_reference(c, np); });
+}
+
 long ecsprt_hip_num_particles(const ecsprt_hip_ctx *c) { return c ? c->np : 0; }
 
 int ecsprt_hip_cell_counts(ecsprt_hip_ctx *c, int *counts) {

@@ -534,9 +534,11 @@ __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__r
     const int *px = tx + g.Nx + i * FOLD_MAXP * 2;
     const int *py = ty + g.Ny + j * FOLD_MAXP * 2;
     const int *pz = tz + g.Nz + k * FOLD_MAXP * 2;
-    // Integer regions are summed in FP64: each slot value converts exactly (mixed) or to
-    // within 2^-53 (FP64 path, up to 2^62), in a fixed order.
-    double acc[4] = {0, 0, 0, 0};
+    // Exact integer sum of the regions in two words (high parts signed, low 32-bit parts
+    // unsigned), rounded to FP64 once at the end: the moments do not depend on how ions
+    // were split among workgroups or ordered within them.
+    long long hi[4] = {0, 0, 0, 0};
+    unsigned long long lo[4] = {0, 0, 0, 0};
     for (int a = 0; a < cz; ++a)
       for (int b = 0; b < cy; ++b)
         for (int c = 0; c < cx; ++c) {
@@ -544,12 +546,20 @@ __global__ void k_fold(TileGeom g, const R *__restrict__ fold, const double *__r
           const int loc = (pz[2 * a + 1] * RAY + py[2 * b + 1]) * RAX + px[2 * c + 1];
           for (int s = 0; s < g.split; ++s) {
             const R *r = fold + (size_t)(t * g.split + s) * 4 * RA3 + loc;
-            acc[0] += (double)r[0];
-            acc[1] += (double)r[RA3];
-            acc[2] += (double)r[2 * RA3];
-            acc[3] += (double)r[3 * RA3];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+              const long long v = r[d * RA3];
+              hi[d] += v >> 32;
+              lo[d] += (unsigned long long)(v & 0xffffffffLL);
+            }
           }
         }
+    double acc[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const long long h = hi[d] + (long long)(lo[d] >> 32);
+      acc[d] = (double)h * 4294967296.0 + (double)(lo[d] & 0xffffffffULL);
+    }
     m[node * 4 + 0] = acc[0] * sn + ovf[node * 4 + 0];
     for (int d = 1; d < 4; ++d) m[node * 4 + d] = acc[d] * sg + ovf[node * 4 + d];
   }

@@ -129,6 +129,11 @@ class Context:
     def init_particles(self, np_, seed, amp=0.0, k=0):
         self._c(self.L.ecsprt_hip_init_particles(self.ptr, np_, seed, amp, k))
 
+    def init_reference(self, np_):
+        """The reference's own initial conditions (ran3 positions, gasdev velocities,
+        seed -12345), generated on the host and binned."""
+        self._c(self.L.ecsprt_hip_init_reference(self.ptr, np_))
+
     @property
     def num_particles(self):
         return self.L.ecsprt_hip_num_particles(self.ptr)

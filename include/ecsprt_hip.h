@@ -149,6 +149,14 @@ int ecsprt_hip_get_particles(ecsprt_hip_ctx *ctx, double *pos, double *vel);
  * x-density perturbation 1 + amp*sin(2*pi*k*x/Lx) (inverse CDF), velocities
  * Maxwellian with per-component thermal speeds viT{x,y,z} and drift vi0{x,y,z}. */
 int ecsprt_hip_init_particles(ecsprt_hip_ctx *ctx, long np, uint64_t seed, double amp, int k);
+
+/* The reference's own initial conditions on one rank: positions ran3 * L per coordinate
+ * (NormalDistribution, src/positions.c:42-137 -- uniform despite the name; no domain
+ * origin offset, as there), then velocities vT * gasdev + v0 per component
+ * (NormalVelocities, src/velocities.c:14-47), each stream seeded with getseed() = -12345
+ * (src/random.c:292-295).  The Numerical Recipes streams are serial and run on the host;
+ * the arrays are uploaded and binned. */
+int ecsprt_hip_init_reference(ecsprt_hip_ctx *ctx, long np);
 long ecsprt_hip_num_particles(const ecsprt_hip_ctx *ctx);
 /* Per-cell counts of (int)(pos/d) per axis, host int[Nz][Ny][Nx]. */
 int ecsprt_hip_cell_counts(ecsprt_hip_ctx *ctx, int *counts);

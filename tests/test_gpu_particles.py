@@ -296,3 +296,70 @@ def test_fused_deposit_of_far_movers(prec, tile, coll):
     if coll:
         assert ctx.collision_stats()[1] > 0
     ctx.close()
+
+
+def test_fp64_deposit_is_order_independent():
+    """FP64 deposit regions accumulate in exact fixed point: the same ions in another
+    order give identical moments, bit for bit."""
+    say("fp64 deposit order independence")
+    wl = configs.small(12, 8)
+    ctx, po, pos, vel = make(wl, tile=4)
+    ctx.set_particles(pos, vel)
+    ctx.collect()
+    m1 = ctx.get_moments()
+    perm = np.random.default_rng(9).permutation(len(pos))
+    ctx.set_particles(np.ascontiguousarray(pos[perm]), np.ascontiguousarray(vel[perm]))
+    ctx.collect()
+    np.testing.assert_array_equal(ctx.get_moments(), m1)
+    assert rel(m1, oracle.collect_moments(po, pos, vel)) <= 1e-12
+    ctx.close()
+
+
+def test_fp64_fast_ions_deposit():
+    """Ions with a velocity component above the fixed-point flux range (vmax = 128 here)
+    take the FP64 global-atomic path; fused steps still match the oracle to 1e-12."""
+    say("fp64 fast ions")
+    wl = configs.small(12, 6)
+    ctx, po, pos, vel = make(wl, tile=4, collide_mode=_lib.COLLIDE_OFF, operator_mode=_lib.OPERATOR_REBUILD,
+                             ksp_rtol=1e-8)
+    dt = 0.05
+    rng = np.random.default_rng(4)
+    u = rng.normal(size=(7, 3))
+    vel[:7] = 150.0 * u / np.linalg.norm(u, axis=1, keepdims=True)
+    phi = 0.05 * rng.normal(size=(12, 12, 12))
+    ctx.set_particles(pos, vel)
+    for _ in range(3):
+        ctx.set_potential(phi)
+        ctx.step(dt)
+        oracle.boris(po, dt, pos, vel, phi)
+        oracle.update_positions(dt, pos, vel)
+        oracle.apply_bc(po, pos)
+    gpos, gvel = ctx.get_particles()
+    assert rel(gvel, vel) <= 1e-12
+    assert rel(gpos, pos) <= 1e-12
+    assert rel(ctx.get_moments(), oracle.collect_moments(po, pos, vel)) <= 1e-12
+    ctx.close()
+
+
+def test_reference_initial_conditions():
+    """ecsprt_hip_init_reference reproduces the reference's own initial particles: ran3
+    positions times L and vT*gasdev + v0 velocities, both streams from seed -12345, as the
+    oracle's restatement of the Numerical Recipes generators draws them."""
+    say("reference initial conditions")
+    cli = configs.base_cli(10, viT=1.3)
+    cli.update(vi0x=0.2, vi0z=-0.1)
+    p = pic.setup_context(**cli)
+    po = oracle.setup_context(**cli)
+    n = 4000
+    ctx = pic.Context(p, pic.default_options(tile=4))
+    ctx.init_reference(n)
+    gpos, gvel = ctx.get_particles()
+    s3, _ = oracle.NR().stream("ran3", -12345, 3 * n)
+    sg, _ = oracle.NR().stream("gasdev", -12345, 3 * n)
+    pos = s3.astype(np.float64).reshape(n, 3) * np.array([po.Lx, po.Ly, po.Lz])
+    vel = np.array([po.viTx, po.viTy, po.viTz]) * sg.astype(np.float64).reshape(n, 3) + \
+        np.array([po.vi0x, po.vi0y, po.vi0z])
+    np.testing.assert_array_equal(gpos, pos)
+    np.testing.assert_array_equal(gvel, vel)
+    np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
+    ctx.close()
