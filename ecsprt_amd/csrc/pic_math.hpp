@@ -159,10 +159,14 @@ __device__ inline double c_sqrt(double x) { return sqrt(x); }
 __device__ inline float c_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ inline double c_div(double a, double b) { return a / b; }
 __device__ inline float c_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-__device__ inline double c_cos2pi(double u) { return cos(2 * 3.14159265358979323846 * u); }
-__device__ inline float c_cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
-__device__ inline double c_sin2pi(double u) { return sin(2 * 3.14159265358979323846 * u); }
-__device__ inline float c_sin2pi(float u) { return __builtin_amdgcn_sinf(u); }
+// cos and sin of 2*pi*u: in FP64 through sincospi(2u), whose reduction by multiples of
+// pi is exact (within an ulp or two of the reference's cos(2*PI*u), and one range
+// reduction instead of two); in FP32 on the hardware's sin/cos of revolutions.
+__device__ inline void c_sincos2pi(double u, double &s, double &c) { sincospi(2.0 * u, &s, &c); }
+__device__ inline void c_sincos2pi(float u, float &s, float &c) {
+  s = __builtin_amdgcn_sinf(u);
+  c = __builtin_amdgcn_cosf(u);
+}
 
 // One attempt of the ScatterElastic loop body (src/collisions.c:77-154) on ion
 // velocity vi.  Returns 0 rejected by the acceptance draw, 1 collided (vf written),
@@ -192,8 +196,10 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   T vcr = c_sqrt(tmp);
   T u0z = T(2) * (T)u_z - T(1);
   T uperp = c_sqrt(T(1) - u0z * u0z);
-  T u0x = uperp * c_cos2pi((T)u_phi);
-  T u0y = uperp * c_sin2pi((T)u_phi);
+  T sphi, cphi;
+  c_sincos2pi((T)u_phi, sphi, cphi);
+  T u0x = uperp * cphi;
+  T u0y = uperp * sphi;
   T costht, sintht, cosphi, sinphi;
   if constexpr (sizeof(T) == sizeof(double)) {
     costht = vr[2] / vrr;

@@ -220,12 +220,14 @@ __device__ inline T relax_planes(const StencilConsts &s, const T *d0, const T *d
 // and 4 planes keep every plane until no thread can still read it, with two barriers
 // per plane.
 template <typename T, int TX, int TY>
-__global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ xin,
-                                                const T *__restrict__ b, T *__restrict__ xout, int first, int KZ) {
+__global__ void __launch_bounds__(TX * TY) k_sweep_z(StencilConsts s, const T *__restrict__ dn,
+                                                    const T *__restrict__ xin, const T *__restrict__ b,
+                                                    T *__restrict__ xout, int first, int KZ) {
+  constexpr int NT = TX * TY;                            // one thread per tile column
   constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;  // potential and density planes
   constexpr int RX = TX + 2, RY = TY + 2, RP = RX * RY;  // first-colour planes
   constexpr int NXO = 4, ND = 5, NSN = 4;
-  constexpr int LPT = (HP + 511) / 512;
+  constexpr int LPT = (HP + NT - 1) / NT;
   __shared__ T sxo[NXO * HP], sd[ND * HP], ssn[NSN * RP];
   const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
   const int ntx = (Nx + TX - 1) / TX, nty = (Ny + TY - 1) / TY;
@@ -238,14 +240,13 @@ __global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__res
   long goff[LPT];
 #pragma unroll
   for (int u = 0; u < LPT; ++u) {
-    const int q = tid + u * 512;
+    const int q = tid + u * NT;
     const int li = q % HX - 2, lj = q / HX - 2;
     lq[u] = (q < HP && li <= ex + 1 && lj <= ey + 1) ? q : -1;
     goff[u] = (long)wrap1(j0 + lj, Ny) * Nx + wrap1(i0 + li, Nx);
   }
-  T rx[LPT], rd[LPT];
   const int Lbeg = k0 - 2, Lend = k0 + ez + 1;
-  auto fetch = [&](int L) {
+  auto fetch = [&](int L, T *rx, T *rd) {
     const long zo = (long)wrap1(L, Nz) * plane;
 #pragma unroll
     for (int u = 0; u < LPT; ++u)
@@ -254,8 +255,9 @@ __global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__res
         rx[u] = xin ? xin[zo + goff[u]] : T(0);
       }
   };
-  fetch(Lbeg);
-  for (int L = Lbeg; L <= Lend; ++L) {
+  // One plane step: plane L enters LDS from registers, which then take plane L+2, so
+  // two planes of loads are in flight while the relaxations run.
+  auto step = [&](int L, T *rx, T *rd) {
     const int r = L - Lbeg;
     {
       T *xo = sxo + (r % NXO) * HP, *dd = sd + (r % ND) * HP;
@@ -267,14 +269,14 @@ __global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__res
         }
     }
     __syncthreads();
-    if (L < Lend) fetch(L + 1);
+    if (L + 2 <= Lend) fetch(L + 2, rx, rd);
     if (r >= 2) {  // first colour on plane L-1
       const int r1 = r - 1;
       const T *x0 = sxo + ((r1 - 1) % NXO) * HP, *x1 = sxo + (r1 % NXO) * HP, *x2 = sxo + ((r1 + 1) % NXO) * HP;
       const T *d0 = sd + ((r1 - 1) % ND) * HP, *d1 = sd + (r1 % ND) * HP, *d2 = sd + ((r1 + 1) % ND) * HP;
       T *sn = ssn + (r1 % NSN) * RP;
       const int gk = wrap1(L - 1, Nz);
-      for (int q = tid; q < RP; q += 512) {
+      for (int q = tid; q < RP; q += NT) {
         const int li = q % RX - 1, lj = q / RX - 1;
         if (li > ex || lj > ey) continue;
         const int p = (lj + 2) * HX + (li + 2);
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__res
       const T *s0 = ssn + ((r2 - 1) % NSN) * RP, *s1 = ssn + (r2 % NSN) * RP, *s2 = ssn + ((r2 + 1) % NSN) * RP;
       const T *d0 = sd + ((r2 - 1) % ND) * HP, *d1 = sd + (r2 % ND) * HP, *d2 = sd + ((r2 + 1) % ND) * HP;
       const int gk = L - 2;
-      for (int q = tid; q < TX * TY; q += 512) {
+      for (int q = tid; q < TX * TY; q += NT) {
         const int li = q % TX, lj = q / TX;
         if (li >= ex || lj >= ey) continue;
         const int pd = (lj + 2) * HX + (li + 2), ps = (lj + 1) * RX + (li + 1);
@@ -302,6 +304,13 @@ __global__ void __launch_bounds__(512) k_sweep_z(StencilConsts s, const T *__res
         xout[g] = v;
       }
     }
+  };
+  T rxa[LPT], rda[LPT], rxb[LPT], rdb[LPT];
+  fetch(Lbeg, rxa, rda);
+  fetch(Lbeg + 1, rxb, rdb);  // Lend >= Lbeg + 4
+  for (int L = Lbeg; L <= Lend; L += 2) {
+    step(L, rxa, rda);
+    if (L + 1 <= Lend) step(L + 1, rxb, rdb);
   }
 }
 
@@ -976,7 +985,7 @@ static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const 
   if (mode == 2 || (mode == 0 && (long)L.s.Nx * L.s.Ny * L.s.Nz >= SZ_MIN_POINTS)) {
     const int kz = sweep_kz(L.s.Nx, L.s.Ny, L.s.Nz);
     const int nt = ((L.s.Nx + SZ_TX - 1) / SZ_TX) * ((L.s.Ny + SZ_TY - 1) / SZ_TY) * ((L.s.Nz + kz - 1) / kz);
-    k_sweep_z<T, SZ_TX, SZ_TY><<<nt, 512, 0, st>>>(L.s, L.dn, xin, b, xout, first, kz);
+    k_sweep_z<T, SZ_TX, SZ_TY><<<nt, SZ_TX * SZ_TY, 0, st>>>(L.s, L.dn, xin, b, xout, first, kz);
   } else {
     const int nt = ((L.s.Nx + SW_TX - 1) / SW_TX) * ((L.s.Ny + SW_TY - 1) / SW_TY) * ((L.s.Nz + SW_TZ - 1) / SW_TZ);
     k_sweep<T, SW_TX, SW_TY, SW_TZ><<<nt, 512, 0, st>>>(L.s, L.dn, xin, b, xout, first);
