@@ -30,12 +30,6 @@ __device__ inline int tile_step(int d, int n) {
   return (d >= -1 && d <= 1) ? d : 2;
 }
 
-// FP64 add into the workgroup's LDS accumulator (ds_add_f64).  Workgroup scope keeps
-// it distinct from the agent-scope global atomics of the overflow path.
-__device__ inline void lds_add(double *p, double v) {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
 // Exact fixed-point accumulation of an FP64 value already scaled to integer units
 // (|v| < 2^51): v + 1.5*2^52 rounds v to the nearest integer into the low mantissa bits,
 // whose difference from the bit pattern of 1.5*2^52 is that integer as a signed 64-bit
@@ -49,10 +43,6 @@ __device__ inline void lds_addfx(long long *p, double scaled) {
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Deposit of an ion that moved beyond the tile's LDS region (about 0.6% of ions at
-// one cell per step): global FP64 atomics into the overflow moments.  Kept out of
-// line so the LDS path stays free of its address arithmetic.
-// Periodic index for i within one period of [0, n) (no integer division).
 // Rank of this ion among the workgroup's ions bound for destination code (0..26; codes
 // 27 and 28 take none), from the LDS counters cnt.  Most ions stay in their tile (code
 // 13): a wave reserves their ranks with one atomic and numbers them by lane, where 64
@@ -72,8 +62,12 @@ __device__ inline int rebin_rank(int *cnt, int code) {
   return rank;
 }
 
+// Periodic index for i within one period of [0, n) (no integer division).
 __device__ inline int wrap_near(int i, int n) { return i >= n ? i - n : (i < 0 ? i + n : i); }
 
+// Deposit of an ion outside the fixed-point LDS path (beyond the tile's region without
+// a late list, or with a velocity component at or above vmax): global FP64 atomics into
+// the overflow moments.
 __device__ inline void deposit_overflow(double *ovf, const TileGeom &g, int ixl, int iyl, int izl,
                                               const double w8[8], double vx, double vy, double vz) {
   const int xs[2] = {wrap_near(ixl, g.Nx), wrap_near(ixl + 1, g.Nx)};
