@@ -6,9 +6,12 @@
  * are function pointers here too and are bound to the libecsprt_hip entry points, so
  * the time loop reads like src/pic.c:238-269.  Options use the reference spellings:
  *   -Nx -Ny -Nz -dx -dy -dz -Np (negative = per cell) -Nt -dt -B0 -E0 -qi -mi -nui
- *   -nue -Te -viT -mn -Tn -vn0 ; plus -rtol, -fused (one-pass timestep), -seed.
+ *   -nue -Te -viT -mn -Tn -vn0 ; plus -rtol, -fused (one-pass timestep), -seed,
+ *   -mixed (mixed-precision particles and solve).
  * Initial conditions come from the device generator (uniform positions, Maxwellian
- * velocities), the analogue of InitializePositions/InitializeVelocities.
+ * velocities), the analogue of InitializePositions/InitializeVelocities; with
+ * --position-dist normal they are the reference's own (NormalDistribution and
+ * NormalVelocities: ran3 positions, gasdev velocities, seed -12345).
  */
 #include <math.h>
 #include <stdio.h>
@@ -41,6 +44,12 @@ static int get_real(int argc, char **argv, const char *name, double *v) {
       return 1;
     }
   return 0;
+}
+
+static const char *get_str(int argc, char **argv, const char *name) {
+  for (int i = 1; i + 1 < argc; ++i)
+    if (!strcmp(argv[i], name)) return argv[i + 1];
+  return NULL;
 }
 
 static int has_flag(int argc, char **argv, const char *name) {
@@ -104,6 +113,9 @@ int main(int argc, char **argv) {
   if (!(cli.nue > 0)) cli.nue = 1.0;
   get_real(argc, argv, "-nue", &cli.nue);
   const int fused = has_flag(argc, argv, "-fused");
+  const int mixed = has_flag(argc, argv, "-mixed");
+  const char *pdist = get_str(argc, argv, "--position-dist");
+  const int reference_ics = pdist && !strcmp(pdist, "normal");
 
   ecsprt_params p;
   ecsprt_hip_ctx *ctx = NULL;
@@ -114,6 +126,10 @@ int main(int argc, char **argv) {
   o.seed = (uint64_t)seed;
   o.timing = 1;
   o.collide_mode = fused ? ECSPRT_COLLIDE_PHILOX : ECSPRT_COLLIDE_OFF;
+  if (mixed) {
+    o.particle_precision = ECSPRT_PARTICLES_MIXED;
+    o.solve_precision = ECSPRT_SOLVE_MIXED;
+  }
   CHECK(ecsprt_hip_create(&p, &o, &ctx));
   const long ncell = (long)p.Nx * p.Ny * p.Nz;
   const long np = Np > 0 ? (long)Np : (long)(-Np) * ncell; /* src/pic.c:100-106 */
@@ -121,9 +137,13 @@ int main(int argc, char **argv) {
   /* Plugin slots bound as in SetUpContext (src/setup.c:106-114). */
   Species ions = {ecsprt_hip_push, ecsprt_hip_collide, ecsprt_hip_collect};
 
-  printf("grid %dx%dx%d, %ld ions, %d steps, dt %g, kappa_e %g, %s timestep\n", p.Nx, p.Ny, p.Nz, np, (int)Nt, dt,
-         p.kappa_e, fused ? "fused" : "staged");
-  CHECK(ecsprt_hip_init_particles(ctx, np, (uint64_t)seed, 0.0, 0));
+  printf("grid %dx%dx%d, %ld ions, %d steps, dt %g, kappa_e %g, %s timestep, %s precision, %s initial conditions\n",
+         p.Nx, p.Ny, p.Nz, np, (int)Nt, dt, p.kappa_e, fused ? "fused" : "staged", mixed ? "mixed" : "FP64",
+         reference_ics ? "reference" : "device");
+  if (reference_ics)
+    CHECK(ecsprt_hip_init_reference(ctx, np)); /* InitializePositions/Velocities, src/pic.c:203-207 */
+  else
+    CHECK(ecsprt_hip_init_particles(ctx, np, (uint64_t)seed, 0.0, 0));
   CHECK(ions.collect(ctx));                 /* src/pic.c:211 */
   CHECK(ecsprt_hip_compute_potential(ctx)); /* src/pic.c:215 */
 

@@ -1,0 +1,30 @@
+"""The C host driver (ecsprt_amd/ecsprt_pic, the counterpart of src/pic.c) runs the
+time loop through the plugin-slot entry points on the GPU."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from ecsprt_amd import build
+
+pytestmark = pytest.mark.gpu
+
+
+def run(*args):
+    r = subprocess.run([build.DRIVER, *args], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    return r
+
+
+@pytest.mark.parametrize("extra", [[], ["-fused"], ["-fused", "-mixed"], ["--position-dist", "normal"]])
+def test_driver_steps(extra):
+    assert os.path.exists(build.DRIVER), "driver not built"
+    r = run("-Nx", "16", "-Ny", "16", "-Nz", "16", "-Np", "-8", "-Nt", "3", "-rtol", "1e-8", *extra)
+    assert r.returncode == 0
+    steps = re.findall(r"< Time step (\d+) > KSP iterations (\d+), \|\|r\|\|/\|\|b\|\| = ([0-9.e+-]+)", r.stdout)
+    assert [int(s[0]) for s in steps] == [0, 1, 2]
+    for _, it, rr in steps:
+        assert int(it) > 0 and float(rr) <= 1e-8
+    if "--position-dist" in extra:
+        assert "reference initial conditions" in r.stdout
