@@ -126,14 +126,16 @@ def copy_probe(gib, reps=5):
 
 def launch_of(p, o, mixed=False):
     """Effective tile shape and block size of the particle pass.  With none given the
-    library chooses: 8-cell tiles once a grid holds at least 512 of them (else 4), as
-    8x8x4 cells with 512 threads for FP64 particles and 8x8x8 with 1024 for mixed."""
+    library chooses: 8-cell tiles once a grid holds at least 512 of them, as 8x8x4 cells
+    with 512 threads for FP64 particles and 8x8x8 with 1024 for mixed; else 4x4x4 cells
+    with 256 threads."""
     n8 = ((p.Nx + 7) // 8) * ((p.Nz + 7) // 8) * ((p.Ny + 7) // 8)
     auto = not (o.get("tile") or o.get("tile_z") or o.get("block"))
     t = o.get("tile") or (8 if n8 >= 512 else 4)
     f64_auto = auto and t == 8 and not mixed
     tz = o.get("tile_z") or (4 if f64_auto else t)
-    return f"{t}x{t}x{tz}/{o.get('block') or (512 if f64_auto else 1024)}"
+    block = o.get("block") or (256 if t == 4 else 512 if f64_auto else 1024)
+    return f"{t}x{t}x{tz}/{block}"
 
 
 def committed_traffic(kernel, workload_name, launch, args):

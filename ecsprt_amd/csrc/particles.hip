@@ -797,9 +797,9 @@ void tiles_init(ecsprt_hip_ctx *c) {
     long n8 = (long)((p.Nx + 7) / 8) * ((p.Ny + 7) / 8) * ((p.Nz + 7) / 8);
     T = n8 >= 512 ? 8 : 4;
   }
-  // Measured on the 128^3 x 100 ions-per-cell workload (ecsprt_amd.autotune): FP64
-  // tiles run fastest as 8x8x4 cells with 512 threads (two workgroups per CU fit in
-  // LDS), mixed tiles as 8x8x8 with 1024 threads.
+  // Measured with ecsprt_amd.autotune: at 128^3 x 100 ions per cell FP64 tiles run
+  // fastest as 8x8x4 cells with 512 threads (two workgroups per CU fit in LDS), mixed
+  // tiles as 8x8x8 with 1024 threads; 4x4x4 tiles (small grids) with 256 threads.
   const bool auto_shape = c->o.tile <= 0 && c->o.tile_z <= 0 && c->o.block <= 0;
   const int Tz = c->o.tile_z > 0 ? c->o.tile_z : ((auto_shape && T == 8 && !c->mixed) ? 4 : T);
   ECS_CHECK((T == 4 && Tz == 4) || (T == 8 && (Tz == 8 || Tz == 4)), ECSPRT_ERR_ARG,
@@ -823,7 +823,7 @@ void tiles_init(ecsprt_hip_ctx *c) {
     if (split < 1) split = 1;
   }
   g.split = split;
-  int block = c->o.block > 0 ? c->o.block : ((auto_shape && T == 8 && !c->mixed) ? 512 : 1024);
+  int block = c->o.block > 0 ? c->o.block : (T == 4 ? 256 : (auto_shape && !c->mixed) ? 512 : 1024);
   c->block = block;
   ECS_CHECK(block % 64 == 0 && block >= 128 && block <= 1024, ECSPRT_ERR_ARG,
             "block must be a multiple of 64 in [128, 1024]");
