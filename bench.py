@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--mg-pre", type=int, default=0)
     ap.add_argument("--inner-rtol", type=float, default=0.0, help="mixed solve: inner FP32 tolerance (0 = library's)")
     ap.add_argument("--mg-post", type=int, default=0)
+    ap.add_argument("--mg-sweep", type=int, default=0, choices=[0, 1, 2],
+                    help="multigrid sweep kernel: 0 by level size, 1 one-shot boxes, 2 marching along z")
     ap.add_argument("--autotune", action="store_true",
                     help="pick the tile shape and block size of each particle path with ecsprt_amd.autotune")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -142,7 +144,7 @@ def committed_traffic(kernel, workload_name, launch, args):
     """HBM bytes per launch of `kernel` on this workload and launch shape, from the
     calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE runs summarised in profiles/hbm_traffic.md;
     None for a configuration that was not profiled."""
-    if args.ppc or args.grid or args.mg_pre or args.mg_post or args.split or args.collide != "philox":
+    if args.ppc or args.grid or args.mg_pre or args.mg_post or args.mg_sweep or args.split or args.collide != "philox":
         return None
     path = os.path.join(HERE, "profiles", "hbm_traffic.md")
     if not os.path.exists(path):
@@ -393,6 +395,8 @@ def main():
         common["inner_rtol"] = args.inner_rtol
     if args.mg_post:
         common["mg_post"] = args.mg_post
+    if args.mg_sweep:
+        common["mg_sweep"] = args.mg_sweep
     mixed_only = args.workload == "mixed"
     if mixed_only:
         common.update(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)

@@ -24,6 +24,8 @@ OPERATOR_FROZEN, OPERATOR_REBUILD = 0, 1
 PC_NONE, PC_JACOBI, PC_GMG = 0, 1, 2
 SOLVE_FP64, SOLVE_MIXED = 0, 1
 PARTICLES_FP64, PARTICLES_MIXED = 0, 1
+# PDistType order, src/positions.c:8-10.
+PDIST_SOBOL, PDIST_REVERSE, PDIST_NORMAL, PDIST_UNIFORM, PDIST_SINUSOIDAL, PDIST_GAUSSIAN = range(6)
 
 _CLI_DOUBLES = ("dx", "dy", "dz", "x0", "y0", "z0", "x1", "y1", "z1", "B0", "E0", "n0",
                 "qi", "mi", "nui", "vi0x", "vi0y", "vi0z", "viTx", "viTy", "viTz", "Ti",
@@ -75,6 +77,7 @@ SIGNATURES = {
     "ecsprt_hip_get_particles": (ctypes.c_int, [_P, _D, _D]),
     "ecsprt_hip_init_particles": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_uint64, ctypes.c_double, ctypes.c_int]),
     "ecsprt_hip_init_reference": (ctypes.c_int, [_P, ctypes.c_long]),
+    "ecsprt_hip_init_distribution": (ctypes.c_int, [_P, ctypes.c_long, ctypes.c_int]),
     "ecsprt_hip_num_particles": (ctypes.c_long, [_P]),
     "ecsprt_hip_cell_counts": (ctypes.c_int, [_P, _I]),
     "ecsprt_hip_push": (ctypes.c_int, [ctypes.c_double, _P]),

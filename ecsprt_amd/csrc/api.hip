@@ -260,7 +260,12 @@ int ecsprt_hip_init_particles(ecsprt_hip_ctx *c, long np, uint64_t seed, double 
 
 int ecsprt_hip_init_reference(ecsprt_hip_ctx *c, long np) {
   if (!c || np <= 0) return ECSPRT_ERR_ARG;
-  return guarded(c, [&] { ecsprt::particles_init_reference(c, np); });
+  return guarded(c, [&] { ecsprt::particles_init_distribution(c, np, ECSPRT_PDIST_NORMAL); });
+}
+
+int ecsprt_hip_init_distribution(ecsprt_hip_ctx *c, long np, int pdist) {
+  if (!c || np <= 0) return ECSPRT_ERR_ARG;
+  return guarded(c, [&] { ecsprt::particles_init_distribution(c, np, pdist); });
 }
 
 long ecsprt_hip_num_particles(const ecsprt_hip_ctx *c) { return c ? c->np : 0; }

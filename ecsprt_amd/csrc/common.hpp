@@ -196,7 +196,7 @@ void tiles_free(ecsprt_hip_ctx *c);
 void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool ids_from_stage);
 void particles_to_stage(ecsprt_hip_ctx *c);
 void particles_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k);
-void particles_init_reference(ecsprt_hip_ctx *c, long np);
+void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist);
 void run_push(ecsprt_hip_ctx *c, double dt);
 void run_drift_rebin(ecsprt_hip_ctx *c, double dt);
 void run_deposit(ecsprt_hip_ctx *c);

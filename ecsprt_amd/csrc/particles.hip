@@ -1171,6 +1171,9 @@ static CollideConsts collide_consts(const ecsprt_params &p) {
                (p.vi0z - p.vn0z) * (p.vi0z - p.vn0z);
   k.vrm = 4.0 * p.viT + sqrt(tmp);
   k.viT = p.viT;
+  k.invM_f = (float)(1.0 / k.M);
+  k.inv_vrm2_f = (float)(1.0 / (k.vrm * k.vrm));
+  k.vf_max_f = (float)(10.0 * p.viT);
   return k;
 }
 

@@ -149,6 +149,9 @@ __device__ inline void gather_nodes(const T *__restrict__ E, int base, T wxh, T 
 struct CollideConsts {
   double vn0[3];
   double vnT, mi, mn, M, vrm, viT;
+  // FP32 path: 1/M, 1/vrm^2 and the refusal bound 10 viT, so its attempts multiply
+  // where the FP64 path divides as the reference does.
+  float invM_f, inv_vrm2_f, vf_max_f;
 };
 
 // Arithmetic of the scattering formula by precision.  FP64 is the reference's
@@ -186,11 +189,17 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   }
   T vrr = c_sqrt(tmp);
   float acc = u_acc1 * u_acc2;
-  if (!(acc < c_div(vrr * vrr, vrm * vrm))) return 0;
+  constexpr bool f64 = sizeof(T) == sizeof(double);
+  if constexpr (f64) {
+    if (!(acc < vrr * vrr / (vrm * vrm))) return 0;
+  } else {
+    if (!(acc < vrr * vrr * k.inv_vrm2_f)) return 0;
+  }
   tmp = 0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    vc[d] = c_div(vi[d] * mi + vn[d] * mn, M);
+    if constexpr (f64) vc[d] = (vi[d] * mi + vn[d] * mn) / M;
+    else vc[d] = (vi[d] * mi + vn[d] * mn) * k.invM_f;
     tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
   }
   T vcr = c_sqrt(tmp);
@@ -201,7 +210,7 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   T u0x = uperp * cphi;
   T u0y = uperp * sphi;
   T costht, sintht, cosphi, sinphi;
-  if constexpr (sizeof(T) == sizeof(double)) {
+  if constexpr (f64) {
     costht = vr[2] / vrr;
     sintht = sqrt(T(1) - costht * costht);
     cosphi = vr[0] / (vrr * sintht);
@@ -231,8 +240,11 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
     tmp += w[d] * w[d];
   }
   T vfr = c_sqrt(tmp);
-  T ratio = c_div(vfr, viT);
-  if (ratio > T(10)) return 2;
+  if constexpr (f64) {
+    if (vfr / viT > T(10)) return 2;
+  } else {
+    if (vfr > k.vf_max_f) return 2;  // |vf| / viT > 10
+  }
   vf[0] = w[0];
   vf[1] = w[1];
   vf[2] = w[2];

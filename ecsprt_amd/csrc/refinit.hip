@@ -1,12 +1,21 @@
 // Initial conditions drawn as the reference draws them on one rank, so a run can start
-// from the reference's own particles: positions from ran3 (NormalDistribution,
-// src/positions.c:42-137, which despite its name places every coordinate uniformly,
-// pos = ran3 * L), velocities vT * gasdev + v0 per component (NormalVelocities,
-// src/velocities.c:14-47), each stream seeded with getseed() = -(rank+1)*12345 = -12345
-// (src/random.c:292-295).  The generators are Numerical Recipes (2nd edition) ran3 and
-// gasdev over ran1, with the reference's float return types (src/random.c:24-204).  The
-// streams are serial, so they run on the host; the arrays are then uploaded and binned
-// like any other host particles.
+// from the reference's own particles.  Positions (InitializePositions,
+// src/positions.c:390-433):
+//   normal      ran3 per coordinate (NormalDistribution, src/positions.c:42-137, which
+//               despite its name places every coordinate uniformly, pos = ran3 * L),
+//               seeded with getseed() = -(rank+1)*12345 = -12345 (src/random.c:292-295);
+//   sobol       the reference's Sobseq, one 3-D point per ion, pos = s * L
+//               (SobolDistribution, src/positions.c:139-234; src/random.c:233-289);
+//   sinusoidal  rejection against 1 + 0.25 sin(2 pi x/Lx) over PETSc's default
+//               PetscRandom, the rand48 generator seeded with 7 (Rejection,
+//               src/positions.c:236-388).
+// Velocities are vT * gasdev + v0 per component (NormalVelocities,
+// src/velocities.c:14-47), seed -12345.  The generators are Numerical Recipes (2nd
+// edition) ran3, gasdev over ran1 and sobseq, with the reference's return types
+// (src/random.c:24-289), and the 48-bit LCG of srand48/drand48.  The streams are
+// serial, so they run on the host; the arrays are then uploaded and binned like any
+// other host particles.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -111,18 +120,105 @@ struct Gasdev {
   }
 };
 
+// sobseq with MAXBIT 30 and MAXDIM 6: the direction numbers of the six primitive
+// polynomials of degree up to 4 (degrees mdeg, interior coefficients ip, initial odd
+// numerators of the listed table), the point sequence in Gray-code order.  For degrees
+// above 2 the reference's recurrence XORs the previous direction number once per set
+// coefficient bit (src/random.c:263-266), where the published algorithm XORs the one l
+// places back; the reference's sequence is the one reproduced.
+struct Sobol {
+  static constexpr int MAXBIT = 30, MAXDIM = 6;
+  unsigned long in = 0, ix[MAXDIM + 1] = {};
+  unsigned long v[MAXBIT + 1][MAXDIM + 1] = {};  // v[j][k]: direction number j of dimension k
+  Sobol() {
+    static const int mdeg[MAXDIM + 1] = {0, 1, 2, 3, 3, 4, 4};
+    static const unsigned ip[MAXDIM + 1] = {0, 0, 1, 1, 2, 1, 4};
+    static const unsigned long m0[4][MAXDIM + 1] = {
+        {0, 1, 1, 1, 1, 1, 1}, {0, 3, 1, 3, 3, 1, 1}, {0, 5, 7, 7, 3, 3, 5}, {0, 15, 11, 5, 15, 13, 9}};
+    for (int k = 1; k <= MAXDIM; ++k) {
+      for (int j = 1; j <= mdeg[k]; ++j) v[j][k] = m0[j - 1][k] << (MAXBIT - j);
+      for (int j = mdeg[k] + 1; j <= MAXBIT; ++j) {
+        unsigned long a = v[j - mdeg[k]][k];
+        a ^= a >> mdeg[k];
+        unsigned bits = ip[k];
+        for (int l = mdeg[k] - 1; l >= 1; --l, bits >>= 1)
+          if (bits & 1) a ^= v[j - 1][k];
+        v[j][k] = a;
+      }
+    }
+  }
+  // Next point in x[0..2]; false once 2^30 - 1 points have been drawn.
+  bool next3(double x[3]) {
+    unsigned long c = in++;
+    int j = 1;
+    while (j <= MAXBIT && (c & 1)) {
+      c >>= 1;
+      ++j;
+    }
+    if (j > MAXBIT) return false;
+    for (int k = 1; k <= 3; ++k) {
+      ix[k] ^= v[j][k];
+      x[k - 1] = (double)ix[k] * (1.0 / (double)(1L << MAXBIT));
+    }
+    return true;
+  }
+};
+
+// srand48 / drand48: x <- (0x5DEECE66D x + 0xB) mod 2^48, seeded x = (seed << 16) | 0x330E,
+// each value x / 2^48 (exact in FP64).
+struct Rand48 {
+  unsigned long long x;
+  explicit Rand48(long seed) : x((((unsigned long long)(unsigned int)seed) << 16) | 0x330EULL) {}
+  double next() {
+    x = (0x5DEECE66DULL * x + 0xBULL) & 0xFFFFFFFFFFFFULL;
+    return std::ldexp((double)x, -48);
+  }
+};
+
+double sinusoidal_density(const double r[3]) { return 1.0 + 0.25 * std::sin(2 * 3.14159265358979323846 * r[0]); }
+
 }  // namespace
 
-void particles_init_reference(ecsprt_hip_ctx *c, long np) {
+void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist) {
   const ecsprt_params &p = c->p;
+  ECS_CHECK(pdist == ECSPRT_PDIST_NORMAL || pdist == ECSPRT_PDIST_SOBOL || pdist == ECSPRT_PDIST_SINUSOIDAL,
+            ECSPRT_ERR_ARG,
+            "position distribution not available: reverse and gaussian are not implemented by the reference, "
+            "uniform is PETSc's DMSwarmSetPointsUniformCoordinates lattice");
+  ECS_CHECK(pdist != ECSPRT_PDIST_SOBOL || np < (1L << Sobol::MAXBIT), ECSPRT_ERR_ARG,
+            "Sobol positions: at most 2^30 - 1 ions (MAXBIT)");
   std::vector<double> pos(3 * (size_t)np), vel(3 * (size_t)np);
   const double L[3] = {p.Lx, p.Ly, p.Lz};
   const double vT[3] = {p.viTx, p.viTy, p.viTz}, v0[3] = {p.vi0x, p.vi0y, p.vi0z};
-  {
+  if (pdist == ECSPRT_PDIST_NORMAL) {
     Ran3 r3;
     long seed = -12345;
     for (long ip = 0; ip < np; ++ip)
       for (int d = 0; d < 3; ++d) pos[3 * ip + d] = (double)r3.next(&seed) * L[d];
+  } else if (pdist == ECSPRT_PDIST_SOBOL) {
+    Sobol sb;
+    double s3[3];
+    for (long ip = 0; ip < np; ++ip) {
+      sb.next3(s3);
+      for (int d = 0; d < 3; ++d) pos[3 * ip + d] = s3[d] * L[d];
+    }
+  } else {
+    // The maximum over the grid nodes (i/Nx, j/Ny, k/Nz); the density depends on x alone.
+    double maxVal = 0.0, s3[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < p.Nx; ++i) {
+      s3[0] = (double)i / p.Nx;
+      maxVal = std::max(maxVal, sinusoidal_density(s3));
+    }
+    const double normVal = 1.0 / maxVal;
+    Rand48 r(7);
+    for (long ip = 0; ip < np;) {
+      for (int d = 0; d < 3; ++d) s3[d] = r.next();
+      const double w = sinusoidal_density(s3);
+      if (w * normVal > r.next()) {
+        for (int d = 0; d < 3; ++d) pos[3 * ip + d] = s3[d] * L[d];
+        ++ip;
+      }
+    }
   }
   {
     Gasdev g;

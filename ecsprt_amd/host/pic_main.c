@@ -10,8 +10,9 @@
  *   -mixed (mixed-precision particles and solve).
  * Initial conditions come from the device generator (uniform positions, Maxwellian
  * velocities), the analogue of InitializePositions/InitializeVelocities; with
- * --position-dist normal they are the reference's own (NormalDistribution and
- * NormalVelocities: ran3 positions, gasdev velocities, seed -12345).
+ * --position-dist normal|sobol|sinusoidal they are the reference's own
+ * (NormalDistribution, SobolDistribution or the sinusoidal Rejection, then
+ * NormalVelocities: gasdev velocities, seed -12345).
  */
 #include <math.h>
 #include <stdio.h>
@@ -115,7 +116,17 @@ int main(int argc, char **argv) {
   const int fused = has_flag(argc, argv, "-fused");
   const int mixed = has_flag(argc, argv, "-mixed");
   const char *pdist = get_str(argc, argv, "--position-dist");
-  const int reference_ics = pdist && !strcmp(pdist, "normal");
+  int pdist_id = -1; /* PDistType order, src/positions.c:8-10 */
+  if (pdist) {
+    static const char *names[] = {"sobol", "reverse", "normal", "uniform", "sinusoidal", "gaussian"};
+    for (int i = 0; i < 6; ++i)
+      if (!strcmp(pdist, names[i])) pdist_id = i;
+    if (pdist_id < 0) {
+      fprintf(stderr, "unknown --position-dist %s\n", pdist);
+      return 1;
+    }
+  }
+  const int reference_ics = pdist_id >= 0;
 
   ecsprt_params p;
   ecsprt_hip_ctx *ctx = NULL;
@@ -137,11 +148,11 @@ int main(int argc, char **argv) {
   /* Plugin slots bound as in SetUpContext (src/setup.c:106-114). */
   Species ions = {ecsprt_hip_push, ecsprt_hip_collide, ecsprt_hip_collect};
 
-  printf("grid %dx%dx%d, %ld ions, %d steps, dt %g, kappa_e %g, %s timestep, %s precision, %s initial conditions\n",
+  printf("grid %dx%dx%d, %ld ions, %d steps, dt %g, kappa_e %g, %s timestep, %s precision, %s initial conditions%s%s\n",
          p.Nx, p.Ny, p.Nz, np, (int)Nt, dt, p.kappa_e, fused ? "fused" : "staged", mixed ? "mixed" : "FP64",
-         reference_ics ? "reference" : "device");
+         reference_ics ? "reference" : "device", reference_ics ? ", positions " : "", reference_ics ? pdist : "");
   if (reference_ics)
-    CHECK(ecsprt_hip_init_reference(ctx, np)); /* InitializePositions/Velocities, src/pic.c:203-207 */
+    CHECK(ecsprt_hip_init_distribution(ctx, np, pdist_id)); /* InitializePositions/Velocities, src/pic.c:203-207 */
   else
     CHECK(ecsprt_hip_init_particles(ctx, np, (uint64_t)seed, 0.0, 0));
   CHECK(ions.collect(ctx));                 /* src/pic.c:211 */

@@ -134,6 +134,15 @@ class Context:
         seed -12345), generated on the host and binned."""
         self._c(self.L.ecsprt_hip_init_reference(self.ptr, np_))
 
+    def init_distribution(self, np_, pdist):
+        """InitializePositions for a PDistType (sobol, normal or sinusoidal; see
+        _lib.PDIST_*) followed by the reference's velocities, on the host, binned."""
+        if isinstance(pdist, str):
+            pdist = {"sobol": _lib.PDIST_SOBOL, "reverse": _lib.PDIST_REVERSE, "normal": _lib.PDIST_NORMAL,
+                     "uniform": _lib.PDIST_UNIFORM, "sinusoidal": _lib.PDIST_SINUSOIDAL,
+                     "gaussian": _lib.PDIST_GAUSSIAN}[pdist]
+        self._c(self.L.ecsprt_hip_init_distribution(self.ptr, np_, pdist))
+
     @property
     def num_particles(self):
         return self.L.ecsprt_hip_num_particles(self.ptr)

@@ -37,6 +37,14 @@ extern "C" {
 #define ECSPRT_ERR_STATE          8 /* call made before the state it needs was set */
 
 /* Command-line parameters of the hot path (subset of CLI, src/ecsprt.h:137-191). */
+/* Initial position distributions, numbered as PDistType (src/positions.c:8-10). */
+#define ECSPRT_PDIST_SOBOL      0
+#define ECSPRT_PDIST_REVERSE    1 /* not implemented by the reference (SETERRQ) */
+#define ECSPRT_PDIST_NORMAL     2
+#define ECSPRT_PDIST_UNIFORM    3 /* PETSc's DMSwarmSetPointsUniformCoordinates lattice: not provided */
+#define ECSPRT_PDIST_SINUSOIDAL 4
+#define ECSPRT_PDIST_GAUSSIAN   5 /* not implemented by the reference (SETERRQ) */
+
 typedef struct {
   int    Nx, Ny, Nz;
   double dx, dy, dz;
@@ -157,6 +165,14 @@ int ecsprt_hip_init_particles(ecsprt_hip_ctx *ctx, long np, uint64_t seed, doubl
  * (src/random.c:292-295).  The Numerical Recipes streams are serial and run on the host;
  * the arrays are uploaded and binned. */
 int ecsprt_hip_init_reference(ecsprt_hip_ctx *ctx, long np);
+/* InitializePositions (src/positions.c:390-433) for pdist, then the velocities of
+ * ecsprt_hip_init_reference: ECSPRT_PDIST_NORMAL is ecsprt_hip_init_reference;
+ * ECSPRT_PDIST_SOBOL the reference's Sobseq (src/random.c:233-289), one 3-D point per
+ * ion times L (SobolDistribution, src/positions.c:139-234; np < 2^30);
+ * ECSPRT_PDIST_SINUSOIDAL rejection against 1 + 0.25 sin(2 pi x/Lx) over PETSc's
+ * default PetscRandom, the rand48 generator seeded with 7 (Rejection,
+ * src/positions.c:236-388).  Other values return ECSPRT_ERR_ARG. */
+int ecsprt_hip_init_distribution(ecsprt_hip_ctx *ctx, long np, int pdist);
 long ecsprt_hip_num_particles(const ecsprt_hip_ctx *ctx);
 /* Per-cell counts of (int)(pos/d) per axis, host int[Nz][Ny][Nx]. */
 int ecsprt_hip_cell_counts(ecsprt_hip_ctx *ctx, int *counts);

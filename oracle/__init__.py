@@ -61,6 +61,15 @@ _L = ctypes.POINTER(ctypes.c_long)
 _lib = None
 
 
+class SobolState(ctypes.Structure):
+    _fields_ = [("in_", ctypes.c_ulong), ("ix", ctypes.c_ulong * 7), ("iv", ctypes.c_ulong * 181),
+                ("fac", ctypes.c_double)]
+
+
+class Rand48(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_ulonglong)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -98,6 +107,12 @@ def lib():
             "oracle_full_rhs3d": (None, [P, _D, _D]),
             "oracle_sinusoidal_rhs3d": (None, [P, _D]),
             "oracle_remove_mean": (None, [ctypes.c_long, _D]),
+            "oracle_sobseq_init": (None, [ctypes.POINTER(SobolState)]),
+            "oracle_sobseq_next": (ctypes.c_int, [ctypes.POINTER(SobolState), ctypes.c_int, _D]),
+            "oracle_rand48_seed": (None, [ctypes.POINTER(Rand48), ctypes.c_long]),
+            "oracle_rand48_next": (ctypes.c_double, [ctypes.POINTER(Rand48)]),
+            "oracle_sobol_positions": (ctypes.c_int, [P, ctypes.c_long, _D]),
+            "oracle_sinusoidal_positions": (ctypes.c_long, [P, ctypes.c_long, _D]),
             "oracle_particle_steps": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_int, ctypes.c_long,
                                                      _D, _D, _D, _D, ctypes.POINTER(NRState),
                                                      ctypes.c_int, _D]),
@@ -191,6 +206,42 @@ class NR:
         for i in range(n):
             out[i] = f(ctypes.byref(self.state), ctypes.byref(s))
         return out, s.value
+
+
+def sobol_points(n, dim=3):
+    """The first n points of the reference's Sobseq (src/random.c:233-289), shape (n, dim)."""
+    st = SobolState()
+    lib().oracle_sobseq_init(ctypes.byref(st))
+    out = np.empty((n, dim))
+    x = np.empty(dim)
+    for i in range(n):
+        if lib().oracle_sobseq_next(ctypes.byref(st), dim, _dp(x)):
+            raise RuntimeError("Sobol sequence exhausted")
+        out[i] = x
+    return out
+
+
+def rand48_stream(seed, n):
+    """n values of the rand48 generator seeded as srand48(seed)."""
+    r = Rand48()
+    lib().oracle_rand48_seed(ctypes.byref(r), seed)
+    return np.array([lib().oracle_rand48_next(ctypes.byref(r)) for _ in range(n)])
+
+
+def sobol_positions(p, np_):
+    """SobolDistribution on one rank (src/positions.c:139-234), pos[np, 3]."""
+    pos = np.empty((np_, 3))
+    if lib().oracle_sobol_positions(ctypes.byref(p), np_, _dp(pos)):
+        raise RuntimeError("Sobol sequence exhausted")
+    return pos
+
+
+def sinusoidal_positions(p, np_):
+    """Rejection of SinusoidalDistribution on one rank (src/positions.c:236-388):
+    (pos[np, 3], trials)."""
+    pos = np.empty((np_, 3))
+    trials = lib().oracle_sinusoidal_positions(ctypes.byref(p), np_, _dp(pos))
+    return pos, trials
 
 
 def collision_count(p, np_, dt):

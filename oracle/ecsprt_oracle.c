@@ -35,6 +35,14 @@
  *   oracle_setup_context        hand-computed derived coefficients.
  *   oracle_scatter_elastic_*    predrawn and NR-stream paths agree when fed the NR
  *                               log; invariants above per collision.
+ *   oracle_sobseq_*             dimensions 1-2 (where the reference's recurrence and
+ *                               Numerical Recipes' agree) against an independent
+ *                               Bratley-Fox implementation of the published
+ *                               algorithm; every dimension's first 2^m points put one
+ *                               point in each interval of width 2^-m.
+ *   oracle_rand48_*             glibc's srand48/drand48 stream, value for value.
+ *   oracle_*_positions          Sobol: the sequence times L; sinusoidal: acceptance
+ *                               ratio and x histogram against 1 + 0.25 sin(2 pi x).
  */
 #include "ecsprt_oracle.h"
 
@@ -395,6 +403,125 @@ void oracle_cell_counts(const oracle_params *p, long np, const double *pos, int 
 void oracle_nr_init(oracle_nr_state *s) { memset(s, 0, sizeof(*s)); }
 
 long oracle_getseed(int rank) { return (long)(-(rank + 1) * 12345); } /* random.c:292-295 */
+
+/* random.c:233-289 (Sobseq).  The table initialisation loop of the reference runs
+ * to j = MAXDIM*MAXBIT+1, one entry past both arrays; entries beyond the 24 listed
+ * initial values are zero either way and the stray write is not restated.  The
+ * recurrence for degrees above 2 XORs iu[j-1] for every set bit of the polynomial
+ * where Numerical Recipes XORs iu[j-l]; the restatement follows the reference. */
+void oracle_sobseq_init(oracle_sobol_state *s)
+{
+  static const unsigned long mdeg[7] = {0, 1, 2, 3, 3, 4, 4};
+  static const unsigned long ip[7] = {0, 0, 1, 1, 2, 1, 4};
+  static const unsigned long ic[25] = {0, 1, 1, 1, 1, 1, 1, 3, 1, 3, 3, 1, 1, 5, 7, 7, 3, 3, 5, 15, 11, 5, 15, 13, 9};
+  const int MAXBIT = 30, MAXDIM = 6;
+  int j, k, l;
+  memset(s, 0, sizeof(*s));
+  for (j = 1; j <= MAXDIM * MAXBIT; j++) s->iv[j] = j < 25 ? ic[j] : 0;
+  /* iu[j] = &iv[(j-1)*MAXDIM], so iu[j][k] = iv[(j-1)*MAXDIM + k]. */
+#define IU(J, K) s->iv[((J) - 1) * MAXDIM + (K)]
+  for (k = 1; k <= MAXDIM; k++) {
+    for (j = 1; j <= (int)mdeg[k]; j++) IU(j, k) <<= (MAXBIT - j);
+    for (j = (int)mdeg[k] + 1; j <= MAXBIT; j++) {
+      unsigned long ipp = ip[k];
+      unsigned long i = IU(j - (int)mdeg[k], k);
+      i ^= (i >> mdeg[k]);
+      for (l = (int)mdeg[k] - 1; l >= 1; l--) {
+        if (ipp & 1) i ^= IU(j - 1, k);
+        ipp >>= 1;
+      }
+      IU(j, k) = i;
+    }
+  }
+#undef IU
+  s->fac = 1.0 / ((long)1 << MAXBIT);
+  s->in = 0;
+}
+
+/* Next point: x[0..min(n,6)-1].  Returns -1 once 2^30 - 1 points have been drawn
+ * (the reference's "MAXBIT too small" error). */
+int oracle_sobseq_next(oracle_sobol_state *s, int n, double *x)
+{
+  const int MAXBIT = 30, MAXDIM = 6;
+  unsigned long im = s->in++;
+  int j, k;
+  for (j = 1; j <= MAXBIT; j++) {
+    if (!(im & 1)) break;
+    im >>= 1;
+  }
+  if (j > MAXBIT) return -1;
+  im = (unsigned long)(j - 1) * MAXDIM;
+  for (k = 1; k <= (n < MAXDIM ? n : MAXDIM); k++) {
+    s->ix[k] ^= s->iv[im + k];
+    x[k - 1] = s->ix[k] * s->fac;
+  }
+  return 0;
+}
+
+/* PETSc's rand48 generators (PETSCRANDER48, and PETSCRAND48 over libc drand48): the
+ * 48-bit LCG x <- (0x5DEECE66D x + 0xB) mod 2^48 seeded as srand48 seeds it,
+ * x = (seed << 16) | 0x330E, each value x / 2^48 (exact in FP64). */
+void oracle_rand48_seed(oracle_rand48 *r, long seed)
+{
+  r->x = (((unsigned long long)(unsigned int)seed) << 16) | 0x330EULL;
+}
+
+double oracle_rand48_next(oracle_rand48 *r)
+{
+  r->x = (0x5DEECE66DULL * r->x + 0xBULL) & 0xFFFFFFFFFFFFULL;
+  return ldexp((double)r->x, -48);
+}
+
+/* positions.c:139-234 (SobolDistribution), one rank: Sobseq(-1) once, then one
+ * 3-D point per ion, pos = s * L (no origin offset). */
+int oracle_sobol_positions(const oracle_params *p, long np, double *pos)
+{
+  oracle_sobol_state st;
+  const double L[3] = {p->Lx, p->Ly, p->Lz};
+  double sv[3];
+  long ip;
+  int d;
+  oracle_sobseq_init(&st);
+  for (ip = 0; ip < np; ip++) {
+    if (oracle_sobseq_next(&st, 3, sv)) return -1;
+    for (d = 0; d < 3; d++) pos[ip * 3 + d] = sv[d] * L[d];
+  }
+  return 0;
+}
+
+/* positions.c:236-250 (SinusoidalDistribution): 1 + 0.25 sin(2 pi x), x in units of Lx. */
+static double sinusoidal_density(const double r[3]) { return 1.0 + 0.25 * sin(2 * 3.14159265358979323846 * r[0]); }
+
+/* positions.c:253-388 (Rejection), one rank: the maximum over grid nodes (i/Nx, j/Ny,
+ * k/Nz) (the density depends on x alone), then per trial three values for the
+ * position and one for the test, accepted when w/max > v.  Returns the trials. */
+long oracle_sinusoidal_positions(const oracle_params *p, long np, double *pos)
+{
+  oracle_rand48 r;
+  const double L[3] = {p->Lx, p->Ly, p->Lz};
+  double maxVal = 0.0, normVal, sv[3], w, v;
+  long ip = 0, it = 0;
+  int i, d;
+  oracle_rand48_seed(&r, 7);
+  for (i = 0; i < p->Nx; i++) {
+    sv[0] = (double)i / p->Nx;
+    sv[1] = sv[2] = 0.0;
+    w = sinusoidal_density(sv);
+    maxVal = maxVal > w ? maxVal : w;
+  }
+  normVal = 1.0 / maxVal;
+  while (ip < np) {
+    for (d = 0; d < 3; d++) sv[d] = oracle_rand48_next(&r);
+    w = sinusoidal_density(sv);
+    v = oracle_rand48_next(&r);
+    if (w * normVal > v) {
+      for (d = 0; d < 3; d++) pos[ip * 3 + d] = sv[d] * L[d];
+      ip++;
+    }
+    it++;
+  }
+  return it;
+}
 
 /* random.c:24-72 (ran1). */
 float oracle_ran1(oracle_nr_state *s, long *idum)

@@ -131,6 +131,29 @@ void oracle_full_rhs3d(const oracle_params *p, const double *moments, double *b)
 void oracle_sinusoidal_rhs3d(const oracle_params *p, double *b);
 void oracle_remove_mean(long n, double *v);
 
+/* random.c:233-289 (Sobseq, Numerical Recipes' sobseq with MAXBIT 30, MAXDIM 6) with its
+ * static state held explicitly. */
+typedef struct {
+  unsigned long in;
+  unsigned long ix[7];
+  unsigned long iv[181];
+  double fac;
+} oracle_sobol_state;
+void oracle_sobseq_init(oracle_sobol_state *s);
+int  oracle_sobseq_next(oracle_sobol_state *s, int n, double *x);
+
+/* PETSc 3.20.2's default PetscRandom (the rand48 linear congruential generator). */
+typedef struct {
+  unsigned long long x;
+} oracle_rand48;
+void   oracle_rand48_seed(oracle_rand48 *r, long seed);
+double oracle_rand48_next(oracle_rand48 *r);
+
+/* positions.c:139-234 (SobolDistribution) and positions.c:236-388 (Rejection of
+ * SinusoidalDistribution), one rank: positions of np ions, pos[3*np]. */
+int  oracle_sobol_positions(const oracle_params *p, long np, double *pos);
+long oracle_sinusoidal_positions(const oracle_params *p, long np, double *pos);
+
 /* CPU baseline: Nt steps of push + collide + drift/wrap + deposit in the
  * reference loop order (src/pic.c:246-254), per-stage seconds returned in t[4]. */
 int oracle_particle_steps(const oracle_params *p, double dt, int nt, long np, double *pos,

@@ -17,7 +17,9 @@ def run(*args):
     return r
 
 
-@pytest.mark.parametrize("extra", [[], ["-fused"], ["-fused", "-mixed"], ["--position-dist", "normal"]])
+@pytest.mark.parametrize("extra", [[], ["-fused"], ["-fused", "-mixed"], ["--position-dist", "normal"],
+                                   ["-fused", "--position-dist", "sobol"],
+                                   ["-fused", "-mixed", "--position-dist", "sinusoidal"]])
 def test_driver_steps(extra):
     assert os.path.exists(build.DRIVER), "driver not built"
     r = run("-Nx", "16", "-Ny", "16", "-Nz", "16", "-Np", "-8", "-Nt", "3", "-rtol", "1e-8", *extra)
@@ -27,4 +29,4 @@ def test_driver_steps(extra):
     for _, it, rr in steps:
         assert int(it) > 0 and float(rr) <= 1e-8
     if "--position-dist" in extra:
-        assert "reference initial conditions" in r.stdout
+        assert f"reference initial conditions, positions {extra[-1]}" in r.stdout

@@ -365,6 +365,34 @@ def test_reference_initial_conditions():
     ctx.close()
 
 
+@pytest.mark.parametrize("dist", ["sobol", "sinusoidal"])
+def test_reference_position_distributions(dist):
+    """ecsprt_hip_init_distribution places ions as the reference's SobolDistribution and
+    sinusoidal Rejection do (positions bit-equal to the oracle's), with the reference's
+    gasdev velocities; the unimplemented PDistTypes are refused."""
+    say(f"reference position distribution {dist}")
+    cli = configs.base_cli(12, viT=0.7)
+    cli.update(dx=0.5, dz=0.25)
+    p = pic.setup_context(**cli)
+    po = oracle.setup_context(**cli)
+    n = 6000
+    ctx = pic.Context(p, pic.default_options(tile=4))
+    ctx.init_distribution(n, dist)
+    gpos, gvel = ctx.get_particles()
+    pos = oracle.sobol_positions(po, n) if dist == "sobol" else oracle.sinusoidal_positions(po, n)[0]
+    sg, _ = oracle.NR().stream("gasdev", -12345, 3 * n)
+    vel = np.array([po.viTx, po.viTy, po.viTz]) * sg.astype(np.float64).reshape(n, 3) + \
+        np.array([po.vi0x, po.vi0y, po.vi0z])
+    np.testing.assert_array_equal(gpos, pos)
+    np.testing.assert_array_equal(gvel, vel)
+    np.testing.assert_array_equal(ctx.cell_counts(), oracle.cell_counts(po, pos))
+    for bad in (_lib.PDIST_REVERSE, _lib.PDIST_UNIFORM, _lib.PDIST_GAUSSIAN, 17):
+        with pytest.raises(pic.HipError) as e:
+            ctx.init_distribution(n, bad)
+        assert e.value.code == _lib.ERR_ARG
+    ctx.close()
+
+
 @pytest.mark.parametrize("prec", [_lib.PARTICLES_FP64, _lib.PARTICLES_MIXED])
 def test_many_fused_steps_conserve(prec):
     """Forty fused steps (collisions, heating, late deposits, re-binning) on a 64^3 grid:

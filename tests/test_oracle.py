@@ -534,3 +534,112 @@ def test_sinusoidal_rhs_closed_form():
     c = np.cos(2 * np.pi * (np.arange(8) + 0.5) / 8)
     exp = 2.0 * p.scale * c[:, None, None] * c[None, :, None] * c[None, None, :]
     np.testing.assert_allclose(b, exp - exp.mean(), atol=1e-14)
+
+
+# ---- initial position distributions (src/positions.c, src/random.c:233-289) ----------
+
+# Numerical Recipes (2nd edition, section 7.7) sobseq table: degree s, interior
+# coefficients of the primitive polynomial (bit s-1-l holds a_l), initial odd m_1..m_s.
+_NR_SOBOL = [(1, 0, [1]), (2, 1, [1, 1]), (3, 1, [1, 3, 7]), (3, 2, [1, 3, 3]), (4, 1, [1, 1, 3, 13]),
+             (4, 4, [1, 1, 5, 9])]
+
+
+def _sobol_direct(n, dim, reference_rule=False):
+    """Points 1..n of one Sobol dimension built from the direction numerators m_j and the
+    Gray code of the point index (Bratley & Fox, ACM TOMS 14 (1988) 88-100):
+    m_j = 2 a_1 m_{j-1} ^ ... ^ 2^(s-1) a_(s-1) m_(j-s+1) ^ 2^s m_(j-s) ^ m_(j-s).
+    reference_rule: every set a_l contributes 2 m_(j-1), as src/random.c:263-266 does."""
+    s, ip, m0 = _NR_SOBOL[dim - 1]
+    m = [0] + list(m0)
+    for j in range(s + 1, 31):
+        v = (m[j - s] << s) ^ m[j - s]
+        for l in range(1, s):
+            if (ip >> (s - 1 - l)) & 1:
+                v ^= (m[j - 1] << 1) if reference_rule else (m[j - l] << l)
+        m.append(v)
+    vdir = [0] + [m[j] << (30 - j) for j in range(1, 31)]
+    out = np.empty(n)
+    for i in range(1, n + 1):
+        g, x, b = i ^ (i >> 1), 0, 1
+        while g:
+            if g & 1:
+                x ^= vdir[b]
+            g >>= 1
+            b += 1
+        out[i - 1] = x / 2.0 ** 30
+    return out
+
+
+def test_sobol_matches_published_construction():
+    """Dimensions whose polynomial has at most one interior coefficient at l = 1 (1, 2, 4,
+    6) follow the published recurrence exactly; dimensions 3 and 5 follow the reference's
+    own rule (it XORs m_(j-1) where the published algorithm XORs m_(j-l)) and differ from
+    the published sequence."""
+    n = 4096
+    pts = oracle.sobol_points(n, 6)
+    for d in (1, 2, 4, 6):
+        np.testing.assert_array_equal(pts[:, d - 1], _sobol_direct(n, d))
+    for d in (3, 5):
+        np.testing.assert_array_equal(pts[:, d - 1], _sobol_direct(n, d, reference_rule=True))
+        assert (pts[:, d - 1] != _sobol_direct(n, d)).any()
+    np.testing.assert_array_equal(pts[0], [0.5] * 6)
+
+
+def test_sobol_dimensions_are_stratified():
+    """Every direction numerator is odd, so the first 2^m points of each dimension put
+    exactly one point in each interval [i 2^-m, (i+1) 2^-m) (the point x = 0 of index 0
+    is not emitted, so 2^m - 1 points leave only [0, 2^-m) empty)."""
+    m = 12
+    pts = oracle.sobol_points(2 ** m - 1, 6)
+    for d in range(6):
+        cnt = np.bincount((pts[:, d] * 2 ** m).astype(int), minlength=2 ** m)
+        assert cnt[0] == 0 and (cnt[1:] == 1).all()
+
+
+def test_rand48_matches_glibc_drand48():
+    """The restated rand48 stream equals glibc's srand48/drand48, value for value."""
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    for seed in (7, 0, 123456789, -5):
+        libc.srand48(seed)
+        ref = np.array([libc.drand48() for _ in range(2000)])
+        np.testing.assert_array_equal(oracle.rand48_stream(seed, 2000), ref)
+
+
+def test_sinusoidal_rejection_statistics_and_stream_order():
+    """Rejection against 1 + 0.25 sin(2 pi x): acceptance 1/1.25 of the trials, x
+    distributed as the density, y and z uniform; the first accepted ions are those of a
+    glibc drand48 stream read as (x, y, z, test) per trial."""
+    N = 16
+    p = oracle.setup_context(Nx=N, Ny=N, Nz=N, dx=0.5, dy=0.5, dz=0.5)
+    n = 100000
+    pos, trials = oracle.sinusoidal_positions(p, n)
+    assert abs(trials / n - 1.25) < 0.015
+    L = np.array([p.Lx, p.Ly, p.Lz])
+    u = pos / L
+    assert (u >= 0).all() and (u < 1).all()
+    nb = 20
+    edges = np.linspace(0, 1, nb + 1)
+    exp = n * (np.diff(edges) - 0.25 / (2 * np.pi) * np.diff(np.cos(2 * np.pi * edges)))
+    hx = np.histogram(u[:, 0], edges)[0]
+    assert ((hx - exp) ** 2 / exp).sum() < 45  # chi-square, 19 degrees of freedom
+    for d in (1, 2):
+        h = np.histogram(u[:, d], edges)[0]
+        assert ((h - n / nb) ** 2 / (n / nb)).sum() < 45
+    libc = ctypes.CDLL("libc.so.6")
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48(7)
+    acc = []
+    while len(acc) < 50:
+        s = [libc.drand48() for _ in range(3)]
+        w = 1.0 + 0.25 * math.sin(2 * math.pi * s[0])
+        if w * (1.0 / 1.25) > libc.drand48():
+            acc.append(np.array(s) * L)
+    np.testing.assert_array_equal(pos[:50], np.array(acc))
+
+
+def test_sobol_positions_scale_the_sequence():
+    p = oracle.setup_context(Nx=8, Ny=4, Nz=16, dx=0.5, dy=2.0, dz=0.25)
+    pos = oracle.sobol_positions(p, 1000)
+    np.testing.assert_array_equal(pos, oracle.sobol_points(1000, 3) * np.array([p.Lx, p.Ly, p.Lz]))
