@@ -49,10 +49,10 @@ __device__ inline void store_particle(const ParticlesF64 &d, long i, double x, d
 
 // Place a particle whose destination slot was reserved: into the destination tile
 // when there is room, else into the spill list (the host re-bins after the step).
-__device__ inline void place(const TileArgs &a, int dt, int slot, double x, double y, double z, double vx, double vy,
-                             double vz, int id) {
-  if (slot < a.cap[dt]) {
-    store_particle(a.dst, (long)a.base[dt] + slot, x, y, z, vx, vy, vz, id);
+__device__ inline void place(const TileArgs &a, int base, int cap, int slot, double x, double y, double z, double vx,
+                             double vy, double vz, int id) {
+  if (slot < cap) {
+    store_particle(a.dst, (long)base + slot, x, y, z, vx, vy, vz, id);
     return;
   }
   const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
@@ -78,11 +78,11 @@ __device__ inline void deposit_direct(const TileArgs &a, double px, double py, d
 // An ion that left its source tile's deposit region lies in its destination tile dt:
 // append its position and velocity to dt's list (one atomic, 48 contiguous bytes) for
 // k_late, or deposit it now when the tile's list is full.
-__device__ inline void defer_deposit(const TileArgs &a, int dt, double px, double py, double pz, double vx,
-                                     double vy, double vz) {
+__device__ inline void defer_deposit(const TileArgs &a, int dt, int base, int cap, double px, double py, double pz,
+                                     double vx, double vy, double vz) {
   const int li = atomicAdd(&a.late_cnt[dt], 1);
-  if (li < (a.cap[dt] >> 2)) {
-    double2 *e = reinterpret_cast<double2 *>(a.late + 6 * ((long)(a.base[dt] >> 2) + li));
+  if (li < (cap >> 2)) {
+    double2 *e = reinterpret_cast<double2 *>(a.late + 6 * ((long)(base >> 2) + li));
     e[0] = make_double2(px, py);
     e[1] = make_double2(pz, vx);
     e[2] = make_double2(vy, vz);
@@ -97,6 +97,7 @@ struct TileWG {
   int RA, RA3, RAx, RAy, RAz, H;
   long long *sacc;
   int *s_cnt, *s_base, *s_nbr;  // counters and bases: two buffers of 27
+  int *s_nslot, *s_ncap;         // slot offset and capacity of each neighbour tile
   int par;                       // buffer of the current batch
 };
 
@@ -197,8 +198,9 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     }
     __syncthreads();
     if (code < 27) {
-      place(a, w.s_nbr[code], bas[code] + rank, px, py, pz, vx, vy, vz, pid);
-      if (late) defer_deposit(a, w.s_nbr[code], px, py, pz, vx, vy, vz);
+      const int nb = w.s_nslot[code], nc = w.s_ncap[code];
+      place(a, nb, nc, bas[code] + rank, px, py, pz, vx, vy, vz, pid);
+      if (late) defer_deposit(a, w.s_nbr[code], nb, nc, px, py, pz, vx, vy, vz);
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
@@ -219,7 +221,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   extern __shared__ double smem[];
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
-  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef, s_n3;
+  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
   __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
@@ -248,6 +250,8 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   w.s_base = s_base;
   w.par = 0;
   w.s_nbr = s_nbr;
+  w.s_nslot = s_nslot;
+  w.s_ncap = s_ncap;
   // LDS: the gather's difference numerators at the tile's (T+1)^3 corner nodes, three
   // components, then the deposit region.
   double *sE = smem;
@@ -271,8 +275,11 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
   if ((MODE & M_REBIN) && tid < 27) {
     const int ox = tid % 3 - 1, oy = (tid / 3) % 3 - 1, oz = tid / 9 - 1;
-    s_nbr[tid] = (wrap_index(w.tz + oz, g.ntz) * g.nty + wrap_index(w.ty + oy, g.nty)) * g.ntx +
-                 wrap_index(w.tx + ox, g.ntx);
+    const int nb = (wrap_index(w.tz + oz, g.ntz) * g.nty + wrap_index(w.ty + oy, g.nty)) * g.ntx +
+                   wrap_index(w.tx + ox, g.ntx);
+    s_nbr[tid] = nb;
+    s_nslot[tid] = a.base[nb];
+    s_ncap[tid] = a.cap[nb];
   }
   if (tid == 0) {
     s_ndef = 0;

@@ -108,10 +108,10 @@ __device__ inline bool drift_axis(float &o, int &i, float d, int N) {
 }
 
 // Stores an ion in tile dt, or in the spill list when the tile is full.
-__device__ inline void place_mx(const TileArgsMx &a, int dt, int slot, uint32_t cell, float ox, float oy, float oz,
-                                float vx, float vy, float vz, int id) {
-  if (slot < a.cap[dt]) {
-    store_mx(a.dst, (long)a.base[dt] + slot, cell, ox, oy, oz, vx, vy, vz, id);
+__device__ inline void place_mx(const TileArgsMx &a, int base, int cap, int slot, uint32_t cell, float ox, float oy,
+                                float oz, float vx, float vy, float vz, int id) {
+  if (slot < cap) {
+    store_mx(a.dst, (long)base + slot, cell, ox, oy, oz, vx, vy, vz, id);
     return;
   }
   const unsigned long long sp = atomicAdd(&a.ctr[1], 1ull);
@@ -148,11 +148,11 @@ __device__ inline void deposit_direct_mx(const TileArgsMx &a, int ix, int iy, in
 // An ion beyond its source tile's deposit region lies in its destination tile dt:
 // append cell, offsets and velocity to dt's list (one atomic, 32 contiguous bytes) for
 // k_late_mx, or deposit it directly when the list is full.
-__device__ inline void defer_deposit_mx(const TileArgsMx &a, int dt, int ix, int iy, int iz, float ox, float oy,
-                                        float oz, float vx, float vy, float vz) {
+__device__ inline void defer_deposit_mx(const TileArgsMx &a, int dt, int base, int cap, int ix, int iy, int iz,
+                                        float ox, float oy, float oz, float vx, float vy, float vz) {
   const int li = atomicAdd(&a.late_cnt[dt], 1);
-  if (li < (a.cap[dt] >> 2)) {
-    uint4 *e = a.late + 2 * ((long)(a.base[dt] >> 2) + li);
+  if (li < (cap >> 2)) {
+    uint4 *e = a.late + 2 * ((long)(base >> 2) + li);
     e[0] = make_uint4(mx_pack(ix, iy, iz), __float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz));
     e[1] = make_uint4(__float_as_uint(vx), __float_as_uint(vy), __float_as_uint(vz), 0u);
   } else {
@@ -165,6 +165,7 @@ struct TileWGMx {
   int RAx, RAy, RAz;
   long long *sacc;
   int *s_cnt, *s_base, *s_nbr;
+  int *s_nslot, *s_ncap;  // slot offset and capacity of each neighbour tile
   int par;
 };
 
@@ -230,9 +231,9 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     }
     __syncthreads();
     if (code < 27) {
-      const int dt = w.s_nbr[code];
-      place_mx(a, dt, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
-      if (late) defer_deposit_mx(a, dt, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+      const int dt = w.s_nbr[code], nb = w.s_nslot[code], nc = w.s_ncap[code];
+      place_mx(a, nb, nc, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+      if (late) defer_deposit_mx(a, dt, nb, nc, ix, iy, iz, ox, oy, oz, vx, vy, vz);
     } else if (code == 27) {
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
       store_mx(a.emi, (long)e, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
@@ -252,7 +253,7 @@ template <int MODE, int TX, int TY, int TZ, int HH>
 __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ long long smemq[];
-  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_ndef, s_n3;
+  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
   __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
@@ -275,6 +276,8 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   w.s_cnt = s_cnt;
   w.s_base = s_base;
   w.s_nbr = s_nbr;
+  w.s_nslot = s_nslot;
+  w.s_ncap = s_ncap;
   w.par = 0;
   // LDS: the accumulators first (8-byte aligned), then the gather's difference
   // numerators at the tile's (T+1)^3 corner nodes (FP64 differences rounded to FP32).
@@ -299,8 +302,11 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
   if ((MODE & M_REBIN) && tid < 27) {
     const int sx = tid % 3 - 1, sy = (tid / 3) % 3 - 1, sz = tid / 9 - 1;
-    s_nbr[tid] = (wrap_index(w.tz + sz, g.ntz) * g.nty + wrap_index(w.ty + sy, g.nty)) * g.ntx +
-                 wrap_index(w.tx + sx, g.ntx);
+    const int nb = (wrap_index(w.tz + sz, g.ntz) * g.nty + wrap_index(w.ty + sy, g.nty)) * g.ntx +
+                   wrap_index(w.tx + sx, g.ntx);
+    s_nbr[tid] = nb;
+    s_nslot[tid] = a.base[nb];
+    s_ncap[tid] = a.cap[nb];
   }
   if (tid == 0) {
     s_ndef = 0;
