@@ -113,6 +113,23 @@ def lib():
             "oracle_rand48_next": (ctypes.c_double, [ctypes.POINTER(Rand48)]),
             "oracle_sobol_positions": (ctypes.c_int, [P, ctypes.c_long, _D]),
             "oracle_sinusoidal_positions": (ctypes.c_long, [P, ctypes.c_long, _D]),
+            "oracle_collide_one_nd": (None, [P, ctypes.c_int, ctypes.POINTER(OracleAttempt), _D, _D, _I, _I]),
+            "oracle_scatter_elastic_nr_nd": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_double, ctypes.c_long, _D,
+                                                            ctypes.POINTER(NRState), ctypes.c_long,
+                                                            ctypes.c_void_p, ctypes.c_long, _L, _I, _I]),
+            "oracle_scatter_elastic_predrawn_nd": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_double, ctypes.c_long,
+                                                                  _D, ctypes.c_void_p, ctypes.c_long, ctypes.c_int,
+                                                                  _L, _I, _I]),
+            "oracle_difference_vector2d": (None, [_D, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                  _D]),
+            "oracle_boris_bz2d": (None, [P, ctypes.c_double, ctypes.c_long, _D, _D, _D]),
+            "oracle_update_positions2d": (None, [ctypes.c_double, ctypes.c_long, _D, _D]),
+            "oracle_apply_bc2d": (None, [P, ctypes.c_long, _D]),
+            "oracle_collect_moments2d": (None, [P, ctypes.c_long, _D, _D, _D]),
+            "oracle_full_lhs2d_coeffs": (None, [P, _D, _D]),
+            "oracle_full_lhs2d_csr": (None, [P, _D, _I, _I, _D]),
+            "oracle_full_lhs2d_apply": (None, [P, _D, _D, _D]),
+            "oracle_full_rhs2d": (None, [P, _D, _D]),
             "oracle_particle_steps": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_int, ctypes.c_long,
                                                      _D, _D, _D, _D, ctypes.POINTER(NRState),
                                                      ctypes.c_int, _D]),
@@ -206,6 +223,108 @@ class NR:
         for i in range(n):
             out[i] = f(ctypes.byref(self.state), ctypes.byref(s))
         return out, s.value
+
+
+# ---- two-dimensional variants (cli.ndim = 2) ----------------------------------------
+
+def difference_vector2d(phi, x, y):
+    Ny, Nx = phi.shape
+    f = np.zeros(2)
+    lib().oracle_difference_vector2d(_dp(phi), Nx, Ny, x, y, _dp(f))
+    return f
+
+
+def boris2d(p, dt, pos, vel, phi):
+    lib().oracle_boris_bz2d(ctypes.byref(p), dt, pos.shape[0], _dp(pos), _dp(vel), _dp(phi))
+
+
+def update_positions2d(dt, pos, vel):
+    lib().oracle_update_positions2d(dt, pos.shape[0], _dp(pos), _dp(vel))
+
+
+def apply_bc2d(p, pos):
+    lib().oracle_apply_bc2d(ctypes.byref(p), pos.shape[0], _dp(pos))
+
+
+def collect_moments2d(p, pos, vel):
+    m = np.zeros((p.Ny, p.Nx, 4))
+    lib().oracle_collect_moments2d(ctypes.byref(p), pos.shape[0], _dp(pos), _dp(vel), _dp(m))
+    return m
+
+
+def lhs2d_coeffs(p, moments):
+    c = np.zeros((p.Ny, p.Nx, 9))
+    lib().oracle_full_lhs2d_coeffs(ctypes.byref(p), _dp(np.ascontiguousarray(moments)), _dp(c))
+    return c
+
+
+def lhs2d_csr(p, moments):
+    import scipy.sparse as sp
+    n = p.Nx * p.Ny
+    rowptr = np.zeros(n + 1, dtype=np.int32)
+    cols = np.zeros(9 * n, dtype=np.int32)
+    vals = np.zeros(9 * n)
+    lib().oracle_full_lhs2d_csr(ctypes.byref(p), _dp(np.ascontiguousarray(moments)), rowptr.ctypes.data_as(_I),
+                                cols.ctypes.data_as(_I), _dp(vals))
+    return sp.csr_matrix((vals, cols, rowptr), shape=(n, n))
+
+
+def lhs2d_apply(p, moments, x):
+    y = np.zeros((p.Ny, p.Nx))
+    lib().oracle_full_lhs2d_apply(ctypes.byref(p), _dp(np.ascontiguousarray(moments)),
+                                  _dp(np.ascontiguousarray(x)), _dp(y))
+    return y
+
+
+def full_rhs2d(p, moments):
+    b = np.zeros((p.Ny, p.Nx))
+    lib().oracle_full_rhs2d(ctypes.byref(p), _dp(np.ascontiguousarray(moments)), _dp(b))
+    return b
+
+
+def _attempt(att_row):
+    a = OracleAttempt()
+    a.u_ip = float(att_row["u_ip"])
+    for d in range(3):
+        a.g[d] = float(att_row["g"][d])
+    a.u_acc1, a.u_acc2 = float(att_row["u_acc1"]), float(att_row["u_acc2"])
+    a.u_z, a.u_phi = float(att_row["u_z"]), float(att_row["u_phi"])
+    return a
+
+
+def collide_one_nd(p, ndim, att_row, vi):
+    """One attempt in ndim (2 or 3) dimensions: (vf[3], accepted, refused)."""
+    a = _attempt(att_row)
+    v = np.zeros(3)
+    v[:ndim] = np.asarray(vi, dtype=np.float64)[:ndim]
+    vf = np.zeros(3)
+    acc, fail = ctypes.c_int(), ctypes.c_int()
+    lib().oracle_collide_one_nd(ctypes.byref(p), ndim, ctypes.byref(a), _dp(v), _dp(vf),
+                                ctypes.byref(acc), ctypes.byref(fail))
+    return vf, bool(acc.value), bool(fail.value)
+
+
+def scatter_elastic_nr_nd(p, ndim, dt, vel, nr, seed=None, log_cap=0):
+    """ScatterElastic on vel[np, ndim] from the reference's NR stream, as
+    scatter_elastic_nr."""
+    if seed is None:
+        seed = lib().oracle_getseed(0)
+    log = np.zeros(max(log_cap, 1), dtype=ATTEMPT_DTYPE)
+    nl, ns, nf = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
+    rc = lib().oracle_scatter_elastic_nr_nd(ctypes.byref(p), ndim, dt, vel.shape[0], _dp(vel),
+                                            ctypes.byref(nr.state), seed,
+                                            log.ctypes.data if log_cap else None, log_cap,
+                                            ctypes.byref(nl), ctypes.byref(ns), ctypes.byref(nf))
+    return rc, nl.value, ns.value, nf.value, log[:min(nl.value, log_cap)]
+
+
+def scatter_elastic_predrawn_nd(p, ndim, dt, vel, att, Nc):
+    att = np.ascontiguousarray(att, dtype=ATTEMPT_DTYPE)
+    nu, ns, nf = ctypes.c_long(), ctypes.c_int(), ctypes.c_int()
+    rc = lib().oracle_scatter_elastic_predrawn_nd(ctypes.byref(p), ndim, dt, vel.shape[0], _dp(vel),
+                                                  att.ctypes.data, att.shape[0], Nc,
+                                                  ctypes.byref(nu), ctypes.byref(ns), ctypes.byref(nf))
+    return rc, nu.value, ns.value, nf.value
 
 
 def sobol_points(n, dim=3):

@@ -638,21 +638,23 @@ int oracle_collision_count(const oracle_params *p, long np, double dt)
   return Nc;
 }
 
-/* One attempt of the collisions.c:70-157 loop body on velocity vi_in. */
-void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const double vi_in[3],
-                        double vf_out[3], int *accepted, int *failed)
+/* One attempt of the collisions.c:70-157 loop body on velocity vi_in, in ndim (2 or 3)
+ * dimensions: every loop runs over ndim components; vr[2] keeps its initial 0 in 2-D
+ * (collisions.c:32), so cos(theta) = 0 there. */
+void oracle_collide_one_nd(const oracle_params *p, int ndim, const oracle_attempt *a, const double vi_in[3],
+                           double vf_out[3], int *accepted, int *failed)
 {
   double vn0[3] = {p->vn0x, p->vn0y, p->vn0z};
   double vnT = p->vnT, mi = p->mi, mn = p->mn, M = mn + mi;
   double vrm = vrm_of(p);
-  double vi[3], vn[3], vr[3], vc[3], u2[3], vf[3];
+  double vi[3] = {0, 0, 0}, vn[3] = {0, 0, 0}, vr[3] = {0.0, 0.0, 0.0}, vc[3] = {0, 0, 0}, u2[3], vf[3] = {0, 0, 0};
   double tmp = 0.0, vrr, vcr, u0x, u0y, u0z, uperp, uphi;
   double costht, sintht, cosphi, sinphi, u1x, u1y, u1z, vfr, ratio;
   float acc;
   int d;
   *accepted = 0;
   *failed = 0;
-  for (d = 0; d < 3; d++) {
+  for (d = 0; d < ndim; d++) {
     vi[d] = vi_in[d];
     vn[d] = a->g[d] * vnT + vn0[d];
     vr[d] = vi[d] - vn[d];
@@ -661,12 +663,12 @@ void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const d
   vrr = sqrt(tmp);
   acc = a->u_acc1 * a->u_acc2; /* PetscSqr(ran3(&seed)): two draws, float product */
   if (!(acc < vrr * vrr / (vrm * vrm))) {
-    for (d = 0; d < 3; d++) vf_out[d] = vi_in[d];
+    for (d = 0; d < ndim; d++) vf_out[d] = vi_in[d];
     return;
   }
   *accepted = 1;
   tmp = 0.0;
-  for (d = 0; d < 3; d++) {
+  for (d = 0; d < ndim; d++) {
     vc[d] = (vi[d] * mi + vn[d] * mn) / M;
     tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
   }
@@ -687,7 +689,7 @@ void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const d
   u2[1] = u1x * sinphi + u1y * cosphi;
   u2[2] = u1z;
   tmp = 0.0;
-  for (d = 0; d < 3; d++) {
+  for (d = 0; d < ndim; d++) {
     vf[d] = vc[d] + vcr * u2[d];
     tmp += vf[d] * vf[d];
   }
@@ -695,10 +697,16 @@ void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const d
   ratio = vfr / p->viT;
   if (ratio > 10) {
     *failed = 1;
-    for (d = 0; d < 3; d++) vf_out[d] = vi_in[d];
+    for (d = 0; d < ndim; d++) vf_out[d] = vi_in[d];
     return;
   }
-  for (d = 0; d < 3; d++) vf_out[d] = vf[d];
+  for (d = 0; d < ndim; d++) vf_out[d] = vf[d];
+}
+
+void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const double vi_in[3],
+                        double vf_out[3], int *accepted, int *failed)
+{
+  oracle_collide_one_nd(p, 3, a, vi_in, vf_out, accepted, failed);
 }
 
 static long pick_ip(long np, float u)
@@ -713,30 +721,29 @@ static long pick_ip(long np, float u)
 }
 
 /* Return codes: 0 ok, 2 collision failure (collisions.c:145-147 Abort), 3 records
- * exhausted before Nc successes. */
-int oracle_scatter_elastic_predrawn(const oracle_params *p, double dt, long np, double *vel,
-                                    const oracle_attempt *att, long n_att, int Nc,
-                                    long *n_used, int *Ns_out, int *Nf_out)
+ * exhausted before Nc successes.  vel is double[np][ndim]. */
+int oracle_scatter_elastic_predrawn_nd(const oracle_params *p, int ndim, double dt, long np, double *vel,
+                                       const oracle_attempt *att, long n_att, int Nc,
+                                       long *n_used, int *Ns_out, int *Nf_out)
 {
   int Ns = 0, Nf = 0, rc = 0;
   long a = 0;
   (void)dt;
   while (Ns < Nc) {
     long ip;
-    int acc, fail;
-    double vf[3];
+    int acc, fail, d;
+    double vi[3] = {0, 0, 0}, vf[3];
     if (a >= n_att) { rc = 3; break; }
     ip = pick_ip(np, att[a].u_ip);
-    oracle_collide_one(p, &att[a], &vel[ip * 3], vf, &acc, &fail);
+    for (d = 0; d < ndim; d++) vi[d] = vel[ip * ndim + d];
+    oracle_collide_one_nd(p, ndim, &att[a], vi, vf, &acc, &fail);
     a++;
     if (!acc) continue;
     if (fail) {
       Nf++;
       if (Nf >= 10) { rc = 2; break; }
     } else {
-      vel[ip * 3 + 0] = vf[0];
-      vel[ip * 3 + 1] = vf[1];
-      vel[ip * 3 + 2] = vf[2];
+      for (d = 0; d < ndim; d++) vel[ip * ndim + d] = vf[d];
       Ns++;
     }
   }
@@ -746,54 +753,60 @@ int oracle_scatter_elastic_predrawn(const oracle_params *p, double dt, long np, 
   return rc;
 }
 
+int oracle_scatter_elastic_predrawn(const oracle_params *p, double dt, long np, double *vel,
+                                    const oracle_attempt *att, long n_att, int Nc,
+                                    long *n_used, int *Ns_out, int *Nf_out)
+{
+  return oracle_scatter_elastic_predrawn_nd(p, 3, dt, np, vel, att, n_att, Nc, n_used, Ns_out, Nf_out);
+}
+
 /* The reference stream: ScatterElastic with ran3/gasdev drawn in the reference's
- * order from the caller-held static state, starting from a local seed (getseed).
- * Each consumed attempt is appended to log (u_z, u_phi = -1 when not drawn).    */
-int oracle_scatter_elastic_nr(const oracle_params *p, double dt, long np, double *vel,
-                              oracle_nr_state *st, long seed, oracle_attempt *log,
-                              long log_cap, long *n_log, int *Ns_out, int *Nf_out)
+ * order from the caller-held static state, starting from a local seed (getseed):
+ * per attempt ran3 for the ion, ndim gasdev for the neutral, two ran3 for the
+ * acceptance and, when accepted, two more for the angles.  Each consumed attempt is
+ * appended to log (g[d] = 0 for d >= ndim; u_z, u_phi = -1 when not drawn).
+ * vel is double[np][ndim]. */
+int oracle_scatter_elastic_nr_nd(const oracle_params *p, int ndim, double dt, long np, double *vel,
+                                 oracle_nr_state *st, long seed, oracle_attempt *log,
+                                 long log_cap, long *n_log, int *Ns_out, int *Nf_out)
 {
   int Nc = oracle_collision_count(p, np, dt);
   double vrm = vrm_of(p);
+  double vn0[3] = {p->vn0x, p->vn0y, p->vn0z};
   int Ns = 0, Nf = 0, rc = 0;
   long nl = 0;
-  (void)vrm;
   while (Ns < Nc) {
     oracle_attempt a;
     long ip;
     int d, acc, fail;
-    double vf[3], tmp = 0.0, vrr;
+    double vi[3] = {0, 0, 0}, vf[3], tmp = 0.0, vrr;
     a.u_ip = oracle_ran3(st, &seed);
     ip = pick_ip(np, a.u_ip);
-    for (d = 0; d < 3; d++) a.g[d] = oracle_gasdev(st, &seed);
+    a.g[0] = a.g[1] = a.g[2] = 0.0f;
+    for (d = 0; d < ndim; d++) {
+      a.g[d] = oracle_gasdev(st, &seed);
+      vi[d] = vel[ip * ndim + d];
+      tmp += (vi[d] - (a.g[d] * p->vnT + vn0[d])) * (vi[d] - (a.g[d] * p->vnT + vn0[d]));
+    }
+    vrr = sqrt(tmp);
     a.u_acc1 = oracle_ran3(st, &seed);
     a.u_acc2 = oracle_ran3(st, &seed);
     a.u_z = -1.0f;
     a.u_phi = -1.0f;
     /* The acceptance decision needs vrr before the angle draws. */
-    {
-      double vn0[3] = {p->vn0x, p->vn0y, p->vn0z};
-      for (d = 0; d < 3; d++) {
-        double vr = vel[ip * 3 + d] - (a.g[d] * p->vnT + vn0[d]);
-        tmp += vr * vr;
-      }
-      vrr = sqrt(tmp);
-    }
-    if ((float)(a.u_acc1 * a.u_acc2) < vrr * vrr / (vrm_of(p) * vrm_of(p))) {
+    if ((float)(a.u_acc1 * a.u_acc2) < vrr * vrr / (vrm * vrm)) {
       a.u_z = oracle_ran3(st, &seed);
       a.u_phi = oracle_ran3(st, &seed);
     }
     if (log && nl < log_cap) log[nl] = a;
     nl++;
-    oracle_collide_one(p, &a, &vel[ip * 3], vf, &acc, &fail);
+    oracle_collide_one_nd(p, ndim, &a, vi, vf, &acc, &fail);
     if (!acc) continue;
     if (fail) {
       Nf++;
       if (Nf >= 10) { rc = 2; break; }
     } else {
-      vel[ip * 3 + 0] = vf[0];
-      vel[ip * 3 + 1] = vf[1];
-      vel[ip * 3 + 2] = vf[2];
+      for (d = 0; d < ndim; d++) vel[ip * ndim + d] = vf[d];
       Ns++;
     }
   }
@@ -801,6 +814,229 @@ int oracle_scatter_elastic_nr(const oracle_params *p, double dt, long np, double
   if (Ns_out) *Ns_out = Ns;
   if (Nf_out) *Nf_out = Nf;
   return rc;
+}
+
+int oracle_scatter_elastic_nr(const oracle_params *p, double dt, long np, double *vel,
+                              oracle_nr_state *st, long seed, oracle_attempt *log,
+                              long log_cap, long *n_log, int *Ns_out, int *Nf_out)
+{
+  return oracle_scatter_elastic_nr_nd(p, 3, dt, np, vel, st, seed, log, log_cap, n_log, Ns_out, Nf_out);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Two-dimensional variants (cli.ndim = 2: src/setup.c:99-104, Nz = 1).        */
+/* Particles are double[np][2]; moments FluidNode[Ny][Nx] with Gz = 0;        */
+/* phi double[Ny][Nx].                                                         */
+/* ------------------------------------------------------------------------- */
+#define PHI2(jj, ii) F[(long)wrapi((jj), Ny) * Nx + wrapi((ii), Nx)]
+
+/* vector-math.c:529-592 (DifferenceVector2D): plain bilinear weights (no swap). */
+void oracle_difference_vector2d(const double *F, int Nx, int Ny, double x0, double y0, double f[2])
+{
+  int ixl = (int)x0, ixh = ixl + 1, iyl = (int)y0, iyh = iyl + 1;
+  double wxh = x0 - (double)ixl, wyh = y0 - (double)iyl;
+  double hh, lh, hl, ll, Ewh, Ewl;
+  if (ixl >= 0) {
+    hl = PHI2(iyh, ixl + 1) - PHI2(iyh, ixl - 1);
+    ll = PHI2(iyl, ixl + 1) - PHI2(iyl, ixl - 1);
+  } else {
+    hl = -1.0 * PHI2(iyh, ixl + 2) + 4.0 * PHI2(iyh, ixl + 1) - 3.0 * PHI2(iyh, ixl);
+    ll = -1.0 * PHI2(iyl, ixl + 2) + 4.0 * PHI2(iyl, ixl + 1) - 3.0 * PHI2(iyl, ixl);
+  }
+  if (ixh < Nx) {
+    hh = PHI2(iyh, ixh + 1) - PHI2(iyh, ixh - 1);
+    lh = PHI2(iyl, ixh + 1) - PHI2(iyl, ixh - 1);
+  } else {
+    hh = +3.0 * PHI2(iyh, ixh) - 4.0 * PHI2(iyh, ixh - 1) + 1.0 * PHI2(iyh, ixh - 2);
+    lh = +3.0 * PHI2(iyl, ixh) - 4.0 * PHI2(iyl, ixh - 1) + 1.0 * PHI2(iyl, ixh - 2);
+  }
+  Ewh = lh + wyh * (hh - lh);
+  Ewl = ll + wyh * (hl - ll);
+  f[0] = Ewl + wxh * (Ewh - Ewl);
+  if (iyl >= 0) {
+    lh = PHI2(iyl + 1, ixh) - PHI2(iyl - 1, ixh);
+    ll = PHI2(iyl + 1, ixl) - PHI2(iyl - 1, ixl);
+  } else {
+    lh = -1.0 * PHI2(iyl + 2, ixh) + 4.0 * PHI2(iyl + 1, ixh) - 3.0 * PHI2(iyl, ixh);
+    ll = -1.0 * PHI2(iyl + 2, ixl) + 4.0 * PHI2(iyl + 1, ixl) - 3.0 * PHI2(iyl, ixl);
+  }
+  if (iyh < Ny) {
+    hh = PHI2(iyh + 1, ixh) - PHI2(iyh - 1, ixh);
+    hl = PHI2(iyh + 1, ixl) - PHI2(iyh - 1, ixl);
+  } else {
+    hh = +3.0 * PHI2(iyh, ixh) - 4.0 * PHI2(iyh - 1, ixh) + 1.0 * PHI2(iyh - 2, ixh);
+    hl = +3.0 * PHI2(iyh, ixl) - 4.0 * PHI2(iyh - 1, ixl) + 1.0 * PHI2(iyh - 2, ixl);
+  }
+  Ewh = lh + wyh * (hh - lh);
+  Ewl = ll + wyh * (hl - ll);
+  f[1] = Ewl + wxh * (Ewh - Ewl);
+}
+#undef PHI2
+
+/* velocities.c:186-273 (BorisMoverBz2D). */
+void oracle_boris_bz2d(const oracle_params *p, double dt, long np, const double *pos, double *vel,
+                       const double *phi)
+{
+  double h[2] = {1.0 / p->dx, 1.0 / p->dy};
+  double tscale = 0.5 * (p->qi / p->mi) * dt;
+  double t = tscale * p->B0;
+  double s = 2.0 * t / (1 + t * t);
+  double Escale[2], E[2];
+  long ip;
+  int d;
+  for (d = 0; d < 2; d++) Escale[d] = -0.5 * h[d] * tscale;
+  for (ip = 0; ip < np; ip++) {
+    double x = pos[ip * 2 + 0] / p->dx;
+    double y = pos[ip * 2 + 1] / p->dy;
+    double vxm, vym, vxt, vxp, vyp;
+    oracle_difference_vector2d(phi, p->Nx, p->Ny, x, y, E);
+    vxm = vel[ip * 2 + 0] + Escale[0] * E[0];
+    vym = vel[ip * 2 + 1] + Escale[1] * E[1];
+    vxt = vxm + vym * t;
+    vyp = vym - vxt * s;
+    vxp = vxt + vyp * t;
+    vel[ip * 2 + 0] = vxp;
+    vel[ip * 2 + 1] = vyp;
+  }
+}
+
+/* positions.c:436-470 (UpdatePositions) and boundaries.c:6-42 (Apply2DBC), ndim = 2. */
+void oracle_update_positions2d(double dt, long np, double *pos, const double *vel)
+{
+  long ip;
+  int d;
+  for (ip = 0; ip < np; ip++)
+    for (d = 0; d < 2; d++) pos[ip * 2 + d] += vel[ip * 2 + d] * dt;
+}
+
+void oracle_apply_bc2d(const oracle_params *p, long np, double *pos)
+{
+  double L[2] = {p->Lx, p->Ly}, r0[2] = {p->x0, p->y0}, r1[2] = {p->x1, p->y1};
+  long ip;
+  int d;
+  for (ip = 0; ip < np; ip++)
+    for (d = 0; d < 2; d++) {
+      double r = pos[ip * 2 + d];
+      if (r < r0[d]) r += L[d];
+      if (r >= r1[d]) r -= L[d];
+      pos[ip * 2 + d] = r;
+    }
+}
+
+/* moments.c:5-104 (Collect2DFluidMoments): m[Ny][Nx][4], Gz left at 0. */
+void oracle_collect_moments2d(const oracle_params *p, long np, const double *pos, const double *vel, double *m)
+{
+  int Nx = p->Nx, Ny = p->Ny;
+  long ip;
+  memset(m, 0, sizeof(double) * 4 * (long)Nx * Ny);
+  for (ip = 0; ip < np; ip++) {
+    double x = pos[ip * 2 + 0] / p->dx, y = pos[ip * 2 + 1] / p->dy;
+    int ixl = (int)x, ixh = ixl + 1, iyl = (int)y, iyh = iyl + 1;
+    double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+    double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+    double w4[4] = {wyh * wxh, wyl * wxh, wyh * wxl, wyl * wxl};
+    long n4[4];
+    int c, d;
+    n4[0] = (long)wrapi(iyh, Ny) * Nx + wrapi(ixh, Nx);
+    n4[1] = (long)wrapi(iyl, Ny) * Nx + wrapi(ixh, Nx);
+    n4[2] = (long)wrapi(iyh, Ny) * Nx + wrapi(ixl, Nx);
+    n4[3] = (long)wrapi(iyl, Ny) * Nx + wrapi(ixl, Nx);
+    for (c = 0; c < 4; c++) m[n4[c] * 4 + 0] += w4[c];
+    for (d = 0; d < 2; d++) {
+      double w = vel[ip * 2 + d];
+      for (c = 0; c < 4; c++) m[n4[c] * 4 + 1 + d] += w * w4[c];
+    }
+  }
+}
+
+/* lhs.c:194-334 (ComputeFullLHS2D): 9 coefficients per row in the reference's
+ * column order (i+1), (i-1), (j+1), (j-1), (i+1,j+1), (i+1,j-1), (i-1,j+1), (i-1,j-1),
+ * (i,j) (lhs.c:268-302). */
+static const int lhs2_off[9][2] = {{+1, 0}, {-1, 0}, {0, +1}, {0, -1}, {+1, +1}, {+1, -1}, {-1, +1}, {-1, -1}, {0, 0}};
+
+void oracle_full_lhs2d_coeffs(const oracle_params *p, const double *m, double *coef)
+{
+  int Nx = p->Nx, Ny = p->Ny, i, j;
+  double kappa = p->kappa_e;
+  double hxx = 1.0 / (2.0 * p->dx * p->dx), hyy = 1.0 / (2.0 * p->dy * p->dy), hod = 1.0 / (8.0 * p->dy * p->dx);
+  double sxx = p->scale * hxx, syy = p->scale * hyy, sod = p->scale * kappa * hod;
+#define N2(ii, jj) m[((long)wrapi((jj), Ny) * Nx + wrapi((ii), Nx)) * 4]
+  for (j = 0; j < Ny; j++)
+    for (i = 0; i < Nx; i++) {
+      double nijk = N2(i, j), nmjk = N2(i - 1, j), npjk = N2(i + 1, j), nimk = N2(i, j - 1), nipk = N2(i, j + 1);
+      double *c = &coef[((long)j * Nx + i) * 9];
+      c[4] = sod * (nipk - npjk);
+      c[5] = sod * (npjk - nimk);
+      c[6] = sod * (nmjk - nipk);
+      c[7] = sod * (nimk - nmjk);
+      c[0] = sxx * (npjk + nijk) + sod * (nipk - nimk);
+      c[1] = sxx * (nijk + nmjk) - sod * (nipk - nimk);
+      c[2] = syy * (nipk + nijk) - sod * (npjk - nmjk);
+      c[3] = syy * (nijk + nimk) + sod * (npjk - nmjk);
+      c[8] = -(sxx * (npjk + 2 * nijk + nmjk) + syy * (nipk + 2 * nijk + nimk));
+    }
+#undef N2
+}
+
+void oracle_full_lhs2d_csr(const oracle_params *p, const double *m, int *rowptr, int *cols, double *vals)
+{
+  int Nx = p->Nx, Ny = p->Ny, i, j, e;
+  long n = (long)Nx * Ny;
+  oracle_full_lhs2d_coeffs(p, m, vals);
+  for (j = 0; j < Ny; j++)
+    for (i = 0; i < Nx; i++) {
+      long r = (long)j * Nx + i;
+      rowptr[r] = (int)(r * 9);
+      for (e = 0; e < 9; e++)
+        cols[r * 9 + e] = (int)((long)wrapi(j + lhs2_off[e][1], Ny) * Nx + wrapi(i + lhs2_off[e][0], Nx));
+    }
+  rowptr[n] = (int)(n * 9);
+}
+
+void oracle_full_lhs2d_apply(const oracle_params *p, const double *m, const double *x, double *y)
+{
+  int Nx = p->Nx, Ny = p->Ny, i, j, e;
+  long n = (long)Nx * Ny;
+  double *coef = (double *)malloc(sizeof(double) * 9 * n);
+  oracle_full_lhs2d_coeffs(p, m, coef);
+  for (j = 0; j < Ny; j++)
+    for (i = 0; i < Nx; i++) {
+      long r = (long)j * Nx + i;
+      double acc = 0.0;
+      for (e = 0; e < 9; e++)
+        acc += coef[r * 9 + e] * x[(long)wrapi(j + lhs2_off[e][1], Ny) * Nx + wrapi(i + lhs2_off[e][0], Nx)];
+      y[r] = acc;
+    }
+  free(coef);
+}
+
+/* rhs.c:151-256 (ComputeFullRHS2D), then the null-space (mean) removal. */
+void oracle_full_rhs2d(const oracle_params *p, const double *m, double *b)
+{
+  int Nx = p->Nx, Ny = p->Ny, i, j;
+  double kappa = p->kappa_e, E0 = p->E0, scale = p->scale;
+  double cth = p->gammaT * OR_KB * p->Te / OR_Q;
+  double cG = (1 + kappa * kappa) * p->me * p->nue / OR_Q;
+  double hx = 1.0 / (2.0 * p->dx), hy = 1.0 / (2.0 * p->dy);
+  double hxx = 1.0 / (p->dx * p->dx), hyy = 1.0 / (p->dy * p->dy);
+#define M2(ii, jj, dof) m[((long)wrapi((jj), Ny) * Nx + wrapi((ii), Nx)) * 4 + (dof)]
+  for (j = 0; j < Ny; j++)
+    for (i = 0; i < Nx; i++) {
+      double nijk = M2(i, j, 0), npjk = M2(i + 1, j, 0), nmjk = M2(i - 1, j, 0);
+      double nipk = M2(i, j + 1, 0), nimk = M2(i, j - 1, 0);
+      double dndx = hx * (npjk - nmjk);
+      double dndy = hy * (nipk - nimk);
+      double d2ndxx = hxx * (npjk - 2 * nijk + nmjk);
+      double d2ndyy = hyy * (nipk - 2 * nijk + nimk);
+      double dGxdx = hx * (M2(i + 1, j, 1) - M2(i - 1, j, 1));
+      double dGydy = hy * (M2(i, j + 1, 2) - M2(i, j - 1, 2));
+      double Eterm = (dndy - kappa * dndx) * E0;
+      double Pterm = cth * (d2ndxx + d2ndyy);
+      double Gterm = cG * (dGxdx + dGydy);
+      b[(long)j * Nx + i] = scale * (Eterm + Pterm + Gterm);
+    }
+#undef M2
+  oracle_remove_mean((long)Nx * Ny, b);
 }
 
 /* ------------------------------------------------------------------------- */

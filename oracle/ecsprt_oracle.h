@@ -120,6 +120,31 @@ int  oracle_scatter_elastic_predrawn(const oracle_params *p, double dt, long np,
 void oracle_collide_one(const oracle_params *p, const oracle_attempt *a, const double vi_in[3],
                         double vf_out[3], int *accepted, int *failed);
 
+/* ndim-generic forms (2 or 3): vel is double[np][ndim]; vi/vf of the single attempt
+ * hold ndim components (the rest ignored). */
+void oracle_collide_one_nd(const oracle_params *p, int ndim, const oracle_attempt *a, const double vi_in[3],
+                           double vf_out[3], int *accepted, int *failed);
+int  oracle_scatter_elastic_nr_nd(const oracle_params *p, int ndim, double dt, long np, double *vel,
+                                  oracle_nr_state *st, long seed, oracle_attempt *log,
+                                  long log_cap, long *n_log, int *Ns, int *Nf);
+int  oracle_scatter_elastic_predrawn_nd(const oracle_params *p, int ndim, double dt, long np, double *vel,
+                                        const oracle_attempt *att, long n_att, int Nc,
+                                        long *n_used, int *Ns, int *Nf);
+
+/* Two-dimensional variants (ndim = 2): particles double[np][2], moments [Ny][Nx][4],
+ * phi and right-hand side [Ny][Nx]. */
+void oracle_difference_vector2d(const double *phi, int Nx, int Ny, double x, double y, double f[2]);
+void oracle_boris_bz2d(const oracle_params *p, double dt, long np, const double *pos, double *vel,
+                       const double *phi);
+void oracle_update_positions2d(double dt, long np, double *pos, const double *vel);
+void oracle_apply_bc2d(const oracle_params *p, long np, double *pos);
+void oracle_collect_moments2d(const oracle_params *p, long np, const double *pos, const double *vel,
+                              double *moments);
+void oracle_full_lhs2d_coeffs(const oracle_params *p, const double *moments, double *coef);
+void oracle_full_lhs2d_csr(const oracle_params *p, const double *moments, int *rowptr, int *cols, double *vals);
+void oracle_full_lhs2d_apply(const oracle_params *p, const double *moments, const double *x, double *y);
+void oracle_full_rhs2d(const oracle_params *p, const double *moments, double *b);
+
 /* lhs.c / rhs.c */
 void oracle_full_lhs3d_coeffs(const oracle_params *p, const double *moments, double *coef);
 void oracle_full_lhs3d_csr(const oracle_params *p, const double *moments,
