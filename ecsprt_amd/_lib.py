@@ -39,11 +39,13 @@ _PARAM_DOUBLES = ("dx", "dy", "dz", "Lx", "Ly", "Lz", "x0", "y0", "z0", "x1", "y
 
 
 class CLI(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int) for n in ("Nx", "Ny", "Nz")] + [(n, ctypes.c_double) for n in _CLI_DOUBLES]
+    _fields_ = [(n, ctypes.c_int) for n in ("Nx", "Ny", "Nz")] + [(n, ctypes.c_double) for n in _CLI_DOUBLES] + \
+        [("ndim", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int) for n in ("Nx", "Ny", "Nz")] + [(n, ctypes.c_double) for n in _PARAM_DOUBLES]
+    _fields_ = [(n, ctypes.c_int) for n in ("Nx", "Ny", "Nz")] + [(n, ctypes.c_double) for n in _PARAM_DOUBLES] + \
+        [("ndim", ctypes.c_int)]
 
 
 class Attempt(ctypes.Structure):

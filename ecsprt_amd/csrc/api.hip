@@ -90,18 +90,24 @@ int ecsprt_hip_setup_context(const ecsprt_cli *c, ecsprt_params *p) {
   p->vn0y = c->vn0y;
   p->vn0z = c->vn0z;
   p->Tn = c->Tn;
+  if (c->ndim != 0 && c->ndim != 2 && c->ndim != 3) return ECSPRT_ERR_ARG;  // src/setup.c:41
+  p->ndim = c->ndim == 2 ? 2 : 3;
   p->Nx = c->Nx > 0 ? c->Nx : 7;
   p->Ny = c->Ny > 0 ? c->Ny : 7;
-  p->Nz = c->Nz > 0 ? c->Nz : 7;
+  p->Nz = p->ndim == 2 ? 1 : (c->Nz > 0 ? c->Nz : 7);  // src/setup.c:178-188
   p->dx = c->dx;
   p->dy = c->dy;
-  p->dz = c->dz;
+  p->dz = p->ndim == 2 ? 0.0 : c->dz;
   p->x0 = c->x0;
   p->y0 = c->y0;
   p->z0 = c->z0;
   p->x1 = c->x1;
   p->y1 = c->y1;
   p->z1 = c->z1;
+  if (p->ndim == 2 && !(p->z1 > p->z0)) {  // one plane of unit thickness; no z coupling in 2-D
+    p->z0 = 0.0;
+    p->z1 = 1.0;
+  }
   if (p->dx <= 0.0) { p->Lx = p->x1 - p->x0; p->dx = p->Lx / (double)p->Nx; }
   else { p->Lx = p->dx * (double)p->Nx; p->x0 = 0.0; p->x1 = p->Lx; }
   if (p->dy <= 0.0) { p->Ly = p->y1 - p->y0; p->dy = p->Ly / (double)p->Ny; }
@@ -175,8 +181,12 @@ int ecsprt_hip_create(const ecsprt_params *p, const ecsprt_options *opt, ecsprt_
   if (opt) c->o = *opt;
   else ecsprt_hip_default_options(&c->o);
   int rc = guarded(c, [&] {
-    ECS_CHECK(p->Nx >= 3 && p->Ny >= 3 && p->Nz >= 3, ECSPRT_ERR_ARG,
-              "grid edges must be >= 3 (the 11-point stencil needs distinct neighbours)");
+    const bool two_d = p->ndim == 2;
+    ECS_CHECK(p->ndim == 2 || p->ndim == 3, ECSPRT_ERR_ARG, "ndim must be 2 or 3");
+    ECS_CHECK(p->Nx >= 3 && p->Ny >= 3 && (two_d ? p->Nz == 1 : p->Nz >= 3), ECSPRT_ERR_ARG,
+              "grid edges must be >= 3 (the 11-point stencil needs distinct neighbours; Nz = 1 in 2-D)");
+    ECS_CHECK(!two_d || c->o.particle_precision == ECSPRT_PARTICLES_FP64, ECSPRT_ERR_ARG,
+              "two-dimensional runs use FP64 particles");
     ECS_CHECK(p->dx > 0 && p->dy > 0 && p->dz > 0, ECSPRT_ERR_ARG, "cell widths must be positive");
     ECS_CHECK(c->o.particle_precision == ECSPRT_PARTICLES_FP64 || c->o.particle_precision == ECSPRT_PARTICLES_MIXED,
               ECSPRT_ERR_ARG, "particle_precision must be ECSPRT_PARTICLES_FP64 or ECSPRT_PARTICLES_MIXED");
@@ -200,7 +210,7 @@ int ecsprt_hip_create(const ecsprt_params *p, const ecsprt_options *opt, ecsprt_
     ECS_HIP(hipMemsetAsync(c->d_phi, 0, sizeof(double) * c->nnode, c->stream));
     ECS_HIP(hipMemsetAsync(c->d_moments, 0, sizeof(double) * 4 * c->nnode, c->stream));
     ECS_HIP(hipMemsetAsync(c->d_rhs, 0, sizeof(double) * c->nnode, c->stream));
-    ecsprt::tiles_init(c);
+    if (!two_d) ecsprt::tiles_init(c);
     ecsprt::solver_create(c);
     ECS_HIP(hipStreamSynchronize(c->stream));
   });
@@ -221,6 +231,7 @@ int ecsprt_hip_destroy(ecsprt_hip_ctx *c) {
   } catch (...) {
   }
   ecsprt::tiles_free(c);
+  ecsprt::p2d_free(c);
   (void)hipFree(c->d_phi);
   (void)hipFree(c->d_moments);
   (void)hipFree(c->d_rhs);
@@ -235,6 +246,7 @@ int ecsprt_hip_destroy(ecsprt_hip_ctx *c) {
 int ecsprt_hip_set_particles(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel) {
   if (!c || !pos || !vel || np <= 0) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
+    if (c->p.ndim == 2) return ecsprt::p2d_set(c, np, pos, vel, hipMemcpyHostToDevice);
     ecsprt::ensure_stage(c, np);
     ECS_HIP(hipMemcpyAsync(c->d_stage, pos, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
     ECS_HIP(hipMemcpyAsync(c->d_stage + 3 * np, vel, sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));
@@ -246,6 +258,7 @@ int ecsprt_hip_get_particles(ecsprt_hip_ctx *c, double *pos, double *vel) {
   if (!c || !pos || !vel) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
+    if (c->p.ndim == 2) return ecsprt::p2d_get(c, pos, vel);
     ecsprt::particles_to_stage(c);
     ECS_HIP(hipMemcpyAsync(pos, c->d_stage, sizeof(double) * 3 * c->np, hipMemcpyDeviceToHost, c->stream));
     ECS_HIP(hipMemcpyAsync(vel, c->d_stage + 3 * c->np, sizeof(double) * 3 * c->np, hipMemcpyDeviceToHost, c->stream));
@@ -255,7 +268,10 @@ int ecsprt_hip_get_particles(ecsprt_hip_ctx *c, double *pos, double *vel) {
 
 int ecsprt_hip_init_particles(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k) {
   if (!c || np <= 0) return ECSPRT_ERR_ARG;
-  return guarded(c, [&] { ecsprt::particles_init_device(c, np, seed, amp, k); });
+  return guarded(c, [&] {
+    if (c->p.ndim == 2) ecsprt::p2d_init_device(c, np, seed, amp, k);
+    else ecsprt::particles_init_device(c, np, seed, amp, k);
+  });
 }
 
 int ecsprt_hip_init_reference(ecsprt_hip_ctx *c, long np) {
@@ -275,7 +291,8 @@ int ecsprt_hip_cell_counts(ecsprt_hip_ctx *c, int *counts) {
   return guarded(c, [&] {
     int *d = nullptr;
     ECS_HIP(hipMallocAsync((void **)&d, sizeof(int) * c->nnode, c->stream));
-    ecsprt::run_cell_counts(c, d);
+    if (c->p.ndim == 2) ecsprt::p2d_cell_counts(c, d);
+    else ecsprt::run_cell_counts(c, d);
     ECS_HIP(hipMemcpyAsync(counts, d, sizeof(int) * c->nnode, hipMemcpyDeviceToHost, c->stream));
     ECS_HIP(hipFreeAsync(d, c->stream));
     ECS_HIP(hipStreamSynchronize(c->stream));
@@ -288,7 +305,8 @@ int ecsprt_hip_push(double dt, void *opts) {
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
     ev_record(c, 0);
-    ecsprt::run_push(c, dt);
+    if (c->p.ndim == 2) ecsprt::p2d_push(c, dt);
+    else ecsprt::run_push(c, dt);
     ev_record(c, 1);
     if (c->o.timing) {
       ECS_HIP(hipStreamSynchronize(c->stream));
@@ -299,11 +317,14 @@ int ecsprt_hip_push(double dt, void *opts) {
 
 int ecsprt_hip_collide(int ndim, double dt, void *opts) {
   ecsprt_hip_ctx *c = as_ctx(opts);
-  if (!c || ndim != 3) return ECSPRT_ERR_ARG;
+  if (!c || ndim != c->p.ndim) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
     if (c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) {
       ecsprt::run_collide_predrawn(c, dt);
+    } else if (c->o.collide_mode == ECSPRT_COLLIDE_PHILOX && ndim == 2) {
+      ecsprt::p2d_collide_philox(c, dt);
+      c->step_index++;
     } else if (c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) {
       throw Error{ECSPRT_ERR_ARG, "Philox collisions run inside ecsprt_hip_step"};
     }
@@ -315,7 +336,8 @@ int ecsprt_hip_apply_bc(int np, double *pos, double *vel, void *opts) {
   (void)vel;
   if (!c || !pos || np < 0) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
-    ecsprt::run_apply_bc_aos(c, np, pos);
+    if (c->p.ndim == 2) ecsprt::p2d_apply_bc(c, np, pos);
+    else ecsprt::run_apply_bc_aos(c, np, pos);
     ECS_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -326,7 +348,8 @@ int ecsprt_hip_collect(void *opts) {
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
     ev_record(c, 2);
-    ecsprt::run_deposit(c);
+    if (c->p.ndim == 2) ecsprt::p2d_deposit(c);
+    else ecsprt::run_deposit(c);
     ev_record(c, 3);
     ECS_HIP(hipStreamSynchronize(c->stream));
     if (c->o.timing) c->stage_ms[3] = ev_ms(c, 2, 3);
@@ -335,19 +358,22 @@ int ecsprt_hip_collect(void *opts) {
 
 int ecsprt_hip_update_velocities(int ndim, double dt, void *opts) {
   ecsprt_hip_ctx *c = as_ctx(opts);
-  if (!c || ndim != 3) return ECSPRT_ERR_ARG;
+  if (!c || ndim != c->p.ndim) return ECSPRT_ERR_ARG;
   int rc = ecsprt_hip_push(dt, opts);
   if (rc != ECSPRT_OK) return rc;
-  if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) return ecsprt_hip_collide(ndim, dt, opts);
+  const bool staged = c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN ||
+                      (ndim == 2 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX);
+  if (c->p.mn > 0.0 && staged) return ecsprt_hip_collide(ndim, dt, opts);
   return ECSPRT_OK;
 }
 
 int ecsprt_hip_update_positions(int ndim, double dt, void *opts) {
   ecsprt_hip_ctx *c = as_ctx(opts);
-  if (!c || ndim != 3) return ECSPRT_ERR_ARG;
+  if (!c || ndim != c->p.ndim) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
-    ecsprt::run_drift_rebin(c, dt);
+    if (ndim == 2) ecsprt::p2d_drift(c, dt);
+    else ecsprt::run_drift_rebin(c, dt);
   });
 }
 
@@ -370,6 +396,27 @@ int ecsprt_hip_step(void *opts, double dt) {
   if (!c) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
+    if (c->p.ndim == 2) {
+      // The reference loop (src/pic.c:246-254) stage by stage: UpdateVelocities (push,
+      // collisions), UpdatePositions, collect, potential.
+      ev_record(c, 0);
+      ecsprt::p2d_push(c, dt);
+      if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) ecsprt::p2d_collide_philox(c, dt);
+      else if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) ecsprt::run_collide_predrawn(c, dt);
+      ecsprt::p2d_drift(c, dt);
+      ecsprt::p2d_deposit(c);
+      ev_record(c, 3);
+      compute_potential(c);
+      c->step_index++;
+      if (c->o.timing) {
+        ECS_HIP(hipStreamSynchronize(c->stream));
+        c->stage_ms[0] = ev_ms(c, 0, 3);
+        c->stage_ms[1] = c->stage_ms[2] = c->stage_ms[3] = 0.0;
+        c->stage_ms[4] = ev_ms(c, 3, 4);
+        c->stage_ms[5] = ev_ms(c, 4, 5);
+      }
+      return;
+    }
     ev_record(c, 0);
     ecsprt::run_fused_particles(c, dt);
     ev_record(c, 3);

@@ -184,6 +184,38 @@ __global__ void k_att_replay(int mode, long np, const ecsprt_attempt *att, const
   }
 }
 
+// The same replay on two-dimensional ions, double[np][2] velocities in ion order.
+__global__ void k_att_replay2d(int mode, long np, const ecsprt_attempt *att, const int *cnt, const long long *off,
+                               const int *list, double *vel, CollideConsts k, int *succ, int *fail,
+                               const long long *cut_ptr) {
+  const long long cut = mode ? *cut_ptr : 0;
+  for (long ip = blockIdx.x * (long)blockDim.x + threadIdx.x; ip < np; ip += (long)gridDim.x * blockDim.x) {
+    const int n = cnt[ip];
+    if (n == 0) continue;
+    double v[3] = {vel[2 * ip], vel[2 * ip + 1], 0.0};
+    const int *L = list + off[ip];
+    for (int q = 0; q < n; ++q) {
+      const int a = L[q];
+      if (mode && a >= cut) break;
+      const ecsprt_attempt r = att[a];
+      double vf[3];
+      const int res = collide_one<double, 2>(k, r.g, r.u_acc1, r.u_acc2, r.u_z, r.u_phi, v, vf);
+      if (res == 1) {
+        v[0] = vf[0];
+        v[1] = vf[1];
+      }
+      if (!mode) {
+        succ[a] = res == 1;
+        fail[a] = res == 2;
+      }
+    }
+    if (mode) {
+      vel[2 * ip] = v[0];
+      vel[2 * ip + 1] = v[1];
+    }
+  }
+}
+
 // res[0] = cut, res[1] = successes in total, res[2] = refusals before the cut.
 __global__ void k_att_cut(long na, int Nc, const int *succ, const long long *succ_ex, const int *fail,
                           const long long *fail_ex, long long *res) {
@@ -238,9 +270,13 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
   scan_exclusive(c, cnt, off, np);
   k_att_place<<<gb, 256, 0, st>>>(na, ipa, off, fill, list);
   k_att_sort<<<gb, 256, 0, st>>>(np, cnt, off, list);
-  k_slot_of_id<<<c->ntiles, 256, 0, st>>>(c->A, c->d_base, c->d_count, slot);
+  const bool two_d = c->p.ndim == 2;
+  if (!two_d) k_slot_of_id<<<c->ntiles, 256, 0, st>>>(c->A, c->d_base, c->d_count, slot);
   CollideConsts k = collide_consts_of(c->p);
-  k_att_replay<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, nullptr);
+  if (two_d)
+    k_att_replay2d<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, c->d_vel2, k, succ, fail, nullptr);
+  else
+    k_att_replay<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, nullptr);
   scan_exclusive(c, succ, succ_ex, na);
   scan_exclusive(c, fail, fail_ex, na);
   k_set_i64<<<1, 1, 0, st>>>(res, (long long)na);
@@ -260,7 +296,10 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
     msg = "pre-drawn records exhausted after " + std::to_string(total_succ) + " of " + std::to_string(Nc) +
           " collisions";
   } else {
-    k_att_replay<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, res);
+    if (two_d)
+      k_att_replay2d<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, c->d_vel2, k, succ, fail, res);
+    else
+      k_att_replay<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, res);
     ECS_HIP(hipGetLastError());
   }
   void *bufs[] = {cnt, fill, slot, off, ipa, list, succ, fail, succ_ex, fail_ex, res};

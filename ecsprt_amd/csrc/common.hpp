@@ -162,6 +162,13 @@ struct ecsprt_hip_ctx {
   int block = 1024;  // threads per tile workgroup in effect
   int fx_bits = 40;  // FP64 path: fixed-point bits of the deposit regions (set at binning)
 
+  // two-dimensional runs (p.ndim == 2): ions double[np][2] in the reference's layout,
+  // fixed-point moment sums [Ny][Nx][3], per-cell counts plus their maximum
+  double *d_pos2 = nullptr, *d_vel2 = nullptr;
+  long cap2 = 0;
+  unsigned long long *d_acc2 = nullptr;
+  int *d_cnt2 = nullptr;
+
   // staging (host<->device AoS)
   double *d_stage = nullptr;
   long stage_cap = 0;
@@ -197,6 +204,19 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool ids_from_stage);
 void particles_to_stage(ecsprt_hip_ctx *c);
 void particles_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k);
 void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist);
+
+// Two-dimensional ion stages (particles2d.hip).
+void p2d_reserve(ecsprt_hip_ctx *c, long np);
+void p2d_free(ecsprt_hip_ctx *c);
+void p2d_set(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel, hipMemcpyKind kind);
+void p2d_get(ecsprt_hip_ctx *c, double *pos, double *vel);
+void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k);
+void p2d_push(ecsprt_hip_ctx *c, double dt);
+void p2d_drift(ecsprt_hip_ctx *c, double dt);
+void p2d_apply_bc(ecsprt_hip_ctx *c, long np, double *pos);
+void p2d_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
+void p2d_deposit(ecsprt_hip_ctx *c);
+void p2d_collide_philox(ecsprt_hip_ctx *c, double dt);
 void run_push(ecsprt_hip_ctx *c, double dt);
 void run_drift_rebin(ecsprt_hip_ctx *c, double dt);
 void run_deposit(ecsprt_hip_ctx *c);

@@ -1,0 +1,397 @@
+// Two-dimensional ion stages (cli.ndim = 2, src/setup.c:99-104): the push
+// (BorisMoverBz2D, src/velocities.c:186-273), the drift and periodic wrap
+// (UpdatePositions and Apply2DBC with ndim = 2, src/positions.c:436-470,
+// src/boundaries.c:6-42), the deposit (Collect2DFluidMoments, src/moments.c:5-104)
+// and the two-dimensional ScatterElastic (src/collisions.c:9-170 with ndim = 2).
+//
+// Ions are kept in the reference's own layout, double[np][2] positions and
+// velocities, in the order they were set: a two-dimensional grid is small enough that
+// the potential and the moments stay in the L2 cache, so the ions are streamed once per
+// stage without binning.  The deposit is exact and order-independent: every weight
+// (and every flux over vmax) is rounded once to a fixed-point grid of 2^-fx and added
+// with 64-bit integer atomics, fx chosen from the fullest cell so that no node sum can
+// overflow; ions with a velocity component at or above vmax take FP64 atomics.
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+#include "philox.hpp"
+#include "pic_math.hpp"
+#include "tiles.hpp"
+
+namespace ecsprt {
+
+int collision_count(const ecsprt_params *p, long np, double dt);
+double mx_vmax(const ecsprt_params &p);
+
+namespace {
+
+// DifferenceVector2D (src/vector-math.c:529-592) on the unghosted periodic potential:
+// plain bilinear weights, one-sided differences at an upper node index N.
+__device__ inline void diff_vec2d(const double *__restrict__ F, int Nx, int Ny, double x0, double y0, double f[2]) {
+  auto P = [&](int j, int i) { return F[(long)wrap_index(j, Ny) * Nx + wrap_index(i, Nx)]; };
+  const int ixl = (int)x0, ixh = ixl + 1, iyl = (int)y0, iyh = iyl + 1;
+  const double wxh = x0 - (double)ixl, wyh = y0 - (double)iyl;
+  double hh, lh, hl, ll;
+  if (ixl >= 0) {
+    hl = P(iyh, ixl + 1) - P(iyh, ixl - 1);
+    ll = P(iyl, ixl + 1) - P(iyl, ixl - 1);
+  } else {
+    hl = -1.0 * P(iyh, ixl + 2) + 4.0 * P(iyh, ixl + 1) - 3.0 * P(iyh, ixl);
+    ll = -1.0 * P(iyl, ixl + 2) + 4.0 * P(iyl, ixl + 1) - 3.0 * P(iyl, ixl);
+  }
+  if (ixh < Nx) {
+    hh = P(iyh, ixh + 1) - P(iyh, ixh - 1);
+    lh = P(iyl, ixh + 1) - P(iyl, ixh - 1);
+  } else {
+    hh = +3.0 * P(iyh, ixh) - 4.0 * P(iyh, ixh - 1) + 1.0 * P(iyh, ixh - 2);
+    lh = +3.0 * P(iyl, ixh) - 4.0 * P(iyl, ixh - 1) + 1.0 * P(iyl, ixh - 2);
+  }
+  double Ewh = lh + wyh * (hh - lh), Ewl = ll + wyh * (hl - ll);
+  f[0] = Ewl + wxh * (Ewh - Ewl);
+  if (iyl >= 0) {
+    lh = P(iyl + 1, ixh) - P(iyl - 1, ixh);
+    ll = P(iyl + 1, ixl) - P(iyl - 1, ixl);
+  } else {
+    lh = -1.0 * P(iyl + 2, ixh) + 4.0 * P(iyl + 1, ixh) - 3.0 * P(iyl, ixh);
+    ll = -1.0 * P(iyl + 2, ixl) + 4.0 * P(iyl + 1, ixl) - 3.0 * P(iyl, ixl);
+  }
+  if (iyh < Ny) {
+    hh = P(iyh + 1, ixh) - P(iyh - 1, ixh);
+    hl = P(iyh + 1, ixl) - P(iyh - 1, ixl);
+  } else {
+    hh = +3.0 * P(iyh, ixh) - 4.0 * P(iyh - 1, ixh) + 1.0 * P(iyh - 2, ixh);
+    hl = +3.0 * P(iyh, ixl) - 4.0 * P(iyh - 1, ixl) + 1.0 * P(iyh - 2, ixl);
+  }
+  Ewh = lh + wyh * (hh - lh);
+  Ewl = ll + wyh * (hl - ll);
+  f[1] = Ewl + wxh * (Ewh - Ewl);
+}
+
+// BorisMoverBz2D body (src/velocities.c:254-262).
+__global__ void k_push2d(long np, const double *__restrict__ phi, int Nx, int Ny, PushConsts k,
+                         const double *__restrict__ pos, double *__restrict__ vel) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    const double x = cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2);
+    const double y = cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2);
+    double E[2];
+    diff_vec2d(phi, Nx, Ny, x, y, E);
+    const double vxm = vel[2 * p] + k.Escale[0] * E[0];
+    const double vym = vel[2 * p + 1] + k.Escale[1] * E[1];
+    const double vxt = vxm + vym * k.t;
+    const double vyp = vym - vxt * k.s;
+    const double vxp = vxt + vyp * k.t;
+    vel[2 * p] = vxp;
+    vel[2 * p + 1] = vyp;
+  }
+}
+
+// UpdatePositions (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28), one
+// wrap per axis; ions still outside [r0, r1] (or not finite) are counted in lost.
+__global__ void k_drift2d(long np, DriftConsts d, double *__restrict__ pos, const double *__restrict__ vel,
+                          unsigned long long *lost) {
+  unsigned long long nl = 0;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    bool out = false;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      double r = pos[2 * p + a] + vel[2 * p + a] * d.dt;
+      if (r < d.r0[a]) r += d.L[a];
+      if (r >= d.r1[a]) r -= d.L[a];
+      pos[2 * p + a] = r;
+      out |= !(r >= d.r0[a] && r <= d.r1[a]);
+    }
+    nl += out ? 1ull : 0ull;
+  }
+  nl = wave_sum(nl);
+  if (__lane_id() == 0 && nl) atomicAdd(lost, nl);
+}
+
+__global__ void k_apply_bc2d(long np, DriftConsts d, double *__restrict__ pos) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      double r = pos[2 * p + a];
+      if (r < d.r0[a]) r += d.L[a];
+      if (r >= d.r1[a]) r -= d.L[a];
+      pos[2 * p + a] = r;
+    }
+  }
+}
+
+// Ions per cell ((int)(pos/d) per axis, a coordinate at L in cell 0) and the largest
+// count, which bounds every node sum of weights by four times it.
+__global__ void k_cell_counts2d(long np, int Nx, int Ny, double dx, double dy, const double *__restrict__ pos,
+                                int *__restrict__ counts) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    const int i = wrap_index((int)(pos[2 * p] / dx), Nx), j = wrap_index((int)(pos[2 * p + 1] / dy), Ny);
+    atomicAdd(&counts[(long)j * Nx + i], 1);
+  }
+}
+
+__global__ void k_max_i32(const int *__restrict__ a, long n, int *out) {
+  int m = 0;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x)
+    m = max(m, a[q]);
+  for (int o = 32; o > 0; o >>= 1) m = max(m, __shfl_down(m, o));
+  if (__lane_id() == 0) atomicMax(out, m);
+}
+
+// Fixed-point bits for node sums bounded by 4 * maxcell (weights <= 1, |flux/vmax| < 1).
+__device__ inline int fx2d_bits(int maxcell) {
+  int b = 0;
+  while (b < 62 && (1ll << b) < 4ll * (long long)maxcell + 1) ++b;
+  return min(50, 62 - b);
+}
+
+// Collect2DFluidMoments (src/moments.c:44-82): bilinear weights hh, lh, hl, ll of the
+// four corner nodes; moments n, Gx, Gy as fixed-point integers acc[node][3].
+__global__ void k_deposit2d(long np, int Nx, int Ny, PushConsts k, double inv_vmax, const int *__restrict__ maxcell,
+                            const double *__restrict__ pos, const double *__restrict__ vel,
+                            unsigned long long *__restrict__ acc, double *__restrict__ ovf) {
+  const double sw = ldexp(1.0, fx2d_bits(*maxcell));
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    const double x = cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2);
+    const double y = cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2);
+    const int ixl = (int)x, ixh = ixl + 1, iyl = (int)y, iyh = iyl + 1;
+    const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+    const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+    const double w4[4] = {wyh * wxh, wyl * wxh, wyh * wxl, wyl * wxl};
+    const int ci[4] = {ixh, ixh, ixl, ixl}, cj[4] = {iyh, iyl, iyh, iyl};
+    const double vx = vel[2 * p], vy = vel[2 * p + 1];
+    const double ux = vx * inv_vmax, uy = vy * inv_vmax;
+    const bool slow = fabs(ux) < 1.0 && fabs(uy) < 1.0;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const long node = (long)wrap_index(cj[c], Ny) * Nx + wrap_index(ci[c], Nx);
+      if (slow) {
+        atomicAdd(&acc[node * 3 + 0], (unsigned long long)fx_round(w4[c] * sw));
+        atomicAdd(&acc[node * 3 + 1], (unsigned long long)fx_round(ux * w4[c] * sw));
+        atomicAdd(&acc[node * 3 + 2], (unsigned long long)fx_round(uy * w4[c] * sw));
+      } else {
+        unsafeAtomicAdd(&ovf[node * 4 + 0], w4[c]);
+        unsafeAtomicAdd(&ovf[node * 4 + 1], vx * w4[c]);
+        unsafeAtomicAdd(&ovf[node * 4 + 2], vy * w4[c]);
+      }
+    }
+  }
+}
+
+// moments[node] = {n, Gx, Gy, 0} from the fixed-point sums and the overflow moments.
+__global__ void k_fold2d(long n, const unsigned long long *__restrict__ acc, const double *__restrict__ ovf,
+                         const int *__restrict__ maxcell, double vmax, double *__restrict__ m) {
+  const double sn = ldexp(1.0, -fx2d_bits(*maxcell)), sg = vmax * sn;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
+    m[q * 4 + 0] = (double)(long long)acc[q * 3 + 0] * sn + ovf[q * 4 + 0];
+    m[q * 4 + 1] = (double)(long long)acc[q * 3 + 1] * sg + ovf[q * 4 + 1];
+    m[q * 4 + 2] = (double)(long long)acc[q * 3 + 2] * sg + ovf[q * 4 + 2];
+    m[q * 4 + 3] = 0.0;
+  }
+}
+
+// Statistical collisions (Philox mode) in two dimensions: a Poisson number of attempts
+// per ion keyed by (ion index, step), each the two-dimensional scattering formula.
+__global__ void k_collide2d(long np, PoissonTable pt, CollideConsts cc, uint32_t key0, uint32_t key1, uint32_t step,
+                            int write, double *__restrict__ vel, unsigned long long *ctr) {
+  __shared__ float s_cdf[24];
+  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
+  __syncthreads();
+  unsigned natt = 0, nsucc = 0, nfail = 0;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    const int k = poisson_attempts(s_cdf, key0, key1, step, (int)p);
+    if (k == 0) continue;
+    double vx = vel[2 * p], vy = vel[2 * p + 1], vz = 0.0;
+    collide_attempts<double, 2>(cc, key0, key1, step, (int)p, 0, k, vx, vy, vz, natt, nsucc, nfail);
+    if (write) {
+      vel[2 * p] = vx;
+      vel[2 * p + 1] = vy;
+    }
+  }
+  unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+  if (__lane_id() == 0) {
+    if (s0) atomicAdd(&ctr[3], s0);
+    if (s1) atomicAdd(&ctr[4], s1);
+    if (s2) atomicAdd(&ctr[5], s2);
+  }
+}
+
+// Device initial conditions: positions uniform in the domain (x optionally displaced
+// to a density 1 + amp cos(2 pi k x / Lx)), velocities Maxwellian per component.
+__global__ void k_init2d(long np, double Lx, double Ly, double x0, double y0, double amp, int kk, double vtx,
+                         double vty, double v0x, double v0y, uint32_t key0, uint32_t key1, double *__restrict__ pos,
+                         double *__restrict__ vel) {
+  const double twopi = 6.283185307179586;
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    u32x4 c0 = {{(uint32_t)p, (uint32_t)(p >> 32), 0x2D2Du, 0u}};
+    u32x4 c1 = {{(uint32_t)p, (uint32_t)(p >> 32), 0x2D2Du, 1u}};
+    u32x4 r0 = philox4x32_10(c0, key0, key1);
+    u32x4 r1 = philox4x32_10(c1, key0, key1);
+    const double ux = ((double)(r0.v[0] >> 5) * 67108864.0 + (double)(r0.v[1] >> 6)) * (1.0 / 9007199254740992.0);
+    const double uy = ((double)(r0.v[2] >> 5) * 67108864.0 + (double)(r0.v[3] >> 6)) * (1.0 / 9007199254740992.0);
+    double xs = ux;
+    if (amp != 0.0 && kk != 0) {
+      const double kw = twopi * kk;
+      for (int it = 0; it < 8; ++it) {
+        const double F = xs - amp / kw * (cos(kw * xs) - 1.0) - ux;
+        const double dF = 1.0 + amp * sin(kw * xs);
+        xs -= F / dF;
+      }
+      xs = xs - floor(xs);
+    }
+    double x = x0 + xs * Lx, y = y0 + uy * Ly;
+    if (x >= x0 + Lx) x = x0;
+    if (y >= y0 + Ly) y = y0;
+    pos[2 * p] = x;
+    pos[2 * p + 1] = y;
+    const double u1 = ((double)r1.v[0] + 0.5) * (1.0 / 4294967296.0);
+    const double u2 = ((double)r1.v[1] + 0.5) * (1.0 / 4294967296.0);
+    const double ra = sqrt(-2.0 * log(u1));
+    vel[2 * p] = vtx * ra * cos(twopi * u2) + v0x;
+    vel[2 * p + 1] = vty * ra * sin(twopi * u2) + v0y;
+  }
+}
+
+int grid2(long n) { return (int)std::max(1L, std::min(65536L, (n + 255) / 256)); }
+
+}  // namespace
+
+// ---- host side -------------------------------------------------------------------
+
+void p2d_reserve(ecsprt_hip_ctx *c, long np) {
+  if (!c->d_acc2) {
+    ECS_HIP(hipMalloc(&c->d_acc2, sizeof(unsigned long long) * 3 * c->nnode));
+    ECS_HIP(hipMalloc(&c->d_cnt2, sizeof(int) * (c->nnode + 1)));
+  }
+  if (!c->d_ovf) ECS_HIP(hipMalloc(&c->d_ovf, sizeof(double) * 4 * c->nnode));
+  if (!c->d_ctr) {
+    ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * 8));
+    ECS_HIP(hipHostMalloc(&c->h_ctr, sizeof(unsigned long long) * 8));
+  }
+  if (c->cap2 >= np && c->d_pos2) return;
+  (void)hipFree(c->d_pos2);
+  (void)hipFree(c->d_vel2);
+  c->d_pos2 = c->d_vel2 = nullptr;
+  c->cap2 = 0;
+  if (hipMalloc(&c->d_pos2, sizeof(double) * 2 * std::max(np, 1L)) != hipSuccess ||
+      hipMalloc(&c->d_vel2, sizeof(double) * 2 * std::max(np, 1L)) != hipSuccess)
+    throw Error{ECSPRT_ERR_NOMEM, "two-dimensional particle arrays"};
+  c->cap2 = np;
+}
+
+void p2d_free(ecsprt_hip_ctx *c) {
+  (void)hipFree(c->d_pos2);
+  (void)hipFree(c->d_vel2);
+  (void)hipFree(c->d_acc2);
+  (void)hipFree(c->d_cnt2);
+  c->d_pos2 = c->d_vel2 = nullptr;
+  c->d_acc2 = nullptr;
+  c->d_cnt2 = nullptr;
+}
+
+void p2d_set(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel, hipMemcpyKind kind) {
+  p2d_reserve(c, np);
+  ECS_HIP(hipMemcpyAsync(c->d_pos2, pos, sizeof(double) * 2 * np, kind, c->stream));
+  ECS_HIP(hipMemcpyAsync(c->d_vel2, vel, sizeof(double) * 2 * np, kind, c->stream));
+  c->np = np;
+  c->coll_lambda = -1.0;
+  ECS_HIP(hipStreamSynchronize(c->stream));
+}
+
+void p2d_get(ecsprt_hip_ctx *c, double *pos, double *vel) {
+  ECS_HIP(hipMemcpyAsync(pos, c->d_pos2, sizeof(double) * 2 * c->np, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipMemcpyAsync(vel, c->d_vel2, sizeof(double) * 2 * c->np, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipStreamSynchronize(c->stream));
+}
+
+void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k) {
+  p2d_reserve(c, np);
+  const ecsprt_params &p = c->p;
+  k_init2d<<<grid2(np), 256, 0, c->stream>>>(np, p.Lx, p.Ly, p.x0, p.y0, amp, k, p.viTx, p.viTy, p.vi0x, p.vi0y,
+                                             (uint32_t)seed, (uint32_t)(seed >> 32), c->d_pos2, c->d_vel2);
+  ECS_HIP(hipGetLastError());
+  c->np = np;
+  c->coll_lambda = -1.0;
+  ECS_HIP(hipStreamSynchronize(c->stream));
+}
+
+void p2d_push(ecsprt_hip_ctx *c, double dt) {
+  k_push2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->d_phi, c->p.Nx, c->p.Ny, push_consts(c->p, dt), c->d_pos2,
+                                                c->d_vel2);
+  ECS_HIP(hipGetLastError());
+}
+
+// Drift and wrap; a lost ion aborts as a changed ion count does in the reference
+// (src/boundaries.c:101-109).
+void p2d_drift(ecsprt_hip_ctx *c, double dt) {
+  zero_ctr(c);
+  k_drift2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, drift_consts(c->p, dt), c->d_pos2, c->d_vel2, c->d_ctr + 2);
+  ECS_HIP(hipGetLastError());
+  read_ctr(c);
+  ECS_CHECK(c->h_ctr[2] == 0, ECSPRT_ERR_COUNT_CHANGED,
+            std::to_string(c->h_ctr[2]) + " ions left the periodic domain (not finite or moved more than L)");
+}
+
+void p2d_apply_bc(ecsprt_hip_ctx *c, long np, double *pos) {
+  k_apply_bc2d<<<grid2(np), 256, 0, c->stream>>>(np, drift_consts(c->p, 0.0), pos);
+  ECS_HIP(hipGetLastError());
+}
+
+void p2d_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
+  fill_i32(c, d_counts, c->nnode, 0);
+  k_cell_counts2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->p.Nx, c->p.Ny, c->p.dx, c->p.dy, c->d_pos2,
+                                                       d_counts);
+  ECS_HIP(hipGetLastError());
+}
+
+void p2d_deposit(ecsprt_hip_ctx *c) {
+  const long n = c->nnode;
+  int *maxcell = c->d_cnt2 + n;
+  p2d_cell_counts(c, c->d_cnt2);
+  fill_i32(c, maxcell, 1, 0);
+  k_max_i32<<<grid2(n), 256, 0, c->stream>>>(c->d_cnt2, n, maxcell);
+  ECS_HIP(hipGetLastError());
+  fill_f64(c, reinterpret_cast<double *>(c->d_acc2), 3 * n, 0.0);  // all-zero bits
+  fill_f64(c, c->d_ovf, 4 * n, 0.0);
+  const double vmax = mx_vmax(c->p);
+  k_deposit2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->p.Nx, c->p.Ny, push_consts(c->p, 0.0), 1.0 / vmax,
+                                                   maxcell, c->d_pos2, c->d_vel2, c->d_acc2, c->d_ovf);
+  ECS_HIP(hipGetLastError());
+  k_fold2d<<<grid2(n), 256, 0, c->stream>>>(n, c->d_acc2, c->d_ovf, maxcell, vmax, c->d_moments);
+  ECS_HIP(hipGetLastError());
+}
+
+// Philox-mode collisions: the attempt rate is set by dry runs the first time (as the
+// three-dimensional pilot does) and steered towards Nc collisions per step after.
+void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
+  const int Nc = collision_count(&c->p, c->np, dt);
+  c->last_Nc = Nc;
+  c->last_Ns = c->last_Nf = c->last_attempts = 0;
+  if (Nc == 0 || c->p.mn <= 0.0) return;
+  const CollideConsts cc = collide_consts_of(c->p);
+  const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
+  auto pass = [&](double lam, uint32_t step, int write) {
+    zero_ctr(c);
+    k_collide2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, poisson_table(lam), cc, k0, k1, step, write, c->d_vel2,
+                                                     c->d_ctr);
+    ECS_HIP(hipGetLastError());
+    read_ctr(c);
+  };
+  if (c->coll_lambda <= 0.0) {
+    double lam = (double)Nc / (double)c->np;
+    for (int it = 0; it < 4 && lam > 0; ++it) {
+      pass(lam, 0xFFFFFFF0u + it, 0);
+      const double s = (double)c->h_ctr[4];
+      lam *= s > 0 ? std::min(16.0, std::max(0.25, (double)Nc / s)) : 16.0;
+    }
+    c->coll_lambda = lam;
+  }
+  pass(c->coll_lambda, (uint32_t)c->step_index, 1);
+  const long natt = (long)c->h_ctr[3], ns = (long)c->h_ctr[4], nf = (long)c->h_ctr[5];
+  c->last_attempts = natt;
+  c->last_Ns = ns;
+  c->last_Nf = nf;
+  if (ns > 0) c->coll_lambda *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
+  ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
+}
+
+}  // namespace ecsprt

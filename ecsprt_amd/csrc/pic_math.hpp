@@ -175,14 +175,16 @@ __device__ inline void c_sincos2pi(float u, float &s, float &c) {
 // velocity vi.  Returns 0 rejected by the acceptance draw, 1 collided (vf written),
 // 2 refused because |vf|/viT > 10 (collisions.c:141).  T = double follows the
 // reference's arithmetic; T = float is the mixed-precision path's.
-template <typename T>
+// ND = 2 is the two-dimensional run (ndim = 2): every loop covers x and y and vr[2]
+// keeps its initial 0 (collisions.c:32), so cos(theta) = 0.
+template <typename T, int ND = 3>
 __device__ inline int collide_one(const CollideConsts &k, const float g[3], float u_acc1, float u_acc2, float u_z,
                                   float u_phi, const T vi[3], T vf[3]) {
   const T vnT = (T)k.vnT, mi = (T)k.mi, mn = (T)k.mn, M = (T)k.M, vrm = (T)k.vrm, viT = (T)k.viT;
-  T vn[3], vr[3], vc[3], u2[3];
+  T vn[3] = {0, 0, 0}, vr[3] = {0, 0, 0}, vc[3] = {0, 0, 0}, u2[3];
   T tmp = 0;
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < ND; d++) {
     vn[d] = g[d] * vnT + (T)k.vn0[d];
     vr[d] = vi[d] - vn[d];
     tmp += vr[d] * vr[d];
@@ -197,7 +199,7 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   }
   tmp = 0;
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < ND; d++) {
     if constexpr (f64) vc[d] = (vi[d] * mi + vn[d] * mn) / M;
     else vc[d] = (vi[d] * mi + vn[d] * mn) * k.invM_f;
     tmp += (vi[d] - vc[d]) * (vi[d] - vc[d]);
@@ -233,9 +235,9 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   u2[1] = u1x * sinphi + u1y * cosphi;
   u2[2] = u1z;
   tmp = 0;
-  T w[3];
+  T w[3] = {0, 0, 0};
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
+  for (int d = 0; d < ND; d++) {
     w[d] = vc[d] + vcr * u2[d];
     tmp += w[d] * w[d];
   }
@@ -245,9 +247,8 @@ __device__ inline int collide_one(const CollideConsts &k, const float g[3], floa
   } else {
     if (vfr > k.vf_max_f) return 2;  // |vf| / viT > 10
   }
-  vf[0] = w[0];
-  vf[1] = w[1];
-  vf[2] = w[2];
+#pragma unroll
+  for (int d = 0; d < ND; d++) vf[d] = w[d];
   return 1;
 }
 
@@ -281,8 +282,8 @@ __device__ inline int poisson_attempts(const float *cdf, uint32_t key0, uint32_t
   return k;
 }
 
-// Attempts first..k-1 of ion pid at this step, in order.
-template <typename T>
+// Attempts first..k-1 of ion pid at this step, in order (ND = 2: vz is not touched).
+template <typename T, int ND = 3>
 __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, uint32_t key1, uint32_t step, int pid,
                                         int first, int k, T &vx, T &vy, T &vz, unsigned &natt, unsigned &nsucc,
                                         unsigned &nfail) {
@@ -294,15 +295,15 @@ __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, 
     float g[3], spare;
     box_muller(r1.v[0], r1.v[1], g[0], g[1]);
     box_muller(r1.v[2], r1.v[3], g[2], spare);
-    T vi[3] = {vx, vy, vz}, vf[3];
-    int res = collide_one<T>(cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
+    T vi[3] = {vx, vy, ND == 3 ? vz : T(0)}, vf[3];
+    int res = collide_one<T, ND>(cc, g, u01(r2.v[0]), u01(r2.v[1]), u01(r2.v[2]), u01(r2.v[3]), vi, vf);
     natt += 1u;
     nsucc += res == 1 ? 1u : 0u;
     nfail += res == 2 ? 1u : 0u;
     if (res == 1) {
       vx = vf[0];
       vy = vf[1];
-      vz = vf[2];
+      if (ND == 3) vz = vf[2];
     }
   }
 }

@@ -9,6 +9,8 @@
 //   sinusoidal  rejection against 1 + 0.25 sin(2 pi x/Lx) over PETSc's default
 //               PetscRandom, the rand48 generator seeded with 7 (Rejection,
 //               src/positions.c:236-388).
+// Each ion takes ndim coordinates and ndim velocity components (two in a
+// two-dimensional run), in the streams' order.
 // Velocities are vT * gasdev + v0 per component (NormalVelocities,
 // src/velocities.c:14-47), seed -12345.  The generators are Numerical Recipes (2nd
 // edition) ran3, gasdev over ran1 and sobseq, with the reference's return types
@@ -187,20 +189,23 @@ void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist) {
             "uniform is PETSc's DMSwarmSetPointsUniformCoordinates lattice");
   ECS_CHECK(pdist != ECSPRT_PDIST_SOBOL || np < (1L << Sobol::MAXBIT), ECSPRT_ERR_ARG,
             "Sobol positions: at most 2^30 - 1 ions (MAXBIT)");
-  std::vector<double> pos(3 * (size_t)np), vel(3 * (size_t)np);
+  const int nd = p.ndim == 2 ? 2 : 3;  // components per ion, as the reference's ndim
+  std::vector<double> pos(nd * (size_t)np), vel(nd * (size_t)np);
   const double L[3] = {p.Lx, p.Ly, p.Lz};
   const double vT[3] = {p.viTx, p.viTy, p.viTz}, v0[3] = {p.vi0x, p.vi0y, p.vi0z};
   if (pdist == ECSPRT_PDIST_NORMAL) {
     Ran3 r3;
     long seed = -12345;
     for (long ip = 0; ip < np; ++ip)
-      for (int d = 0; d < 3; ++d) pos[3 * ip + d] = (double)r3.next(&seed) * L[d];
+      for (int d = 0; d < nd; ++d) pos[nd * ip + d] = (double)r3.next(&seed) * L[d];
   } else if (pdist == ECSPRT_PDIST_SOBOL) {
+    // Sobseq(&ndim, ...) advances dimensions 1..ndim only; each dimension is independent,
+    // so the first two of a three-dimensional draw are the two-dimensional point.
     Sobol sb;
     double s3[3];
     for (long ip = 0; ip < np; ++ip) {
       sb.next3(s3);
-      for (int d = 0; d < 3; ++d) pos[3 * ip + d] = s3[d] * L[d];
+      for (int d = 0; d < nd; ++d) pos[nd * ip + d] = s3[d] * L[d];
     }
   } else {
     // The maximum over the grid nodes (i/Nx, j/Ny, k/Nz); the density depends on x alone.
@@ -212,10 +217,10 @@ void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist) {
     const double normVal = 1.0 / maxVal;
     Rand48 r(7);
     for (long ip = 0; ip < np;) {
-      for (int d = 0; d < 3; ++d) s3[d] = r.next();
+      for (int d = 0; d < nd; ++d) s3[d] = r.next();
       const double w = sinusoidal_density(s3);
       if (w * normVal > r.next()) {
-        for (int d = 0; d < 3; ++d) pos[3 * ip + d] = s3[d] * L[d];
+        for (int d = 0; d < nd; ++d) pos[nd * ip + d] = s3[d] * L[d];
         ++ip;
       }
     }
@@ -224,7 +229,11 @@ void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist) {
     Gasdev g;
     long seed = -12345;
     for (long ip = 0; ip < np; ++ip)
-      for (int d = 0; d < 3; ++d) vel[3 * ip + d] = vT[d] * (double)g.next(&seed) + v0[d];
+      for (int d = 0; d < nd; ++d) vel[nd * ip + d] = vT[d] * (double)g.next(&seed) + v0[d];
+  }
+  if (nd == 2) {
+    p2d_set(c, np, pos.data(), vel.data(), hipMemcpyHostToDevice);
+    return;
   }
   ensure_stage(c, np);
   ECS_HIP(hipMemcpyAsync(c->d_stage, pos.data(), sizeof(double) * 3 * np, hipMemcpyHostToDevice, c->stream));

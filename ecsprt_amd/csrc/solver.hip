@@ -116,6 +116,8 @@ __global__ void k_residual(StencilConsts s, const T *__restrict__ dn, const T *_
 
 // Periodic index for a within one period of [0, n).
 __host__ __device__ inline int wrap1(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+// z index of a halo plane: one plane (the two-dimensional run) is its own periodic image.
+__host__ __device__ inline int wrapz(int a, int n) { return n == 1 ? 0 : wrap1(a, n); }
 
 // 1/x from the hardware reciprocal refined by Newton steps (two in FP64, one in
 // FP32): within an ulp of the quotient, for the smoother's diagonal scaling.
@@ -164,7 +166,7 @@ __global__ void __launch_bounds__(512) k_sweep(StencilConsts s, const T *__restr
   for (int q = threadIdx.x; q < H3; q += blockDim.x) {
     const int li = q % HX - 2, lj = (q / HX) % HY - 2, lk = q / (HX * HY) - 2;
     if (li > ex + 1 || lj > ey + 1 || lk > ez + 1) continue;
-    const long g = ((long)wrap1(k0 + lk, Nz) * Ny + wrap1(j0 + lj, Ny)) * Nx + wrap1(i0 + li, Nx);
+    const long g = ((long)wrapz(k0 + lk, Nz) * Ny + wrap1(j0 + lj, Ny)) * Nx + wrap1(i0 + li, Nx);
     sd[q] = dn[g];
     sx[q] = xin ? xin[g] : T(0);
   }
@@ -172,7 +174,7 @@ __global__ void __launch_bounds__(512) k_sweep(StencilConsts s, const T *__restr
   for (int q = threadIdx.x; q < R3; q += blockDim.x) {
     const int li = q % RX - 1, lj = (q / RX) % RY - 1, lk = q / (RX * RY) - 1;
     if (li > ex || lj > ey || lk > ez) continue;
-    const int gi = wrap1(i0 + li, Nx), gj = wrap1(j0 + lj, Ny), gk = wrap1(k0 + lk, Nz);
+    const int gi = wrap1(i0 + li, Nx), gj = wrap1(j0 + lj, Ny), gk = wrapz(k0 + lk, Nz);
     const int p = ((lk + 2) * HY + (lj + 2)) * HX + (li + 2);
     T v = sx[p];
     if (((gi + gj + gk) & 1) == first)
@@ -247,7 +249,7 @@ __global__ void __launch_bounds__(TX * TY) k_sweep_z(StencilConsts s, const T *_
   }
   const int Lbeg = k0 - 2, Lend = k0 + ez + 1;
   auto fetch = [&](int L, T *rx, T *rd) {
-    const long zo = (long)wrap1(L, Nz) * plane;
+    const long zo = (long)wrapz(L, Nz) * plane;
 #pragma unroll
     for (int u = 0; u < LPT; ++u)
       if (lq[u] >= 0) {
@@ -275,7 +277,7 @@ __global__ void __launch_bounds__(TX * TY) k_sweep_z(StencilConsts s, const T *_
       const T *x0 = sxo + ((r1 - 1) % NXO) * HP, *x1 = sxo + (r1 % NXO) * HP, *x2 = sxo + ((r1 + 1) % NXO) * HP;
       const T *d0 = sd + ((r1 - 1) % ND) * HP, *d1 = sd + (r1 % ND) * HP, *d2 = sd + ((r1 + 1) % ND) * HP;
       T *sn = ssn + (r1 % NSN) * RP;
-      const int gk = wrap1(L - 1, Nz);
+      const int gk = wrapz(L - 1, Nz);
       for (int q = tid; q < RP; q += NT) {
         const int li = q % RX - 1, lj = q / RX - 1;
         if (li > ex || lj > ey) continue;
@@ -780,7 +782,7 @@ static StencilConsts stencil_consts(const ecsprt_params &p, int Nx, int Ny, int 
   const double hod = 1.0 / (8.0 * dy * dx);
   s.sxx = p.scale * hxx;
   s.syy = p.scale * hyy;
-  s.szz = p.scale * (1 + kappa * kappa) * hzz;
+  s.szz = p.ndim == 2 ? 0.0 : p.scale * (1 + kappa * kappa) * hzz;  // ComputeFullLHS2D: no z coupling
   s.sod = p.scale * kappa * hod;
   return s;
 }
@@ -1346,6 +1348,7 @@ void solver_rhs(ecsprt_hip_ctx *c, double *b) {
   k.hxx = 1.0 / (p.dx * p.dx);
   k.hyy = 1.0 / (p.dy * p.dy);
   k.hzz = 1.0 / (p.dz * p.dz);
+  if (p.ndim == 2) k.hz = k.hzz = 0.0;  // ComputeFullRHS2D (src/rhs.c:151-256) has no z terms
   const Launch3 g3 = launch3(p.Nx, p.Ny, p.Nz);
   k_rhs<<<g3.grid, g3.block, 0, c->stream>>>(k, c->d_moments, b);
   ECS_HIP(hipGetLastError());

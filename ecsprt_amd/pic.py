@@ -97,7 +97,9 @@ class Context:
         self.options = options if options is not None else default_options()
         self.ptr = ctypes.c_void_p()
         _check(self.L.ecsprt_hip_create(ctypes.byref(params), ctypes.byref(self.options), ctypes.byref(self.ptr)))
-        self.shape = (params.Nz, params.Ny, params.Nx)
+        self.ndim = params.ndim if params.ndim in (2, 3) else 3
+        # grid arrays: [Nz][Ny][Nx] in 3-D, [Ny][Nx] in 2-D (Nz = 1)
+        self.shape = (params.Nz, params.Ny, params.Nx) if self.ndim == 3 else (params.Ny, params.Nx)
 
     def close(self):
         if self.ptr:
@@ -117,12 +119,13 @@ class Context:
     def set_particles(self, pos, vel):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         vel = np.ascontiguousarray(vel, dtype=np.float64)
+        assert pos.shape[1:] == (self.ndim,) and vel.shape == pos.shape
         self._c(self.L.ecsprt_hip_set_particles(self.ptr, pos.shape[0], _dp(pos), _dp(vel)))
 
     def get_particles(self):
         n = self.num_particles
-        pos = np.empty((n, 3))
-        vel = np.empty((n, 3))
+        pos = np.empty((n, self.ndim))
+        vel = np.empty((n, self.ndim))
         self._c(self.L.ecsprt_hip_get_particles(self.ptr, _dp(pos), _dp(vel)))
         return pos, vel
 
@@ -157,16 +160,16 @@ class Context:
         self._c(self.L.ecsprt_hip_push(dt, self.ptr))
 
     def collide(self, dt):
-        self._c(self.L.ecsprt_hip_collide(3, dt, self.ptr))
+        self._c(self.L.ecsprt_hip_collide(self.ndim, dt, self.ptr))
 
     def collect(self):
         self._c(self.L.ecsprt_hip_collect(self.ptr))
 
     def update_velocities(self, dt):
-        self._c(self.L.ecsprt_hip_update_velocities(3, dt, self.ptr))
+        self._c(self.L.ecsprt_hip_update_velocities(self.ndim, dt, self.ptr))
 
     def update_positions(self, dt):
-        self._c(self.L.ecsprt_hip_update_positions(3, dt, self.ptr))
+        self._c(self.L.ecsprt_hip_update_positions(self.ndim, dt, self.ptr))
 
     def compute_potential(self):
         self._c(self.L.ecsprt_hip_compute_potential(self.ptr))

@@ -36,7 +36,6 @@ extern "C" {
 #define ECSPRT_ERR_NOMEM          7 /* device allocation failed */
 #define ECSPRT_ERR_STATE          8 /* call made before the state it needs was set */
 
-/* Command-line parameters of the hot path (subset of CLI, src/ecsprt.h:137-191). */
 /* Initial position distributions, numbered as PDistType (src/positions.c:8-10). */
 #define ECSPRT_PDIST_SOBOL      0
 #define ECSPRT_PDIST_REVERSE    1 /* not implemented by the reference (SETERRQ) */
@@ -45,6 +44,7 @@ extern "C" {
 #define ECSPRT_PDIST_SINUSOIDAL 4
 #define ECSPRT_PDIST_GAUSSIAN   5 /* not implemented by the reference (SETERRQ) */
 
+/* Command-line parameters of the hot path (subset of CLI, src/ecsprt.h:137-191). */
 typedef struct {
   int    Nx, Ny, Nz;
   double dx, dy, dz;
@@ -57,6 +57,7 @@ typedef struct {
   double nue, Te;
   double veTx, veTy, veTz;
   double mn, vn0x, vn0y, vn0z, Tn;
+  int    ndim;  /* 3 (0 also means 3) or 2: -ndim, src/setup.c:31-112 (in 2-D, Nz = 1) */
 } ecsprt_cli;
 
 /* Derived context scalars (Context, src/ecsprt.h:118-134, as SetUpContext fills them). */
@@ -69,6 +70,7 @@ typedef struct {
   double mn, vn0x, vn0y, vn0z, vn0, vnT, Tn;
   double scale;
   double gammaT;
+  int    ndim;  /* 2 or 3 */
 } ecsprt_params;
 
 /* One collision attempt's random numbers in the reference's draw order

@@ -113,6 +113,7 @@ def lib():
             "oracle_rand48_next": (ctypes.c_double, [ctypes.POINTER(Rand48)]),
             "oracle_sobol_positions": (ctypes.c_int, [P, ctypes.c_long, _D]),
             "oracle_sinusoidal_positions": (ctypes.c_long, [P, ctypes.c_long, _D]),
+            "oracle_sinusoidal_positions_nd": (ctypes.c_long, [P, ctypes.c_int, ctypes.c_long, _D]),
             "oracle_collide_one_nd": (None, [P, ctypes.c_int, ctypes.POINTER(OracleAttempt), _D, _D, _I, _I]),
             "oracle_scatter_elastic_nr_nd": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_double, ctypes.c_long, _D,
                                                             ctypes.POINTER(NRState), ctypes.c_long,
@@ -355,11 +356,11 @@ def sobol_positions(p, np_):
     return pos
 
 
-def sinusoidal_positions(p, np_):
+def sinusoidal_positions(p, np_, ndim=3):
     """Rejection of SinusoidalDistribution on one rank (src/positions.c:236-388):
-    (pos[np, 3], trials)."""
-    pos = np.empty((np_, 3))
-    trials = lib().oracle_sinusoidal_positions(ctypes.byref(p), np_, _dp(pos))
+    (pos[np, ndim], trials)."""
+    pos = np.empty((np_, ndim))
+    trials = lib().oracle_sinusoidal_positions_nd(ctypes.byref(p), ndim, np_, _dp(pos))
     return pos, trials
 
 

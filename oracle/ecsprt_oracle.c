@@ -495,7 +495,7 @@ static double sinusoidal_density(const double r[3]) { return 1.0 + 0.25 * sin(2 
 /* positions.c:253-388 (Rejection), one rank: the maximum over grid nodes (i/Nx, j/Ny,
  * k/Nz) (the density depends on x alone), then per trial three values for the
  * position and one for the test, accepted when w/max > v.  Returns the trials. */
-long oracle_sinusoidal_positions(const oracle_params *p, long np, double *pos)
+long oracle_sinusoidal_positions_nd(const oracle_params *p, int ndim, long np, double *pos)
 {
   oracle_rand48 r;
   const double L[3] = {p->Lx, p->Ly, p->Lz};
@@ -511,16 +511,21 @@ long oracle_sinusoidal_positions(const oracle_params *p, long np, double *pos)
   }
   normVal = 1.0 / maxVal;
   while (ip < np) {
-    for (d = 0; d < 3; d++) sv[d] = oracle_rand48_next(&r);
+    for (d = 0; d < ndim; d++) sv[d] = oracle_rand48_next(&r);
     w = sinusoidal_density(sv);
     v = oracle_rand48_next(&r);
     if (w * normVal > v) {
-      for (d = 0; d < 3; d++) pos[ip * 3 + d] = sv[d] * L[d];
+      for (d = 0; d < ndim; d++) pos[ip * ndim + d] = sv[d] * L[d];
       ip++;
     }
     it++;
   }
   return it;
+}
+
+long oracle_sinusoidal_positions(const oracle_params *p, long np, double *pos)
+{
+  return oracle_sinusoidal_positions_nd(p, 3, np, pos);
 }
 
 /* random.c:24-72 (ran1). */

@@ -178,6 +178,7 @@ double oracle_rand48_next(oracle_rand48 *r);
  * SinusoidalDistribution), one rank: positions of np ions, pos[3*np]. */
 int  oracle_sobol_positions(const oracle_params *p, long np, double *pos);
 long oracle_sinusoidal_positions(const oracle_params *p, long np, double *pos);
+long oracle_sinusoidal_positions_nd(const oracle_params *p, int ndim, long np, double *pos);
 
 /* CPU baseline: Nt steps of push + collide + drift/wrap + deposit in the
  * reference loop order (src/pic.c:246-254), per-stage seconds returned in t[4]. */

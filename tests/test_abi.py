@@ -57,8 +57,24 @@ def test_philox2x32_known_answers():
 def test_setup_context_matches_oracle(wl):
     a = pic.setup_context(**wl.cli)
     b = oracle.setup_context(**wl.cli)
+    assert a.ndim == 3
     for name, _ in a._fields_:
-        assert getattr(a, name) == getattr(b, name), name
+        if name != "ndim":
+            assert getattr(a, name) == getattr(b, name), name
+
+
+def test_setup_context_two_dimensional():
+    """-ndim 2 (src/setup.c:178-188): Nz = 1 and dz from a unit z extent, every other
+    parameter as in three dimensions; other ndim values are refused (src/setup.c:41)."""
+    cli = dict(configs.small(12).cli, Nz=9, dz=0.3)
+    a = pic.setup_context(**dict(cli, ndim=2))
+    b = oracle.setup_context(**dict(cli, Nz=1, dz=1.0))
+    assert a.ndim == 2 and a.Nz == 1 and a.dz == 1.0
+    for name, _ in a._fields_:
+        if name not in ("ndim", "z0", "z1", "Lz"):
+            assert getattr(a, name) == getattr(b, name), name
+    with pytest.raises(pic.HipError):
+        pic.setup_context(**dict(cli, ndim=4))
 
 
 @pytest.mark.parametrize("Tn_over_Ti,nui,np_", [(1.0, 0.02, 10 ** 6), (0.01, 0.01, 2097152), (1e-30, 0.01, 10 ** 6),
