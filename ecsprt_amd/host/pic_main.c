@@ -7,7 +7,8 @@
  * the time loop reads like src/pic.c:238-269.  Options use the reference spellings:
  *   -Nx -Ny -Nz -dx -dy -dz -Np (negative = per cell) -Nt -dt -B0 -E0 -qi -mi -nui
  *   -nue -Te -viT -mn -Tn -vn0 ; plus -rtol, -fused (one-pass timestep), -seed,
- *   -mixed (mixed-precision particles and solve).
+ *   -mixed (mixed-precision particles and solve), -ndim 2 (a two-dimensional run:
+ *   BorisMoverBz2D, Collect2DFluidMoments, ComputeFullLHS2D/RHS2D; Nz = 1).
  * Initial conditions come from the device generator (uniform positions, Maxwellian
  * velocities), the analogue of InitializePositions/InitializeVelocities; with
  * --position-dist normal|sobol|sinusoidal they are the reference's own
@@ -113,6 +114,10 @@ int main(int argc, char **argv) {
   cli.nue = fabs(Q * cli.B0 / ME) / kappa_e;
   if (!(cli.nue > 0)) cli.nue = 1.0;
   get_real(argc, argv, "-nue", &cli.nue);
+  double ndim_arg = 3;
+  get_real(argc, argv, "-ndim", &ndim_arg);
+  const int ndim = (int)ndim_arg;
+  cli.ndim = ndim;
   const int fused = has_flag(argc, argv, "-fused");
   const int mixed = has_flag(argc, argv, "-mixed");
   const char *pdist = get_str(argc, argv, "--position-dist");
@@ -166,8 +171,8 @@ int main(int argc, char **argv) {
       for (int k = 0; k < 6; ++k) acc[k] += ms[k];
     } else {
       CHECK(ions.push(dt, ctx)); /* UpdateVelocities, src/pic.c:246 */
-      if (p.mn > 0.0 && o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) CHECK(ions.collide(3, dt, ctx));
-      CHECK(ecsprt_hip_update_positions(3, dt, ctx)); /* src/pic.c:250 */
+      if (p.mn > 0.0 && o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) CHECK(ions.collide(ndim, dt, ctx));
+      CHECK(ecsprt_hip_update_positions(ndim, dt, ctx)); /* src/pic.c:250 */
       CHECK(ions.collect(ctx));                       /* src/pic.c:254 */
       CHECK(ecsprt_hip_compute_potential(ctx));       /* src/pic.c:258 */
     }

@@ -19,7 +19,8 @@ def run(*args):
 
 @pytest.mark.parametrize("extra", [[], ["-fused"], ["-fused", "-mixed"], ["--position-dist", "normal"],
                                    ["-fused", "--position-dist", "sobol"],
-                                   ["-fused", "-mixed", "--position-dist", "sinusoidal"]])
+                                   ["-fused", "-mixed", "--position-dist", "sinusoidal"],
+                                   ["-ndim", "2"], ["-ndim", "2", "-fused", "--position-dist", "sobol"]])
 def test_driver_steps(extra):
     assert os.path.exists(build.DRIVER), "driver not built"
     r = run("-Nx", "16", "-Ny", "16", "-Nz", "16", "-Np", "-8", "-Nt", "3", "-rtol", "1e-8", *extra)
