@@ -28,9 +28,14 @@ namespace {
 
 // DifferenceVector2D (src/vector-math.c:529-592) on the unghosted periodic potential:
 // plain bilinear weights, one-sided differences at an upper node index N.
+// Periodic index of base + d for a base already in [0, n) and |d| <= 2 < n.
+__device__ inline int near_wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
 __device__ inline void diff_vec2d(const double *__restrict__ F, int Nx, int Ny, double x0, double y0, double f[2]) {
-  auto P = [&](int j, int i) { return F[(long)wrap_index(j, Ny) * Nx + wrap_index(i, Nx)]; };
   const int ixl = (int)x0, ixh = ixl + 1, iyl = (int)y0, iyh = iyl + 1;
+  // Every node read is (ixl + a, iyl + b) with a, b in [-1, 2]: one modulo per axis.
+  const int bx = wrap_index(ixl, Nx), by = wrap_index(iyl, Ny);
+  auto P = [&](int j, int i) { return F[(long)near_wrap(by + (j - iyl), Ny) * Nx + near_wrap(bx + (i - ixl), Nx)]; };
   const double wxh = x0 - (double)ixl, wyh = y0 - (double)iyl;
   double hh, lh, hl, ll;
   if (ixl >= 0) {
@@ -153,17 +158,19 @@ __global__ void k_deposit2d(long np, int Nx, int Ny, PushConsts k, double inv_vm
   for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
     const double x = cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2);
     const double y = cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2);
-    const int ixl = (int)x, ixh = ixl + 1, iyl = (int)y, iyh = iyl + 1;
+    const int ixl = (int)x, iyl = (int)y;
     const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
     const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
     const double w4[4] = {wyh * wxh, wyl * wxh, wyh * wxl, wyl * wxl};
-    const int ci[4] = {ixh, ixh, ixl, ixl}, cj[4] = {iyh, iyl, iyh, iyl};
+    const int bx = wrap_index(ixl, Nx), by = wrap_index(iyl, Ny);
+    const int bx1 = near_wrap(bx + 1, Nx), by1 = near_wrap(by + 1, Ny);
+    const int ci[4] = {bx1, bx1, bx, bx}, cj[4] = {by1, by, by1, by};
     const double vx = vel[2 * p], vy = vel[2 * p + 1];
     const double ux = vx * inv_vmax, uy = vy * inv_vmax;
     const bool slow = fabs(ux) < 1.0 && fabs(uy) < 1.0;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const long node = (long)wrap_index(cj[c], Ny) * Nx + wrap_index(ci[c], Nx);
+      const long node = (long)cj[c] * Nx + ci[c];
       if (slow) {
         atomicAdd(&acc[node * 3 + 0], (unsigned long long)fx_round(w4[c] * sw));
         atomicAdd(&acc[node * 3 + 1], (unsigned long long)fx_round(ux * w4[c] * sw));

@@ -18,7 +18,7 @@ import os
 
 
 def short(name, width=72):
-    n = name.split("(")[0].replace("void ", "").replace("ecsprt::", "")
+    n = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("ecsprt::", "")
     return n if len(n) <= width else n[: width - 3] + "..."
 
 

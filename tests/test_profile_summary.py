@@ -40,3 +40,8 @@ def test_pmc_means(tmp_path):
     ps.pmc(str(tmp_path), str(out), "cmd", drop=False)
     line = [l for l in out.read_text().splitlines() if l.startswith("k_tile_mx")][0]
     assert line.split()[-2:] == ["2", "20.0"]
+
+
+def test_short_names_drop_anonymous_namespaces():
+    assert ps.short("void ecsprt::(anonymous namespace)::k_push2d(long, double const*)") == "k_push2d"
+    assert ps.short("void ecsprt::k_tile<31, 8, 8, 4, 2>(ecsprt::TileArgs)") == "k_tile<31, 8, 8, 4, 2>"
