@@ -168,6 +168,10 @@ struct ecsprt_hip_ctx {
   long cap2 = 0;
   unsigned long long *d_acc2 = nullptr;
   int *d_cnt2 = nullptr;
+  int *d_perm2 = nullptr, *d_tiles2 = nullptr;  // ion indices binned by tile; tile counts and offsets
+  long perm_cap2 = 0;
+  int *d_seg2 = nullptr;  // colliding ions per list segment
+  long seg_cap2 = 0;
 
   // staging (host<->device AoS)
   double *d_stage = nullptr;

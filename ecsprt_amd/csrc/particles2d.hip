@@ -9,8 +9,8 @@
 // the potential and the moments stay in the L2 cache, so the ions are streamed once per
 // stage without binning.  The deposit is exact and order-independent: every weight
 // (and every flux over vmax) is rounded once to a fixed-point grid of 2^-fx and added
-// with 64-bit integer atomics, fx chosen from the fullest cell so that no node sum can
-// overflow; ions with a velocity component at or above vmax take FP64 atomics.
+// with 64-bit integer atomics, fx chosen from the fullest tile (or cell) so that no node
+// sum can overflow; ions with a velocity component at or above vmax take FP64 atomics.
 #include <algorithm>
 #include <cmath>
 
@@ -142,7 +142,8 @@ __global__ void k_max_i32(const int *__restrict__ a, long n, int *out) {
   if (__lane_id() == 0) atomicMax(out, m);
 }
 
-// Fixed-point bits for node sums bounded by 4 * maxcell (weights <= 1, |flux/vmax| < 1).
+// Fixed-point bits for node sums bounded by 4 * maxcell (weights <= 1, |flux/vmax| < 1);
+// maxcell is the fullest cell, or on the tiled path the fullest tile.
 __device__ inline int fx2d_bits(int maxcell) {
   int b = 0;
   while (b < 62 && (1ll << b) < 4ll * (long long)maxcell + 1) ++b;
@@ -196,17 +197,221 @@ __global__ void k_fold2d(long n, const unsigned long long *__restrict__ acc, con
   }
 }
 
+// ---- binned deposit --------------------------------------------------------------
+// Ion indices are binned by tile of T2 x T2 cells (a counting sort of indices: the ions
+// themselves stay in place, in the reference's order), then one workgroup per tile sums
+// its ions into fixed-point LDS accumulators over the tile's (T2+1)^2 nodes and adds the
+// region to the global sums: 12 LDS atomics per ion instead of 12 global ones.
+constexpr int T2 = 32;
+constexpr int MAX_TILES2 = 4096;
+constexpr int BIN_THREADS2 = 256;
+
+struct Bin2 {
+  int Nx, Ny, ntx, nty;
+};
+
+// The ion's wrapped cell (ix, iy) from its cell coordinates.
+__device__ inline void cell_of(const PushConsts &k, const Bin2 &b, const double *__restrict__ pos, long p, int &ix,
+                               int &iy) {
+  ix = wrap_index((int)cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2), b.Nx);
+  iy = wrap_index((int)cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2), b.Ny);
+}
+
+// Per-tile counts (LDS histogram per workgroup, added to tile_counts once per non-empty
+// bin).  Workgroup w takes ions [w*chunk, (w+1)*chunk).
+__global__ void __launch_bounds__(BIN_THREADS2) k_bin_hist2d(long np, long chunk, Bin2 b, PushConsts k,
+                                                            const double *__restrict__ pos,
+                                                            int *__restrict__ tile_counts) {
+  __shared__ int h[MAX_TILES2];
+  const int nt = b.ntx * b.nty;
+  for (int q = threadIdx.x; q < nt; q += blockDim.x) h[q] = 0;
+  __syncthreads();
+  const long c0 = blockIdx.x * chunk, c1 = min(np, c0 + chunk);
+  for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
+    int ix, iy;
+    cell_of(k, b, pos, p, ix, iy);
+    atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nt; q += blockDim.x)
+    if (h[q]) atomicAdd(&tile_counts[q], h[q]);
+}
+
+// Exclusive scan of the tile counts into off[] and fill[] (one workgroup), and the
+// largest tile count into *maxc: a node lies in at most four tile regions, so its sums
+// are bounded by four times it (the fixed-point scale is chosen from that bound).
+__global__ void __launch_bounds__(BIN_THREADS2) k_scan_tiles2d(int nt, const int *__restrict__ counts,
+                                                              int *__restrict__ off, int *__restrict__ fill,
+                                                              int *__restrict__ maxc) {
+  __shared__ long long part[BIN_THREADS2];
+  __shared__ int pmax[BIN_THREADS2];
+  const int per = (nt + BIN_THREADS2 - 1) / BIN_THREADS2;
+  const int q0 = threadIdx.x * per, q1 = min(nt, q0 + per);
+  long long s = 0;
+  int mx = 0;
+  for (int q = q0; q < q1; ++q) {
+    s += counts[q];
+    mx = max(mx, counts[q]);
+  }
+  part[threadIdx.x] = s;
+  pmax[threadIdx.x] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long a = 0;
+    int m = 0;
+    for (int t = 0; t < BIN_THREADS2; ++t) {
+      const long long v = part[t];
+      part[t] = a;
+      a += v;
+      m = max(m, pmax[t]);
+    }
+    *maxc = m;
+  }
+  __syncthreads();
+  long long a = part[threadIdx.x];
+  for (int q = q0; q < q1; ++q) {
+    off[q] = (int)a;
+    fill[q] = (int)a;
+    a += counts[q];
+  }
+}
+
+// Scatter of ion indices to their tile's range of perm: each workgroup counts its
+// chunk per tile in LDS, reserves one range per non-empty tile, then places its ions.
+__global__ void __launch_bounds__(BIN_THREADS2) k_bin_scatter2d(long np, long chunk, Bin2 b, PushConsts k,
+                                                               const double *__restrict__ pos,
+                                                               int *__restrict__ fill, int *__restrict__ perm) {
+  __shared__ int h[MAX_TILES2];
+  const int nt = b.ntx * b.nty;
+  for (int q = threadIdx.x; q < nt; q += blockDim.x) h[q] = 0;
+  __syncthreads();
+  const long c0 = blockIdx.x * chunk, c1 = min(np, c0 + chunk);
+  for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
+    int ix, iy;
+    cell_of(k, b, pos, p, ix, iy);
+    atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+    const int n = h[q];
+    h[q] = n ? atomicAdd(&fill[q], n) : 0;  // the workgroup's first slot in tile q
+  }
+  __syncthreads();
+  for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
+    int ix, iy;
+    cell_of(k, b, pos, p, ix, iy);
+    perm[atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1)] = (int)p;
+  }
+}
+
+// One workgroup per tile: fixed-point LDS sums of n, Gx/vmax, Gy/vmax over the tile's
+// (T2+1)^2 nodes from its listed ions, then added (wrapped) into the global sums.
+__global__ void __launch_bounds__(BIN_THREADS2) k_deposit2d_tiles(Bin2 b, PushConsts k, double inv_vmax,
+                                                                 const int *__restrict__ maxcell,
+                                                                 const int *__restrict__ off,
+                                                                 const int *__restrict__ counts,
+                                                                 const int *__restrict__ perm,
+                                                                 const double *__restrict__ pos,
+                                                                 const double *__restrict__ vel,
+                                                                 unsigned long long *__restrict__ acc,
+                                                                 double *__restrict__ ovf) {
+  constexpr int R = T2 + 1, R2 = R * R;
+  __shared__ unsigned long long s[3 * R2];
+  for (int q = threadIdx.x; q < 3 * R2; q += blockDim.x) s[q] = 0;
+  __syncthreads();
+  const int t = blockIdx.x;
+  const int i0 = (t % b.ntx) * T2, j0 = (t / b.ntx) * T2;
+  const double sw = ldexp(1.0, fx2d_bits(*maxcell));
+  const int beg = off[t], n = counts[t];
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const long p = perm[beg + q];
+    const double x = cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2);
+    const double y = cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2);
+    const int ixl = (int)x, iyl = (int)y;
+    const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
+    const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
+    const double w4[4] = {wyh * wxh, wyl * wxh, wyh * wxl, wyl * wxl};
+    const double vx = vel[2 * p], vy = vel[2 * p + 1];
+    const double ux = vx * inv_vmax, uy = vy * inv_vmax;
+    const int la = wrap_index(ixl, b.Nx) - i0, lb = wrap_index(iyl, b.Ny) - j0;  // in [0, T2)
+    const int na[4] = {la + 1, la + 1, la, la}, nb[4] = {lb + 1, lb, lb + 1, lb};
+    if (fabs(ux) < 1.0 && fabs(uy) < 1.0) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int r = nb[c] * R + na[c];
+        lds_addfx(reinterpret_cast<long long *>(&s[r]), w4[c] * sw);
+        lds_addfx(reinterpret_cast<long long *>(&s[R2 + r]), ux * w4[c] * sw);
+        lds_addfx(reinterpret_cast<long long *>(&s[2 * R2 + r]), uy * w4[c] * sw);
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const long node = (long)near_wrap(j0 + nb[c], b.Ny) * b.Nx + near_wrap(i0 + na[c], b.Nx);
+        unsafeAtomicAdd(&ovf[node * 4 + 0], w4[c]);
+        unsafeAtomicAdd(&ovf[node * 4 + 1], vx * w4[c]);
+        unsafeAtomicAdd(&ovf[node * 4 + 2], vy * w4[c]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int r = threadIdx.x; r < R2; r += blockDim.x) {
+    const int a = r % R, bb = r / R;
+    if (i0 + a > b.Nx || j0 + bb > b.Ny) continue;  // beyond the grid's last node (partial tile)
+    const unsigned long long v0 = s[r], v1 = s[R2 + r], v2 = s[2 * R2 + r];
+    if (!(v0 | v1 | v2)) continue;
+    const long node = (long)near_wrap(j0 + bb, b.Ny) * b.Nx + near_wrap(i0 + a, b.Nx);
+    atomicAdd(&acc[node * 3 + 0], v0);
+    atomicAdd(&acc[node * 3 + 1], v1);
+    atomicAdd(&acc[node * 3 + 2], v2);
+  }
+}
+
 // Statistical collisions (Philox mode) in two dimensions: a Poisson number of attempts
-// per ion keyed by (ion index, step), each the two-dimensional scattering formula.
-__global__ void k_collide2d(long np, PoissonTable pt, CollideConsts cc, uint32_t key0, uint32_t key1, uint32_t step,
-                            int write, double *__restrict__ vel, unsigned long long *ctr) {
+// per ion keyed by (ion index, step), each the two-dimensional scattering formula.  The
+// draw runs over every ion; workgroup w lists the ions of its chunk that have attempts
+// (a few in a hundred) in its own segment of the list (LDS ranks, no global atomics),
+// and a second kernel runs the attempts of each segment with every lane busy instead
+// of leaving most of each wave idle through the scattering arithmetic.
+constexpr int COLL_CHUNK2 = 4096;  // ions per selecting workgroup (16 per thread)
+
+__global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable pt, uint32_t key0, uint32_t key1,
+                                                          uint32_t step, int *__restrict__ list,
+                                                          int *__restrict__ seg_count) {
+  __shared__ float s_cdf[24];
+  __shared__ int s_n;
+  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  const long c0 = (long)blockIdx.x * COLL_CHUNK2, c1 = min(np, c0 + COLL_CHUNK2);
+  for (long p0 = c0; p0 < c1; p0 += blockDim.x) {  // uniform per wave
+    const long p = p0 + threadIdx.x;
+    const int k = p < c1 ? poisson_attempts(s_cdf, key0, key1, step, (int)p) : 0;
+    const unsigned long long m = __ballot(k > 0);
+    if (!m) continue;
+    const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    int base = 0;
+    if (k > 0 && below == 0) base = atomicAdd(&s_n, (int)__popcll(m));
+    base = __builtin_amdgcn_readlane(base, __ffsll((long long)m) - 1);
+    if (k > 0) list[c0 + base + below] = (int)p;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) seg_count[blockIdx.x] = s_n;
+}
+
+__global__ void __launch_bounds__(256) k_collide2d_apply(const int *__restrict__ list,
+                                                         const int *__restrict__ seg_count, PoissonTable pt,
+                                                         CollideConsts cc, uint32_t key0, uint32_t key1,
+                                                         uint32_t step, int write, double *__restrict__ vel,
+                                                         unsigned long long *ctr) {
   __shared__ float s_cdf[24];
   if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
   __syncthreads();
+  const long c0 = (long)blockIdx.x * COLL_CHUNK2;
+  const int n = seg_count[blockIdx.x];
   unsigned natt = 0, nsucc = 0, nfail = 0;
-  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const long p = list[c0 + q];
     const int k = poisson_attempts(s_cdf, key0, key1, step, (int)p);
-    if (k == 0) continue;
     double vx = vel[2 * p], vy = vel[2 * p + 1], vz = 0.0;
     collide_attempts<double, 2>(cc, key0, key1, step, (int)p, 0, k, vx, vy, vz, natt, nsucc, nfail);
     if (write) {
@@ -265,6 +470,7 @@ int grid2(long n) { return (int)std::max(1L, std::min(65536L, (n + 255) / 256));
 // ---- host side -------------------------------------------------------------------
 
 void p2d_reserve(ecsprt_hip_ctx *c, long np) {
+  ECS_CHECK(np > 0 && np < 2147483647L, ECSPRT_ERR_ARG, "np must be in [1, 2^31-1)");
   if (!c->d_acc2) {
     ECS_HIP(hipMalloc(&c->d_acc2, sizeof(unsigned long long) * 3 * c->nnode));
     ECS_HIP(hipMalloc(&c->d_cnt2, sizeof(int) * (c->nnode + 1)));
@@ -290,6 +496,12 @@ void p2d_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_vel2);
   (void)hipFree(c->d_acc2);
   (void)hipFree(c->d_cnt2);
+  (void)hipFree(c->d_perm2);
+  (void)hipFree(c->d_tiles2);
+  (void)hipFree(c->d_seg2);
+  c->d_perm2 = c->d_tiles2 = c->d_seg2 = nullptr;
+  c->seg_cap2 = 0;
+  c->perm_cap2 = 0;
   c->d_pos2 = c->d_vel2 = nullptr;
   c->d_acc2 = nullptr;
   c->d_cnt2 = nullptr;
@@ -352,17 +564,45 @@ void p2d_cell_counts(ecsprt_hip_ctx *c, int *d_counts) {
 
 void p2d_deposit(ecsprt_hip_ctx *c) {
   const long n = c->nnode;
+  const int Nx = c->p.Nx, Ny = c->p.Ny;
+  const Bin2 b{Nx, Ny, (Nx + T2 - 1) / T2, (Ny + T2 - 1) / T2};
+  const int nt = b.ntx * b.nty;
   int *maxcell = c->d_cnt2 + n;
-  p2d_cell_counts(c, c->d_cnt2);
-  fill_i32(c, maxcell, 1, 0);
-  k_max_i32<<<grid2(n), 256, 0, c->stream>>>(c->d_cnt2, n, maxcell);
-  ECS_HIP(hipGetLastError());
+  const PushConsts k = push_consts(c->p, 0.0);
+  const double vmax = mx_vmax(c->p);
   fill_f64(c, reinterpret_cast<double *>(c->d_acc2), 3 * n, 0.0);  // all-zero bits
   fill_f64(c, c->d_ovf, 4 * n, 0.0);
-  const double vmax = mx_vmax(c->p);
-  k_deposit2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->p.Nx, c->p.Ny, push_consts(c->p, 0.0), 1.0 / vmax,
-                                                   maxcell, c->d_pos2, c->d_vel2, c->d_acc2, c->d_ovf);
-  ECS_HIP(hipGetLastError());
+  const long np = c->np;
+  if (nt <= MAX_TILES2) {
+    if (!c->d_perm2 || c->perm_cap2 < np) {
+      (void)hipFree(c->d_perm2);
+      c->d_perm2 = nullptr;
+      c->perm_cap2 = 0;
+      if (hipMalloc(&c->d_perm2, sizeof(int) * std::max(np, 1L)) != hipSuccess)
+        throw Error{ECSPRT_ERR_NOMEM, "two-dimensional tile lists"};
+      c->perm_cap2 = np;
+    }
+    if (!c->d_tiles2) ECS_HIP(hipMalloc(&c->d_tiles2, sizeof(int) * 3 * MAX_TILES2));
+    int *tcnt = c->d_tiles2, *toff = tcnt + MAX_TILES2, *tfill = toff + MAX_TILES2;
+    fill_i32(c, tcnt, nt, 0);
+    const long nwg = std::max(1L, std::min(4096L, (np + 65535) / 65536));
+    const long chunk = (np + nwg - 1) / nwg;
+    k_bin_hist2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tcnt);
+    k_scan_tiles2d<<<1, BIN_THREADS2, 0, c->stream>>>(nt, tcnt, toff, tfill, maxcell);
+    k_bin_scatter2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tfill, c->d_perm2);
+    k_deposit2d_tiles<<<nt, BIN_THREADS2, 0, c->stream>>>(b, k, 1.0 / vmax, maxcell, toff, tcnt, c->d_perm2,
+                                                          c->d_pos2, c->d_vel2, c->d_acc2, c->d_ovf);
+    ECS_HIP(hipGetLastError());
+  } else {
+    // Grids of more than 4096 tiles: global fixed-point atomics per ion, the scale bounded
+    // by the fullest cell.
+    p2d_cell_counts(c, c->d_cnt2);
+    fill_i32(c, maxcell, 1, 0);
+    k_max_i32<<<grid2(n), 256, 0, c->stream>>>(c->d_cnt2, n, maxcell);
+    k_deposit2d<<<grid2(np), 256, 0, c->stream>>>(np, Nx, Ny, k, 1.0 / vmax, maxcell, c->d_pos2, c->d_vel2,
+                                                  c->d_acc2, c->d_ovf);
+    ECS_HIP(hipGetLastError());
+  }
   k_fold2d<<<grid2(n), 256, 0, c->stream>>>(n, c->d_acc2, c->d_ovf, maxcell, vmax, c->d_moments);
   ECS_HIP(hipGetLastError());
 }
@@ -376,10 +616,28 @@ void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
   if (Nc == 0 || c->p.mn <= 0.0) return;
   const CollideConsts cc = collide_consts_of(c->p);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
+  if (!c->d_perm2 || c->perm_cap2 < c->np) {  // the collision list shares the tile-list buffer
+    (void)hipFree(c->d_perm2);
+    c->d_perm2 = nullptr;
+    c->perm_cap2 = 0;
+    if (hipMalloc(&c->d_perm2, sizeof(int) * std::max(c->np, 1L)) != hipSuccess)
+      throw Error{ECSPRT_ERR_NOMEM, "two-dimensional collision list"};
+    c->perm_cap2 = c->np;
+  }
+  const long nseg = (c->np + COLL_CHUNK2 - 1) / COLL_CHUNK2;
+  if (!c->d_seg2 || c->seg_cap2 < nseg) {
+    (void)hipFree(c->d_seg2);
+    c->d_seg2 = nullptr;
+    c->seg_cap2 = 0;
+    ECS_HIP(hipMalloc(&c->d_seg2, sizeof(int) * nseg));
+    c->seg_cap2 = nseg;
+  }
   auto pass = [&](double lam, uint32_t step, int write) {
     zero_ctr(c);
-    k_collide2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, poisson_table(lam), cc, k0, k1, step, write, c->d_vel2,
-                                                     c->d_ctr);
+    const PoissonTable pt = poisson_table(lam);
+    k_collide2d_select<<<nseg, 256, 0, c->stream>>>(c->np, pt, k0, k1, step, c->d_perm2, c->d_seg2);
+    k_collide2d_apply<<<nseg, 256, 0, c->stream>>>(c->d_perm2, c->d_seg2, pt, cc, k0, k1, step, write, c->d_vel2,
+                                                   c->d_ctr);
     ECS_HIP(hipGetLastError());
     read_ctr(c);
   };
