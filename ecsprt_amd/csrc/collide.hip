@@ -186,13 +186,14 @@ __global__ void k_att_replay(int mode, long np, const ecsprt_attempt *att, const
 
 // The same replay on two-dimensional ions, double[np][2] velocities in ion order.
 __global__ void k_att_replay2d(int mode, long np, const ecsprt_attempt *att, const int *cnt, const long long *off,
-                               const int *list, double *vel, CollideConsts k, int *succ, int *fail,
-                               const long long *cut_ptr) {
+                               const int *list, const int *slot_of_id, double *vel, CollideConsts k, int *succ,
+                               int *fail, const long long *cut_ptr) {
   const long long cut = mode ? *cut_ptr : 0;
   for (long ip = blockIdx.x * (long)blockDim.x + threadIdx.x; ip < np; ip += (long)gridDim.x * blockDim.x) {
     const int n = cnt[ip];
     if (n == 0) continue;
-    double v[3] = {vel[2 * ip], vel[2 * ip + 1], 0.0};
+    const long s = slot_of_id[ip];
+    double v[3] = {vel[2 * s], vel[2 * s + 1], 0.0};
     const int *L = list + off[ip];
     for (int q = 0; q < n; ++q) {
       const int a = L[q];
@@ -210,8 +211,8 @@ __global__ void k_att_replay2d(int mode, long np, const ecsprt_attempt *att, con
       }
     }
     if (mode) {
-      vel[2 * ip] = v[0];
-      vel[2 * ip + 1] = v[1];
+      vel[2 * s] = v[0];
+      vel[2 * s + 1] = v[1];
     }
   }
 }
@@ -271,10 +272,11 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
   k_att_place<<<gb, 256, 0, st>>>(na, ipa, off, fill, list);
   k_att_sort<<<gb, 256, 0, st>>>(np, cnt, off, list);
   const bool two_d = c->p.ndim == 2;
-  if (!two_d) k_slot_of_id<<<c->ntiles, 256, 0, st>>>(c->A, c->d_base, c->d_count, slot);
+  if (two_d) p2d_slot_of_id(c, slot);
+  else k_slot_of_id<<<c->ntiles, 256, 0, st>>>(c->A, c->d_base, c->d_count, slot);
   CollideConsts k = collide_consts_of(c->p);
   if (two_d)
-    k_att_replay2d<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, c->d_vel2, k, succ, fail, nullptr);
+    k_att_replay2d<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, slot, c->d_vel2, k, succ, fail, nullptr);
   else
     k_att_replay<<<gb, 256, 0, st>>>(0, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, nullptr);
   scan_exclusive(c, succ, succ_ex, na);
@@ -297,7 +299,7 @@ void run_collide_predrawn(ecsprt_hip_ctx *c, double dt) {
           " collisions";
   } else {
     if (two_d)
-      k_att_replay2d<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, c->d_vel2, k, succ, fail, res);
+      k_att_replay2d<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, slot, c->d_vel2, k, succ, fail, res);
     else
       k_att_replay<<<gb, 256, 0, st>>>(1, np, c->d_att, cnt, off, list, slot, c->A, k, succ, fail, res);
     ECS_HIP(hipGetLastError());

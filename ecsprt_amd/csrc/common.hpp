@@ -165,6 +165,8 @@ struct ecsprt_hip_ctx {
   // two-dimensional runs (p.ndim == 2): ions double[np][2] in the reference's layout,
   // fixed-point moment sums [Ny][Nx][3], per-cell counts plus their maximum
   double *d_pos2 = nullptr, *d_vel2 = nullptr;
+  double *d_pos2b = nullptr, *d_vel2b = nullptr;  // the other buffer of the tile-order copy
+  int *d_id2 = nullptr, *d_id2b = nullptr;        // index each ion was set with
   long cap2 = 0;
   unsigned long long *d_acc2 = nullptr;
   int *d_cnt2 = nullptr;
@@ -214,6 +216,7 @@ void p2d_reserve(ecsprt_hip_ctx *c, long np);
 void p2d_free(ecsprt_hip_ctx *c);
 void p2d_set(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel, hipMemcpyKind kind);
 void p2d_get(ecsprt_hip_ctx *c, double *pos, double *vel);
+void p2d_slot_of_id(ecsprt_hip_ctx *c, int *slot);
 void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k);
 void p2d_push(ecsprt_hip_ctx *c, double dt);
 void p2d_drift(ecsprt_hip_ctx *c, double dt);

@@ -5,9 +5,11 @@
 // and the two-dimensional ScatterElastic (src/collisions.c:9-170 with ndim = 2).
 //
 // Ions are kept in the reference's own layout, double[np][2] positions and
-// velocities, in the order they were set: a two-dimensional grid is small enough that
-// the potential and the moments stay in the L2 cache, so the ions are streamed once per
-// stage without binning.  The deposit is exact and order-independent: every weight
+// velocities, with an id (the index they were set with).  Each deposit bins their
+// indices by tile and, while it reads them in that order, writes them out again in tile
+// order (a second buffer), so the next push, collisions and drift stream them with a
+// local potential.  Collision draws are keyed by id, and get_particles returns the ions
+// in id order.  The deposit is exact and order-independent: every weight
 // (and every flux over vmax) is rounded once to a fixed-point grid of 2^-fx and added
 // with 64-bit integer atomics, fx chosen from the fullest tile (or cell) so that no node
 // sum can overflow; ions with a velocity component at or above vmax take FP64 atomics.
@@ -313,6 +315,10 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_deposit2d_tiles(Bin2 b, PushCo
                                                                  const int *__restrict__ perm,
                                                                  const double *__restrict__ pos,
                                                                  const double *__restrict__ vel,
+                                                                 const int *__restrict__ id,
+                                                                 double *__restrict__ pos_out,
+                                                                 double *__restrict__ vel_out,
+                                                                 int *__restrict__ id_out,
                                                                  unsigned long long *__restrict__ acc,
                                                                  double *__restrict__ ovf) {
   constexpr int R = T2 + 1, R2 = R * R;
@@ -325,13 +331,21 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_deposit2d_tiles(Bin2 b, PushCo
   const int beg = off[t], n = counts[t];
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
     const long p = perm[beg + q];
-    const double x = cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2);
-    const double y = cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2);
+    const double px = pos[2 * p], py = pos[2 * p + 1];
+    const double x = cell_coord(px, k.dx, k.inv[0], k.pow2);
+    const double y = cell_coord(py, k.dy, k.inv[1], k.pow2);
     const int ixl = (int)x, iyl = (int)y;
     const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
     const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
     const double w4[4] = {wyh * wxh, wyl * wxh, wyh * wxl, wyl * wxl};
     const double vx = vel[2 * p], vy = vel[2 * p + 1];
+    // the ion, copied in tile order: the following stages stream it with a local potential
+    const long o = beg + q;
+    pos_out[2 * o] = px;
+    pos_out[2 * o + 1] = py;
+    vel_out[2 * o] = vx;
+    vel_out[2 * o + 1] = vy;
+    id_out[o] = id[p];
     const double ux = vx * inv_vmax, uy = vy * inv_vmax;
     const int la = wrap_index(ixl, b.Nx) - i0, lb = wrap_index(iyl, b.Ny) - j0;  // in [0, T2)
     const int na[4] = {la + 1, la + 1, la, la}, nb[4] = {lb + 1, lb, lb + 1, lb};
@@ -375,8 +389,8 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_deposit2d_tiles(Bin2 b, PushCo
 constexpr int COLL_CHUNK2 = 4096;  // ions per selecting workgroup (16 per thread)
 
 __global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable pt, uint32_t key0, uint32_t key1,
-                                                          uint32_t step, int *__restrict__ list,
-                                                          int *__restrict__ seg_count) {
+                                                          uint32_t step, const int *__restrict__ id,
+                                                          int *__restrict__ list, int *__restrict__ seg_count) {
   __shared__ float s_cdf[24];
   __shared__ int s_n;
   if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
@@ -385,7 +399,7 @@ __global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable 
   const long c0 = (long)blockIdx.x * COLL_CHUNK2, c1 = min(np, c0 + COLL_CHUNK2);
   for (long p0 = c0; p0 < c1; p0 += blockDim.x) {  // uniform per wave
     const long p = p0 + threadIdx.x;
-    const int k = p < c1 ? poisson_attempts(s_cdf, key0, key1, step, (int)p) : 0;
+    const int k = p < c1 ? poisson_attempts(s_cdf, key0, key1, step, id[p]) : 0;
     const unsigned long long m = __ballot(k > 0);
     if (!m) continue;
     const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -401,8 +415,8 @@ __global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable 
 __global__ void __launch_bounds__(256) k_collide2d_apply(const int *__restrict__ list,
                                                          const int *__restrict__ seg_count, PoissonTable pt,
                                                          CollideConsts cc, uint32_t key0, uint32_t key1,
-                                                         uint32_t step, int write, double *__restrict__ vel,
-                                                         unsigned long long *ctr) {
+                                                         uint32_t step, int write, const int *__restrict__ id,
+                                                         double *__restrict__ vel, unsigned long long *ctr) {
   __shared__ float s_cdf[24];
   if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
   __syncthreads();
@@ -411,9 +425,10 @@ __global__ void __launch_bounds__(256) k_collide2d_apply(const int *__restrict__
   unsigned natt = 0, nsucc = 0, nfail = 0;
   for (int q = threadIdx.x; q < n; q += blockDim.x) {
     const long p = list[c0 + q];
-    const int k = poisson_attempts(s_cdf, key0, key1, step, (int)p);
+    const int pid = id[p];
+    const int k = poisson_attempts(s_cdf, key0, key1, step, pid);
     double vx = vel[2 * p], vy = vel[2 * p + 1], vz = 0.0;
-    collide_attempts<double, 2>(cc, key0, key1, step, (int)p, 0, k, vx, vy, vz, natt, nsucc, nfail);
+    collide_attempts<double, 2>(cc, key0, key1, step, pid, 0, k, vx, vy, vz, natt, nsucc, nfail);
     if (write) {
       vel[2 * p] = vx;
       vel[2 * p + 1] = vy;
@@ -463,6 +478,26 @@ __global__ void k_init2d(long np, double Lx, double Ly, double x0, double y0, do
   }
 }
 
+__global__ void k_iota2d(long np, int *__restrict__ id) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) id[p] = (int)p;
+}
+
+__global__ void k_unsort2d(long np, const int *__restrict__ id, const double *__restrict__ pos,
+                           const double *__restrict__ vel, double *__restrict__ pos_out, double *__restrict__ vel_out) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x) {
+    const long q = id[p];
+    pos_out[2 * q] = pos[2 * p];
+    pos_out[2 * q + 1] = pos[2 * p + 1];
+    vel_out[2 * q] = vel[2 * p];
+    vel_out[2 * q + 1] = vel[2 * p + 1];
+  }
+}
+
+__global__ void k_slot_of_id2d(long np, const int *__restrict__ id, int *__restrict__ slot) {
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < np; p += (long)gridDim.x * blockDim.x)
+    slot[id[p]] = (int)p;
+}
+
 int grid2(long n) { return (int)std::max(1L, std::min(65536L, (n + 255) / 256)); }
 
 }  // namespace
@@ -481,19 +516,35 @@ void p2d_reserve(ecsprt_hip_ctx *c, long np) {
     ECS_HIP(hipHostMalloc(&c->h_ctr, sizeof(unsigned long long) * 8));
   }
   if (c->cap2 >= np && c->d_pos2) return;
-  (void)hipFree(c->d_pos2);
-  (void)hipFree(c->d_vel2);
-  c->d_pos2 = c->d_vel2 = nullptr;
+  double **dv[4] = {&c->d_pos2, &c->d_vel2, &c->d_pos2b, &c->d_vel2b};
+  int **iv[2] = {&c->d_id2, &c->d_id2b};
+  for (double **a : dv) {
+    (void)hipFree(*a);
+    *a = nullptr;
+  }
+  for (int **a : iv) {
+    (void)hipFree(*a);
+    *a = nullptr;
+  }
   c->cap2 = 0;
-  if (hipMalloc(&c->d_pos2, sizeof(double) * 2 * std::max(np, 1L)) != hipSuccess ||
-      hipMalloc(&c->d_vel2, sizeof(double) * 2 * std::max(np, 1L)) != hipSuccess)
-    throw Error{ECSPRT_ERR_NOMEM, "two-dimensional particle arrays"};
+  for (double **a : dv)
+    if (hipMalloc(a, sizeof(double) * 2 * std::max(np, 1L)) != hipSuccess)
+      throw Error{ECSPRT_ERR_NOMEM, "two-dimensional particle arrays"};
+  for (int **a : iv)
+    if (hipMalloc(a, sizeof(int) * std::max(np, 1L)) != hipSuccess)
+      throw Error{ECSPRT_ERR_NOMEM, "two-dimensional particle ids"};
   c->cap2 = np;
 }
 
 void p2d_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_pos2);
   (void)hipFree(c->d_vel2);
+  (void)hipFree(c->d_pos2b);
+  (void)hipFree(c->d_vel2b);
+  (void)hipFree(c->d_id2);
+  (void)hipFree(c->d_id2b);
+  c->d_pos2b = c->d_vel2b = nullptr;
+  c->d_id2 = c->d_id2b = nullptr;
   (void)hipFree(c->d_acc2);
   (void)hipFree(c->d_cnt2);
   (void)hipFree(c->d_perm2);
@@ -511,15 +562,28 @@ void p2d_set(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel, h
   p2d_reserve(c, np);
   ECS_HIP(hipMemcpyAsync(c->d_pos2, pos, sizeof(double) * 2 * np, kind, c->stream));
   ECS_HIP(hipMemcpyAsync(c->d_vel2, vel, sizeof(double) * 2 * np, kind, c->stream));
+  k_iota2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2);
+  ECS_HIP(hipGetLastError());
   c->np = np;
   c->coll_lambda = -1.0;
   ECS_HIP(hipStreamSynchronize(c->stream));
 }
 
+// Ions back in the order they were set (by id).
 void p2d_get(ecsprt_hip_ctx *c, double *pos, double *vel) {
-  ECS_HIP(hipMemcpyAsync(pos, c->d_pos2, sizeof(double) * 2 * c->np, hipMemcpyDeviceToHost, c->stream));
-  ECS_HIP(hipMemcpyAsync(vel, c->d_vel2, sizeof(double) * 2 * c->np, hipMemcpyDeviceToHost, c->stream));
+  const long np = c->np;
+  ensure_stage(c, np);
+  k_unsort2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2, c->d_pos2, c->d_vel2, c->d_stage, c->d_stage + 2 * np);
+  ECS_HIP(hipGetLastError());
+  ECS_HIP(hipMemcpyAsync(pos, c->d_stage, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipMemcpyAsync(vel, c->d_stage + 2 * np, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, c->stream));
   ECS_HIP(hipStreamSynchronize(c->stream));
+}
+
+// Slot of each ion id (for the pre-drawn replay, whose records name ions by id).
+void p2d_slot_of_id(ecsprt_hip_ctx *c, int *slot) {
+  k_slot_of_id2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->d_id2, slot);
+  ECS_HIP(hipGetLastError());
 }
 
 void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k) {
@@ -527,6 +591,7 @@ void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int 
   const ecsprt_params &p = c->p;
   k_init2d<<<grid2(np), 256, 0, c->stream>>>(np, p.Lx, p.Ly, p.x0, p.y0, amp, k, p.viTx, p.viTy, p.vi0x, p.vi0y,
                                              (uint32_t)seed, (uint32_t)(seed >> 32), c->d_pos2, c->d_vel2);
+  k_iota2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2);
   ECS_HIP(hipGetLastError());
   c->np = np;
   c->coll_lambda = -1.0;
@@ -591,8 +656,12 @@ void p2d_deposit(ecsprt_hip_ctx *c) {
     k_scan_tiles2d<<<1, BIN_THREADS2, 0, c->stream>>>(nt, tcnt, toff, tfill, maxcell);
     k_bin_scatter2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tfill, c->d_perm2);
     k_deposit2d_tiles<<<nt, BIN_THREADS2, 0, c->stream>>>(b, k, 1.0 / vmax, maxcell, toff, tcnt, c->d_perm2,
-                                                          c->d_pos2, c->d_vel2, c->d_acc2, c->d_ovf);
+                                                          c->d_pos2, c->d_vel2, c->d_id2, c->d_pos2b, c->d_vel2b,
+                                                          c->d_id2b, c->d_acc2, c->d_ovf);
     ECS_HIP(hipGetLastError());
+    std::swap(c->d_pos2, c->d_pos2b);
+    std::swap(c->d_vel2, c->d_vel2b);
+    std::swap(c->d_id2, c->d_id2b);
   } else {
     // Grids of more than 4096 tiles: global fixed-point atomics per ion, the scale bounded
     // by the fullest cell.
@@ -635,9 +704,9 @@ void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
   auto pass = [&](double lam, uint32_t step, int write) {
     zero_ctr(c);
     const PoissonTable pt = poisson_table(lam);
-    k_collide2d_select<<<nseg, 256, 0, c->stream>>>(c->np, pt, k0, k1, step, c->d_perm2, c->d_seg2);
-    k_collide2d_apply<<<nseg, 256, 0, c->stream>>>(c->d_perm2, c->d_seg2, pt, cc, k0, k1, step, write, c->d_vel2,
-                                                   c->d_ctr);
+    k_collide2d_select<<<nseg, 256, 0, c->stream>>>(c->np, pt, k0, k1, step, c->d_id2, c->d_perm2, c->d_seg2);
+    k_collide2d_apply<<<nseg, 256, 0, c->stream>>>(c->d_perm2, c->d_seg2, pt, cc, k0, k1, step, write, c->d_id2,
+                                                   c->d_vel2, c->d_ctr);
     ECS_HIP(hipGetLastError());
     read_ctr(c);
   };

@@ -36,9 +36,9 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
-@pytest.mark.parametrize("N,Ny,dx", [(12, 12, 1.0), (16, 10, 0.5), (9, 13, 0.3)])
-def test_push2d_and_drift_bit_exact(N, Ny, dx):
-    say(f"2-D push/drift N={N} Ny={Ny} dx={dx}")
+@pytest.mark.parametrize("N,Ny,dx,sort", [(12, 12, 1.0, False), (16, 10, 0.5, True), (9, 13, 0.3, True)])
+def test_push2d_and_drift_bit_exact(N, Ny, dx, sort):
+    say(f"2-D push/drift N={N} Ny={Ny} dx={dx} sort={sort}")
     cli = cli2d(N, Ny, B0=0.2)
     cli.update(dx=dx, dy=0.75)
     ctx, po, pos, vel = make(cli, 5000)
@@ -47,6 +47,8 @@ def test_push2d_and_drift_bit_exact(N, Ny, dx):
     pos[:4] = [[0.0, 0.0], [po.Lx * (1 - 1e-16), po.Ly * 0.5], [po.Lx - 0.5 * po.dx, po.Ly - 0.25 * po.dy],
                [0.5 * po.dx, po.Ly * (1 - 1e-16)]]
     ctx.set_particles(pos, vel)
+    if sort:
+        ctx.collect()  # the deposit stores the ions in tile order; results are still by set index
     ctx.set_potential(phi)
     ctx.push(0.9)
     gpos, gvel = ctx.get_particles()
@@ -132,14 +134,17 @@ def test_rhs2d_matmult_and_solve():
     ctx.close()
 
 
-def test_collide2d_predrawn_matches_reference_stream():
-    say("2-D predrawn collisions")
+@pytest.mark.parametrize("sort", [False, True])
+def test_collide2d_predrawn_matches_reference_stream(sort):
+    say(f"2-D predrawn collisions sort={sort}")
     cli = cli2d(10, nui=0.05, Tn_over_Ti=2.0)
     ctx, po, pos, vel = make(cli, 4000, seed=7, collide_mode=_lib.COLLIDE_PREDRAWN)
     v_nr = vel.copy()
     rc, nl, Ns, Nf, log = oracle.scatter_elastic_nr_nd(po, 2, 1.0, v_nr, oracle.NR(), log_cap=200000)
     assert rc == 0 and nl == len(log)
     ctx.set_particles(pos, vel)
+    if sort:
+        ctx.collect()  # records name ions by set index, whatever order they are stored in
     ctx.set_attempts(log)
     ctx.collide(1.0)
     _, gvel = ctx.get_particles()
@@ -173,6 +178,37 @@ def test_step2d_philox_conserves_and_collides():
     assert np.isfinite(ctx.get_potential()).all()
     assert ctx.num_particles == n
     ctx.close()
+
+
+def test_step2d_philox_independent_of_storage_order():
+    """Philox draws are keyed by the index an ion was set with, so a step gives the same
+    ions whether they start in set order or in the tile order a deposit leaves them in."""
+    say("2-D steps, storage order")
+    N = 24
+    cli = cli2d(N, nui=0.05, E0=0.05)
+    runs, phi0 = [], None
+    for sorted_first in (True, False):
+        p = pic.setup_context(**dict(cli, ndim=2))
+        ctx = pic.Context(p, pic.default_options(collide_mode=_lib.COLLIDE_PHILOX, ksp_rtol=1e-10,
+                                                 operator_mode=_lib.OPERATOR_REBUILD))
+        ctx.init_particles(32 * N * N, 5, 1e-2, 2)
+        if sorted_first:
+            ctx.collect()
+            ctx.compute_potential()
+            phi0 = ctx.get_potential()
+        else:
+            ctx.set_potential(phi0)
+        out = []
+        for step in range(2):
+            ctx.step(1.0)
+            out.append((*ctx.get_particles(), ctx.get_moments(), ctx.collision_stats()))
+        runs.append(out)
+        ctx.close()
+    for (pa, va, ma, ca), (pb, vb, mb, cb) in zip(*runs):
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(va, vb)
+        np.testing.assert_array_equal(ma, mb)
+        assert ca == cb
 
 
 @pytest.mark.parametrize("dist", ["normal", "sobol", "sinusoidal"])
