@@ -397,13 +397,16 @@ int ecsprt_hip_step(void *opts, double dt) {
   return guarded(c, [&] {
     ECS_CHECK(c->np > 0, ECSPRT_ERR_STATE, "no particles");
     if (c->p.ndim == 2) {
-      // The reference loop (src/pic.c:246-254) stage by stage: UpdateVelocities (push,
-      // collisions), UpdatePositions, collect, potential.
+      // The reference loop (src/pic.c:246-254): UpdateVelocities (push, collisions),
+      // UpdatePositions, collect, potential; push, Philox collisions and drift fused
+      // when the ions are in tile order.
       ev_record(c, 0);
-      ecsprt::p2d_push(c, dt);
-      if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) ecsprt::p2d_collide_philox(c, dt);
-      else if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) ecsprt::run_collide_predrawn(c, dt);
-      ecsprt::p2d_drift(c, dt);
+      if (!ecsprt::p2d_push_collide_drift(c, dt)) {
+        ecsprt::p2d_push(c, dt);
+        if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) ecsprt::p2d_collide_philox(c, dt);
+        else if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) ecsprt::run_collide_predrawn(c, dt);
+        ecsprt::p2d_drift(c, dt);
+      }
       ecsprt::p2d_deposit(c);
       ev_record(c, 3);
       compute_potential(c);

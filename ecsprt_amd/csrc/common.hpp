@@ -167,6 +167,7 @@ struct ecsprt_hip_ctx {
   double *d_pos2 = nullptr, *d_vel2 = nullptr;
   double *d_pos2b = nullptr, *d_vel2b = nullptr;  // the other buffer of the tile-order copy
   int *d_id2 = nullptr, *d_id2b = nullptr;        // index each ion was set with
+  bool tiled2 = false;  // ions in the tile order and tile ranges of the last deposit, not moved since
   long cap2 = 0;
   unsigned long long *d_acc2 = nullptr;
   int *d_cnt2 = nullptr;
@@ -219,6 +220,7 @@ void p2d_get(ecsprt_hip_ctx *c, double *pos, double *vel);
 void p2d_slot_of_id(ecsprt_hip_ctx *c, int *slot);
 void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int k);
 void p2d_push(ecsprt_hip_ctx *c, double dt);
+bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt);
 void p2d_drift(ecsprt_hip_ctx *c, double dt);
 void p2d_apply_bc(ecsprt_hip_ctx *c, long np, double *pos);
 void p2d_cell_counts(ecsprt_hip_ctx *c, int *d_counts);

@@ -33,11 +33,10 @@ namespace {
 // Periodic index of base + d for a base already in [0, n) and |d| <= 2 < n.
 __device__ inline int near_wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-__device__ inline void diff_vec2d(const double *__restrict__ F, int Nx, int Ny, double x0, double y0, double f[2]) {
+// P(j, i) is the potential at node (i, j), always with i - ixl and j - iyl in [-1, 2].
+template <class Acc>
+__device__ inline void diff_vec2d_at(const Acc &P, int Nx, int Ny, double x0, double y0, double f[2]) {
   const int ixl = (int)x0, ixh = ixl + 1, iyl = (int)y0, iyh = iyl + 1;
-  // Every node read is (ixl + a, iyl + b) with a, b in [-1, 2]: one modulo per axis.
-  const int bx = wrap_index(ixl, Nx), by = wrap_index(iyl, Ny);
-  auto P = [&](int j, int i) { return F[(long)near_wrap(by + (j - iyl), Ny) * Nx + near_wrap(bx + (i - ixl), Nx)]; };
   const double wxh = x0 - (double)ixl, wyh = y0 - (double)iyl;
   double hh, lh, hl, ll;
   if (ixl >= 0) {
@@ -73,6 +72,14 @@ __device__ inline void diff_vec2d(const double *__restrict__ F, int Nx, int Ny, 
   Ewh = lh + wyh * (hh - lh);
   Ewl = ll + wyh * (hl - ll);
   f[1] = Ewl + wxh * (Ewh - Ewl);
+}
+
+__device__ inline void diff_vec2d(const double *__restrict__ F, int Nx, int Ny, double x0, double y0, double f[2]) {
+  const int ixl = (int)x0, iyl = (int)y0;
+  // Every node read is (ixl + a, iyl + b) with a, b in [-1, 2]: one modulo per axis.
+  const int bx = wrap_index(ixl, Nx), by = wrap_index(iyl, Ny);
+  auto P = [&](int j, int i) { return F[(long)near_wrap(by + (j - iyl), Ny) * Nx + near_wrap(bx + (i - ixl), Nx)]; };
+  diff_vec2d_at(P, Nx, Ny, x0, y0, f);
 }
 
 // BorisMoverBz2D body (src/velocities.c:254-262).
@@ -390,7 +397,7 @@ constexpr int COLL_CHUNK2 = 4096;  // ions per selecting workgroup (16 per threa
 
 __global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable pt, uint32_t key0, uint32_t key1,
                                                           uint32_t step, const int *__restrict__ id,
-                                                          int *__restrict__ list, int *__restrict__ seg_count) {
+                                                          int *__restrict__ list, int *__restrict__ seg) {
   __shared__ float s_cdf[24];
   __shared__ int s_n;
   if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
@@ -409,22 +416,50 @@ __global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable 
     if (k > 0) list[c0 + base + below] = (int)p;
   }
   __syncthreads();
-  if (threadIdx.x == 0) seg_count[blockIdx.x] = s_n;
+  if (threadIdx.x == 0) {
+    seg[2 * blockIdx.x] = (int)c0;
+    seg[2 * blockIdx.x + 1] = s_n;
+  }
 }
 
-__global__ void __launch_bounds__(256) k_collide2d_apply(const int *__restrict__ list,
-                                                         const int *__restrict__ seg_count, PoissonTable pt,
+constexpr int COLL_SEGS2 = 8;  // most list segments per applying workgroup
+
+// Segments per applying workgroup: about 8192 workgroups of a few hundred listed ions.
+int apply_segs(long nseg) { return (int)std::max(1L, std::min((long)COLL_SEGS2, nseg / 8192)); }
+
+// Apply the attempts of the ions listed in G consecutive segments (segment g holds
+// seg[2g + 1] ion slots from list[seg[2g]] on), so every thread has work.  The
+// workgroup's attempt, success and failure counts go to the counters with one atomic
+// each.
+__global__ void __launch_bounds__(256) k_collide2d_apply(long nseg, int G, const int *__restrict__ list,
+                                                         const int *__restrict__ seg, PoissonTable pt,
                                                          CollideConsts cc, uint32_t key0, uint32_t key1,
                                                          uint32_t step, int write, const int *__restrict__ id,
                                                          double *__restrict__ vel, unsigned long long *ctr) {
   __shared__ float s_cdf[24];
+  __shared__ int s_off[COLL_SEGS2 + 1], s_beg[COLL_SEGS2];
+  __shared__ unsigned long long s_red[3][4];
   if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
+  const long g0 = (long)blockIdx.x * G;
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int g = 0; g < COLL_SEGS2; ++g) {
+      s_off[g] = t;
+      s_beg[g] = 0;
+      if (g < G && g0 + g < nseg) {
+        s_beg[g] = seg[2 * (g0 + g)];
+        t += seg[2 * (g0 + g) + 1];
+      }
+    }
+    s_off[COLL_SEGS2] = t;
+  }
   __syncthreads();
-  const long c0 = (long)blockIdx.x * COLL_CHUNK2;
-  const int n = seg_count[blockIdx.x];
+  const int total = s_off[COLL_SEGS2];
   unsigned natt = 0, nsucc = 0, nfail = 0;
-  for (int q = threadIdx.x; q < n; q += blockDim.x) {
-    const long p = list[c0 + q];
+  for (int q = threadIdx.x; q < total; q += blockDim.x) {
+    int g = 0;
+    while (q >= s_off[g + 1]) ++g;
+    const long p = list[(long)s_beg[g] + (q - s_off[g])];
     const int pid = id[p];
     const int k = poisson_attempts(s_cdf, key0, key1, step, pid);
     double vx = vel[2 * p], vy = vel[2 * p + 1], vz = 0.0;
@@ -434,11 +469,175 @@ __global__ void __launch_bounds__(256) k_collide2d_apply(const int *__restrict__
       vel[2 * p + 1] = vy;
     }
   }
-  unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+  const unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+  const int w = threadIdx.x >> 6;
   if (__lane_id() == 0) {
-    if (s0) atomicAdd(&ctr[3], s0);
-    if (s1) atomicAdd(&ctr[4], s1);
-    if (s2) atomicAdd(&ctr[5], s2);
+    s_red[0][w] = s0;
+    s_red[1][w] = s1;
+    s_red[2][w] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const unsigned long long v = s_red[threadIdx.x][0] + s_red[threadIdx.x][1] + s_red[threadIdx.x][2] +
+                                 s_red[threadIdx.x][3];
+    if (v) atomicAdd(&ctr[3 + threadIdx.x], v);
+  }
+}
+
+constexpr int HALO2 = T2 + 3;  // a tile's nodes -1 .. T2 + 1 per axis
+
+constexpr int QCAP2 = 512;  // LDS collision queue of k_push2d_tiles: two workgroups' worth
+
+// UpdatePositions (src/positions.c:455-459) and the wrap of one ion; true if it is
+// still outside [r0, r1] (or not finite).
+__device__ inline bool drift_one2d(const DriftConsts &d, long p, double px, double py, double vx, double vy,
+                                   double *__restrict__ pos) {
+  const double r0v[2] = {px, py}, v[2] = {vx, vy};
+  bool out = false;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    double r = r0v[a] + v[a] * d.dt;
+    if (r < d.r0[a]) r += d.L[a];
+    if (r >= d.r1[a]) r -= d.L[a];
+    pos[2 * p + a] = r;
+    out |= !(r >= d.r0[a] && r <= d.r1[a]);
+  }
+  return out;
+}
+
+// BorisMoverBz2D (src/velocities.c:254-262) over ions stored in tile order.  Workgroup
+// (t, s) takes the s-th contiguous share of tile t's ions and reads the potential from
+// the tile's nodes staged in LDS; an ion whose cell is outside the tile (not where the
+// last deposit put it) reads it from memory.  With fused set it also draws each ion's
+// collision attempt count and moves it (UpdatePositions and the wrap): an ion with no
+// attempt at once, the others through an LDS queue that the whole workgroup works off
+// 256 at a time (attempts, then the move), so collisions run on full waves while the
+// ions' lines are still in the cache.
+__global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const int *__restrict__ toff,
+                                                      const int *__restrict__ tcnt, const double *__restrict__ phi,
+                                                      PushConsts k, int fused, int collide, DriftConsts d,
+                                                      PoissonTable pt, CollideConsts cc, uint32_t key0,
+                                                      uint32_t key1, uint32_t step, const int *__restrict__ id,
+                                                      double *__restrict__ pos, double *__restrict__ vel,
+                                                      unsigned long long *ctr) {
+  __shared__ double s_phi[HALO2 * HALO2];
+  __shared__ double s_qd[QCAP2][4];  // position, then pushed velocity
+  __shared__ int s_qp[QCAP2], s_qid[QCAP2];
+  __shared__ float s_cdf[24];
+  __shared__ int s_qn;
+  __shared__ unsigned long long s_red[4][4];
+  const int t = blockIdx.x / split, sp = blockIdx.x - t * split;
+  const int i0 = (t % b.ntx) * T2, j0 = (t / b.ntx) * T2;
+  for (int q = threadIdx.x; q < HALO2 * HALO2; q += blockDim.x) {
+    const int ly = q / HALO2, lx = q - ly * HALO2;
+    s_phi[q] = phi[(long)wrap_index(j0 - 1 + ly, b.Ny) * b.Nx + wrap_index(i0 - 1 + lx, b.Nx)];
+  }
+  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
+  if (threadIdx.x == 0) s_qn = 0;
+  __syncthreads();
+  const int n = tcnt[t], m = (n + split - 1) / split;
+  const long beg = (long)toff[t] + (long)sp * m;
+  const int cnt = max(0, min(m, n - sp * m));
+  unsigned natt = 0, nsucc = 0, nfail = 0, nl = 0;
+  // Entries from .. from + nw - 1 of the queue, one per thread.
+  auto work_off = [&](int from, int nw) {
+    if ((int)threadIdx.x < nw) {
+      const int e = from + threadIdx.x;
+      const long p = s_qp[e];
+      const int pid = s_qid[e];
+      const double px = s_qd[e][0], py = s_qd[e][1];
+      double vx = s_qd[e][2], vy = s_qd[e][3], vz = 0.0;
+      const int ka = poisson_attempts(s_cdf, key0, key1, step, pid);
+      collide_attempts<double, 2>(cc, key0, key1, step, pid, 0, ka, vx, vy, vz, natt, nsucc, nfail);
+      vel[2 * p] = vx;
+      vel[2 * p + 1] = vy;
+      nl += drift_one2d(d, p, px, py, vx, vy, pos) ? 1u : 0u;
+    }
+  };
+  for (int q0 = 0; q0 < cnt; q0 += blockDim.x) {  // uniform per workgroup
+    const int q = q0 + threadIdx.x;
+    bool listed = false;
+    long p = 0;
+    int pid = 0;
+    double px = 0, py = 0, vxp = 0, vyp = 0;
+    if (q < cnt) {
+      p = beg + q;
+      px = pos[2 * p];
+      py = pos[2 * p + 1];
+      const double x = cell_coord(px, k.dx, k.inv[0], k.pow2);
+      const double y = cell_coord(py, k.dy, k.inv[1], k.pow2);
+      const int ixl = (int)x, iyl = (int)y;
+      const int lx = wrap_index(ixl, b.Nx) - i0, ly = wrap_index(iyl, b.Ny) - j0;
+      double E[2];
+      if (lx >= 0 && lx < T2 && ly >= 0 && ly < T2) {
+        auto P = [&](int j, int i) { return s_phi[(ly + 1 + (j - iyl)) * HALO2 + lx + 1 + (i - ixl)]; };
+        diff_vec2d_at(P, b.Nx, b.Ny, x, y, E);
+      } else {
+        diff_vec2d(phi, b.Nx, b.Ny, x, y, E);
+      }
+      const double vxm = vel[2 * p] + k.Escale[0] * E[0];
+      const double vym = vel[2 * p + 1] + k.Escale[1] * E[1];
+      const double vxt = vxm + vym * k.t;
+      vyp = vym - vxt * k.s;
+      vxp = vxt + vyp * k.t;
+      if (fused && collide) {
+        pid = id[p];
+        listed = poisson_attempts(s_cdf, key0, key1, step, pid) > 0;
+      }
+      if (!listed) {
+        vel[2 * p] = vxp;
+        vel[2 * p + 1] = vyp;
+        if (fused) nl += drift_one2d(d, p, px, py, vxp, vyp, pos) ? 1u : 0u;
+      }
+    }
+    if (fused && collide) {
+      const unsigned long long mk = __ballot(listed);
+      if (mk) {
+        const int below =
+            (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mk >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mk, 0u));
+        int base = 0;
+        if (listed && below == 0) base = atomicAdd(&s_qn, (int)__popcll(mk));
+        base = __builtin_amdgcn_readlane(base, __ffsll((long long)mk) - 1);
+        if (listed) {
+          const int e = base + below;  // < QCAP2: the queue holds < 256 entries before this append
+          s_qp[e] = (int)p;
+          s_qid[e] = pid;
+          s_qd[e][0] = px;
+          s_qd[e][1] = py;
+          s_qd[e][2] = vxp;
+          s_qd[e][3] = vyp;
+        }
+      }
+      __syncthreads();
+      const int nq = s_qn;
+      __syncthreads();  // every thread has read s_qn before it changes
+      if (nq >= (int)blockDim.x) {
+        work_off(nq - blockDim.x, blockDim.x);
+        if (threadIdx.x == 0) s_qn = nq - blockDim.x;
+        __syncthreads();
+      }
+    }
+  }
+  if (fused) {
+    if (collide) {
+      __syncthreads();
+      work_off(0, s_qn);
+    }
+    const unsigned long long r0 = wave_sum(natt), r1 = wave_sum(nsucc), r2 = wave_sum(nfail), r3 = wave_sum(nl);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if (__lane_id() == 0) {
+      s_red[0][w] = r0;
+      s_red[1][w] = r1;
+      s_red[2][w] = r2;
+      s_red[3][w] = r3;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      const unsigned long long v = s_red[threadIdx.x][0] + s_red[threadIdx.x][1] + s_red[threadIdx.x][2] +
+                                   s_red[threadIdx.x][3];
+      if (v) atomicAdd(&ctr[threadIdx.x < 3 ? 3 + threadIdx.x : 2], v);
+    }
   }
 }
 
@@ -504,6 +703,35 @@ int grid2(long n) { return (int)std::max(1L, std::min(65536L, (n + 255) / 256));
 
 // ---- host side -------------------------------------------------------------------
 
+namespace {
+
+Bin2 bin2_of(const ecsprt_params &p) { return Bin2{p.Nx, p.Ny, (p.Nx + T2 - 1) / T2, (p.Ny + T2 - 1) / T2}; }
+
+// Workgroups per tile in k_push2d_tiles: about 8192 over the grid.
+int push_split(int nt) { return std::max(1, std::min(64, (8192 + nt - 1) / nt)); }
+
+void ensure_perm(ecsprt_hip_ctx *c, long np) {
+  if (c->d_perm2 && c->perm_cap2 >= np) return;
+  (void)hipFree(c->d_perm2);
+  c->d_perm2 = nullptr;
+  c->perm_cap2 = 0;
+  if (hipMalloc(&c->d_perm2, sizeof(int) * std::max(np, 1L)) != hipSuccess)
+    throw Error{ECSPRT_ERR_NOMEM, "two-dimensional ion lists"};
+  c->perm_cap2 = np;
+}
+
+// (start, count) of each list segment.
+void ensure_segments(ecsprt_hip_ctx *c, long nseg) {
+  if (c->d_seg2 && c->seg_cap2 >= nseg) return;
+  (void)hipFree(c->d_seg2);
+  c->d_seg2 = nullptr;
+  c->seg_cap2 = 0;
+  ECS_HIP(hipMalloc(&c->d_seg2, sizeof(int) * 2 * std::max(nseg, 1L)));
+  c->seg_cap2 = nseg;
+}
+
+}  // namespace
+
 void p2d_reserve(ecsprt_hip_ctx *c, long np) {
   ECS_CHECK(np > 0 && np < 2147483647L, ECSPRT_ERR_ARG, "np must be in [1, 2^31-1)");
   if (!c->d_acc2) {
@@ -565,6 +793,7 @@ void p2d_set(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel, h
   k_iota2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2);
   ECS_HIP(hipGetLastError());
   c->np = np;
+  c->tiled2 = false;
   c->coll_lambda = -1.0;
   ECS_HIP(hipStreamSynchronize(c->stream));
 }
@@ -594,19 +823,73 @@ void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int 
   k_iota2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2);
   ECS_HIP(hipGetLastError());
   c->np = np;
+  c->tiled2 = false;
   c->coll_lambda = -1.0;
   ECS_HIP(hipStreamSynchronize(c->stream));
 }
 
 void p2d_push(ecsprt_hip_ctx *c, double dt) {
-  k_push2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->d_phi, c->p.Nx, c->p.Ny, push_consts(c->p, dt), c->d_pos2,
-                                                c->d_vel2);
+  if (c->tiled2) {
+    const Bin2 b = bin2_of(c->p);
+    const int nt = b.ntx * b.nty, split = push_split(nt);
+    int *toff = c->d_tiles2 + MAX_TILES2, *tcnt = c->d_tiles2;
+    k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 0, 0,
+                                                      drift_consts(c->p, dt), poisson_table(0.0),
+                                                      collide_consts_of(c->p), 0, 0, 0, c->d_id2, c->d_pos2,
+                                                      c->d_vel2, nullptr);
+  } else {
+    k_push2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->d_phi, c->p.Nx, c->p.Ny, push_consts(c->p, dt),
+                                                  c->d_pos2, c->d_vel2);
+  }
   ECS_HIP(hipGetLastError());
+}
+
+// Push, Philox collisions and drift of one step in one pass over the tile-ordered ions
+// (k_push2d_tiles with fused set): the same ions, draws and arithmetic as p2d_push,
+// p2d_collide_philox and p2d_drift in turn.  False (nothing done) when the ions are not
+// in tile order, collisions are pre-drawn or the Philox attempt rate has not been set
+// yet (its dry runs are done by p2d_collide_philox).
+bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
+  if (!c->tiled2) return false;
+  const bool philox = c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX;
+  if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) return false;
+  const int Nc = philox ? collision_count(&c->p, c->np, dt) : 0;
+  const bool collide = philox && Nc > 0;
+  if (collide && c->coll_lambda <= 0.0) return false;
+  if (philox) {
+    c->last_Nc = Nc;
+    c->last_Ns = c->last_Nf = c->last_attempts = 0;
+  }
+  const Bin2 b = bin2_of(c->p);
+  const int nt = b.ntx * b.nty, split = push_split(nt);
+  int *toff = c->d_tiles2 + MAX_TILES2, *tcnt = c->d_tiles2;
+  const PoissonTable pt = poisson_table(collide ? c->coll_lambda : 0.0);
+  const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32), step = (uint32_t)c->step_index;
+  zero_ctr(c);
+  k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 1,
+                                                    collide ? 1 : 0, drift_consts(c->p, dt), pt,
+                                                    collide_consts_of(c->p), k0, k1, step, c->d_id2, c->d_pos2,
+                                                    c->d_vel2, c->d_ctr);
+  ECS_HIP(hipGetLastError());
+  c->tiled2 = false;
+  read_ctr(c);
+  if (collide) {
+    const long natt = (long)c->h_ctr[3], ns = (long)c->h_ctr[4], nf = (long)c->h_ctr[5];
+    c->last_attempts = natt;
+    c->last_Ns = ns;
+    c->last_Nf = nf;
+    if (ns > 0) c->coll_lambda *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
+    ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
+  }
+  ECS_CHECK(c->h_ctr[2] == 0, ECSPRT_ERR_COUNT_CHANGED,
+            std::to_string(c->h_ctr[2]) + " ions left the periodic domain (not finite or moved more than L)");
+  return true;
 }
 
 // Drift and wrap; a lost ion aborts as a changed ion count does in the reference
 // (src/boundaries.c:101-109).
 void p2d_drift(ecsprt_hip_ctx *c, double dt) {
+  c->tiled2 = false;
   zero_ctr(c);
   k_drift2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, drift_consts(c->p, dt), c->d_pos2, c->d_vel2, c->d_ctr + 2);
   ECS_HIP(hipGetLastError());
@@ -616,6 +899,7 @@ void p2d_drift(ecsprt_hip_ctx *c, double dt) {
 }
 
 void p2d_apply_bc(ecsprt_hip_ctx *c, long np, double *pos) {
+  c->tiled2 = false;
   k_apply_bc2d<<<grid2(np), 256, 0, c->stream>>>(np, drift_consts(c->p, 0.0), pos);
   ECS_HIP(hipGetLastError());
 }
@@ -638,15 +922,9 @@ void p2d_deposit(ecsprt_hip_ctx *c) {
   fill_f64(c, reinterpret_cast<double *>(c->d_acc2), 3 * n, 0.0);  // all-zero bits
   fill_f64(c, c->d_ovf, 4 * n, 0.0);
   const long np = c->np;
+  c->tiled2 = false;
   if (nt <= MAX_TILES2) {
-    if (!c->d_perm2 || c->perm_cap2 < np) {
-      (void)hipFree(c->d_perm2);
-      c->d_perm2 = nullptr;
-      c->perm_cap2 = 0;
-      if (hipMalloc(&c->d_perm2, sizeof(int) * std::max(np, 1L)) != hipSuccess)
-        throw Error{ECSPRT_ERR_NOMEM, "two-dimensional tile lists"};
-      c->perm_cap2 = np;
-    }
+    ensure_perm(c, np);
     if (!c->d_tiles2) ECS_HIP(hipMalloc(&c->d_tiles2, sizeof(int) * 3 * MAX_TILES2));
     int *tcnt = c->d_tiles2, *toff = tcnt + MAX_TILES2, *tfill = toff + MAX_TILES2;
     fill_i32(c, tcnt, nt, 0);
@@ -662,6 +940,7 @@ void p2d_deposit(ecsprt_hip_ctx *c) {
     std::swap(c->d_pos2, c->d_pos2b);
     std::swap(c->d_vel2, c->d_vel2b);
     std::swap(c->d_id2, c->d_id2b);
+    c->tiled2 = true;
   } else {
     // Grids of more than 4096 tiles: global fixed-point atomics per ion, the scale bounded
     // by the fullest cell.
@@ -685,28 +964,16 @@ void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
   if (Nc == 0 || c->p.mn <= 0.0) return;
   const CollideConsts cc = collide_consts_of(c->p);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
-  if (!c->d_perm2 || c->perm_cap2 < c->np) {  // the collision list shares the tile-list buffer
-    (void)hipFree(c->d_perm2);
-    c->d_perm2 = nullptr;
-    c->perm_cap2 = 0;
-    if (hipMalloc(&c->d_perm2, sizeof(int) * std::max(c->np, 1L)) != hipSuccess)
-      throw Error{ECSPRT_ERR_NOMEM, "two-dimensional collision list"};
-    c->perm_cap2 = c->np;
-  }
+  ensure_perm(c, c->np);  // the collision list shares the tile-list buffer
   const long nseg = (c->np + COLL_CHUNK2 - 1) / COLL_CHUNK2;
-  if (!c->d_seg2 || c->seg_cap2 < nseg) {
-    (void)hipFree(c->d_seg2);
-    c->d_seg2 = nullptr;
-    c->seg_cap2 = 0;
-    ECS_HIP(hipMalloc(&c->d_seg2, sizeof(int) * nseg));
-    c->seg_cap2 = nseg;
-  }
+  ensure_segments(c, nseg);
   auto pass = [&](double lam, uint32_t step, int write) {
     zero_ctr(c);
     const PoissonTable pt = poisson_table(lam);
     k_collide2d_select<<<nseg, 256, 0, c->stream>>>(c->np, pt, k0, k1, step, c->d_id2, c->d_perm2, c->d_seg2);
-    k_collide2d_apply<<<nseg, 256, 0, c->stream>>>(c->d_perm2, c->d_seg2, pt, cc, k0, k1, step, write, c->d_id2,
-                                                   c->d_vel2, c->d_ctr);
+    const int G = apply_segs(nseg);
+    k_collide2d_apply<<<(nseg + G - 1) / G, 256, 0, c->stream>>>(nseg, G, c->d_perm2, c->d_seg2, pt, cc, k0, k1, step,
+                                                                 write, c->d_id2, c->d_vel2, c->d_ctr);
     ECS_HIP(hipGetLastError());
     read_ctr(c);
   };

@@ -182,33 +182,44 @@ def test_step2d_philox_conserves_and_collides():
 
 def test_step2d_philox_independent_of_storage_order():
     """Philox draws are keyed by the index an ion was set with, so a step gives the same
-    ions whether they start in set order or in the tile order a deposit leaves them in."""
-    say("2-D steps, storage order")
-    N = 24
+    ions whether they start in set order or in the tile order a deposit leaves them in,
+    and the fused step (push, collisions and drift over tile-ordered ions) gives the same
+    ions as the stages called one by one."""
+    say("2-D steps, storage order and fused vs staged")
+    N = 40
     cli = cli2d(N, nui=0.05, E0=0.05)
     runs, phi0 = [], None
-    for sorted_first in (True, False):
+    for how in ("step", "unsorted", "staged"):
         p = pic.setup_context(**dict(cli, ndim=2))
         ctx = pic.Context(p, pic.default_options(collide_mode=_lib.COLLIDE_PHILOX, ksp_rtol=1e-10,
                                                  operator_mode=_lib.OPERATOR_REBUILD))
         ctx.init_particles(32 * N * N, 5, 1e-2, 2)
-        if sorted_first:
+        if how == "unsorted":
+            ctx.set_potential(phi0)
+        else:
             ctx.collect()
             ctx.compute_potential()
             phi0 = ctx.get_potential()
-        else:
-            ctx.set_potential(phi0)
         out = []
-        for step in range(2):
-            ctx.step(1.0)
+        for step in range(3):
+            if how == "staged":
+                ctx.push(1.0)
+                ctx.collide(1.0)
+                ctx.update_positions(1.0)
+                ctx.collect()
+                ctx.compute_potential()
+            else:
+                ctx.step(1.0)
             out.append((*ctx.get_particles(), ctx.get_moments(), ctx.collision_stats()))
         runs.append(out)
         ctx.close()
-    for (pa, va, ma, ca), (pb, vb, mb, cb) in zip(*runs):
-        np.testing.assert_array_equal(pa, pb)
-        np.testing.assert_array_equal(va, vb)
-        np.testing.assert_array_equal(ma, mb)
-        assert ca == cb
+    for other in runs[1:]:
+        for (pa, va, ma, ca), (pb, vb, mb, cb) in zip(runs[0], other):
+            np.testing.assert_array_equal(pa, pb)
+            np.testing.assert_array_equal(va, vb)
+            np.testing.assert_array_equal(ma, mb)
+            assert ca == cb
+            assert ca[0] > 0 and ca[1] > 0
 
 
 @pytest.mark.parametrize("dist", ["normal", "sobol", "sinusoidal"])
