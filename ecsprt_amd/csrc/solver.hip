@@ -209,6 +209,52 @@ __device__ inline T relax_planes(const StencilConsts &s, const T *d0, const T *d
   return (bq - off) * smooth_rcp(c[10]);
 }
 
+// The sweep of k_sweep on a single plane (Nz = 1, the two-dimensional run): the
+// periodic z neighbours of a node are the node itself, so one plane of density and
+// potential with an xy halo of 2 is staged, and each relaxation is relax_planes with
+// the same plane below, at and above (the arithmetic of the box sweep on that grid).
+template <typename T, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_sweep2d(StencilConsts s, const T *__restrict__ dn,
+                                                    const T *__restrict__ xin, const T *__restrict__ b,
+                                                    T *__restrict__ xout, int first) {
+  constexpr int HX = TX + 4, HY = TY + 4, H2 = HX * HY;
+  constexpr int RX = TX + 2, RY = TY + 2, R2 = RX * RY;
+  __shared__ T sx[H2], sn[H2], sd[H2];
+  const int Nx = s.Nx, Ny = s.Ny;
+  const int ntx = (Nx + TX - 1) / TX;
+  const int i0 = (blockIdx.x % ntx) * TX, j0 = (blockIdx.x / ntx) * TY;
+  const int ex = min(TX, Nx - i0), ey = min(TY, Ny - j0);
+  for (int q = threadIdx.x; q < H2; q += blockDim.x) {
+    const int li = q % HX - 2, lj = q / HX - 2;
+    if (li > ex + 1 || lj > ey + 1) continue;
+    const long g = (long)wrap1(j0 + lj, Ny) * Nx + wrap1(i0 + li, Nx);
+    sd[q] = dn[g];
+    sx[q] = xin ? xin[g] : T(0);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < R2; q += blockDim.x) {
+    const int li = q % RX - 1, lj = q / RX - 1;
+    if (li > ex || lj > ey) continue;
+    const int gi = wrap1(i0 + li, Nx), gj = wrap1(j0 + lj, Ny);
+    const int p = (lj + 2) * HX + (li + 2);
+    T v = sx[p];
+    if (((gi + gj) & 1) == first)
+      v = relax_planes<T, HX, HX>(s, sd, sd, sd, p, sx, sx, sx, p, b[(long)gj * Nx + gi], v);
+    sn[p] = v;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < TX * TY; q += blockDim.x) {
+    const int li = q % TX, lj = q / TX;
+    if (li >= ex || lj >= ey) continue;
+    const int gi = i0 + li, gj = j0 + lj;
+    const int p = (lj + 2) * HX + (li + 2);
+    const long g = (long)gj * Nx + gi;
+    T v = sn[p];
+    if (((gi + gj) & 1) != first) v = relax_planes<T, HX, HX>(s, sd, sd, sd, p, sn, sn, sn, p, b[g], v);
+    xout[g] = v;
+  }
+}
+
 // One red-black Gauss-Seidel sweep (colour `first`, then the other; colour of node
 // (i,j,k) = (i+j+k)&1) from xin (zero when null) into xout, marching along z.  A
 // workgroup owns a TX x TY column of nodes over KZ planes.  Planes of the old potential
@@ -317,6 +363,7 @@ __global__ void __launch_bounds__(TX * TY) k_sweep_z(StencilConsts s, const T *_
 }
 
 constexpr int SW_TX = 16, SW_TY = 8, SW_TZ = 4;  // box of the one-shot sweep
+constexpr int S2_TX = 32, S2_TY = 16;             // tile of the single-plane sweep
 constexpr int SZ_TX = 32, SZ_TY = 16;            // xy tile of the marching sweep
 constexpr long SZ_MIN_POINTS = 1L << 21;         // levels at least this large march
 constexpr int DENSE_MAX = 1024;                // largest coarsest level solved by a dense inverse
@@ -984,7 +1031,10 @@ static int sweep_kz(int Nx, int Ny, int Nz) {
 // points, slower from 2^20 points down).
 template <typename T>
 static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first, int mode) {
-  if (mode == 2 || (mode == 0 && (long)L.s.Nx * L.s.Ny * L.s.Nz >= SZ_MIN_POINTS)) {
+  if (L.s.Nz == 1) {
+    const int nt = ((L.s.Nx + S2_TX - 1) / S2_TX) * ((L.s.Ny + S2_TY - 1) / S2_TY);
+    k_sweep2d<T, S2_TX, S2_TY><<<nt, S2_TX * S2_TY, 0, st>>>(L.s, L.dn, xin, b, xout, first);
+  } else if (mode == 2 || (mode == 0 && (long)L.s.Nx * L.s.Ny * L.s.Nz >= SZ_MIN_POINTS)) {
     const int kz = sweep_kz(L.s.Nx, L.s.Ny, L.s.Nz);
     const int nt = ((L.s.Nx + SZ_TX - 1) / SZ_TX) * ((L.s.Ny + SZ_TY - 1) / SZ_TY) * ((L.s.Nz + kz - 1) / kz);
     k_sweep_z<T, SZ_TX, SZ_TY><<<nt, SZ_TX * SZ_TY, 0, st>>>(L.s, L.dn, xin, b, xout, first, kz);
