@@ -368,12 +368,12 @@ def bench_ensemble(args, pic, _lib):
     print(json.dumps(out))
 
 
-# Two-dimensional run (ndim = 2): bytes per ion of the staged particle pass as built
-# (push: read x, read and write v; drift: read x and v, write x; cell counts: read x;
-# deposit: read x and v; 16 B per two-component vector), and per node (fold: read three
-# 8-byte sums and the 32-byte overflow moments, write 32 bytes of moments).
-BYTES_PER_PARTICLE_2D = 48 + 48 + 16 + 32
-BYTES_PER_NODE_2D = 24 + 32 + 32
+# Algorithmic bytes of the two-dimensional particle stages, as for the three-dimensional
+# step: per ion its position and velocity read and written once (two 16-byte vectors,
+# 64 B); per node the potential read (8 B), the fold's three 8-byte sums and 32-byte
+# overflow moments read and 32 bytes of moments written.
+BYTES_PER_PARTICLE_2D = 64
+BYTES_PER_NODE_2D = 8 + 24 + 32 + 32
 
 
 def cpu_baseline_2d(p2, target_s=15.0):
@@ -406,8 +406,9 @@ def cpu_baseline_2d(p2, target_s=15.0):
 
 def bench_2d(args, pic, _lib):
     """The two-dimensional run: 1024^2 cells, 100 ions per cell, FP64, Philox
-    collisions, staged stages (BorisMoverBz2D, ScatterElastic, UpdatePositions,
-    Collect2DFluidMoments, ComputeFullRHS2D + GMRES/GMG on ComputeFullLHS2D)."""
+    collisions: BorisMoverBz2D, ScatterElastic and UpdatePositions in one pass over
+    the tile-ordered ions, Collect2DFluidMoments (binned by tile, which re-orders the
+    ions), ComputeFullRHS2D + GMRES/GMG on ComputeFullLHS2D."""
     import oracle
     from ecsprt_amd import configs
     N = args.grid or 1024
@@ -451,7 +452,7 @@ def bench_2d(args, pic, _lib):
         "stage_ms_mean": {"particle_stages": float(st[0]), "rhs": float(st[4]), "solve": float(st[5])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_push2d+k_collide2d+k_drift2d+k_cell_counts2d+k_deposit2d+k_fold2d",
+                     "kernel": "k_push2d_tiles+k_bin_hist2d+k_bin_scatter2d+k_deposit2d_tiles+k_fold2d",
                      "algorithmic_bytes_per_launch": alg},
     }
     if not args.no_cpu:
