@@ -33,6 +33,12 @@ namespace {
 // Periodic index of base + d for a base already in [0, n) and |d| <= 2 < n.
 __device__ inline int near_wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// wrap_index without the integer division for an index already in [0, n] (every
+// in-domain coordinate, a coordinate at L giving n).
+__device__ inline int wrap_fast(int i, int n) {
+  return (unsigned)i < (unsigned)n ? i : (i == n ? 0 : wrap_index(i, n));
+}
+
 // P(j, i) is the potential at node (i, j), always with i - ixl and j - iyl in [-1, 2].
 template <class Acc>
 __device__ inline void diff_vec2d_at(const Acc &P, int Nx, int Ny, double x0, double y0, double f[2]) {
@@ -77,7 +83,7 @@ __device__ inline void diff_vec2d_at(const Acc &P, int Nx, int Ny, double x0, do
 __device__ inline void diff_vec2d(const double *__restrict__ F, int Nx, int Ny, double x0, double y0, double f[2]) {
   const int ixl = (int)x0, iyl = (int)y0;
   // Every node read is (ixl + a, iyl + b) with a, b in [-1, 2]: one modulo per axis.
-  const int bx = wrap_index(ixl, Nx), by = wrap_index(iyl, Ny);
+  const int bx = wrap_fast(ixl, Nx), by = wrap_fast(iyl, Ny);
   auto P = [&](int j, int i) { return F[(long)near_wrap(by + (j - iyl), Ny) * Nx + near_wrap(bx + (i - ixl), Nx)]; };
   diff_vec2d_at(P, Nx, Ny, x0, y0, f);
 }
@@ -172,7 +178,7 @@ __global__ void k_deposit2d(long np, int Nx, int Ny, PushConsts k, double inv_vm
     const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
     const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
     const double w4[4] = {wyh * wxh, wyl * wxh, wyh * wxl, wyl * wxl};
-    const int bx = wrap_index(ixl, Nx), by = wrap_index(iyl, Ny);
+    const int bx = wrap_fast(ixl, Nx), by = wrap_fast(iyl, Ny);
     const int bx1 = near_wrap(bx + 1, Nx), by1 = near_wrap(by + 1, Ny);
     const int ci[4] = {bx1, bx1, bx, bx}, cj[4] = {by1, by, by1, by};
     const double vx = vel[2 * p], vy = vel[2 * p + 1];
@@ -220,10 +226,14 @@ struct Bin2 {
 };
 
 // The ion's wrapped cell (ix, iy) from its cell coordinates.
+__device__ inline void cell_of_xy(const PushConsts &k, const Bin2 &b, double px, double py, int &ix, int &iy) {
+  ix = wrap_fast((int)cell_coord(px, k.dx, k.inv[0], k.pow2), b.Nx);
+  iy = wrap_fast((int)cell_coord(py, k.dy, k.inv[1], k.pow2), b.Ny);
+}
+
 __device__ inline void cell_of(const PushConsts &k, const Bin2 &b, const double *__restrict__ pos, long p, int &ix,
                                int &iy) {
-  ix = wrap_index((int)cell_coord(pos[2 * p], k.dx, k.inv[0], k.pow2), b.Nx);
-  iy = wrap_index((int)cell_coord(pos[2 * p + 1], k.dy, k.inv[1], k.pow2), b.Ny);
+  cell_of_xy(k, b, pos[2 * p], pos[2 * p + 1], ix, iy);
 }
 
 // Per-tile counts (LDS histogram per workgroup, added to tile_counts once per non-empty
@@ -249,7 +259,7 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_bin_hist2d(long np, long chunk
 // Exclusive scan of the tile counts into off[] and fill[] (one workgroup), and the
 // largest tile count into *maxc: a node lies in at most four tile regions, so its sums
 // are bounded by four times it (the fixed-point scale is chosen from that bound).
-__global__ void __launch_bounds__(BIN_THREADS2) k_scan_tiles2d(int nt, const int *__restrict__ counts,
+__global__ void __launch_bounds__(BIN_THREADS2) k_scan_tiles2d(int nt, const int *counts, int *counts_out,
                                                               int *__restrict__ off, int *__restrict__ fill,
                                                               int *__restrict__ maxc) {
   __shared__ long long part[BIN_THREADS2];
@@ -279,9 +289,11 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_scan_tiles2d(int nt, const int
   __syncthreads();
   long long a = part[threadIdx.x];
   for (int q = q0; q < q1; ++q) {
+    const int n = counts[q];
     off[q] = (int)a;
     fill[q] = (int)a;
-    a += counts[q];
+    counts_out[q] = n;  // the counts the deposit and the next push read
+    a += n;
   }
 }
 
@@ -309,7 +321,8 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_bin_scatter2d(long np, long ch
   for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
     int ix, iy;
     cell_of(k, b, pos, p, ix, iy);
-    perm[atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1)] = (int)p;
+    const int slot = atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1);
+    if (slot < np) perm[slot] = (int)p;  // always, when the counts are those of these positions
   }
 }
 
@@ -354,7 +367,7 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_deposit2d_tiles(Bin2 b, PushCo
     vel_out[2 * o + 1] = vy;
     id_out[o] = id[p];
     const double ux = vx * inv_vmax, uy = vy * inv_vmax;
-    const int la = wrap_index(ixl, b.Nx) - i0, lb = wrap_index(iyl, b.Ny) - j0;  // in [0, T2)
+    const int la = wrap_fast(ixl, b.Nx) - i0, lb = wrap_fast(iyl, b.Ny) - j0;  // in [0, T2)
     const int na[4] = {la + 1, la + 1, la, la}, nb[4] = {lb + 1, lb, lb + 1, lb};
     if (fabs(ux) < 1.0 && fabs(uy) < 1.0) {
 #pragma unroll
@@ -488,10 +501,10 @@ constexpr int HALO2 = T2 + 3;  // a tile's nodes -1 .. T2 + 1 per axis
 
 constexpr int QCAP2 = 512;  // LDS collision queue of k_push2d_tiles: two workgroups' worth
 
-// UpdatePositions (src/positions.c:455-459) and the wrap of one ion; true if it is
-// still outside [r0, r1] (or not finite).
+// UpdatePositions (src/positions.c:455-459) and the wrap of one ion, stored and
+// returned in q; true if it is still outside [r0, r1] (or not finite).
 __device__ inline bool drift_one2d(const DriftConsts &d, long p, double px, double py, double vx, double vy,
-                                   double *__restrict__ pos) {
+                                   double *__restrict__ pos, double q[2]) {
   const double r0v[2] = {px, py}, v[2] = {vx, vy};
   bool out = false;
 #pragma unroll
@@ -500,6 +513,7 @@ __device__ inline bool drift_one2d(const DriftConsts &d, long p, double px, doub
     if (r < d.r0[a]) r += d.L[a];
     if (r >= d.r1[a]) r -= d.L[a];
     pos[2 * p + a] = r;
+    q[a] = r;
     out |= !(r >= d.r0[a] && r <= d.r1[a]);
   }
   return out;
@@ -512,15 +526,18 @@ __device__ inline bool drift_one2d(const DriftConsts &d, long p, double px, doub
 // collision attempt count and moves it (UpdatePositions and the wrap): an ion with no
 // attempt at once, the others through an LDS queue that the whole workgroup works off
 // 256 at a time (attempts, then the move), so collisions run on full waves while the
-// ions' lines are still in the cache.
-__global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const int *__restrict__ toff,
+// ions' lines are still in the cache.  The moved ions are counted per tile into
+// tnext (the deposit's binning then needs no counting pass): the tile and its eight
+// neighbours in LDS, any other tile with a global atomic.
+__global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, const int *__restrict__ toff,
                                                       const int *__restrict__ tcnt, const double *__restrict__ phi,
                                                       PushConsts k, int fused, int collide, DriftConsts d,
                                                       PoissonTable pt, CollideConsts cc, uint32_t key0,
                                                       uint32_t key1, uint32_t step, const int *__restrict__ id,
                                                       double *__restrict__ pos, double *__restrict__ vel,
-                                                      unsigned long long *ctr) {
+                                                      int *__restrict__ tnext, unsigned long long *ctr) {
   __shared__ double s_phi[HALO2 * HALO2];
+  __shared__ int s_h[9];
   __shared__ double s_qd[QCAP2][4];  // position, then pushed velocity
   __shared__ int s_qp[QCAP2], s_qid[QCAP2];
   __shared__ float s_cdf[24];
@@ -533,14 +550,38 @@ __global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const i
     s_phi[q] = phi[(long)wrap_index(j0 - 1 + ly, b.Ny) * b.Nx + wrap_index(i0 - 1 + lx, b.Nx)];
   }
   if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
+  if (threadIdx.x < 9) s_h[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_qn = 0;
   __syncthreads();
+  const int tx = t % b.ntx, ty = t / b.ntx;
+  // Tile of a moved ion: its own (bin 4, one LDS atomic per wave), a neighbour's bin, or
+  // a global atomic.  Called by every lane of the wave; an ion still outside the domain
+  // (the step fails) is not counted.
+  auto count_tile = [&](bool moved, double qx, double qy) {
+    int bin = -1, tile = 0;
+    if (moved) {
+      int ix, iy;
+      cell_of_xy(k, b, qx, qy, ix, iy);
+      const int ux = ix / T2, uy = iy / T2;
+      tile = uy * b.ntx + ux;
+      const int dx = tile_step(ux - tx, b.ntx), dy = tile_step(uy - ty, b.nty);
+      bin = (dx < 2 && dy < 2) ? (dy + 1) * 3 + (dx + 1) : 9;
+    }
+    const unsigned long long own = __ballot(bin == 4);
+    if (own && bin == 4 &&
+        (int)__builtin_amdgcn_mbcnt_hi((unsigned)(own >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)own, 0u)) == 0)
+      atomicAdd(&s_h[4], (int)__popcll(own));
+    if (bin >= 0 && bin < 9 && bin != 4) atomicAdd(&s_h[bin], 1);
+    if (bin == 9) atomicAdd(&tnext[tile], 1);
+  };
   const int n = tcnt[t], m = (n + split - 1) / split;
   const long beg = (long)toff[t] + (long)sp * m;
   const int cnt = max(0, min(m, n - sp * m));
   unsigned natt = 0, nsucc = 0, nfail = 0, nl = 0;
   // Entries from .. from + nw - 1 of the queue, one per thread.
   auto work_off = [&](int from, int nw) {
+    bool moved = false;
+    double qx = 0.0, qy = 0.0;
     if ((int)threadIdx.x < nw) {
       const int e = from + threadIdx.x;
       const long p = s_qp[e];
@@ -551,12 +592,19 @@ __global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const i
       collide_attempts<double, 2>(cc, key0, key1, step, pid, 0, ka, vx, vy, vz, natt, nsucc, nfail);
       vel[2 * p] = vx;
       vel[2 * p + 1] = vy;
-      nl += drift_one2d(d, p, px, py, vx, vy, pos) ? 1u : 0u;
+      double qn[2];
+      const bool out = drift_one2d(d, p, px, py, vx, vy, pos, qn);
+      nl += out ? 1u : 0u;
+      moved = !out;
+      qx = qn[0];
+      qy = qn[1];
     }
+    count_tile(moved, qx, qy);
   };
   for (int q0 = 0; q0 < cnt; q0 += blockDim.x) {  // uniform per workgroup
     const int q = q0 + threadIdx.x;
-    bool listed = false;
+    bool listed = false, moved = false;
+    double qx = 0.0, qy = 0.0;
     long p = 0;
     int pid = 0;
     double px = 0, py = 0, vxp = 0, vyp = 0;
@@ -567,7 +615,7 @@ __global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const i
       const double x = cell_coord(px, k.dx, k.inv[0], k.pow2);
       const double y = cell_coord(py, k.dy, k.inv[1], k.pow2);
       const int ixl = (int)x, iyl = (int)y;
-      const int lx = wrap_index(ixl, b.Nx) - i0, ly = wrap_index(iyl, b.Ny) - j0;
+      const int lx = wrap_fast(ixl, b.Nx) - i0, ly = wrap_fast(iyl, b.Ny) - j0;
       double E[2];
       if (lx >= 0 && lx < T2 && ly >= 0 && ly < T2) {
         auto P = [&](int j, int i) { return s_phi[(ly + 1 + (j - iyl)) * HALO2 + lx + 1 + (i - ixl)]; };
@@ -587,9 +635,17 @@ __global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const i
       if (!listed) {
         vel[2 * p] = vxp;
         vel[2 * p + 1] = vyp;
-        if (fused) nl += drift_one2d(d, p, px, py, vxp, vyp, pos) ? 1u : 0u;
+        if (fused) {
+          double qn[2];
+          const bool out = drift_one2d(d, p, px, py, vxp, vyp, pos, qn);
+          nl += out ? 1u : 0u;
+          moved = !out;
+          qx = qn[0];
+          qy = qn[1];
+        }
       }
     }
+    if (fused) count_tile(moved, qx, qy);
     if (fused && collide) {
       const unsigned long long mk = __ballot(listed);
       if (mk) {
@@ -637,6 +693,10 @@ __global__ void __launch_bounds__(256) k_push2d_tiles(Bin2 b, int split, const i
       const unsigned long long v = s_red[threadIdx.x][0] + s_red[threadIdx.x][1] + s_red[threadIdx.x][2] +
                                    s_red[threadIdx.x][3];
       if (v) atomicAdd(&ctr[threadIdx.x < 3 ? 3 + threadIdx.x : 2], v);
+    } else if (threadIdx.x >= 64 && threadIdx.x < 64 + 9) {
+      const int q = threadIdx.x - 64, h = s_h[q];
+      const int dx = q % 3 - 1, dy = q / 3 - 1;
+      if (h) atomicAdd(&tnext[wrap_index(ty + dy, b.nty) * b.ntx + wrap_index(tx + dx, b.ntx)], h);
     }
   }
 }
@@ -793,7 +853,7 @@ void p2d_set(ecsprt_hip_ctx *c, long np, const double *pos, const double *vel, h
   k_iota2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2);
   ECS_HIP(hipGetLastError());
   c->np = np;
-  c->tiled2 = false;
+  c->tiled2 = c->counted2 = false;
   c->coll_lambda = -1.0;
   ECS_HIP(hipStreamSynchronize(c->stream));
 }
@@ -823,7 +883,7 @@ void p2d_init_device(ecsprt_hip_ctx *c, long np, uint64_t seed, double amp, int 
   k_iota2d<<<grid2(np), 256, 0, c->stream>>>(np, c->d_id2);
   ECS_HIP(hipGetLastError());
   c->np = np;
-  c->tiled2 = false;
+  c->tiled2 = c->counted2 = false;
   c->coll_lambda = -1.0;
   ECS_HIP(hipStreamSynchronize(c->stream));
 }
@@ -836,7 +896,7 @@ void p2d_push(ecsprt_hip_ctx *c, double dt) {
     k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 0, 0,
                                                       drift_consts(c->p, dt), poisson_table(0.0),
                                                       collide_consts_of(c->p), 0, 0, 0, c->d_id2, c->d_pos2,
-                                                      c->d_vel2, nullptr);
+                                                      c->d_vel2, nullptr, nullptr);
   } else {
     k_push2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->d_phi, c->p.Nx, c->p.Ny, push_consts(c->p, dt),
                                                   c->d_pos2, c->d_vel2);
@@ -850,6 +910,7 @@ void p2d_push(ecsprt_hip_ctx *c, double dt) {
 // in tile order, collisions are pre-drawn or the Philox attempt rate has not been set
 // yet (its dry runs are done by p2d_collide_philox).
 bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
+  c->counted2 = false;
   if (!c->tiled2) return false;
   const bool philox = c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX;
   if (c->p.mn > 0.0 && c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) return false;
@@ -865,11 +926,13 @@ bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
   int *toff = c->d_tiles2 + MAX_TILES2, *tcnt = c->d_tiles2;
   const PoissonTable pt = poisson_table(collide ? c->coll_lambda : 0.0);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32), step = (uint32_t)c->step_index;
+  int *tnext = c->d_tiles2 + 3 * MAX_TILES2;
   zero_ctr(c);
+  fill_i32(c, tnext, nt, 0);
   k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 1,
                                                     collide ? 1 : 0, drift_consts(c->p, dt), pt,
                                                     collide_consts_of(c->p), k0, k1, step, c->d_id2, c->d_pos2,
-                                                    c->d_vel2, c->d_ctr);
+                                                    c->d_vel2, tnext, c->d_ctr);
   ECS_HIP(hipGetLastError());
   c->tiled2 = false;
   read_ctr(c);
@@ -883,13 +946,14 @@ bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
   }
   ECS_CHECK(c->h_ctr[2] == 0, ECSPRT_ERR_COUNT_CHANGED,
             std::to_string(c->h_ctr[2]) + " ions left the periodic domain (not finite or moved more than L)");
+  c->counted2 = true;
   return true;
 }
 
 // Drift and wrap; a lost ion aborts as a changed ion count does in the reference
 // (src/boundaries.c:101-109).
 void p2d_drift(ecsprt_hip_ctx *c, double dt) {
-  c->tiled2 = false;
+  c->tiled2 = c->counted2 = false;
   zero_ctr(c);
   k_drift2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, drift_consts(c->p, dt), c->d_pos2, c->d_vel2, c->d_ctr + 2);
   ECS_HIP(hipGetLastError());
@@ -899,7 +963,7 @@ void p2d_drift(ecsprt_hip_ctx *c, double dt) {
 }
 
 void p2d_apply_bc(ecsprt_hip_ctx *c, long np, double *pos) {
-  c->tiled2 = false;
+  c->tiled2 = c->counted2 = false;
   k_apply_bc2d<<<grid2(np), 256, 0, c->stream>>>(np, drift_consts(c->p, 0.0), pos);
   ECS_HIP(hipGetLastError());
 }
@@ -922,16 +986,19 @@ void p2d_deposit(ecsprt_hip_ctx *c) {
   fill_f64(c, reinterpret_cast<double *>(c->d_acc2), 3 * n, 0.0);  // all-zero bits
   fill_f64(c, c->d_ovf, 4 * n, 0.0);
   const long np = c->np;
-  c->tiled2 = false;
+  const bool counted = c->counted2;  // tile counts of these positions left by the fused push
+  c->tiled2 = c->counted2 = false;
   if (nt <= MAX_TILES2) {
     ensure_perm(c, np);
-    if (!c->d_tiles2) ECS_HIP(hipMalloc(&c->d_tiles2, sizeof(int) * 3 * MAX_TILES2));
-    int *tcnt = c->d_tiles2, *toff = tcnt + MAX_TILES2, *tfill = toff + MAX_TILES2;
-    fill_i32(c, tcnt, nt, 0);
+    if (!c->d_tiles2) ECS_HIP(hipMalloc(&c->d_tiles2, sizeof(int) * 4 * MAX_TILES2));
+    int *tcnt = c->d_tiles2, *toff = tcnt + MAX_TILES2, *tfill = toff + MAX_TILES2, *tnext = tfill + MAX_TILES2;
     const long nwg = std::max(1L, std::min(4096L, (np + 65535) / 65536));
     const long chunk = (np + nwg - 1) / nwg;
-    k_bin_hist2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tcnt);
-    k_scan_tiles2d<<<1, BIN_THREADS2, 0, c->stream>>>(nt, tcnt, toff, tfill, maxcell);
+    if (!counted) {
+      fill_i32(c, tcnt, nt, 0);
+      k_bin_hist2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tcnt);
+    }
+    k_scan_tiles2d<<<1, BIN_THREADS2, 0, c->stream>>>(nt, counted ? tnext : tcnt, tcnt, toff, tfill, maxcell);
     k_bin_scatter2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tfill, c->d_perm2);
     k_deposit2d_tiles<<<nt, BIN_THREADS2, 0, c->stream>>>(b, k, 1.0 / vmax, maxcell, toff, tcnt, c->d_perm2,
                                                           c->d_pos2, c->d_vel2, c->d_id2, c->d_pos2b, c->d_vel2b,
