@@ -180,20 +180,25 @@ def test_step2d_philox_conserves_and_collides():
     ctx.close()
 
 
-def test_step2d_philox_independent_of_storage_order():
+@pytest.mark.parametrize("N,Ny,dx,viT,collide", [(40, 40, 1.0, 1.0, True), (48, 20, 0.5, 1.0, True),
+                                                  (70, 36, 0.3, 3.0, False)])
+def test_step2d_independent_of_storage_order(N, Ny, dx, viT, collide):
     """Philox draws are keyed by the index an ion was set with, so a step gives the same
     ions whether they start in set order or in the tile order a deposit leaves them in,
-    and the fused step (push, collisions and drift over tile-ordered ions) gives the same
-    ions as the stages called one by one."""
-    say("2-D steps, storage order and fused vs staged")
-    N = 40
-    cli = cli2d(N, nui=0.05, E0=0.05)
+    and the fused step (push, collisions, drift and the count of the ions' new tiles over
+    tile-ordered ions) gives the same ions and moments as the stages called one by one.
+    Cases: one tile row (Ny < 32), partial tiles and cells of 0.3, hot ions of which some
+    move further than a neighbouring tile."""
+    say(f"2-D steps, storage order and fused vs staged N={N} Ny={Ny} dx={dx} viT={viT} collide={collide}")
+    cli = cli2d(N, Ny, nui=0.05 if collide else 0.0, E0=0.05, viT=viT)
+    cli.update(dx=dx)
     runs, phi0 = [], None
     for how in ("step", "unsorted", "staged"):
         p = pic.setup_context(**dict(cli, ndim=2))
-        ctx = pic.Context(p, pic.default_options(collide_mode=_lib.COLLIDE_PHILOX, ksp_rtol=1e-10,
+        mode = _lib.COLLIDE_PHILOX if collide else _lib.COLLIDE_OFF
+        ctx = pic.Context(p, pic.default_options(collide_mode=mode, ksp_rtol=1e-10,
                                                  operator_mode=_lib.OPERATOR_REBUILD))
-        ctx.init_particles(32 * N * N, 5, 1e-2, 2)
+        ctx.init_particles(32 * N * Ny, 5, 1e-2, 2)
         if how == "unsorted":
             ctx.set_potential(phi0)
         else:
@@ -219,7 +224,7 @@ def test_step2d_philox_independent_of_storage_order():
             np.testing.assert_array_equal(va, vb)
             np.testing.assert_array_equal(ma, mb)
             assert ca == cb
-            assert ca[0] > 0 and ca[1] > 0
+            assert (ca[0] > 0 and ca[1] > 0) == collide
 
 
 @pytest.mark.parametrize("dist", ["normal", "sobol", "sinusoidal"])
