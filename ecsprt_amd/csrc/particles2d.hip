@@ -236,6 +236,24 @@ __device__ inline void cell_of(const PushConsts &k, const Bin2 &b, const double 
   cell_of_xy(k, b, pos[2 * p], pos[2 * p + 1], ix, iy);
 }
 
+// LDS atomicAdd(&h[key], 1) of every active lane, returning each lane's old value;
+// the lanes whose key is that of the first active lane (most of them: ions are stored
+// in tile order) take theirs from one atomic, numbered by lane.
+__device__ inline int wave_count(int *h, int key) {
+  const int k0 = __builtin_amdgcn_readfirstlane(key);
+  const unsigned long long same = __ballot(key == k0);
+  int r = 0;
+  if (key == k0) {
+    const int below =
+        (int)__builtin_amdgcn_mbcnt_hi((unsigned)(same >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)same, 0u));
+    if (below == 0) r = atomicAdd(&h[k0], (int)__popcll(same));
+    r = __builtin_amdgcn_readlane(r, __ffsll((long long)same) - 1) + below;
+  } else {
+    r = atomicAdd(&h[key], 1);
+  }
+  return r;
+}
+
 // Per-tile counts (LDS histogram per workgroup, added to tile_counts once per non-empty
 // bin).  Workgroup w takes ions [w*chunk, (w+1)*chunk).
 __global__ void __launch_bounds__(BIN_THREADS2) k_bin_hist2d(long np, long chunk, Bin2 b, PushConsts k,
@@ -249,7 +267,7 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_bin_hist2d(long np, long chunk
   for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
     int ix, iy;
     cell_of(k, b, pos, p, ix, iy);
-    atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1);
+    wave_count(h, (iy / T2) * b.ntx + ix / T2);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < nt; q += blockDim.x)
@@ -310,7 +328,7 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_bin_scatter2d(long np, long ch
   for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
     int ix, iy;
     cell_of(k, b, pos, p, ix, iy);
-    atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1);
+    wave_count(h, (iy / T2) * b.ntx + ix / T2);
   }
   __syncthreads();
   for (int q = threadIdx.x; q < nt; q += blockDim.x) {
@@ -321,7 +339,7 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_bin_scatter2d(long np, long ch
   for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
     int ix, iy;
     cell_of(k, b, pos, p, ix, iy);
-    const int slot = atomicAdd(&h[(iy / T2) * b.ntx + ix / T2], 1);
+    const int slot = wave_count(h, (iy / T2) * b.ntx + ix / T2);
     if (slot < np) perm[slot] = (int)p;  // always, when the counts are those of these positions
   }
 }
