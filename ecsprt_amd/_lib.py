@@ -61,7 +61,7 @@ class Options(ctypes.Structure):
                 ("ksp_max_it", ctypes.c_int), ("ksp_rtol", ctypes.c_double), ("ksp_atol", ctypes.c_double),
                 ("mg_pre", ctypes.c_int), ("mg_post", ctypes.c_int), ("mg_coarse_sweeps", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("timing", ctypes.c_int),
-                ("particle_precision", ctypes.c_int), ("mg_sweep", ctypes.c_int)]
+                ("particle_precision", ctypes.c_int), ("mg_sweep", ctypes.c_int), ("gmres_refine", ctypes.c_int)]
 
 
 _P = ctypes.c_void_p

@@ -170,6 +170,7 @@ int ecsprt_hip_default_options(ecsprt_options *o) {
   o->timing = 0;
   o->particle_precision = ECSPRT_PARTICLES_FP64;
   o->mg_sweep = 0;
+  o->gmres_refine = 0;
   return ECSPRT_OK;
 }
 

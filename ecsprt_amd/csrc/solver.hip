@@ -1238,6 +1238,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
   Level<T> &L0 = mg.L[0];
   const long n = L0.n;
   const int m = g.m;
+  const bool refine = c->o.gmres_refine != 0;
   const Launch3 g0 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1);
   int it = 0;
@@ -1264,10 +1265,20 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       precond<T>(c, mg, g.V[j], g.Z[j]);
       k_apply<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, g.Z[j], g.w);
       ECS_HIP(hipGetLastError());
-      // CGS2: h = V^T w; w -= V h; h2 = V^T w; w -= V h2; |w|^2.  Up to 8 basis
-      // vectors the subtractions carry the following products in the same pass.
+      // Classical Gram-Schmidt, h = V^T w; w -= V h; |w|^2 (PETSc's default), or with
+      // gmres_refine CGS2: h2 = V^T w; w -= V h2 before the norm.  Up to 8 basis vectors
+      // the subtractions carry the following products in the same pass.
       dots<T>(c, g, j + 1, g.dV, g.w, g.dh, 0);
-      if (j + 1 <= 8) {
+      if (!refine) {
+        if (j + 1 <= 8) {
+          const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
+          k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh, g.w, n, g.partial, 1);
+          k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (j + 1), 0);
+        } else {
+          k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
+          dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
+        }
+      } else if (j + 1 <= 8) {
         const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
         k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh, g.w, n, g.partial, 0);
         k_reduce_partials<<<1, 256, 0, st>>>(j + 1, nb, g.partial, g.dh + (m + 2), 0);
@@ -1282,7 +1293,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       ECS_HIP(hipGetLastError());
       ECS_HIP(hipMemcpyAsync(g.hpin, g.dh, sizeof(double) * 2 * (m + 2), hipMemcpyDeviceToHost, st));
       ECS_HIP(hipStreamSynchronize(st));
-      for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = g.hpin[q] + g.hpin[(m + 2) + q];
+      for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = g.hpin[q] + (refine ? g.hpin[(m + 2) + q] : 0.0);
       const double hn = std::sqrt(std::max(g.hpin[j + 1], 0.0));
       if (!std::isfinite(hn)) {
         it = maxit;

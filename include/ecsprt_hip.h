@@ -132,6 +132,8 @@ typedef struct {
   int      particle_precision; /* ECSPRT_PARTICLES_* */
   int      mg_sweep;       /* smoother kernels: 0 = by level size, 1 = box sweeps, 2 = marching sweeps
                               (identical arithmetic; a choice of speed only) */
+  int      gmres_refine;   /* 0 = classical Gram-Schmidt once per Arnoldi step (PETSc's default,
+                              KSP_GMRES_CGS_REFINE_NEVER), 1 = with one reorthogonalisation */
 } ecsprt_options;
 
 typedef struct ecsprt_hip_ctx ecsprt_hip_ctx;
