@@ -119,6 +119,30 @@ __host__ __device__ inline int wrap1(int a, int n) { return a < 0 ? a + n : (a >
 // z index of a halo plane: one plane (the two-dimensional run) is its own periodic image.
 __host__ __device__ inline int wrapz(int a, int n) { return n == 1 ? 0 : wrap1(a, n); }
 
+struct CoarsenMap {
+  int Nxf, Nyf, Nzf, Nxc, Nyc, Nzc;
+  int cx, cy, cz;
+};
+
+// Linear interpolation along the coarsened axes of the coarse vector at fine node (i,j,k).
+template <typename T>
+__device__ inline T prolong_point(const CoarsenMap &m, const T *xc, int i, int j, int k) {
+  int ia[2], ja[2], ka[2];
+  T wa[2], wb[2], wc[2];
+  int na = 1, nb = 1, nc = 1;
+  if (m.cx && (i & 1)) { ia[0] = i >> 1; ia[1] = wrap1((i + 1) >> 1, m.Nxc); wa[0] = wa[1] = (T)0.5; na = 2; }
+  else { ia[0] = m.cx ? i >> 1 : i; wa[0] = (T)1; }
+  if (m.cy && (j & 1)) { ja[0] = j >> 1; ja[1] = wrap1((j + 1) >> 1, m.Nyc); wb[0] = wb[1] = (T)0.5; nb = 2; }
+  else { ja[0] = m.cy ? j >> 1 : j; wb[0] = (T)1; }
+  if (m.cz && (k & 1)) { ka[0] = k >> 1; ka[1] = wrap1((k + 1) >> 1, m.Nzc); wc[0] = wc[1] = (T)0.5; nc = 2; }
+  else { ka[0] = m.cz ? k >> 1 : k; wc[0] = (T)1; }
+  T acc = 0;
+  for (int c = 0; c < nc; ++c)
+    for (int b = 0; b < nb; ++b)
+      for (int a = 0; a < na; ++a) acc += wc[c] * wb[b] * wa[a] * xc[((long)ka[c] * m.Nyc + ja[b]) * m.Nxc + ia[a]];
+  return acc;
+}
+
 // 1/x from the hardware reciprocal refined by Newton steps (two in FP64, one in
 // FP32): within an ulp of the quotient, for the smoother's diagonal scaling.
 __device__ inline double smooth_rcp(double x) {
@@ -216,7 +240,8 @@ __device__ inline T relax_planes(const StencilConsts &s, const T *d0, const T *d
 template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) k_sweep2d(StencilConsts s, const T *__restrict__ dn,
                                                     const T *__restrict__ xin, const T *__restrict__ b,
-                                                    T *__restrict__ xout, int first) {
+                                                    T *__restrict__ xout, int first, const T *__restrict__ xc,
+                                                    CoarsenMap pm) {
   constexpr int HX = TX + 4, HY = TY + 4, H2 = HX * HY;
   constexpr int RX = TX + 2, RY = TY + 2, R2 = RX * RY;
   __shared__ T sx[H2], sn[H2], sd[H2];
@@ -227,9 +252,12 @@ __global__ void __launch_bounds__(TX * TY) k_sweep2d(StencilConsts s, const T *_
   for (int q = threadIdx.x; q < H2; q += blockDim.x) {
     const int li = q % HX - 2, lj = q / HX - 2;
     if (li > ex + 1 || lj > ey + 1) continue;
-    const long g = (long)wrap1(j0 + lj, Ny) * Nx + wrap1(i0 + li, Nx);
+    const int gi = wrap1(i0 + li, Nx), gj = wrap1(j0 + lj, Ny);
+    const long g = (long)gj * Nx + gi;
     sd[q] = dn[g];
-    sx[q] = xin ? xin[g] : T(0);
+    T v = xin ? xin[g] : T(0);
+    if (xc) v += prolong_point(pm, xc, gi, gj, 0);  // the coarse correction, as k_prolong_add adds it
+    sx[q] = v;
   }
   __syncthreads();
   for (int q = threadIdx.x; q < R2; q += blockDim.x) {
@@ -377,11 +405,6 @@ __global__ void k_jacobi(StencilConsts s, const T *__restrict__ dn, const T *__r
   z[q] = c[10] != (T)0 ? r[q] / c[10] : (T)0;
 }
 
-struct CoarsenMap {
-  int Nxf, Nyf, Nzf, Nxc, Nyc, Nzc;
-  int cx, cy, cz;
-};
-
 // Full weighting [1/4 1/2 1/4] along the coarsened axes, injection along the others.
 template <typename T>
 __device__ inline T restrict_point(const CoarsenMap &m, const T *f, int I, int J, int K) {
@@ -402,24 +425,6 @@ __device__ inline T restrict_point(const CoarsenMap &m, const T *f, int I, int J
   return (T)acc;
 }
 
-// Linear interpolation along the coarsened axes of the coarse vector at fine node (i,j,k).
-template <typename T>
-__device__ inline T prolong_point(const CoarsenMap &m, const T *xc, int i, int j, int k) {
-  int ia[2], ja[2], ka[2];
-  T wa[2], wb[2], wc[2];
-  int na = 1, nb = 1, nc = 1;
-  if (m.cx && (i & 1)) { ia[0] = i >> 1; ia[1] = wrap1((i + 1) >> 1, m.Nxc); wa[0] = wa[1] = (T)0.5; na = 2; }
-  else { ia[0] = m.cx ? i >> 1 : i; wa[0] = (T)1; }
-  if (m.cy && (j & 1)) { ja[0] = j >> 1; ja[1] = wrap1((j + 1) >> 1, m.Nyc); wb[0] = wb[1] = (T)0.5; nb = 2; }
-  else { ja[0] = m.cy ? j >> 1 : j; wb[0] = (T)1; }
-  if (m.cz && (k & 1)) { ka[0] = k >> 1; ka[1] = wrap1((k + 1) >> 1, m.Nzc); wc[0] = wc[1] = (T)0.5; nc = 2; }
-  else { ka[0] = m.cz ? k >> 1 : k; wc[0] = (T)1; }
-  T acc = 0;
-  for (int c = 0; c < nc; ++c)
-    for (int b = 0; b < nb; ++b)
-      for (int a = 0; a < na; ++a) acc += wc[c] * wb[b] * wa[a] * xc[((long)ka[c] * m.Nyc + ja[b]) * m.Nxc + ia[a]];
-  return acc;
-}
 
 template <typename T>
 __global__ void k_restrict(CoarsenMap m, const T *__restrict__ f, T *__restrict__ cvec) {
@@ -1030,11 +1035,21 @@ static int sweep_kz(int Nx, int Ny, int Nz) {
 // enough points to keep every CU streaming (measured: 33 against 43 us at 2^21 FP64
 // points, slower from 2^20 points down).
 template <typename T>
-static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first, int mode) {
+static bool marching_level(const Level<T> &L, int mode) {
+  return L.s.Nz != 1 && (mode == 2 || (mode == 0 && (long)L.s.Nx * L.s.Ny * L.s.Nz >= SZ_MIN_POINTS));
+}
+
+// xc (with its map pm) adds the prolonged coarse correction to xin as the sweep loads
+// it; single-plane levels only (in the box sweep, whose halo is 3.75 times its tile,
+// the repeated interpolation cost more than the pass it saved: 4.53 against 4.37 ms
+// per 128^3 solve; on single planes 2.14 against 2.22 ms per 1024^2 solve).
+template <typename T>
+static void launch_sweep(hipStream_t st, const Level<T> &L, const T *xin, const T *b, T *xout, int first, int mode,
+                         const T *xc = nullptr, const CoarsenMap &pm = CoarsenMap{}) {
   if (L.s.Nz == 1) {
     const int nt = ((L.s.Nx + S2_TX - 1) / S2_TX) * ((L.s.Ny + S2_TY - 1) / S2_TY);
-    k_sweep2d<T, S2_TX, S2_TY><<<nt, S2_TX * S2_TY, 0, st>>>(L.s, L.dn, xin, b, xout, first);
-  } else if (mode == 2 || (mode == 0 && (long)L.s.Nx * L.s.Ny * L.s.Nz >= SZ_MIN_POINTS)) {
+    k_sweep2d<T, S2_TX, S2_TY><<<nt, S2_TX * S2_TY, 0, st>>>(L.s, L.dn, xin, b, xout, first, xc, pm);
+  } else if (marching_level(L, mode)) {
     const int kz = sweep_kz(L.s.Nx, L.s.Ny, L.s.Nz);
     const int nt = ((L.s.Nx + SZ_TX - 1) / SZ_TX) * ((L.s.Ny + SZ_TY - 1) / SZ_TY) * ((L.s.Nz + kz - 1) / kz);
     k_sweep_z<T, SZ_TX, SZ_TY><<<nt, SZ_TX * SZ_TY, 0, st>>>(L.s, L.dn, xin, b, xout, first, kz);
@@ -1093,11 +1108,20 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
   }
   for (int l = std::min(lt, nl - 1) - 1; l >= 0; --l) {
     Level<T> &L = mg.L[l];
-    const Launch3 gf = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
-    k_prolong_add<T><<<gf.grid, gf.block, 0, st>>>(L.to_coarse, cur[l + 1], cur[l]);
     const T *in = cur[l];
     const int total = pre + post;
-    for (int k = pre + 1; k <= total; ++k) {
+    int k = pre + 1;
+    if (post > 0 && L.s.Nz == 1) {
+      // the first post-smoothing sweep adds the coarse correction as it loads x
+      T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
+      launch_sweep(st, L, in, rhs(l), out, 1, c->o.mg_sweep, (const T *)cur[l + 1], L.to_coarse);
+      in = out;
+      ++k;
+    } else {
+      const Launch3 gf = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
+      k_prolong_add<T><<<gf.grid, gf.block, 0, st>>>(L.to_coarse, cur[l + 1], cur[l]);
+    }
+    for (; k <= total; ++k) {
       T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
       launch_sweep(st, L, in, rhs(l), out, 1, c->o.mg_sweep);
       in = out;
