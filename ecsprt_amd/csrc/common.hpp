@@ -167,6 +167,7 @@ struct ecsprt_hip_ctx {
   double *d_pos2 = nullptr, *d_vel2 = nullptr;
   double *d_pos2b = nullptr, *d_vel2b = nullptr;  // the other buffer of the tile-order copy
   int *d_id2 = nullptr, *d_id2b = nullptr;        // index each ion was set with
+  int *d_tile2 = nullptr;                          // each ion's tile, recorded by the fused push
   bool tiled2 = false;    // ions in the tile order and tile ranges of the last deposit, not moved since
   bool counted2 = false;  // per-tile counts of the current positions left by the fused push
   long cap2 = 0;

@@ -319,17 +319,21 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_scan_tiles2d(int nt, const int
 // chunk per tile in LDS, reserves one range per non-empty tile, then places its ions.
 __global__ void __launch_bounds__(BIN_THREADS2) k_bin_scatter2d(long np, long chunk, Bin2 b, PushConsts k,
                                                                const double *__restrict__ pos,
+                                                               const int *__restrict__ tile_in,
                                                                int *__restrict__ fill, int *__restrict__ perm) {
+  // The tile of ion p: recorded by the fused push (tile_in), or from its position.
+  auto tile_of = [&](long p) {
+    if (tile_in) return tile_in[p];
+    int ix, iy;
+    cell_of(k, b, pos, p, ix, iy);
+    return (iy / T2) * b.ntx + ix / T2;
+  };
   __shared__ int h[MAX_TILES2];
   const int nt = b.ntx * b.nty;
   for (int q = threadIdx.x; q < nt; q += blockDim.x) h[q] = 0;
   __syncthreads();
   const long c0 = blockIdx.x * chunk, c1 = min(np, c0 + chunk);
-  for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
-    int ix, iy;
-    cell_of(k, b, pos, p, ix, iy);
-    wave_count(h, (iy / T2) * b.ntx + ix / T2);
-  }
+  for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) wave_count(h, tile_of(p));
   __syncthreads();
   for (int q = threadIdx.x; q < nt; q += blockDim.x) {
     const int n = h[q];
@@ -337,9 +341,7 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_bin_scatter2d(long np, long ch
   }
   __syncthreads();
   for (long p = c0 + threadIdx.x; p < c1; p += blockDim.x) {
-    int ix, iy;
-    cell_of(k, b, pos, p, ix, iy);
-    const int slot = wave_count(h, (iy / T2) * b.ntx + ix / T2);
+    const int slot = wave_count(h, tile_of(p));
     if (slot < np) perm[slot] = (int)p;  // always, when the counts are those of these positions
   }
 }
@@ -553,7 +555,8 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
                                                       PoissonTable pt, CollideConsts cc, uint32_t key0,
                                                       uint32_t key1, uint32_t step, const int *__restrict__ id,
                                                       double *__restrict__ pos, double *__restrict__ vel,
-                                                      int *__restrict__ tnext, unsigned long long *ctr) {
+                                                      int *__restrict__ tnext, int *__restrict__ tile_out,
+                                                      unsigned long long *ctr) {
   __shared__ double s_phi[HALO2 * HALO2];
   __shared__ int s_h[9];
   __shared__ double s_qd[QCAP2][4];  // position, then pushed velocity
@@ -575,13 +578,14 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
   // Tile of a moved ion: its own (bin 4, one LDS atomic per wave), a neighbour's bin, or
   // a global atomic.  Called by every lane of the wave; an ion still outside the domain
   // (the step fails) is not counted.
-  auto count_tile = [&](bool moved, double qx, double qy) {
+  auto count_tile = [&](bool moved, long p, double qx, double qy) {
     int bin = -1, tile = 0;
     if (moved) {
       int ix, iy;
       cell_of_xy(k, b, qx, qy, ix, iy);
       const int ux = ix / T2, uy = iy / T2;
       tile = uy * b.ntx + ux;
+      tile_out[p] = tile;
       const int dx = tile_step(ux - tx, b.ntx), dy = tile_step(uy - ty, b.nty);
       bin = (dx < 2 && dy < 2) ? (dy + 1) * 3 + (dx + 1) : 9;
     }
@@ -600,9 +604,10 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
   auto work_off = [&](int from, int nw) {
     bool moved = false;
     double qx = 0.0, qy = 0.0;
+    long p = 0;
     if ((int)threadIdx.x < nw) {
       const int e = from + threadIdx.x;
-      const long p = s_qp[e];
+      p = s_qp[e];
       const int pid = s_qid[e];
       const double px = s_qd[e][0], py = s_qd[e][1];
       double vx = s_qd[e][2], vy = s_qd[e][3], vz = 0.0;
@@ -617,7 +622,7 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
       qx = qn[0];
       qy = qn[1];
     }
-    count_tile(moved, qx, qy);
+    count_tile(moved, p, qx, qy);
   };
   for (int q0 = 0; q0 < cnt; q0 += blockDim.x) {  // uniform per workgroup
     const int q = q0 + threadIdx.x;
@@ -663,7 +668,7 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
         }
       }
     }
-    if (fused) count_tile(moved, qx, qy);
+    if (fused) count_tile(moved, p, qx, qy);
     if (fused && collide) {
       const unsigned long long mk = __ballot(listed);
       if (mk) {
@@ -823,7 +828,7 @@ void p2d_reserve(ecsprt_hip_ctx *c, long np) {
   }
   if (c->cap2 >= np && c->d_pos2) return;
   double **dv[4] = {&c->d_pos2, &c->d_vel2, &c->d_pos2b, &c->d_vel2b};
-  int **iv[2] = {&c->d_id2, &c->d_id2b};
+  int **iv[3] = {&c->d_id2, &c->d_id2b, &c->d_tile2};
   for (double **a : dv) {
     (void)hipFree(*a);
     *a = nullptr;
@@ -849,8 +854,9 @@ void p2d_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_vel2b);
   (void)hipFree(c->d_id2);
   (void)hipFree(c->d_id2b);
+  (void)hipFree(c->d_tile2);
   c->d_pos2b = c->d_vel2b = nullptr;
-  c->d_id2 = c->d_id2b = nullptr;
+  c->d_id2 = c->d_id2b = c->d_tile2 = nullptr;
   (void)hipFree(c->d_acc2);
   (void)hipFree(c->d_cnt2);
   (void)hipFree(c->d_perm2);
@@ -914,7 +920,7 @@ void p2d_push(ecsprt_hip_ctx *c, double dt) {
     k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 0, 0,
                                                       drift_consts(c->p, dt), poisson_table(0.0),
                                                       collide_consts_of(c->p), 0, 0, 0, c->d_id2, c->d_pos2,
-                                                      c->d_vel2, nullptr, nullptr);
+                                                      c->d_vel2, nullptr, nullptr, nullptr);
   } else {
     k_push2d<<<grid2(c->np), 256, 0, c->stream>>>(c->np, c->d_phi, c->p.Nx, c->p.Ny, push_consts(c->p, dt),
                                                   c->d_pos2, c->d_vel2);
@@ -950,7 +956,7 @@ bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
   k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 1,
                                                     collide ? 1 : 0, drift_consts(c->p, dt), pt,
                                                     collide_consts_of(c->p), k0, k1, step, c->d_id2, c->d_pos2,
-                                                    c->d_vel2, tnext, c->d_ctr);
+                                                    c->d_vel2, tnext, c->d_tile2, c->d_ctr);
   ECS_HIP(hipGetLastError());
   c->tiled2 = false;
   read_ctr(c);
@@ -1017,7 +1023,8 @@ void p2d_deposit(ecsprt_hip_ctx *c) {
       k_bin_hist2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tcnt);
     }
     k_scan_tiles2d<<<1, BIN_THREADS2, 0, c->stream>>>(nt, counted ? tnext : tcnt, tcnt, toff, tfill, maxcell);
-    k_bin_scatter2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, tfill, c->d_perm2);
+    k_bin_scatter2d<<<nwg, BIN_THREADS2, 0, c->stream>>>(np, chunk, b, k, c->d_pos2, counted ? c->d_tile2 : nullptr,
+                                                         tfill, c->d_perm2);
     k_deposit2d_tiles<<<nt, BIN_THREADS2, 0, c->stream>>>(b, k, 1.0 / vmax, maxcell, toff, tcnt, c->d_perm2,
                                                           c->d_pos2, c->d_vel2, c->d_id2, c->d_pos2b, c->d_vel2b,
                                                           c->d_id2b, c->d_acc2, c->d_ovf);
