@@ -243,9 +243,8 @@ def bench_batch(args, pic, _lib):
     {32, 48, 64, 96, 128, 192} per axis (seed 3), mixed precision to an FP64 residual of
     1e-8 relative; the FP64 batch on the same fields is the reference."""
     from ecsprt_amd import configs
-    rng = np.random.default_rng(3)
     nprob = args.problems
-    dims = [tuple(int(v) for v in rng.choice([32, 48, 64, 96, 128, 192], size=3)) for _ in range(nprob)]
+    dims = configs.batch_dims(nprob)
     p = pic.setup_context(**configs.base_cli(32, E0=0.05))
     t0 = time.perf_counter()
     mom = [pic.lognormal_moments(d, seed=1000 + q, sigma=0.3, smooth=3.0) for q, d in enumerate(dims)]
@@ -291,23 +290,15 @@ def bench_batch(args, pic, _lib):
     print(json.dumps(out))
 
 
-def ensemble_runs(n=16, seed=4):
-    """The ensemble: per run a cubic grid edge from {48, 64, 80, 96}, 16-64 ions per cell,
-    Omega_i dt in [0.02, 0.1], nu dt in [0.005, 0.05], seed r + 1."""
-    rng = np.random.default_rng(seed)
-    return [dict(N=int(rng.choice([48, 64, 80, 96])), ppc=int(rng.integers(16, 65)), B0=float(rng.uniform(0.02, 0.1)),
-                 nui=float(rng.uniform(0.005, 0.05)), seed=r + 1) for r in range(n)]
-
-
 def bench_ensemble(args, pic, _lib):
     """16 independent PIC runs of uneven size, mixed precision (particles and solve, rtol
     1e-10 as in the 128^3 workload), stepped concurrently (one stream and host thread per run)."""
     from ecsprt_amd import configs
-    runs = ensemble_runs()
+    runs = configs.ensemble_runs()
     mixed = dict(particle_precision=_lib.PARTICLES_MIXED, solve_precision=_lib.SOLVE_MIXED)
 
     def make(r, extra, collide=_lib.COLLIDE_PHILOX):
-        cli = configs.base_cli(r["N"], B0=r["B0"], nui=r["nui"], E0=0.05)
+        cli = configs.ensemble_cli(r)
         c = pic.Context(pic.setup_context(**cli), pic.default_options(ksp_rtol=1e-10, collide_mode=collide, **extra))
         c.init_particles(r["ppc"] * r["N"] ** 3, r["seed"], 1e-3, 2)
         c.collect()

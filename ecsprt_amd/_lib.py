@@ -108,6 +108,7 @@ SIGNATURES = {
     "ecsprt_hip_stage_times": (ctypes.c_int, [_P, _D]),
     "ecsprt_hip_synchronize": (ctypes.c_int, [_P]),
     "ecsprt_hip_last_error": (ctypes.c_char_p, [_P]),
+    "ecsprt_hip_pointer_is_device": (ctypes.c_int, [_P, _I]),
     "ecsprt_hip_dmalloc": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_long]),
     "ecsprt_hip_dfree": (ctypes.c_int, [_P]),
     "ecsprt_hip_h2d": (ctypes.c_int, [_P, _P, ctypes.c_long]),
@@ -117,6 +118,9 @@ SIGNATURES = {
                                         ctypes.POINTER(ctypes.c_uint32)]),
     "ecsprt_hip_philox2x32_10": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32,
                                         ctypes.POINTER(ctypes.c_uint32)]),
+    "ecsprt_hip_poisson_attempts": (ctypes.c_int, [ctypes.c_double, ctypes.c_long, _D, _I]),
+    "ecsprt_hip_nr_stream": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long,
+                                            ctypes.POINTER(ctypes.c_float)]),
     "ecsprt_hip_batch_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(Options), ctypes.c_int, _I,
                                                ctypes.c_int, ctypes.POINTER(_P)]),
     "ecsprt_hip_batch_destroy": (ctypes.c_int, [_P]),

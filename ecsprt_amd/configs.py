@@ -73,3 +73,25 @@ def small(N=12, ppc=8, **kw):
     args = dict(nui=0.01, Tn_over_Ti=1.0, E0=0.05)
     args.update(kw)
     return Workload(f"parity_{N}", base_cli(N, **args), ppc=ppc, dt=1.0, steps=3, rtol=1e-10)
+
+
+def ensemble_runs(n=16, seed=4):
+    """The 16-run ensemble: per run a cubic grid edge from {48, 64, 80, 96}, 16-64 ions per
+    cell, Omega_i dt in [0.02, 0.1], nu dt in [0.005, 0.05], Philox seed r + 1."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [dict(N=int(rng.choice([48, 64, 80, 96])), ppc=int(rng.integers(16, 65)), B0=float(rng.uniform(0.02, 0.1)),
+                 nui=float(rng.uniform(0.005, 0.05)), seed=r + 1) for r in range(n)]
+
+
+def ensemble_cli(run):
+    """Command-line parameters of one ensemble run (E0 = 0.05 along x, as the 128^3 case)."""
+    return base_cli(run["N"], B0=run["B0"], nui=run["nui"], E0=0.05)
+
+
+def batch_dims(nprob=256, seed=3):
+    """Grid shapes of the batched standalone solves: each edge drawn uniformly from
+    {32, 48, 64, 96, 128, 192}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [tuple(int(v) for v in rng.choice([32, 48, 64, 96, 128, 192], size=3)) for _ in range(nprob)]

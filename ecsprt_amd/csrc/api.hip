@@ -7,6 +7,7 @@
 
 #include "common.hpp"
 #include "philox.hpp"
+#include "pic_math.hpp"
 
 using ecsprt::Error;
 
@@ -30,6 +31,19 @@ int guarded(ecsprt_hip_ctx *c, F &&f) {
 }
 
 ecsprt_hip_ctx *as_ctx(void *p) { return static_cast<ecsprt_hip_ctx *>(p); }
+
+// True only for memory a kernel may dereference as a device address: hipMalloc'd or
+// managed.  Host memory, pinned or not, is false.  For an address the runtime does not
+// know (pageable host memory) hipPointerGetAttributes fails; its error is consumed so a
+// later hipGetLastError does not report it.
+bool pointer_is_device(const void *p) {
+  hipPointerAttribute_t at{};
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged;
+}
 
 void ev_record(ecsprt_hip_ctx *c, int i) {
   if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[i], c->stream));
@@ -59,6 +73,8 @@ void compute_potential(ecsprt_hip_ctx *c) {
 }
 
 }  // namespace
+
+void ecsprt::set_global_error(const std::string &msg) { g_last_global_error = msg; }
 
 extern "C" {
 
@@ -327,7 +343,7 @@ int ecsprt_hip_collide(int ndim, double dt, void *opts) {
       ecsprt::p2d_collide_philox(c, dt);
       c->step_index++;
     } else if (c->o.collide_mode == ECSPRT_COLLIDE_PHILOX) {
-      throw Error{ECSPRT_ERR_ARG, "Philox collisions run inside ecsprt_hip_step"};
+      ecsprt::run_collide_philox(c, dt);
     }
   });
 }
@@ -337,8 +353,21 @@ int ecsprt_hip_apply_bc(int np, double *pos, double *vel, void *opts) {
   (void)vel;
   if (!c || !pos || np < 0) return ECSPRT_ERR_ARG;
   return guarded(c, [&] {
-    if (c->p.ndim == 2) ecsprt::p2d_apply_bc(c, np, pos);
-    else ecsprt::run_apply_bc_aos(c, np, pos);
+    if (np == 0) return;
+    const long n = (long)np * c->p.ndim;
+    // The reference's caller hands over the swarm's host array (src/boundaries.c:67-80);
+    // a kernel must never dereference host memory, so anything that is not device
+    // memory goes through the staging buffer and back.
+    double *d = pos;
+    const bool on_device = pointer_is_device(pos);
+    if (!on_device) {
+      ecsprt::ensure_stage(c, (n + 5) / 6);
+      d = c->d_stage;
+      ECS_HIP(hipMemcpyAsync(d, pos, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    }
+    if (c->p.ndim == 2) ecsprt::p2d_apply_bc(c, np, d);
+    else ecsprt::run_apply_bc_aos(c, np, d);
+    if (!on_device) ECS_HIP(hipMemcpyAsync(pos, d, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     ECS_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -362,9 +391,8 @@ int ecsprt_hip_update_velocities(int ndim, double dt, void *opts) {
   if (!c || ndim != c->p.ndim) return ECSPRT_ERR_ARG;
   int rc = ecsprt_hip_push(dt, opts);
   if (rc != ECSPRT_OK) return rc;
-  const bool staged = c->o.collide_mode == ECSPRT_COLLIDE_PREDRAWN ||
-                      (ndim == 2 && c->o.collide_mode == ECSPRT_COLLIDE_PHILOX);
-  if (c->p.mn > 0.0 && staged) return ecsprt_hip_collide(ndim, dt, opts);
+  // src/velocities.c:384-386: collide when the neutrals have mass.
+  if (c->p.mn > 0.0 && c->o.collide_mode != ECSPRT_COLLIDE_OFF) return ecsprt_hip_collide(ndim, dt, opts);
   return ECSPRT_OK;
 }
 
@@ -583,6 +611,12 @@ const char *ecsprt_hip_last_error(const ecsprt_hip_ctx *c) {
   return c ? c->last_error.c_str() : g_last_global_error.c_str();
 }
 
+int ecsprt_hip_pointer_is_device(const void *ptr, int *is_device) {
+  if (!is_device) return ECSPRT_ERR_ARG;
+  *is_device = ptr && pointer_is_device(ptr) ? 1 : 0;
+  return ECSPRT_OK;
+}
+
 int ecsprt_hip_dmalloc(void **ptr, long bytes) {
   if (!ptr || bytes <= 0) return ECSPRT_ERR_ARG;
   return hipMalloc(ptr, (size_t)bytes) == hipSuccess ? ECSPRT_OK : ECSPRT_ERR_NOMEM;
@@ -615,6 +649,18 @@ void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint
   ecsprt::u32x4 c = {{ctr[0], ctr[1], ctr[2], ctr[3]}};
   ecsprt::u32x4 r = ecsprt::philox4x32_10(c, key[0], key[1]);
   for (int i = 0; i < 4; ++i) out[i] = r.v[i];
+}
+
+int ecsprt_hip_poisson_attempts(double lam, long n, const double *u, int *k) {
+  if (n < 0 || (n > 0 && (!u || !k))) return ECSPRT_ERR_ARG;
+  try {
+    const ecsprt::PoissonParams pp = ecsprt::poisson_params(lam);
+    for (long i = 0; i < n; ++i) k[i] = ecsprt::poisson_invert(pp, u[i]);
+  } catch (const Error &e) {
+    g_last_global_error = e.msg;
+    return e.code;
+  }
+  return ECSPRT_OK;
 }
 
 void ecsprt_hip_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]) {

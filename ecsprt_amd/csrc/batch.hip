@@ -16,22 +16,36 @@
 
 namespace ecsprt {
 
+// The generator's kernels address a field as base[q * stride]: the allocating entry point
+// works on plain arrays (stride 1), the allocation-free one on the components of the
+// caller's FluidNode output (stride 4), with the same arithmetic in the same order.
+
 // n = exp(sigma g - sigma^2 / 2), g ~ N(0, 1) per node: Box-Muller (cosine branch)
 // on two 53-bit uniforms from Philox4x32-10, counter (node, node >> 32, 0x106E, 0).
-__global__ void k_lognormal(double *out, long n, double sigma, uint32_t k0, uint32_t k1) {
+__global__ void k_lognormal(double *out, int so, long n, double sigma, uint32_t k0, uint32_t k1) {
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
     const u32x4 c = {{(uint32_t)q, (uint32_t)(q >> 32), 0x106Eu, 0u}};
     const u32x4 r = philox4x32_10(c, k0, k1);
     const double u1 = ((double)(r.v[0] >> 5) * 67108864.0 + (double)(r.v[1] >> 6) + 0.5) * (1.0 / 9007199254740992.0);
     const double u2 = ((double)(r.v[2] >> 5) * 67108864.0 + (double)(r.v[3] >> 6)) * (1.0 / 9007199254740992.0);
     const double g = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-    out[q] = exp(sigma * g - 0.5 * sigma * sigma);
+    out[q * so] = exp(sigma * g - 0.5 * sigma * sigma);
   }
 }
 
+// Normalised weights of a periodic Gaussian of `smooth` cells over offsets -R..R:
+// w[d + R] = exp(-d^2 / (2 smooth^2)) / sum, summed in offset order by one thread
+// (`smooth` = 0 gives the single weight 1).
+__global__ void k_gauss_weights(double *w, int R, double smooth) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double ws = 0.0;
+  for (int d = -R; d <= R; ++d) ws += w[d + R] = smooth > 0 ? exp(-0.5 * d * d / (smooth * smooth)) : 1.0;
+  for (int d = -R; d <= R; ++d) w[d + R] /= ws;
+}
+
 // Periodic convolution along one axis with weights w[0..2R] (offsets -R..R).
-__global__ void k_smooth_axis(const double *__restrict__ a, double *__restrict__ b, int Nx, int Ny, int Nz, int axis,
-                              const double *__restrict__ w, int R) {
+__global__ void k_smooth_axis(const double *__restrict__ a, int sa, double *__restrict__ b, int sb, int Nx, int Ny,
+                              int Nz, int axis, const double *__restrict__ w, int R) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.z;
   if (i >= Nx || j >= Ny) return;
   const int N = axis == 0 ? Nx : (axis == 1 ? Ny : Nz);
@@ -42,15 +56,15 @@ __global__ void k_smooth_axis(const double *__restrict__ a, double *__restrict__
   for (int d = -R; d <= R; ++d) {
     int t = (at + d) % N;
     if (t < 0) t += N;
-    acc += w[d + R] * a[base + (long)t * stride];
+    acc += w[d + R] * a[(base + (long)t * stride) * sa];
   }
-  b[((long)k * Ny + j) * Nx + i] = acc;
+  b[(((long)k * Ny + j) * Nx + i) * sb] = acc;
 }
 
-__global__ void k_block_sums(const double *__restrict__ a, long n, double *__restrict__ partial) {
+__global__ void k_block_sums(const double *__restrict__ a, int sa, long n, double *__restrict__ partial) {
   __shared__ double red[256];
   double acc = 0.0;
-  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) acc += a[q];
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) acc += a[q * sa];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
@@ -61,19 +75,45 @@ __global__ void k_block_sums(const double *__restrict__ a, long n, double *__res
 }
 
 // FluidNode moments (n, G = fs n per component) with n normalised to mean 1; the sum
-// of the node values is the fixed-order sum of the block partial sums.
-__global__ void k_moments_from(const double *__restrict__ a, long n, const double *__restrict__ partial, int nb,
-                               double fs, double *__restrict__ m) {
+// of the node values is the fixed-order sum of the block partial sums.  The thread of
+// node q reads a[q * sa] before it writes m[4q .. 4q+3], so a may be a component of m.
+__global__ void k_moments_from(const double *a, int sa, long n, const double *__restrict__ partial, int nb, double fs,
+                               double *m) {
   double sum = 0.0;
   for (int b = 0; b < nb; ++b) sum += partial[b];
   const double inv = (double)n / sum;
   for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < n; q += (long)gridDim.x * blockDim.x) {
-    const double v = a[q] * inv;
+    const double v = a[q * sa] * inv;
     m[q * 4] = v;
     m[q * 4 + 1] = fs * v;
     m[q * 4 + 2] = fs * v;
     m[q * 4 + 3] = fs * v;
   }
+}
+
+// The whole generator on stream st: the field into `gen` (stride sg), smoothed through two
+// buffers (x: gen -> b1, y: b1 -> b2, z: b2 -> b1; the result is in b1 when smoothing,
+// gen otherwise), block partial sums, moments.  w holds 2R+1 doubles, partial LN_BLOCKS.
+hipError_t lognormal_launch(hipStream_t st, int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth,
+                                   int R, double fs, double *gen, int sg, double *b1, int s1, double *b2, int s2,
+                                   double *w, double *partial, double *moments) {
+  const long n = (long)Nx * Ny * Nz;
+  const int g = (int)std::min<long>((n + 255) / 256, 65536);
+  k_gauss_weights<<<1, 64, 0, st>>>(w, R, smooth);
+  k_lognormal<<<g, 256, 0, st>>>(gen, sg, n, sigma, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const dim3 g3((Nx + 63) / 64, (Ny + 3) / 4, Nz), b3(64, 4, 1);
+  const double *src = gen;
+  int ss = sg;
+  if (R > 0) {
+    k_smooth_axis<<<g3, b3, 0, st>>>(gen, sg, b1, s1, Nx, Ny, Nz, 0, w, R);
+    k_smooth_axis<<<g3, b3, 0, st>>>(b1, s1, b2, s2, Nx, Ny, Nz, 1, w, R);
+    k_smooth_axis<<<g3, b3, 0, st>>>(b2, s2, b1, s1, Nx, Ny, Nz, 2, w, R);
+    src = b1;
+    ss = s1;
+  }
+  k_block_sums<<<LN_BLOCKS, 256, 0, st>>>(src, ss, n, partial);
+  k_moments_from<<<g, 256, 0, st>>>(src, ss, n, partial, LN_BLOCKS, fs, moments);
+  return hipGetLastError();
 }
 
 }  // namespace ecsprt
@@ -115,30 +155,14 @@ int ecsprt_hip_lognormal_moments(int Nx, int Ny, int Nz, uint64_t seed, double s
     ECS_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     double *a = nullptr, *b = nullptr, *w = nullptr, *partial = nullptr;
     const int R = smooth > 0 ? (int)std::ceil(4.0 * smooth) : 0;
-    std::vector<double> hw(2 * R + 1);
-    double ws = 0.0;
-    for (int d = -R; d <= R; ++d) ws += hw[d + R] = smooth > 0 ? std::exp(-0.5 * d * d / (smooth * smooth)) : 1.0;
-    for (auto &x : hw) x /= ws;
-    const int nb = 256;
     hipError_t e = hipMalloc(&a, sizeof(double) * n);
     if (e == hipSuccess) e = hipMalloc(&b, sizeof(double) * n);
-    if (e == hipSuccess) e = hipMalloc(&w, sizeof(double) * hw.size());
-    if (e == hipSuccess) e = hipMalloc(&partial, sizeof(double) * nb);
-    if (e == hipSuccess) e = hipMemcpyAsync(w, hw.data(), sizeof(double) * hw.size(), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) {
-      const int g = (int)std::min<long>((n + 255) / 256, 65536);
-      ecsprt::k_lognormal<<<g, 256, 0, st>>>(a, n, sigma, (uint32_t)seed, (uint32_t)(seed >> 32));
-      const dim3 g3((Nx + 63) / 64, (Ny + 3) / 4, Nz), b3(64, 4, 1);
-      double *src = a, *dst = b;
-      for (int ax = 0; ax < 3 && R > 0; ++ax) {
-        ecsprt::k_smooth_axis<<<g3, b3, 0, st>>>(src, dst, Nx, Ny, Nz, ax, w, R);
-        std::swap(src, dst);
-      }
-      ecsprt::k_block_sums<<<nb, 256, 0, st>>>(src, n, partial);
-      ecsprt::k_moments_from<<<g, 256, 0, st>>>(src, n, partial, nb, flux_scale, moments);
-      e = hipGetLastError();
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-    }
+    if (e == hipSuccess) e = hipMalloc(&w, sizeof(double) * (2 * R + 1));
+    if (e == hipSuccess) e = hipMalloc(&partial, sizeof(double) * ecsprt::LN_BLOCKS);
+    if (e == hipSuccess)
+      e = ecsprt::lognormal_launch(st, Nx, Ny, Nz, seed, sigma, smooth, R, flux_scale, a, 1, b, 1, a, 1, w, partial,
+                                   moments);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     (void)hipFree(a);
     (void)hipFree(b);
     (void)hipFree(w);

@@ -15,6 +15,7 @@
 #include "common.hpp"
 #include "philox.hpp"
 #include "pic_math.hpp"
+#include "tiles.hpp"
 
 namespace ecsprt {
 
@@ -330,13 +331,22 @@ __device__ inline void pilot_ion(const ParticlesMix &A, long s, double &vx, doub
   id = (int)b[64 * MX_ID];
 }
 
+// Tile t takes part in pilot pass `tag` with sampling stride `stride` (a power of two):
+// a hashed selection, so a sample is not a regular lattice of tiles.
+__device__ inline bool pilot_tile(int t, uint32_t tag, int stride) {
+  uint32_t h = (uint32_t)t * 0x9E3779B1u ^ tag * 0x85EBCA6Bu;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return (h & (uint32_t)(stride - 1)) == 0;
+}
+
+// ctr[6] attempts, ctr[7] collisions, ctr[8] ions visited.
 template <class P>
-__global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k, PoissonTable pt,
-                        uint32_t key0, uint32_t key1, uint32_t tag, unsigned long long *ctr) {
-  __shared__ float s_cdf[24];
-  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
-  __syncthreads();
+__global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k, PoissonParams pt, uint32_t key0,
+                        uint32_t key1, uint32_t tag, int stride, unsigned long long *ctr) {
   const int t = blockIdx.x;
+  if (stride > 1 && !pilot_tile(t, tag, stride)) return;
   const long b = base[t], n = count[t];
   unsigned natt = 0, nsucc = 0, nfail = 0;
   for (long q = threadIdx.x; q < n; q += blockDim.x) {
@@ -344,46 +354,67 @@ __global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k,
     double vx, vy, vz;
     int id;
     pilot_ion(A, s, vx, vy, vz, id);
-    collide_philox(k, s_cdf, key0, key1, tag, id, vx, vy, vz, natt, nsucc, nfail);
+    collide_philox(k, pt, key0, key1, tag, id, vx, vy, vz, natt, nsucc, nfail);
   }
-  unsigned long long na = natt, ns = nsucc;
-  for (int o = 32; o > 0; o >>= 1) {
-    ns += __shfl_down(ns, o);
-    na += __shfl_down(na, o);
-  }
+  const unsigned long long na = wave_sum(natt), ns = wave_sum(nsucc);
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&ctr[6], na);
     atomicAdd(&ctr[7], ns);
   }
+  if (threadIdx.x == 0) atomicAdd(&ctr[8], (unsigned long long)n);
 }
 
-// Returns the number of collisions of one dry run.
-static double pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam) {
-  unsigned long long zero[2] = {0, 0};
-  ECS_HIP(hipMemcpyAsync(c->d_ctr + 6, zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+__global__ void k_zero_pilot_ctr(unsigned long long *ctr) {
+  if (threadIdx.x < 3) ctr[6 + threadIdx.x] = 0ull;
+}
+
+struct PilotResult {
+  double collisions, ions;
+};
+
+// One dry run of the Poisson process at rate lam on the tiles of sample `tag`.
+static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int stride) {
+  k_zero_pilot_ctr<<<1, 64, 0, c->stream>>>(c->d_ctr);
   CollideConsts k = collide_consts_of(c->p);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
+  const PoissonParams pp = poisson_params(lam);
   if (c->mixed)
-    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, k, poisson_table(lam), k0, k1, tag,
-                                               c->d_ctr);
+    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, k, pp, k0, k1, tag, stride, c->d_ctr);
   else
-    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, poisson_table(lam), k0, k1, tag,
-                                               c->d_ctr);
+    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, pp, k0, k1, tag, stride, c->d_ctr);
   ECS_HIP(hipGetLastError());
-  unsigned long long h[2];
-  ECS_HIP(hipMemcpyAsync(h, c->d_ctr + 6, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  ECS_HIP(hipStreamSynchronize(c->stream));
-  return (double)h[1];
+  read_ctr(c);
+  return {(double)c->h_ctr[7], (double)c->h_ctr[8]};
 }
 
-// Initial attempt rate: dry runs of the Poisson process, each rescaling lambda by
-// Nc / (collisions it produced), starting from one attempt per collision.
+// Initial attempt rate: dry runs of the Poisson process over every ion, each rescaling
+// lambda by Nc / (collisions it produced), starting from one attempt per collision.
 void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt) {
   const int Nc = collision_count(&c->p, c->np, dt);
   double lam = c->np > 0 ? (double)Nc / (double)c->np : 0.0;
-  for (int it = 0; it < 4 && Nc > 0 && lam > 0; ++it) {
-    const double s = pilot_pass(c, 0xFFFFFFF0u + it, lam);
-    lam *= s > 0 ? std::min(16.0, std::max(0.25, (double)Nc / s)) : 16.0;
+  for (int it = 0; it < 4 && Nc > 0 && lam > 0; ++it)
+    lam = pilot_rescale(lam, Nc, pilot_pass(c, 0xFFFFFFF0u + it, std::min(lam, POISSON_LAMBDA_MAX), 1).collisions);
+  pilot_check(lam);
+  c->coll_lambda = lam;
+}
+
+// The attempt rate of this step, set from the current velocities before the step's
+// draws: two dry-run passes on a sample of the tiles (about 2 * 10^4 expected
+// collisions, at least 1/64 of the ions), each rescaling lambda by Nc / (predicted
+// collisions), on the same sample so that the second corrects the first's bias.  A rate
+// carried over from the last step lags the state: in a run that cools on cold neutrals
+// the acceptance falls every step and the collisions would fall short of Nc.
+void collide_predict(ecsprt_hip_ctx *c, int Nc) {
+  if (Nc <= 0 || c->np <= 0 || c->coll_lambda <= 0.0) return;
+  int stride = 1;
+  while (stride < 64 && (double)Nc / (2 * stride) >= 2.0e4 && c->ntiles / (2 * stride) >= 64) stride *= 2;
+  const uint32_t tag = 0x5A17E000u ^ (uint32_t)c->step_index;
+  double lam = c->coll_lambda;
+  for (int it = 0; it < 2; ++it) {
+    const PilotResult r = pilot_pass(c, tag, lam, stride);
+    if (!(r.collisions > 0.0) || !(r.ions > 0.0)) break;
+    const double predicted = r.collisions * (double)c->np / r.ions;
+    lam = std::min(lam * std::min(2.0, std::max(0.5, (double)Nc / predicted)), POISSON_LAMBDA_MAX);
   }
   c->coll_lambda = lam;
 }

@@ -143,7 +143,8 @@ struct ecsprt_hip_ctx {
   ecsprt::ParticlesMix Am, Bm, Em, Sm, Dm;
   long scap = 0;
   // counters: [0] emigrants, [1] spilled, [2] lost, [3] collision attempts,
-  // [4] collision successes, [5] collision refusals (accumulated in long long below)
+  // [4] collision successes, [5] collision refusals, [6..8] collision pilot (attempts,
+  // collisions, ions visited); CTR_N in all
   unsigned long long *d_ctr = nullptr;
   unsigned long long *h_ctr = nullptr;  // pinned mirror
 
@@ -191,6 +192,7 @@ struct ecsprt_hip_ctx {
   long last_Ns = 0, last_Nf = 0, last_attempts = 0;
   double coll_lambda = -1.0;  // Philox mode: mean attempts per ion per step
   uint64_t step_index = 0;
+  uint64_t lambda_step = ~0ull;  // step whose attempt rate coll_lambda was set for
 
   // solver
   ecsprt::Solver *solver = nullptr;
@@ -239,7 +241,19 @@ void ensure_stage(ecsprt_hip_ctx *c, long np);
 // collide.hip
 void run_collide_predrawn(ecsprt_hip_ctx *c, double dt);
 void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt);
+void collide_predict(ecsprt_hip_ctx *c, int Nc);
+void run_collide_philox(ecsprt_hip_ctx *c, double dt);  // particles.hip
 int collision_count(const ecsprt_params *p, long np, double dt);
+
+// batch.hip: the synthetic density generator on stream st (fields addressed as
+// base[q * stride]; see ecsprt_hip_lognormal_moments).
+constexpr int LN_BLOCKS = 256;
+hipError_t lognormal_launch(hipStream_t st, int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth, int R,
+                            double fs, double *gen, int sg, double *b1, int s1, double *b2, int s2, double *w,
+                            double *partial, double *moments);
+
+// api.hip: the message ecsprt_hip_last_error(NULL) returns on this thread.
+void set_global_error(const std::string &msg);
 
 // solver.hip
 void solver_create(ecsprt_hip_ctx *c);

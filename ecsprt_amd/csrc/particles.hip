@@ -20,7 +20,7 @@ struct TileArgs {
   TileGeom g;
   ParticlesF64 src, dst, emi, spill, def;
   long scap;
-  PoissonTable pt;
+  PoissonParams pt;
   const int *base, *count, *cap;
   int *next;
   unsigned long long *ctr;
@@ -60,36 +60,43 @@ __device__ inline void place(const TileArgs &a, int base, int cap, int slot, dou
   else atomicAdd(&a.ctr[2], 1ull);
 }
 
-// Deposit of one ion with global FP64 atomics (Collect3DFluidMoments weights).
-__device__ inline void deposit_direct(const TileArgs &a, double px, double py, double pz, double vx, double vy,
-                                      double vz) {
-  const double x = cell_coord(px, a.pc.dx, a.pc.inv[0], a.pc.pow2);
-  const double y = cell_coord(py, a.pc.dy, a.pc.inv[1], a.pc.pow2);
-  const double z = cell_coord(pz, a.pc.dz, a.pc.inv[2], a.pc.pow2);
+// Deposit of one ion with global FP64 atomics (Collect3DFluidMoments weights): the
+// rare ion whose tile's late list is full.
+__device__ inline void deposit_direct(double *ovf, const TileGeom &g, const PushConsts &pc, double px, double py,
+                                      double pz, double vx, double vy, double vz) {
+  const double x = cell_coord(px, pc.dx, pc.inv[0], pc.pow2);
+  const double y = cell_coord(py, pc.dy, pc.inv[1], pc.pow2);
+  const double z = cell_coord(pz, pc.dz, pc.inv[2], pc.pow2);
   const int ixl = (int)x, iyl = (int)y, izl = (int)z;
   const double wxh = x - (double)ixl, wxl = 1.0 - wxh;
   const double wyh = y - (double)iyl, wyl = 1.0 - wyh;
   const double wzh = z - (double)izl, wzl = 1.0 - wzh;
   const double w8[8] = {wzh * wyh * wxh, wzl * wyh * wxh, wzh * wyl * wxh, wzl * wyl * wxh,
                         wzh * wyh * wxl, wzl * wyh * wxl, wzh * wyl * wxl, wzl * wyl * wxl};
-  deposit_overflow(a.ovf, a.g, ixl, iyl, izl, w8, vx, vy, vz);
+  deposit_overflow(ovf, g, ixl, iyl, izl, w8, vx, vy, vz);
 }
 
 // An ion that left its source tile's deposit region lies in its destination tile dt:
 // append its position and velocity to dt's list (one atomic, 48 contiguous bytes) for
-// k_late, or deposit it now when the tile's list is full.
-__device__ inline void defer_deposit(const TileArgs &a, int dt, int base, int cap, double px, double py, double pz,
-                                     double vx, double vy, double vz) {
-  const int li = atomicAdd(&a.late_cnt[dt], 1);
+// k_late, which adds it in the exact fixed point of the tile regions, or deposit it now
+// when the tile's list is full.
+__device__ inline void late_append(double *late, int *late_cnt, double *ovf, const TileGeom &g, const PushConsts &pc,
+                                   int dt, int base, int cap, double px, double py, double pz, double vx, double vy,
+                                   double vz) {
+  const int li = atomicAdd(&late_cnt[dt], 1);
   if (li < (cap >> 2)) {
-    double2 *e = reinterpret_cast<double2 *>(a.late + 6 * ((long)(base >> 2) + li));
+    double2 *e = reinterpret_cast<double2 *>(late + 6 * ((long)(base >> 2) + li));
     e[0] = make_double2(px, py);
     e[1] = make_double2(pz, vx);
     e[2] = make_double2(vy, vz);
   } else {
-    deposit_direct(a, px, py, pz, vx, vy, vz);
+    deposit_direct(ovf, g, pc, px, py, pz, vx, vy, vz);
   }
 }
+
+// Bit 31 of an emigrant's id marks an ion whose deposit is still owed: k_rebin lists it
+// in its destination tile (ids are below 2^31 - 1).
+constexpr int EMI_DEPOSIT = (int)0x80000000u;
 
 // Per-workgroup tile constants and LDS pointers used by the particle tail.
 struct TileWG {
@@ -200,11 +207,11 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     if (code < 27) {
       const int nb = w.s_nslot[code], nc = w.s_ncap[code];
       place(a, nb, nc, bas[code] + rank, px, py, pz, vx, vy, vz, pid);
-      if (late) defer_deposit(a, w.s_nbr[code], nb, nc, px, py, pz, vx, vy, vz);
+      if (late) late_append(a.late, a.late_cnt, a.ovf, g, a.pc, w.s_nbr[code], nb, nc, px, py, pz, vx, vy, vz);
     } else if (code == 27) {
+      // further than one tile: k_rebin places it, and lists its deposit when owed
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
-      store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, pid);
-      if (late) deposit_direct(a, px, py, pz, vx, vy, vz);
+      store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, late ? (pid | EMI_DEPOSIT) : pid);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
     w.par ^= 1;
@@ -222,7 +229,6 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
   __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
-  __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
   int wi = blockIdx.x;
@@ -285,7 +291,6 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     s_ndef = 0;
     s_n3 = 0;
   }
-  if ((MODE & M_COLLIDE) && tid < 24) s_cdf[tid] = a.pt.cdf[tid];
   __syncthreads();
 
   const long cnt = a.count[w.t];
@@ -350,7 +355,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       vz = vzm;
     }
     bool defer = false;
-    if ((MODE & M_COLLIDE) && valid && poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid) > 0) {
+    if ((MODE & M_COLLIDE) && valid && poisson_any(a.pt, a.key0, a.key1, a.step, pid)) {
       // About one ion in seven has a collision attempt this step.  It is set aside
       // (post-push state, in this workgroup's own slot range) and finished in
       // phase 2 with every lane busy.
@@ -382,7 +387,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
         vy = a.def.vy(d);
         vz = a.def.vz(d);
         pid = a.def.id(d);
-        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
         collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
         if (k > 1) {
           const int r = atomicAdd(&s_n3, 1);
@@ -413,7 +418,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
         vy = a.def.vy(d);
         vz = a.def.vz(d);
         pid = a.def.id(d);
-        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
         collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
       particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
@@ -496,23 +501,26 @@ __global__ void __launch_bounds__(256) k_late(TileGeom g, const double *__restri
   }
 }
 
-// Emigrants -> destination tiles (slots after the stayers), spill when a tile is full.
+// Emigrants -> destination tiles (slots after the stayers), spill when a tile is full;
+// an emigrant whose deposit is owed joins its destination tile's late list.
 __global__ void k_rebin(TileGeom g, ParticlesF64 emi, ParticlesF64 dst, ParticlesF64 spill, long scap,
-                        const int *base, const int *cap, int *next, unsigned long long *ctr, double dx,
-                        double dy, double dz) {
+                        const int *base, const int *cap, int *next, unsigned long long *ctr, PushConsts pc,
+                        double *late, int *late_cnt, double *ovf) {
   const unsigned long long n = ctr[0];
   for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
        e += (unsigned long long)gridDim.x * blockDim.x) {
-    double x = emi.x(e), y = emi.y(e), z = emi.z(e);
-    int t = tile_of(g, x / dx, y / dy, z / dz);
-    int slot = atomicAdd(&next[t], 1);
+    const double x = emi.x(e), y = emi.y(e), z = emi.z(e), vx = emi.vx(e), vy = emi.vy(e), vz = emi.vz(e);
+    const int tagged = emi.id(e), id = tagged & ~EMI_DEPOSIT;
+    const int t = tile_of(g, x / pc.dx, y / pc.dy, z / pc.dz);
+    const int slot = atomicAdd(&next[t], 1);
     if (slot < cap[t]) {
-      store_particle(dst, (long)base[t] + slot, x, y, z, emi.vx(e), emi.vy(e), emi.vz(e), emi.id(e));
+      store_particle(dst, (long)base[t] + slot, x, y, z, vx, vy, vz, id);
     } else {
       unsigned long long sp = atomicAdd(&ctr[1], 1ull);
-      if ((long)sp < scap) store_particle(spill, (long)sp, x, y, z, emi.vx(e), emi.vy(e), emi.vz(e), emi.id(e));
+      if ((long)sp < scap) store_particle(spill, (long)sp, x, y, z, vx, vy, vz, id);
       else atomicAdd(&ctr[2], 1ull);
     }
+    if (tagged & EMI_DEPOSIT) late_append(late, late_cnt, ovf, g, pc, t, base[t], cap[t], x, y, z, vx, vy, vz);
   }
 }
 
@@ -729,7 +737,7 @@ static void clear_overflow(ecsprt_hip_ctx *c) {
   fill_f64(c, c->d_ovf, 4 * c->nnode, 0.0);
 }
 void zero_ctr(ecsprt_hip_ctx *c) {
-  k_fill_u64<<<1, 64, 0, c->stream>>>(c->d_ctr, 8);
+  k_fill_u64<<<1, 64, 0, c->stream>>>(c->d_ctr, CTR_N);
   ECS_HIP(hipGetLastError());
 }
 
@@ -867,8 +875,8 @@ void tiles_init(ecsprt_hip_ctx *c) {
   const size_t fold_bytes = sizeof(double) * 4 * (size_t)g.RAx * g.RAy * g.RAz * nreg;
   if (hipMalloc(&c->d_fold, fold_bytes) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "fold buffer"};
   ECS_HIP(hipMalloc(&c->d_ovf, sizeof(double) * 4 * c->nnode));
-  ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * 8));
-  ECS_HIP(hipHostMalloc((void **)&c->h_ctr, sizeof(unsigned long long) * 8, hipHostMallocDefault));
+  ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * CTR_N));
+  ECS_HIP(hipHostMalloc((void **)&c->h_ctr, sizeof(unsigned long long) * CTR_N, hipHostMallocDefault));
   ECS_HIP(hipMalloc(&c->d_base, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_cap, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_count, sizeof(int) * c->ntiles));
@@ -1073,7 +1081,8 @@ static void finish_rebin(ecsprt_hip_ctx *c) {
     mx_rebin(c);
   } else {
     k_rebin<<<2048, 256, 0, c->stream>>>(g, c->E, c->B, c->S, c->scap, c->d_base, c->d_cap, c->d_next, c->d_ctr,
-                                         p.dx, p.dy, p.dz);
+                                         push_consts(p, 0.0), reinterpret_cast<double *>(c->d_late), c->d_late_cnt,
+                                         c->d_ovf);
     ECS_HIP(hipGetLastError());
   }
   k_finalize_counts<<<(c->ntiles + 255) / 256, 256, 0, c->stream>>>(c->ntiles, c->d_count, c->d_next, c->d_cap);
@@ -1094,7 +1103,21 @@ static void check_after_rebin(ecsprt_hip_ctx *c) {
   if (nspill > 0) rebuild_from_spill(c, nspill);
 }
 
+// The attempt rate of this step's Philox collisions, from the ions' state at the start
+// of the step: the full pilot on first use, else the sampled prediction.  The fused pass
+// and the push slot both set it before the push, so the staged push + collide draws
+// exactly what the fused step draws.
+static void prepare_lambda(ecsprt_hip_ctx *c, double dt, int Nc) {
+  if (c->coll_lambda <= 0.0) run_collide_philox_pilot(c, dt);
+  else collide_predict(c, Nc);
+  c->lambda_step = c->step_index;
+}
+
 void run_push(ecsprt_hip_ctx *c, double dt) {
+  if (c->o.collide_mode == ECSPRT_COLLIDE_PHILOX && c->p.mn > 0.0) {
+    const int Nc = collision_count(&c->p, c->np, dt);
+    if (Nc > 0) prepare_lambda(c, dt, Nc);
+  }
   if (c->mixed) {
     mx_launch(c, M_PUSH, dt, nullptr, 0, 0, 0);
     return;
@@ -1197,12 +1220,12 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
   if (coll) {
     Nc = collision_count(&c->p, c->np, dt);
     c->last_Nc = Nc;
-    if (c->coll_lambda <= 0.0) run_collide_philox_pilot(c, dt);
+    prepare_lambda(c, dt, Nc);
     a.cc = collide_consts(c->p);
     a.key0 = (uint32_t)c->o.seed;
     a.key1 = (uint32_t)(c->o.seed >> 32);
     a.step = (uint32_t)c->step_index;
-    a.pt = poisson_table(Nc > 0 ? c->coll_lambda : 0.0);
+    a.pt = poisson_params(Nc > 0 ? c->coll_lambda : 0.0);
     if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[6], c->stream));
     if (c->mixed) mx_launch(c, M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN, dt, &a.pt, a.key0, a.key1, a.step);
     else launch_tile<M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN>(c, a);
@@ -1222,8 +1245,71 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
     c->last_attempts = natt;
     c->last_Ns = ns;
     c->last_Nf = nf;
-    // Steer the attempt rate so that the expected number of collisions is Nc.
-    if (ns > 0 && Nc > 0) c->coll_lambda *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
+    ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
+  }
+  c->step_index++;
+}
+
+// ScatterElastic (src/collisions.c:9-167) in the Philox statistical mode outside the
+// fused pass, the CollideFunc slot: every binned ion draws its Poisson number of
+// attempts for this step and runs them in order on its current velocity.  The draws
+// (keyed by id, step and attempt) and the arithmetic are those of phases 2 and 3 of the
+// fused pass, so push followed by this kernel gives the fused step's velocities bit for
+// bit.  One workgroup per tile.
+__global__ void __launch_bounds__(256) k_collide_philox(ParticlesF64 A, const int *__restrict__ base,
+                                                        const int *__restrict__ count, CollideConsts cc,
+                                                        PoissonParams pt, uint32_t key0, uint32_t key1, uint32_t step,
+                                                        unsigned long long *ctr) {
+  __syncthreads();
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  unsigned natt = 0, nsucc = 0, nfail = 0;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    const long s = b + q;
+    const int pid = A.id(s);
+    const int k = poisson_attempts(pt, key0, key1, step, pid);
+    if (k == 0) continue;
+    double vx = A.vx(s), vy = A.vy(s), vz = A.vz(s);
+    collide_attempts<double>(cc, key0, key1, step, pid, 0, k, vx, vy, vz, natt, nsucc, nfail);
+    A.vx(s) = vx;
+    A.vy(s) = vy;
+    A.vz(s) = vz;
+  }
+  const unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+  if (__lane_id() == 0) {
+    if (s0) atomicAdd(&ctr[3], s0);
+    if (s1) atomicAdd(&ctr[4], s1);
+    if (s2) atomicAdd(&ctr[5], s2);
+  }
+}
+
+// The CollideFunc slot in the Philox mode (3-D, FP64 or mixed ions): Nc from the
+// reference formula, the attempt rate from the pilot on first use, one pass over the
+// ions, the per-step counters and the rate steering, then the step counter advances (the
+// draws of the next push + collide are those of the next fused step).  Ten refusals are
+// the reference's Abort (src/collisions.c:141-148).
+void run_collide_philox(ecsprt_hip_ctx *c, double dt) {
+  const int Nc = collision_count(&c->p, c->np, dt);
+  c->last_Nc = Nc;
+  c->last_Ns = c->last_Nf = c->last_attempts = 0;
+  if (Nc > 0 && c->p.mn > 0.0) {
+    if (c->lambda_step != c->step_index) prepare_lambda(c, dt, Nc);
+    const PoissonParams pt = poisson_params(c->coll_lambda);
+    const CollideConsts cc = collide_consts(c->p);
+    const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32), step = (uint32_t)c->step_index;
+    zero_ctr(c);
+    if (c->mixed) {
+      mx_collide_philox(c, pt, cc, k0, k1, step);
+    } else {
+      k_collide_philox<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, cc, pt, k0, k1, step,
+                                                          c->d_ctr);
+      ECS_HIP(hipGetLastError());
+    }
+    read_ctr(c);
+    const long natt = (long)c->h_ctr[3], ns = (long)c->h_ctr[4], nf = (long)c->h_ctr[5];
+    c->last_attempts = natt;
+    c->last_Ns = ns;
+    c->last_Nf = nf;
     ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
   }
   c->step_index++;

@@ -428,18 +428,16 @@ __global__ void __launch_bounds__(BIN_THREADS2) k_deposit2d_tiles(Bin2 b, PushCo
 // of leaving most of each wave idle through the scattering arithmetic.
 constexpr int COLL_CHUNK2 = 4096;  // ions per selecting workgroup (16 per thread)
 
-__global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonTable pt, uint32_t key0, uint32_t key1,
+__global__ void __launch_bounds__(256) k_collide2d_select(long np, PoissonParams pt, uint32_t key0, uint32_t key1,
                                                           uint32_t step, const int *__restrict__ id,
                                                           int *__restrict__ list, int *__restrict__ seg) {
-  __shared__ float s_cdf[24];
   __shared__ int s_n;
-  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
   if (threadIdx.x == 0) s_n = 0;
   __syncthreads();
   const long c0 = (long)blockIdx.x * COLL_CHUNK2, c1 = min(np, c0 + COLL_CHUNK2);
   for (long p0 = c0; p0 < c1; p0 += blockDim.x) {  // uniform per wave
     const long p = p0 + threadIdx.x;
-    const int k = p < c1 ? poisson_attempts(s_cdf, key0, key1, step, id[p]) : 0;
+    const int k = p < c1 && poisson_any(pt, key0, key1, step, id[p]) ? 1 : 0;
     const unsigned long long m = __ballot(k > 0);
     if (!m) continue;
     const int below = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -465,14 +463,12 @@ int apply_segs(long nseg) { return (int)std::max(1L, std::min((long)COLL_SEGS2, 
 // workgroup's attempt, success and failure counts go to the counters with one atomic
 // each.
 __global__ void __launch_bounds__(256) k_collide2d_apply(long nseg, int G, const int *__restrict__ list,
-                                                         const int *__restrict__ seg, PoissonTable pt,
+                                                         const int *__restrict__ seg, PoissonParams pt,
                                                          CollideConsts cc, uint32_t key0, uint32_t key1,
                                                          uint32_t step, int write, const int *__restrict__ id,
                                                          double *__restrict__ vel, unsigned long long *ctr) {
-  __shared__ float s_cdf[24];
   __shared__ int s_off[COLL_SEGS2 + 1], s_beg[COLL_SEGS2];
   __shared__ unsigned long long s_red[3][4];
-  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
   const long g0 = (long)blockIdx.x * G;
   if (threadIdx.x == 0) {
     int t = 0;
@@ -494,7 +490,7 @@ __global__ void __launch_bounds__(256) k_collide2d_apply(long nseg, int G, const
     while (q >= s_off[g + 1]) ++g;
     const long p = list[(long)s_beg[g] + (q - s_off[g])];
     const int pid = id[p];
-    const int k = poisson_attempts(s_cdf, key0, key1, step, pid);
+    const int k = poisson_attempts(pt, key0, key1, step, pid);
     double vx = vel[2 * p], vy = vel[2 * p + 1], vz = 0.0;
     collide_attempts<double, 2>(cc, key0, key1, step, pid, 0, k, vx, vy, vz, natt, nsucc, nfail);
     if (write) {
@@ -552,7 +548,7 @@ __device__ inline bool drift_one2d(const DriftConsts &d, long p, double px, doub
 __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, const int *__restrict__ toff,
                                                       const int *__restrict__ tcnt, const double *__restrict__ phi,
                                                       PushConsts k, int fused, int collide, DriftConsts d,
-                                                      PoissonTable pt, CollideConsts cc, uint32_t key0,
+                                                      PoissonParams pt, CollideConsts cc, uint32_t key0,
                                                       uint32_t key1, uint32_t step, const int *__restrict__ id,
                                                       double *__restrict__ pos, double *__restrict__ vel,
                                                       int *__restrict__ tnext, int *__restrict__ tile_out,
@@ -561,7 +557,6 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
   __shared__ int s_h[9];
   __shared__ double s_qd[QCAP2][4];  // position, then pushed velocity
   __shared__ int s_qp[QCAP2], s_qid[QCAP2];
-  __shared__ float s_cdf[24];
   __shared__ int s_qn;
   __shared__ unsigned long long s_red[4][4];
   const int t = blockIdx.x / split, sp = blockIdx.x - t * split;
@@ -570,7 +565,6 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
     const int ly = q / HALO2, lx = q - ly * HALO2;
     s_phi[q] = phi[(long)wrap_index(j0 - 1 + ly, b.Ny) * b.Nx + wrap_index(i0 - 1 + lx, b.Nx)];
   }
-  if (threadIdx.x < 24) s_cdf[threadIdx.x] = pt.cdf[threadIdx.x];
   if (threadIdx.x < 9) s_h[threadIdx.x] = 0;
   if (threadIdx.x == 0) s_qn = 0;
   __syncthreads();
@@ -611,7 +605,7 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
       const int pid = s_qid[e];
       const double px = s_qd[e][0], py = s_qd[e][1];
       double vx = s_qd[e][2], vy = s_qd[e][3], vz = 0.0;
-      const int ka = poisson_attempts(s_cdf, key0, key1, step, pid);
+      const int ka = poisson_attempts(pt, key0, key1, step, pid);
       collide_attempts<double, 2>(cc, key0, key1, step, pid, 0, ka, vx, vy, vz, natt, nsucc, nfail);
       vel[2 * p] = vx;
       vel[2 * p + 1] = vy;
@@ -653,7 +647,7 @@ __global__ void __launch_bounds__(256, 5) k_push2d_tiles(Bin2 b, int split, cons
       vxp = vxt + vyp * k.t;
       if (fused && collide) {
         pid = id[p];
-        listed = poisson_attempts(s_cdf, key0, key1, step, pid) > 0;
+        listed = poisson_any(pt, key0, key1, step, pid);
       }
       if (!listed) {
         vel[2 * p] = vxp;
@@ -823,8 +817,8 @@ void p2d_reserve(ecsprt_hip_ctx *c, long np) {
   }
   if (!c->d_ovf) ECS_HIP(hipMalloc(&c->d_ovf, sizeof(double) * 4 * c->nnode));
   if (!c->d_ctr) {
-    ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * 8));
-    ECS_HIP(hipHostMalloc(&c->h_ctr, sizeof(unsigned long long) * 8));
+    ECS_HIP(hipMalloc(&c->d_ctr, sizeof(unsigned long long) * CTR_N));
+    ECS_HIP(hipHostMalloc(&c->h_ctr, sizeof(unsigned long long) * CTR_N));
   }
   if (c->cap2 >= np && c->d_pos2) return;
   double **dv[4] = {&c->d_pos2, &c->d_vel2, &c->d_pos2b, &c->d_vel2b};
@@ -918,7 +912,7 @@ void p2d_push(ecsprt_hip_ctx *c, double dt) {
     const int nt = b.ntx * b.nty, split = push_split(nt);
     int *toff = c->d_tiles2 + MAX_TILES2, *tcnt = c->d_tiles2;
     k_push2d_tiles<<<nt * split, 256, 0, c->stream>>>(b, split, toff, tcnt, c->d_phi, push_consts(c->p, dt), 0, 0,
-                                                      drift_consts(c->p, dt), poisson_table(0.0),
+                                                      drift_consts(c->p, dt), poisson_params(0.0),
                                                       collide_consts_of(c->p), 0, 0, 0, c->d_id2, c->d_pos2,
                                                       c->d_vel2, nullptr, nullptr, nullptr);
   } else {
@@ -948,7 +942,7 @@ bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
   const Bin2 b = bin2_of(c->p);
   const int nt = b.ntx * b.nty, split = push_split(nt);
   int *toff = c->d_tiles2 + MAX_TILES2, *tcnt = c->d_tiles2;
-  const PoissonTable pt = poisson_table(collide ? c->coll_lambda : 0.0);
+  const PoissonParams pt = poisson_params(collide ? c->coll_lambda : 0.0);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32), step = (uint32_t)c->step_index;
   int *tnext = c->d_tiles2 + 3 * MAX_TILES2;
   zero_ctr(c);
@@ -965,7 +959,7 @@ bool p2d_push_collide_drift(ecsprt_hip_ctx *c, double dt) {
     c->last_attempts = natt;
     c->last_Ns = ns;
     c->last_Nf = nf;
-    if (ns > 0) c->coll_lambda *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
+    if (ns > 0) c->coll_lambda = steer_lambda(c->coll_lambda, Nc, ns);
     ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
   }
   ECS_CHECK(c->h_ctr[2] == 0, ECSPRT_ERR_COUNT_CHANGED,
@@ -1061,7 +1055,7 @@ void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
   ensure_segments(c, nseg);
   auto pass = [&](double lam, uint32_t step, int write) {
     zero_ctr(c);
-    const PoissonTable pt = poisson_table(lam);
+    const PoissonParams pt = poisson_params(lam);
     k_collide2d_select<<<nseg, 256, 0, c->stream>>>(c->np, pt, k0, k1, step, c->d_id2, c->d_perm2, c->d_seg2);
     const int G = apply_segs(nseg);
     k_collide2d_apply<<<(nseg + G - 1) / G, 256, 0, c->stream>>>(nseg, G, c->d_perm2, c->d_seg2, pt, cc, k0, k1, step,
@@ -1072,10 +1066,10 @@ void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
   if (c->coll_lambda <= 0.0) {
     double lam = (double)Nc / (double)c->np;
     for (int it = 0; it < 4 && lam > 0; ++it) {
-      pass(lam, 0xFFFFFFF0u + it, 0);
-      const double s = (double)c->h_ctr[4];
-      lam *= s > 0 ? std::min(16.0, std::max(0.25, (double)Nc / s)) : 16.0;
+      pass(std::min(lam, POISSON_LAMBDA_MAX), 0xFFFFFFF0u + it, 0);
+      lam = pilot_rescale(lam, Nc, (double)c->h_ctr[4]);
     }
+    pilot_check(lam);
     c->coll_lambda = lam;
   }
   pass(c->coll_lambda, (uint32_t)c->step_index, 1);
@@ -1083,7 +1077,7 @@ void p2d_collide_philox(ecsprt_hip_ctx *c, double dt) {
   c->last_attempts = natt;
   c->last_Ns = ns;
   c->last_Nf = nf;
-  if (ns > 0) c->coll_lambda *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
+  if (ns > 0) c->coll_lambda = steer_lambda(c->coll_lambda, Nc, ns);
   ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
 }
 

@@ -34,7 +34,7 @@ struct TileArgsMx {
   TileGeom g;
   ParticlesMix src, dst, emi, spill, def;
   long scap;
-  PoissonTable pt;
+  PoissonParams pt;
   const int *base, *count, *cap;
   int *next;
   unsigned long long *ctr;
@@ -136,29 +136,39 @@ __device__ inline void weights_mx(float ox, float oy, float oz, float w8[8]) {
 }
 
 // Deposit through FP64 global atomics on the overflow moments.
-__device__ inline void deposit_direct_mx(const TileArgsMx &a, int ix, int iy, int iz, float ox, float oy, float oz,
-                                         float vx, float vy, float vz) {
+__device__ inline void deposit_direct_mx(double *ovf, const TileGeom &g, int ix, int iy, int iz, float ox, float oy,
+                                         float oz, float vx, float vy, float vz) {
   float w8[8];
   weights_mx(ox, oy, oz, w8);
   double wd[8];
   for (int c = 0; c < 8; ++c) wd[c] = (double)w8[c] * (1.0 / 1073741824.0);
-  deposit_overflow(a.ovf, a.g, ix, iy, iz, wd, (double)vx, (double)vy, (double)vz);
+  deposit_overflow(ovf, g, ix, iy, iz, wd, (double)vx, (double)vy, (double)vz);
+}
+__device__ inline void deposit_direct_mx(const TileArgsMx &a, int ix, int iy, int iz, float ox, float oy, float oz,
+                                         float vx, float vy, float vz) {
+  deposit_direct_mx(a.ovf, a.g, ix, iy, iz, ox, oy, oz, vx, vy, vz);
 }
 
 // An ion beyond its source tile's deposit region lies in its destination tile dt:
 // append cell, offsets and velocity to dt's list (one atomic, 32 contiguous bytes) for
-// k_late_mx, or deposit it directly when the list is full.
-__device__ inline void defer_deposit_mx(const TileArgsMx &a, int dt, int base, int cap, int ix, int iy, int iz,
-                                        float ox, float oy, float oz, float vx, float vy, float vz) {
-  const int li = atomicAdd(&a.late_cnt[dt], 1);
+// k_late_mx, which adds it in the exact fixed point of the tile regions, or deposit it
+// directly when the list is full.
+__device__ inline void late_append_mx(uint4 *late, int *late_cnt, double *ovf, const TileGeom &g, int dt, int base,
+                                      int cap, int ix, int iy, int iz, float ox, float oy, float oz, float vx, float vy,
+                                      float vz) {
+  const int li = atomicAdd(&late_cnt[dt], 1);
   if (li < (cap >> 2)) {
-    uint4 *e = a.late + 2 * ((long)(base >> 2) + li);
+    uint4 *e = late + 2 * ((long)(base >> 2) + li);
     e[0] = make_uint4(mx_pack(ix, iy, iz), __float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz));
     e[1] = make_uint4(__float_as_uint(vx), __float_as_uint(vy), __float_as_uint(vz), 0u);
   } else {
-    deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+    deposit_direct_mx(ovf, g, ix, iy, iz, ox, oy, oz, vx, vy, vz);
   }
 }
+
+// Bit 31 of an emigrant's id marks an ion whose deposit is still owed (ids are below
+// 2^31 - 1): k_rebin_mx lists it in its destination tile.
+constexpr uint32_t EMI_DEPOSIT_MX = 0x80000000u;
 
 struct TileWGMx {
   int t, tx, ty, tz, i0, j0, k0;
@@ -233,11 +243,12 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     if (code < 27) {
       const int dt = w.s_nbr[code], nb = w.s_nslot[code], nc = w.s_ncap[code];
       place_mx(a, nb, nc, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
-      if (late) defer_deposit_mx(a, dt, nb, nc, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+      if (late) late_append_mx(a.late, a.late_cnt, a.ovf, g, dt, nb, nc, ix, iy, iz, ox, oy, oz, vx, vy, vz);
     } else if (code == 27) {
+      // further than one tile: k_rebin_mx places it, and lists its deposit when owed
       const unsigned long long e = atomicAdd(&a.ctr[0], 1ull);
-      store_mx(a.emi, (long)e, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
-      if (late) deposit_direct_mx(a, ix, iy, iz, ox, oy, oz, vx, vy, vz);
+      store_mx(a.emi, (long)e, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz,
+               late ? (int)((uint32_t)pid | EMI_DEPOSIT_MX) : pid);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
     w.par ^= 1;
@@ -254,7 +265,6 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   extern __shared__ long long smemq[];
   __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
-  __shared__ float s_cdf[24];
   const TileGeom g = a.g;
   const int nwg = gridDim.x;
   int wi = blockIdx.x;
@@ -312,7 +322,6 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
     s_ndef = 0;
     s_n3 = 0;
   }
-  if ((MODE & M_COLLIDE) && tid < 24) s_cdf[tid] = a.pt.cdf[tid];
   __syncthreads();
 
   const long cnt = a.count[w.t];
@@ -347,7 +356,7 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
       vz = vzm;
     }
     bool defer = false;
-    if ((MODE & M_COLLIDE) && valid && poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid) > 0) {
+    if ((MODE & M_COLLIDE) && valid && poisson_any(a.pt, a.key0, a.key1, a.step, pid)) {
       defer = true;
       const int q = atomicAdd(&s_ndef, 1);
       store_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
@@ -371,7 +380,7 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
       int pid = 0;
       if (valid) {
         load_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
-        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
         collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
         if (k > 1) {
           const int r = atomicAdd(&s_n3, 1);
@@ -397,7 +406,7 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
       int pid = 0;
       if (valid) {
         load_mx(a.def, beg + ndef + q, cell, ox, oy, oz, vx, vy, vz, pid);
-        const int k = poisson_attempts(s_cdf, a.key0, a.key1, a.step, pid);
+        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
         collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
       tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
@@ -480,14 +489,16 @@ __global__ void __launch_bounds__(256) k_late_mx(TileGeom g, const uint4 *__rest
 }
 
 __global__ void k_rebin_mx(TileGeom g, ParticlesMix emi, ParticlesMix dst, ParticlesMix spill, long scap,
-                           const int *base, const int *cap, int *next, unsigned long long *ctr) {
+                           const int *base, const int *cap, int *next, unsigned long long *ctr, uint4 *late,
+                           int *late_cnt, double *ovf) {
   const unsigned long long n = ctr[0];
   for (unsigned long long e = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; e < n;
        e += (unsigned long long)gridDim.x * blockDim.x) {
     uint32_t cell;
     float ox, oy, oz, vx, vy, vz;
-    int id;
-    load_mx(emi, (long)e, cell, ox, oy, oz, vx, vy, vz, id);
+    int tagged;
+    load_mx(emi, (long)e, cell, ox, oy, oz, vx, vy, vz, tagged);
+    const int id = (int)((uint32_t)tagged & ~EMI_DEPOSIT_MX);
     const int t = ((mx_k(cell) / g.Tz) * g.nty + mx_j(cell) / g.Ty) * g.ntx + mx_i(cell) / g.Tx;
     const int slot = atomicAdd(&next[t], 1);
     if (slot < cap[t]) {
@@ -497,6 +508,9 @@ __global__ void k_rebin_mx(TileGeom g, ParticlesMix emi, ParticlesMix dst, Parti
       if ((long)sp < scap) store_mx(spill, (long)sp, cell, ox, oy, oz, vx, vy, vz, id);
       else atomicAdd(&ctr[2], 1ull);
     }
+    if ((uint32_t)tagged & EMI_DEPOSIT_MX)
+      late_append_mx(late, late_cnt, ovf, g, t, base[t], cap[t], mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx,
+                     vy, vz);
   }
 }
 
@@ -644,7 +658,7 @@ static void launch_mx(ecsprt_hip_ctx *c, const TileArgsMx &a) {
   ECS_HIP(hipGetLastError());
 }
 
-void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, uint32_t key0, uint32_t key1,
+void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonParams *pt, uint32_t key0, uint32_t key1,
                uint32_t step) {
   TileArgsMx a{};
   a.g = c->g;
@@ -684,9 +698,46 @@ void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, u
   }
 }
 
+// The Philox collide pass of the CollideFunc slot on mixed ions: the fused pass's
+// phases 2 and 3 (FP32 scattering on the same draws, in this file's arithmetic), applied
+// in place to every ion of every tile.  One workgroup per tile.
+__global__ void __launch_bounds__(256) k_collide_philox_mx(ParticlesMix A, const int *__restrict__ base,
+                                                           const int *__restrict__ count, CollideConsts cc,
+                                                           PoissonParams pt, uint32_t key0, uint32_t key1,
+                                                           uint32_t step, unsigned long long *ctr) {
+  __syncthreads();
+  const int t = blockIdx.x;
+  const long b = base[t], n = count[t];
+  unsigned natt = 0, nsucc = 0, nfail = 0;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    uint32_t *w = mx_at(A, b + q);
+    const int pid = (int)w[64 * MX_ID];
+    const int k = poisson_attempts(pt, key0, key1, step, pid);
+    if (k == 0) continue;
+    float vx = __uint_as_float(w[64 * MX_VX]), vy = __uint_as_float(w[64 * MX_VY]), vz = __uint_as_float(w[64 * MX_VZ]);
+    collide_attempts<float>(cc, key0, key1, step, pid, 0, k, vx, vy, vz, natt, nsucc, nfail);
+    w[64 * MX_VX] = __float_as_uint(vx);
+    w[64 * MX_VY] = __float_as_uint(vy);
+    w[64 * MX_VZ] = __float_as_uint(vz);
+  }
+  const unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
+  if (__lane_id() == 0) {
+    if (s0) atomicAdd(&ctr[3], s0);
+    if (s1) atomicAdd(&ctr[4], s1);
+    if (s2) atomicAdd(&ctr[5], s2);
+  }
+}
+
+void mx_collide_philox(ecsprt_hip_ctx *c, const PoissonParams &pt, const CollideConsts &cc, uint32_t key0,
+                       uint32_t key1, uint32_t step) {
+  k_collide_philox_mx<<<c->ntiles, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, cc, pt, key0, key1, step,
+                                                        c->d_ctr);
+  ECS_HIP(hipGetLastError());
+}
+
 void mx_rebin(ecsprt_hip_ctx *c) {
   k_rebin_mx<<<2048, 256, 0, c->stream>>>(c->g, c->Em, c->Bm, c->Sm, c->scap, c->d_base, c->d_cap, c->d_next,
-                                          c->d_ctr);
+                                          c->d_ctr, reinterpret_cast<uint4 *>(c->d_late), c->d_late_cnt, c->d_ovf);
   ECS_HIP(hipGetLastError());
 }
 

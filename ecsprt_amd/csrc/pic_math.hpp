@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.hpp"
 #include "philox.hpp"
 
@@ -264,22 +266,63 @@ __device__ inline void box_muller(uint32_t a, uint32_t b, float &g0, float &g1) 
 // Statistical collision mode: a Poisson(lambda) number of attempts on each ion, each
 // attempt the reference's scattering formula on Philox draws keyed by (id, step,
 // attempt).  With lambda = Nc / (np * acceptance) the expected number of collisions
-// is Nc.
-// Cumulative Poisson(lambda) probabilities P(K <= k), k < 24, tabulated on the host
-// in FP32 (the draw it is compared with is a 24-bit uniform).  Kernels stage it in LDS.
-struct PoissonTable {
-  float cdf[24];
+// is Nc.  The count is drawn by inversion of one 24-bit uniform, searching the CDF from
+// the mode m = floor(lambda) with the pmf recurrences p(k+1) = p(k) lambda/(k+1) and
+// p(k-1) = p(k) k/lambda in FP64: about 0.8 sqrt(lambda) steps for an ion with
+// attempts, one comparison (u <= P(K = 0)) for one without, and no bound on lambda:
+// the cold-neutral runs, whose ions cool and whose acceptance falls step by step, need
+// ever more attempts per collision, as the reference's loop does (src/collisions.c:70-153).
+struct PoissonParams {
+  double lam;     // mean attempts per ion
+  double F0;      // P(K = 0)
+  double F_mode;  // P(K <= mode)
+  double p_mode;  // P(K = mode)
+  int mode;       // floor(lam)
+  int kmax;       // search bound, far beyond any representable tail
 };
 
-// The number of attempts of ion pid at this step: one Philox2x32-10 draw, counter
-// (pid, step), key mixed from both seed words.
-__device__ inline int poisson_attempts(const float *cdf, uint32_t key0, uint32_t key1, uint32_t step, int pid) {
+// K for the uniform u in (0, 1).
+__host__ __device__ inline int poisson_invert(const PoissonParams &pp, double u) {
+  if (u <= pp.F0) return 0;
+  int k = pp.mode;
+  double F = pp.F_mode, p = pp.p_mode;
+  if (u <= F) {
+    // smallest k with u <= F(k), walking down: F(k-1) = F(k) - p(k)
+    while (k > 0) {
+      const double Fd = F - p;
+      if (u > Fd) break;
+      p *= (double)k / pp.lam;
+      F = Fd;
+      --k;
+    }
+    return k;
+  }
+  while (k < pp.kmax) {
+    p *= pp.lam / (double)(k + 1);
+    F += p;
+    ++k;
+    if (u <= F) break;
+  }
+  return k;
+}
+
+// The attempt draw of ion pid at this step: one Philox2x32-10 draw, counter (pid, step),
+// key mixed from both seed words.
+__device__ inline float poisson_uniform(uint32_t key0, uint32_t key1, uint32_t step, int pid) {
   const u32x2 c0 = {{(uint32_t)pid, step}};
   const u32x2 r0 = philox2x32_10(c0, key0 ^ (key1 * 0x9E3779B9u));
-  const float u = u01(r0.v[0]);
-  int k = 0;
-  while (k < 24 && u > cdf[k]) ++k;
-  return k;
+  return u01(r0.v[0]);
+}
+
+// The number of attempts of ion pid at this step.
+__device__ inline int poisson_attempts(const PoissonParams &pp, uint32_t key0, uint32_t key1, uint32_t step,
+                                       int pid) {
+  return poisson_invert(pp, (double)poisson_uniform(key0, key1, step, pid));
+}
+
+// Whether ion pid has at least one attempt at this step (the same draw, K > 0).
+__device__ inline bool poisson_any(const PoissonParams &pp, uint32_t key0, uint32_t key1, uint32_t step, int pid) {
+  return (double)poisson_uniform(key0, key1, step, pid) > pp.F0;
 }
 
 // Attempts first..k-1 of ion pid at this step, in order (ND = 2: vz is not touched).
@@ -308,24 +351,52 @@ __device__ inline void collide_attempts(const CollideConsts &cc, uint32_t key0, 
   }
 }
 
-__device__ inline void collide_philox(const CollideConsts &cc, const float *cdf, uint32_t key0, uint32_t key1,
+__device__ inline void collide_philox(const CollideConsts &cc, const PoissonParams &pp, uint32_t key0, uint32_t key1,
                                       uint32_t step, int pid, double &vx, double &vy, double &vz, unsigned &natt,
                                       unsigned &nsucc, unsigned &nfail) {
-  const int k = poisson_attempts(cdf, key0, key1, step, pid);
+  const int k = poisson_attempts(pp, key0, key1, step, pid);
   collide_attempts<double>(cc, key0, key1, step, pid, 0, k, vx, vy, vz, natt, nsucc, nfail);
 }
 
-// Host: P(K <= k) for K ~ Poisson(lam); the last entry closes the table at 1.
-inline PoissonTable poisson_table(double lam) {
-  PoissonTable t;
-  double pk = exp(-lam), F = pk;
-  for (int k = 0; k < 24; ++k) {
-    t.cdf[k] = (float)F;
-    pk *= lam / (k + 1);
-    F += pk;
+// Host: the search start for K ~ Poisson(lam), lam in [0, POISSON_LAMBDA_MAX]: the
+// pmf at the mode from its logarithm, the CDF there summed downward from the mode
+// (terms below 1e-300 of it dropped).
+constexpr double POISSON_LAMBDA_MAX = 1.0e6;
+inline PoissonParams poisson_params(double lam) {
+  if (!(lam >= 0.0)) lam = 0.0;
+  if (lam > POISSON_LAMBDA_MAX) throw Error{ECSPRT_ERR_COLLISION, "Poisson attempt rate above 1e6 per ion and step"};
+  PoissonParams pp;
+  pp.lam = lam;
+  pp.F0 = exp(-lam);
+  pp.mode = (int)floor(lam);
+  pp.p_mode = lam > 0.0 ? exp(-lam + pp.mode * log(lam) - lgamma(pp.mode + 1.0)) : 1.0;
+  double F = 0.0, p = pp.p_mode;
+  for (int k = pp.mode; k >= 0 && p > 1e-300 * pp.p_mode; --k) {
+    F += p;
+    p *= (double)k / lam;
   }
-  t.cdf[23] = 1.0f;
-  return t;
+  pp.F_mode = std::min(F, 1.0);
+  pp.kmax = pp.mode + 64 + (int)(64.0 * sqrt(lam));
+  return pp;
+}
+
+// One pilot pass's update of lambda: rescaled by Nc / (collisions of the dry run at
+// min(lam, POISSON_LAMBDA_MAX)), by at most 16 per pass.  A result above the bound after
+// the last pass is an error, not a cap.
+inline double pilot_rescale(double lam, long Nc, double s) {
+  return lam * (s > 0 ? std::min(16.0, std::max(0.25, (double)Nc / s)) : 16.0);
+}
+inline void pilot_check(double lam) {
+  if (lam > POISSON_LAMBDA_MAX)
+    throw Error{ECSPRT_ERR_COLLISION, "collision acceptance too low: the attempt rate needed (" + std::to_string(lam) +
+                                          " per ion and step) exceeds 1e6"};
+}
+
+// Attempt-rate steering shared by the particle paths: lambda scaled by Nc / successes
+// (within a factor two per step), kept within POISSON_LAMBDA_MAX.
+inline double steer_lambda(double lam, long Nc, long ns) {
+  if (ns > 0 && Nc > 0) lam *= std::min(2.0, std::max(0.5, (double)Nc / (double)ns));
+  return std::min(lam, POISSON_LAMBDA_MAX);
 }
 
 // (PetscInt)(np*ran3(&seed)): the int*float product is a float (src/collisions.c:72);

@@ -35,7 +35,10 @@ namespace {
 struct Ran1 {
   static constexpr long IA = 16807, IM = 2147483647, IQ = 127773, IR = 2836, NTAB = 32;
   static constexpr long NDIV = 1 + (IM - 1) / NTAB;
-  static constexpr double AM = 1.0 / IM, RNMX = 1.0 - 1.2e-7;
+  // RNMX is 1 - PETSC_MACHINE_EPSILON (src/random.c:10-11), not the book's 1 - 1.2e-7:
+  // the float temp is compared with it after promotion to double, so only a temp that
+  // rounded to 1.0f is replaced, by (float)RNMX = 1.0f.
+  static constexpr double AM = 1.0 / IM, RNMX = 1.0 - 2.220446049250313e-16;
   long iy = 0, iv[NTAB] = {};
   float next(long *idum) {
     long k;
@@ -243,3 +246,19 @@ void particles_init_distribution(ecsprt_hip_ctx *c, long np, int pdist) {
 }
 
 }  // namespace ecsprt
+
+extern "C" int ecsprt_hip_nr_stream(int generator, long seed, long n, float *out) {
+  if (n < 0 || (n > 0 && !out) || generator < 0 || generator > 2) return ECSPRT_ERR_ARG;
+  long idum = seed;
+  if (generator == 0) {
+    ecsprt::Ran1 r;
+    for (long i = 0; i < n; ++i) out[i] = r.next(&idum);
+  } else if (generator == 1) {
+    ecsprt::Ran3 r;
+    for (long i = 0; i < n; ++i) out[i] = r.next(&idum);
+  } else {
+    ecsprt::Gasdev g;
+    for (long i = 0; i < n; ++i) out[i] = g.next(&idum);
+  }
+  return ECSPRT_OK;
+}

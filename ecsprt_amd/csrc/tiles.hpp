@@ -102,7 +102,7 @@ CollideConsts collide_consts_of(const ecsprt_params &p);
 void mx_free(ParticlesMix &P);
 void mx_alloc(ParticlesMix &P, long n);
 void mx_set_lds_attrs();
-void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonTable *pt, uint32_t key0, uint32_t key1,
+void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonParams *pt, uint32_t key0, uint32_t key1,
                uint32_t step);
 void mx_rebin(ecsprt_hip_ctx *c);
 void mx_bin_scatter(ecsprt_hip_ctx *c, long np);
@@ -110,6 +110,8 @@ void mx_unbin(ecsprt_hip_ctx *c);
 void mx_unbin_list(ecsprt_hip_ctx *c, long n);
 void mx_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
 void mx_late(ecsprt_hip_ctx *c);
+void mx_collide_philox(ecsprt_hip_ctx *c, const PoissonParams &pt, const CollideConsts &cc, uint32_t key0,
+                       uint32_t key1, uint32_t step);
 double mx_vmax(const ecsprt_params &p);
 int fx_bits(const ecsprt_hip_ctx *c);
 
@@ -154,8 +156,10 @@ inline PushConsts push_consts(const ecsprt_params &p, double dt) {
   return k;
 }
 
+constexpr int CTR_N = 16;  // device counters per context (and their pinned mirror)
+
 inline void read_ctr(ecsprt_hip_ctx *c) {
-  ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * 8, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * CTR_N, hipMemcpyDeviceToHost, c->stream));
   ECS_HIP(hipStreamSynchronize(c->stream));
 }
 

@@ -7,6 +7,7 @@
  * the time loop reads like src/pic.c:238-269.  Options use the reference spellings:
  *   -Nx -Ny -Nz -dx -dy -dz -Np (negative = per cell) -Nt -dt -B0 -E0 -qi -mi -nui
  *   -nue -Te -viT -mn -Tn -vn0 ; plus -rtol, -fused (one-pass timestep), -seed,
+ *   -no-collide (collision mode off; by default the Philox statistical mode),
  *   -mixed (mixed-precision particles and solve), -ndim 2 (a two-dimensional run:
  *   BorisMoverBz2D, Collect2DFluidMoments, ComputeFullLHS2D/RHS2D; Nz = 1).
  * Initial conditions come from the device generator (uniform positions, Maxwellian
@@ -141,7 +142,7 @@ int main(int argc, char **argv) {
   o.ksp_rtol = rtol;
   o.seed = (uint64_t)seed;
   o.timing = 1;
-  o.collide_mode = fused ? ECSPRT_COLLIDE_PHILOX : ECSPRT_COLLIDE_OFF;
+  o.collide_mode = has_flag(argc, argv, "-no-collide") ? ECSPRT_COLLIDE_OFF : ECSPRT_COLLIDE_PHILOX;
   if (mixed) {
     o.particle_precision = ECSPRT_PARTICLES_MIXED;
     o.solve_precision = ECSPRT_SOLVE_MIXED;
@@ -170,8 +171,10 @@ int main(int argc, char **argv) {
       ecsprt_hip_stage_times(ctx, ms);
       for (int k = 0; k < 6; ++k) acc[k] += ms[k];
     } else {
-      CHECK(ions.push(dt, ctx)); /* UpdateVelocities, src/pic.c:246 */
-      if (p.mn > 0.0 && o.collide_mode == ECSPRT_COLLIDE_PREDRAWN) CHECK(ions.collide(ndim, dt, ctx));
+      /* UpdateVelocities (src/pic.c:246, src/velocities.c:375-390): push, then the
+       * collision slot when the neutrals have mass. */
+      CHECK(ions.push(dt, ctx));
+      if (p.mn > 0.0) CHECK(ions.collide(ndim, dt, ctx));
       CHECK(ecsprt_hip_update_positions(ndim, dt, ctx)); /* src/pic.c:250 */
       CHECK(ions.collect(ctx));                       /* src/pic.c:254 */
       CHECK(ecsprt_hip_compute_potential(ctx));       /* src/pic.c:258 */
@@ -179,7 +182,11 @@ int main(int argc, char **argv) {
     int iters;
     double rn, bn;
     ecsprt_hip_solver_stats(ctx, &iters, &rn, &bn);
-    printf("< Time step %d > KSP iterations %d, ||r||/||b|| = %.2e\n", it, iters, bn > 0 ? rn / bn : 0.0);
+    int Nc;
+    long Ns, Nf, natt;
+    ecsprt_hip_collision_stats(ctx, &Nc, &Ns, &Nf, &natt);
+    printf("< Time step %d > KSP iterations %d, ||r||/||b|| = %.2e, collisions %ld of Nc %d (%ld attempts)\n", it,
+           iters, bn > 0 ? rn / bn : 0.0, Ns, Nc, natt);
   }
   ecsprt_hip_synchronize(ctx);
   double el = now_s() - t0;

@@ -180,6 +180,12 @@ class Context:
     def apply_bc_device(self, pos_dev, np_):
         self._c(self.L.ecsprt_hip_apply_bc(np_, pos_dev, None, self.ptr))
 
+    def apply_bc_host(self, pos):
+        """BCFunc on a host array pos[np, ndim] in place, as the reference's caller passes
+        the swarm's coordinates (src/boundaries.c:67-80)."""
+        assert pos.dtype == np.float64 and pos.flags.c_contiguous and pos.shape[1:] == (self.ndim,)
+        self._c(self.L.ecsprt_hip_apply_bc(pos.shape[0], pos.ctypes.data, None, self.ptr))
+
     # ---- linear system ---------------------------------------------------------------
     def lhs(self):
         self._c(self.L.ecsprt_hip_lhs(self.ptr))
@@ -254,6 +260,29 @@ class Context:
 
 def collision_count(params, np_, dt):
     return _lib.load().ecsprt_hip_collision_count(ctypes.byref(params), np_, dt)
+
+
+def pointer_is_device(ptr):
+    """True when ptr (an int address or c_void_p) is device memory."""
+    flag = ctypes.c_int()
+    _check(_lib.load().ecsprt_hip_pointer_is_device(ptr, ctypes.byref(flag)))
+    return bool(flag.value)
+
+
+def poisson_attempts(lam, u):
+    """The Philox collision mode's attempt counts for uniforms u at rate lam."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    k = np.empty(u.shape, dtype=np.int32)
+    _check(_lib.load().ecsprt_hip_poisson_attempts(lam, u.size, _dp(u), k.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return k
+
+
+def nr_stream(generator, seed, n):
+    """The library's ran1 / ran3 / gasdev stream (generator "ran1", "ran3" or "gasdev")."""
+    g = {"ran1": 0, "ran3": 1, "gasdev": 2}[generator]
+    out = np.empty(n, dtype=np.float32)
+    _check(_lib.load().ecsprt_hip_nr_stream(g, seed, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+    return out
 
 
 def philox4x32_10(ctr, key):

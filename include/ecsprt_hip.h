@@ -185,16 +185,23 @@ int ecsprt_hip_cell_counts(ecsprt_hip_ctx *ctx, int *counts);
 
 /* PushFunc: BorisMoverBz3D (src/velocities.c:281-371). */
 int ecsprt_hip_push(double dt, void *ctx);
-/* CollideFunc: ScatterElastic (src/collisions.c:9-167), in the context's collide mode. */
+/* CollideFunc: ScatterElastic (src/collisions.c:9-167), in the context's collide mode:
+ * pre-drawn records replayed per particle, or the Philox mode's Poisson attempts on every
+ * ion (the same draws as the fused step's, so push + collide equals the fused step's
+ * velocities); collide mode off does nothing.  Advances the Philox step counter. */
 int ecsprt_hip_collide(int ndim, double dt, void *ctx);
-/* BCFunc: ApplyBC (src/boundaries.c:6-34) on device AoS arrays pos[np][3] (vel unused). */
+/* BCFunc: ApplyBC (src/boundaries.c:6-34) on AoS positions pos[np][ndim] (vel unused).
+ * pos may be device memory (wrapped in place by a kernel) or host memory, as the
+ * reference's caller passes the swarm's array (src/boundaries.c:67-80): host arrays are
+ * copied to the device, wrapped and copied back; no kernel reads host memory. */
 int ecsprt_hip_apply_bc(int np, double *pos, double *vel, void *ctx);
 /* CollectFunc: Collect3DFluidMoments (src/moments.c:108-226). */
 int ecsprt_hip_collect(void *ctx);
 
 /* ---- direct entry points ---------------------------------------------------- */
 
-/* UpdateVelocities (src/velocities.c:375-390): push, then collide if mn > 0. */
+/* UpdateVelocities (src/velocities.c:375-390): push, then ecsprt_hip_collide if mn > 0
+ * (src/velocities.c:384-386) and the collide mode is not off. */
 int ecsprt_hip_update_velocities(int ndim, double dt, void *ctx);
 /* UpdatePositions (src/positions.c:436-472): drift, ApplyBC, re-binning and the
  * count-conservation check of ApplyBCAndMigrate (src/boundaries.c:57-113). */
@@ -252,6 +259,11 @@ int ecsprt_hip_synchronize(ecsprt_hip_ctx *ctx);
 /* Text of the last error on this context. */
 const char *ecsprt_hip_last_error(const ecsprt_hip_ctx *ctx);
 
+/* *is_device = 1 when ptr is device (or managed) memory a kernel may dereference, 0 for
+ * host memory, pinned or pageable: the test ecsprt_hip_apply_bc applies before launching
+ * a kernel on a caller's array.  No counterpart in the reference. */
+int ecsprt_hip_pointer_is_device(const void *ptr, int *is_device);
+
 /* Device memory helpers for callers without their own allocator. */
 int ecsprt_hip_dmalloc(void **ptr, long bytes);
 int ecsprt_hip_dfree(void *ptr);
@@ -265,6 +277,18 @@ int ecsprt_hip_d2d(void *dst, const void *src, long bytes, double *ms);
  * tests of the generators the statistical collision mode draws from. */
 void ecsprt_hip_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void ecsprt_hip_philox2x32_10(const uint32_t ctr[2], uint32_t key, uint32_t out[2]);
+
+/* The statistical collision mode's number of attempts per ion and step: k[i] = the
+ * smallest k with P(K <= k) >= u[i] for K ~ Poisson(lam) (inversion searched from the
+ * mode), the function every Philox-mode kernel applies to its 24-bit uniform draw.  Host
+ * code, for known-answer tests against a reference Poisson quantile.  lam in [0, 1e6]. */
+int ecsprt_hip_poisson_attempts(double lam, long n, const double *u, int *k);
+
+/* The Numerical Recipes streams the library draws the reference's initial conditions
+ * from (src/random.c:24-204), for comparison with the reference's generators: n values of
+ * ran1 (generator 0), ran3 (1) or gasdev (2) from fresh static state and seed `seed`
+ * (getseed() gives -12345, src/random.c:292-295).  Host code; no device is used. */
+int ecsprt_hip_nr_stream(int generator, long seed, long n, float *out);
 
 /* ---- Batched standalone potential solves -------------------------------------------
  * No counterpart entry point in the reference, which solves one field per KSPSolve

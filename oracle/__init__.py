@@ -92,6 +92,8 @@ def lib():
             "oracle_ran3": (ctypes.c_float, [ctypes.POINTER(NRState), _L]),
             "oracle_gasdev": (ctypes.c_float, [ctypes.POINTER(NRState), _L]),
             "oracle_getseed": (ctypes.c_long, [ctypes.c_int]),
+            "oracle_nr_stream": (ctypes.c_int, [ctypes.c_int, ctypes.c_long, ctypes.c_long,
+                                                ctypes.POINTER(ctypes.c_float)]),
             "oracle_collision_count": (ctypes.c_int, [P, ctypes.c_long, ctypes.c_double]),
             "oracle_scatter_elastic_nr": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_long, _D,
                                                          ctypes.POINTER(NRState), ctypes.c_long,
@@ -224,6 +226,15 @@ class NR:
         for i in range(n):
             out[i] = f(ctypes.byref(self.state), ctypes.byref(s))
         return out, s.value
+
+
+def nr_stream(generator, seed, n):
+    """n values of ran1, ran3 or gasdev from fresh state (the bulk form of NR.stream)."""
+    which = {"ran1": 0, "ran3": 1, "gasdev": 2}[generator]
+    out = np.empty(n, dtype=np.float32)
+    if lib().oracle_nr_stream(which, seed, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))):
+        raise ValueError(generator)
+    return out
 
 
 # ---- two-dimensional variants (cli.ndim = 2) ----------------------------------------

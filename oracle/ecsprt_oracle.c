@@ -611,6 +611,19 @@ float oracle_gasdev(oracle_nr_state *s, long *idum)
   return s->g_gset;
 }
 
+/* n consecutive values of ran1 (which = 0), ran3 (1) or gasdev (2) from fresh static
+ * state and seed `seed`: the bulk form of the three calls above. */
+int oracle_nr_stream(int which, long seed, long n, float *out)
+{
+  oracle_nr_state s;
+  long idum = seed, i;
+  if (which < 0 || which > 2) return 1;
+  oracle_nr_init(&s);
+  for (i = 0; i < n; i++)
+    out[i] = which == 0 ? oracle_ran1(&s, &idum) : which == 1 ? oracle_ran3(&s, &idum) : oracle_gasdev(&s, &idum);
+  return 0;
+}
+
 /* ------------------------------------------------------------------------- */
 /* collisions.c:9-167 (ScatterElastic).                                      */
 /* ------------------------------------------------------------------------- */

@@ -107,6 +107,7 @@ void  oracle_nr_init(oracle_nr_state *s);
 float oracle_ran1(oracle_nr_state *s, long *idum);
 float oracle_ran3(oracle_nr_state *s, long *idum);
 float oracle_gasdev(oracle_nr_state *s, long *idum);
+int   oracle_nr_stream(int which, long seed, long n, float *out);
 long  oracle_getseed(int rank);
 
 /* collisions.c */

@@ -94,3 +94,58 @@ def test_workload_collision_counts():
     n1 = oracle.collision_count(p1, configs.PROD128.np, configs.PROD128.dt)
     assert 0.3 * configs.CPU32.np < n0 < 0.5 * configs.CPU32.np
     assert 0.05 * configs.PROD128.np < n1 < 0.1 * configs.PROD128.np
+
+
+# Index of the first ran1 output of at least 1 - 2^-24 (0.99999994f) from seed -12345.
+# ran1 compares the float temp with 1 - DBL_EPSILON (src/random.c:10-11,67-68), so this
+# value passes through unchanged; the Numerical Recipes book's 1 - 1.2e-7 would return
+# 0.99999988f here.
+RAN1_FIRST_NEAR_ONE = 16392033
+
+
+@pytest.mark.parametrize("gen,n", [("ran1", RAN1_FIRST_NEAR_ONE + 200_000), ("ran3", 3_000_000),
+                                   ("gasdev", 14_000_000)])
+def test_library_nr_streams_equal_oracle(gen, n):
+    """The library's host ran1 / ran3 / gasdev streams (the reference's initial conditions
+    are drawn from them) equal the oracle's value for value, through the first ran1 output
+    that rounds to 1 - 2^-24 (gasdev's 14M deviates consume about 17.8M ran1 values)."""
+    ours = pic.nr_stream(gen, -12345, n)
+    ref = oracle.nr_stream(gen, -12345, n)
+    np.testing.assert_array_equal(ours, ref)
+    if gen == "ran1":
+        near_one = np.nonzero(ref >= np.float32(1) - np.float32(2.0 ** -24))[0]
+        assert near_one[0] == RAN1_FIRST_NEAR_ONE
+        assert ref[RAN1_FIRST_NEAR_ONE] == np.float32(1) - np.float32(2.0 ** -24)
+
+
+def test_nr_stream_rejects_bad_generator():
+    import ctypes
+    out = np.zeros(1, np.float32)
+    rc = _lib.load().ecsprt_hip_nr_stream(3, -1, 1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    assert rc == _lib.ERR_ARG
+
+
+@pytest.mark.parametrize("lam", [0.0, 1e-3, 0.08, 1.0, 2.0, 16.5, 63.7, 100.0, 5000.0, 2.5e5])
+def test_poisson_attempts_equal_quantile(lam):
+    """The Philox mode's attempt count is the Poisson(lam) quantile of its uniform, value
+    for value against scipy's, from a tenth of an attempt per ion (the 128^3 workload) to
+    the rates a cooling cold-neutral run reaches (16.5 per ion at 32^3 step 10) and far
+    beyond: no truncation of the tail at any rate."""
+    from scipy import stats
+    rng = np.random.default_rng(1)
+    u = (rng.integers(0, 2 ** 24, 200_000) + 0.5) / 2 ** 24  # the kernels' 24-bit uniforms
+    u = np.concatenate([u, [0.5 / 2 ** 24, 1 - 0.5 / 2 ** 24]])  # the extreme draws
+    k = pic.poisson_attempts(lam, u)
+    ref = stats.poisson.ppf(u, lam).astype(np.int64) if lam > 0 else np.zeros(u.size, np.int64)
+    if lam <= 5000.0:
+        np.testing.assert_array_equal(k, ref)
+    else:
+        # hundreds of FP64 CDF terms summed from the mode: a draw within rounding of a CDF
+        # value may land one count over, in either implementation
+        assert np.abs(k - ref).max() <= 1 and np.count_nonzero(k != ref) <= 5
+
+
+def test_poisson_attempts_rate_bound():
+    with pytest.raises(pic.HipError) as e:
+        pic.poisson_attempts(2e6, np.array([0.5]))
+    assert e.value.code == _lib.ERR_COLLISION

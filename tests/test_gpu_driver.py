@@ -25,9 +25,12 @@ def test_driver_steps(extra):
     assert os.path.exists(build.DRIVER), "driver not built"
     r = run("-Nx", "16", "-Ny", "16", "-Nz", "16", "-Np", "-8", "-Nt", "3", "-rtol", "1e-8", *extra)
     assert r.returncode == 0
-    steps = re.findall(r"< Time step (\d+) > KSP iterations (\d+), \|\|r\|\|/\|\|b\|\| = ([0-9.e+-]+)", r.stdout)
+    steps = re.findall(r"< Time step (\d+) > KSP iterations (\d+), \|\|r\|\|/\|\|b\|\| = ([0-9.e+-]+), "
+                       r"collisions (\d+) of Nc (\d+)", r.stdout)
     assert [int(s[0]) for s in steps] == [0, 1, 2]
-    for _, it, rr in steps:
+    for _, it, rr, ns, nc in steps:
         assert int(it) > 0 and float(rr) <= 1e-8
+        # UpdateVelocities collides every step in the staged loop as in the fused one
+        assert int(nc) > 0 and abs(int(ns) - int(nc)) <= 0.2 * int(nc) + 30, (ns, nc)
     if "--position-dist" in extra:
         assert f"reference initial conditions, positions {extra[-1]}" in r.stdout

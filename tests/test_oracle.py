@@ -364,6 +364,36 @@ def test_collide_acceptance_uses_two_draws():
     assert not acc
 
 
+@pytest.mark.parametrize("vi", [(0.4, -0.3, 0.9), (0.05, 0.02, -1.1), (1.2, 0.0, 0.0)])
+def test_collide_scattering_direction_is_isotropic(vi):
+    """For fixed ion and neutral velocities, the post-collision direction e = (vf - vc)/vcr
+    over uniform (u_z, u_phi) is uniform on the sphere (collisions.c:100-132: cos(theta) =
+    2 u_z - 1, azimuth 2 pi u_phi, rotated into the frame of vr): mean 0 and second moments
+    I/3.  A midpoint grid in (u_z, u_phi) makes the quadrature accurate to O(1/n^2); a wrong
+    azimuth scale (pi u_phi), polar map or rotation leaves a mean or a covariance off."""
+    p = _col_params(mn=1.7)
+    vi = np.array(vi)
+    a = _random_attempts(1, np.random.default_rng(12))[0]
+    a["g"] = np.array([0.3, -0.8, 0.5], dtype=np.float32)
+    a["u_acc1"], a["u_acc2"] = 0.0, 0.0  # always accepted
+    vn = a["g"].astype(np.float64) * p.vnT + np.array([p.vn0x, p.vn0y, p.vn0z])
+    vc = (vi * p.mi + vn * p.mn) / (p.mi + p.mn)
+    vcr = np.linalg.norm(vi - vc)
+    n = 96
+    us = ((np.arange(n) + 0.5) / n).astype(np.float32)
+    es = []
+    for uz in us:
+        for uphi in us:
+            a["u_z"], a["u_phi"] = uz, uphi
+            vf, acc, fail = oracle.collide_one(p, a, vi)
+            assert acc and not fail
+            es.append((vf - vc) / vcr)
+    e = np.array(es)
+    np.testing.assert_allclose(np.linalg.norm(e, axis=1), 1.0, rtol=1e-12)
+    np.testing.assert_allclose(e.mean(axis=0), 0.0, atol=2e-3)
+    np.testing.assert_allclose(e.T @ e / len(e), np.eye(3) / 3.0, atol=2e-3)
+
+
 def test_scatter_nr_log_replays_through_predrawn_and_counts_draws():
     p = _col_params(nui=0.05)
     rng = np.random.default_rng(10)
