@@ -130,6 +130,9 @@ SIGNATURES = {
     "ecsprt_hip_ensemble_step": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_double]),
     "ecsprt_hip_lognormal_moments": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _P]),
+    "ecsprt_hip_lognormal_moments_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, _P,
+                                                            _P]),
 }
 
 _lib = None

@@ -322,6 +322,19 @@ int ecsprt_hip_ensemble_step(ecsprt_hip_ctx *const *ctxs, int n, double dt);
 int ecsprt_hip_lognormal_moments(int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth,
                                  double flux_scale, double *moments);
 
+/* The allocation-free variant of ecsprt_hip_lognormal_moments.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and the same result bit
+ * for bit, plus `stream` (a hipStream_t; NULL = the null stream): the kernels are
+ * enqueued there and the call returns without synchronising, allocating nothing -- the
+ * field lives in the flux components of `moments` until they are overwritten, and the
+ * partial sums and smoothing weights in an 8 KB device array of the library that each
+ * device holds until the process ends.  Limit: ceil(4 smooth) <= 383 cells (smooth <=
+ * 95.75); larger returns ECSPRT_ERR_ARG.  Calls that overlap on one device race for that
+ * array. */
+int ecsprt_hip_lognormal_moments_noalloc(int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth,
+                                         double flux_scale, double *moments, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
