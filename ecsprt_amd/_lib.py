@@ -127,6 +127,7 @@ SIGNATURES = {
     "ecsprt_hip_batch_solve": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
     "ecsprt_hip_batch_stats": (ctypes.c_int, [_P, _I, _D]),
     "ecsprt_hip_batch_last_error": (ctypes.c_char_p, [_P]),
+    "ecsprt_hip_batch_traffic": (ctypes.c_int, [_P, _D]),
     "ecsprt_hip_ensemble_step": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_double]),
     "ecsprt_hip_lognormal_moments": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _P]),

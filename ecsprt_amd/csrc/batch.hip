@@ -124,6 +124,7 @@ struct ecsprt_hip_batch {
   ecsprt_params p;
   ecsprt_options o;
   int device = 0;
+  ecsprt::LockstepBatch *lockstep = nullptr;  // slots == 0: every problem in every launch
   std::vector<int> dims;
   std::vector<ecsprt_hip_ctx *> slots;
   std::vector<ecsprt::Arena> arenas;
@@ -176,7 +177,7 @@ int ecsprt_hip_lognormal_moments(int Nx, int Ny, int Nz, uint64_t seed, double s
 
 int ecsprt_hip_batch_create(const ecsprt_params *p, const ecsprt_options *o, int nprob, const int *dims, int nslots,
                             ecsprt_hip_batch **out) {
-  if (!p || !o || !dims || !out || nprob <= 0 || nslots <= 0) return ECSPRT_ERR_ARG;
+  if (!p || !o || !dims || !out || nprob <= 0 || nslots < 0) return ECSPRT_ERR_ARG;
   *out = nullptr;
   auto *b = new ecsprt_hip_batch();
   b->p = *p;
@@ -189,6 +190,20 @@ int ecsprt_hip_batch_create(const ecsprt_params *p, const ecsprt_options *o, int
       return ECSPRT_ERR_ARG;
     }
     nmax = std::max(nmax, (long)dims[3 * q] * dims[3 * q + 1] * dims[3 * q + 2]);
+  }
+  b->iters.assign(nprob, 0);
+  b->rnorm.assign(nprob, 0.0);
+  if (nslots == 0) {
+    try {
+      ECS_HIP(hipGetDevice(&b->device));
+      b->lockstep = ecsprt::lockstep_create(*p, *o, nprob, dims);
+    } catch (const Error &e) {
+      ecsprt::set_global_error(e.msg);
+      delete b;
+      return e.code;
+    }
+    *out = b;
+    return ECSPRT_OK;
   }
   nslots = std::min(nslots, nprob);
   try {
@@ -222,6 +237,7 @@ int ecsprt_hip_batch_create(const ecsprt_params *p, const ecsprt_options *o, int
 
 int ecsprt_hip_batch_destroy(ecsprt_hip_batch *b) {
   if (!b) return ECSPRT_OK;
+  if (b->lockstep) ecsprt::lockstep_destroy(b->lockstep);
   for (size_t s = 0; s < b->slots.size(); ++s) free_slot(b->slots[s], b->arenas[s]);
   delete b;
   return ECSPRT_OK;
@@ -229,6 +245,18 @@ int ecsprt_hip_batch_destroy(ecsprt_hip_batch *b) {
 
 int ecsprt_hip_batch_solve(ecsprt_hip_batch *b, double *const *moments, double *const *phi) {
   if (!b || !moments || !phi) return ECSPRT_ERR_ARG;
+  if (b->lockstep) {
+    try {
+      ECS_HIP(hipSetDevice(b->device));
+      const int rc = ecsprt::lockstep_solve(*b->lockstep, moments, phi);
+      ecsprt::lockstep_stats(*b->lockstep, b->iters.data(), b->rnorm.data(), nullptr);
+      if (rc != ECSPRT_OK) b->last_error = "KSP did not converge on every problem of the batch";
+      return rc;
+    } catch (const Error &e) {
+      b->last_error = e.msg;
+      return e.code;
+    }
+  }
   const int nprob = (int)b->iters.size(), nslots = (int)b->slots.size();
   std::vector<int> rc(nslots, ECSPRT_OK);
   std::vector<std::string> msg(nslots);
@@ -295,6 +323,13 @@ int ecsprt_hip_batch_stats(const ecsprt_hip_batch *b, int *iters, double *rnorm)
 }
 
 const char *ecsprt_hip_batch_last_error(const ecsprt_hip_batch *b) { return b ? b->last_error.c_str() : ""; }
+
+int ecsprt_hip_batch_traffic(const ecsprt_hip_batch *b, double *bytes) {
+  if (!b || !bytes) return ECSPRT_ERR_ARG;
+  *bytes = 0.0;
+  if (b->lockstep) ecsprt::lockstep_stats(*b->lockstep, nullptr, nullptr, bytes);
+  return ECSPRT_OK;
+}
 
 // One timestep of every context of an ensemble of independent runs (each created by
 // ecsprt_hip_create on this device, each with its own stream): the steps run

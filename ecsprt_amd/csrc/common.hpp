@@ -252,6 +252,13 @@ hipError_t lognormal_launch(hipStream_t st, int Nx, int Ny, int Nz, uint64_t see
                             double fs, double *gen, int sg, double *b1, int s1, double *b2, int s2, double *w,
                             double *partial, double *moments);
 
+// bsolve.hip: the batched solves in lockstep (one launch per phase for the whole batch).
+struct LockstepBatch;
+LockstepBatch *lockstep_create(const ecsprt_params &p, const ecsprt_options &o, int nprob, const int *dims);
+void lockstep_destroy(LockstepBatch *b);
+int lockstep_solve(LockstepBatch &b, double *const *moments, double *const *phi);
+void lockstep_stats(const LockstepBatch &b, int *iters, double *rnorm, double *bytes);
+
 // api.hip: the message ecsprt_hip_last_error(NULL) returns on this thread.
 void set_global_error(const std::string &msg);
 

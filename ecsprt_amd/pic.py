@@ -323,8 +323,9 @@ def ensemble_step(contexts, dt):
 
 
 class Batch:
-    """Batched standalone potential solves (one per grid in `dims`) on `slots`
-    concurrent streams; the physical parameters and solver options are shared."""
+    """Batched standalone potential solves (one per grid in `dims`): on `slots`
+    concurrent streams, or with slots = 0 in lockstep, every kernel launched once for
+    all problems; the physical parameters and solver options are shared."""
 
     def __init__(self, params, options, dims, slots=8):
         L = _lib.load()
@@ -353,6 +354,12 @@ class Batch:
         _check(_lib.load().ecsprt_hip_batch_stats(self.ptr, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                   _dp(rn)))
         return it, rn
+
+    def traffic(self):
+        """Algorithmic HBM bytes of the last solve (lockstep mode)."""
+        b = ctypes.c_double()
+        _check(_lib.load().ecsprt_hip_batch_traffic(self.ptr, ctypes.byref(b)))
+        return b.value
 
     def close(self):
         if self.ptr:

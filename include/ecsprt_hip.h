@@ -298,6 +298,11 @@ int ecsprt_hip_nr_stream(int generator, long seed, long n, float *out);
  * Krylov/multigrid solve (ksp_rtol etc. from the options).  Problems share the physical
  * parameters of `p` and differ in grid shape (dims = Nx, Ny, Nz per problem). */
 typedef struct ecsprt_hip_batch ecsprt_hip_batch;
+/* nslots > 0: that many concurrent solver contexts, each with its own stream and host
+ * thread, take the problems in turn.  nslots == 0: all problems in lockstep from one
+ * host thread -- every kernel of the V-cycle, the operator and the Krylov updates is
+ * launched once for all problems still iterating (multigrid preconditioning only; each
+ * problem's coarsest level must have at most 1024 nodes, else ECSPRT_ERR_ARG). */
 int ecsprt_hip_batch_create(const ecsprt_params *p, const ecsprt_options *o, int nprob, const int *dims,
                             int nslots, ecsprt_hip_batch **out);
 int ecsprt_hip_batch_destroy(ecsprt_hip_batch *b);
@@ -307,6 +312,9 @@ int ecsprt_hip_batch_solve(ecsprt_hip_batch *b, double *const *moments, double *
 /* Per-problem Krylov iterations and final residual norms of the last batch_solve. */
 int ecsprt_hip_batch_stats(const ecsprt_hip_batch *b, int *iters, double *rnorm);
 const char *ecsprt_hip_batch_last_error(const ecsprt_hip_batch *b);
+/* Algorithmic HBM bytes of the last batch_solve in lockstep mode (each kernel's
+ * compulsory reads and writes of the problems it worked on); 0 with slots. */
+int ecsprt_hip_batch_traffic(const ecsprt_hip_batch *b, double *bytes);
 
 /* ---- Ensembles of independent runs --------------------------------------------------
  * One ecsprt_hip_step (the timestep of src/pic.c:246-258) on each of n contexts, run

@@ -133,6 +133,8 @@ def lib():
             "oracle_full_lhs2d_csr": (None, [P, _D, _I, _I, _D]),
             "oracle_full_lhs2d_apply": (None, [P, _D, _D, _D]),
             "oracle_full_rhs2d": (None, [P, _D, _D]),
+            "oracle_solve_gmres_mg": (ctypes.c_int, [P, _D, _D, _D, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                     _I, _D]),
             "oracle_particle_steps": (ctypes.c_int, [P, ctypes.c_double, ctypes.c_int, ctypes.c_long,
                                                      _D, _D, _D, _D, ctypes.POINTER(NRState),
                                                      ctypes.c_int, _D]),
@@ -462,6 +464,18 @@ def particle_steps(p, dt, nt, pos, vel, phi, nr, collide=True):
     rc = lib().oracle_particle_steps(ctypes.byref(p), dt, nt, pos.shape[0], _dp(pos), _dp(vel),
                                      _dp(phi), _dp(m), ctypes.byref(nr.state), int(collide), _dp(t))
     return rc, m, t
+
+
+def solve_gmres_mg(p, moments, b, rtol, restart=30, maxit=10000):
+    """FP64 GMRES(restart) + geometric multigrid on one core (ecsprt_solver.c): A from the
+    density of `moments`, zero initial guess, mean removed.  Returns (x, iterations,
+    ||b - A x||, converged)."""
+    x = np.zeros((p.Nz, p.Ny, p.Nx))
+    it, rn = ctypes.c_int(), ctypes.c_double()
+    rc = lib().oracle_solve_gmres_mg(ctypes.byref(p), _dp(np.ascontiguousarray(moments)),
+                                     _dp(np.ascontiguousarray(b, dtype=np.float64)), _dp(x), rtol, restart, maxit,
+                                     ctypes.byref(it), ctypes.byref(rn))
+    return x, it.value, rn.value, rc == 0
 
 
 def solve_sparse(p, moments, b):

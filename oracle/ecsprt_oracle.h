@@ -190,4 +190,10 @@ int oracle_particle_steps(const oracle_params *p, double dt, int nt, long np, do
 #ifdef __cplusplus
 }
 #endif
+/* ecsprt_solver.c: FP64 GMRES(restart) + geometric multigrid on one core, A from the
+ * density of m_op (FluidNode moments), x overwritten (zero initial guess), converged on
+ * ||b - A x|| <= rtol ||b||; returns 0 on convergence. */
+int oracle_solve_gmres_mg(const oracle_params *p, const double *m_op, const double *b, double *x, double rtol,
+                          int restart, int maxit, int *iters, double *rnorm);
+
 #endif

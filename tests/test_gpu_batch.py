@@ -166,3 +166,67 @@ def test_ensemble_step_matches_sequential_steps():
         assert a.collision_stats()[1] > 0
     for c in ens + seq:
         c.close()
+
+
+LOCKSTEP_DIMS = DIMS + [(32, 32, 32), (48, 32, 64), (64, 24, 40), (20, 28, 36)]
+
+
+@pytest.mark.parametrize("prec", [_lib.SOLVE_FP64, _lib.SOLVE_MIXED])
+def test_lockstep_batch_matches_single_context_solves(prec):
+    """The lockstep batch (slots = 0: every kernel launched once for all problems, tails at
+    level 0 and beyond, marching and box sweeps mixed in one launch) solves each problem as
+    its own context does: the same method, summed in other partial orders, so within the
+    accuracy rtol 1e-10 allows; every true residual meets rtol."""
+    say(f"lockstep batch vs single contexts, precision {prec}")
+    p = c3_params()
+    opts = pic.default_options(ksp_rtol=1e-10, solve_precision=prec)
+    mom = fields(LOCKSTEP_DIMS, base_seed=300)
+    phi = [pic.DeviceVector(d[0] * d[1] * d[2]) for d in LOCKSTEP_DIMS]
+    b = pic.Batch(p, opts, LOCKSTEP_DIMS, slots=0)
+    b.solve(mom, phi)
+    it, rn = b.stats()
+    say("  iterations", it.tolist(), "traffic GB", b.traffic() / 1e9)
+    assert (it > 0).all() and b.traffic() > 0
+    for q, d in enumerate(LOCKSTEP_DIMS):
+        cli = configs.base_cli(8, E0=0.05)
+        cli.update(Nx=d[0], Ny=d[1], Nz=d[2])
+        m = mom[q].download().reshape(d[2], d[1], d[0], 4)
+        ctx = pic.Context(pic.setup_context(**cli), pic.default_options(ksp_rtol=1e-10, solve_precision=prec))
+        ctx.set_moments(m)
+        ctx.compute_potential()
+        ref = ctx.get_potential().reshape(-1)
+        ctx.close()
+        got = phi[q].download()
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        po = oracle.setup_context(**cli)
+        bq = oracle.full_rhs(po, m)
+        r = bq - oracle.lhs_apply(po, m, got.reshape(d[2], d[1], d[0]))
+        rr = np.linalg.norm(r) / np.linalg.norm(bq)
+        say(f"  problem {q} {d}: vs single context {err:.1e}, ||b - A phi||/||b|| {rr:.1e}")
+        assert err <= 1e-7
+        assert rr <= 1.01e-10
+        assert abs(got.mean()) <= 1e-12 * np.abs(got).max()
+    b.close()
+    for v in mom + phi:
+        v.free()
+
+
+def test_lockstep_batch_repeats_bitwise():
+    """Two solves of the same batch give the same potentials bit for bit (fixed reduction
+    orders, no atomics)."""
+    p = c3_params()
+    dims = LOCKSTEP_DIMS[-4:]
+    mom = fields(dims, base_seed=400)
+    out = []
+    b = pic.Batch(p, pic.default_options(ksp_rtol=1e-8, solve_precision=_lib.SOLVE_MIXED), dims, slots=0)
+    for _ in range(2):
+        phi = [pic.DeviceVector(d[0] * d[1] * d[2]) for d in dims]
+        b.solve(mom, phi)
+        out.append([f.download() for f in phi])
+        for f in phi:
+            f.free()
+    for a, c in zip(*out):
+        np.testing.assert_array_equal(a, c)
+    b.close()
+    for v in mom:
+        v.free()
