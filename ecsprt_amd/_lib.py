@@ -105,6 +105,7 @@ SIGNATURES = {
     "ecsprt_hip_collision_count": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_long, ctypes.c_double]),
     "ecsprt_hip_collision_stats": (ctypes.c_int, [_P, _I, _L, _L, _L]),
     "ecsprt_hip_solver_stats": (ctypes.c_int, [_P, _I, _D, _D]),
+    "ecsprt_hip_solver_traffic": (ctypes.c_int, [_P, _D]),
     "ecsprt_hip_stage_times": (ctypes.c_int, [_P, _D]),
     "ecsprt_hip_synchronize": (ctypes.c_int, [_P]),
     "ecsprt_hip_last_error": (ctypes.c_char_p, [_P]),

@@ -63,6 +63,7 @@ void maybe_build_operator(ecsprt_hip_ctx *c) {
 
 void compute_potential(ecsprt_hip_ctx *c) {
   maybe_build_operator(c);
+  c->solve_bytes = 0.0;
   ecsprt::solver_rhs(c, c->d_rhs);
   ev_record(c, 4);
   int rc = ecsprt::solver_solve(c, c->d_rhs, c->d_phi);
@@ -593,6 +594,12 @@ int ecsprt_hip_solver_stats(ecsprt_hip_ctx *c, int *iters, double *rnorm, double
   if (iters) *iters = c->last_iters;
   if (rnorm) *rnorm = c->last_rnorm;
   if (bnorm) *bnorm = c->last_bnorm;
+  return ECSPRT_OK;
+}
+
+int ecsprt_hip_solver_traffic(ecsprt_hip_ctx *c, double *bytes) {
+  if (!c || !bytes) return ECSPRT_ERR_ARG;
+  *bytes = c->solve_bytes;
   return ECSPRT_OK;
 }
 

@@ -331,25 +331,28 @@ __device__ inline void pilot_ion(const ParticlesMix &A, long s, double &vx, doub
   id = (int)b[64 * MX_ID];
 }
 
-// Tile t takes part in pilot pass `tag` with sampling stride `stride` (a power of two):
-// a hashed selection, so a sample is not a regular lattice of tiles.
-__device__ inline bool pilot_tile(int t, uint32_t tag, int stride) {
-  uint32_t h = (uint32_t)t * 0x9E3779B1u ^ tag * 0x85EBCA6Bu;
+// Pilot pass `tag` with sampling stride `stride` (a power of two) visits every ion of one
+// tile out of each run of `stride` consecutive tiles, picked by a hash of (run, tag): a
+// stratified sample of whole tiles, so the result does not depend on the order of the
+// ions inside a tile.  Each sampled tile is shared by gridDim.y blocks.
+__device__ inline int pilot_tile(int j, uint32_t tag, int stride) {
+  uint32_t h = (uint32_t)j * 0x9E3779B1u ^ tag * 0x85EBCA6Bu;
   h ^= h >> 15;
   h *= 0x2C1B3C6Du;
   h ^= h >> 12;
-  return (h & (uint32_t)(stride - 1)) == 0;
+  return j * stride + (int)(h & (uint32_t)(stride - 1));
 }
 
 // ctr[6] attempts, ctr[7] collisions, ctr[8] ions visited.
 template <class P>
-__global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k, PoissonParams pt, uint32_t key0,
-                        uint32_t key1, uint32_t tag, int stride, unsigned long long *ctr) {
-  const int t = blockIdx.x;
-  if (stride > 1 && !pilot_tile(t, tag, stride)) return;
+__global__ void k_pilot(P A, const int *base, const int *count, int ntiles, CollideConsts k, PoissonParams pt,
+                        uint32_t key0, uint32_t key1, uint32_t tag, int stride, unsigned long long *ctr) {
+  const int t = pilot_tile(blockIdx.x, tag, stride);
+  if (t >= ntiles) return;
   const long b = base[t], n = count[t];
   unsigned natt = 0, nsucc = 0, nfail = 0;
-  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+  const long step = (long)blockDim.x * gridDim.y;
+  for (long q = (long)blockIdx.y * blockDim.x + threadIdx.x; q < n; q += step) {
     const long s = b + q;
     double vx, vy, vz;
     int id;
@@ -361,7 +364,7 @@ __global__ void k_pilot(P A, const int *base, const int *count, CollideConsts k,
     atomicAdd(&ctr[6], na);
     atomicAdd(&ctr[7], ns);
   }
-  if (threadIdx.x == 0) atomicAdd(&ctr[8], (unsigned long long)n);
+  if (threadIdx.x == 0 && blockIdx.y == 0) atomicAdd(&ctr[8], (unsigned long long)n);
 }
 
 __global__ void k_zero_pilot_ctr(unsigned long long *ctr) {
@@ -378,10 +381,18 @@ static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int s
   CollideConsts k = collide_consts_of(c->p);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
   const PoissonParams pp = poisson_params(lam);
+  // Blocks per sampled tile: enough for about 2048 blocks in all, at most one per 256
+  // ions of an average tile, so that a sampled pass is not a few long serial blocks.
+  const int ns = (c->ntiles + stride - 1) / stride;
+  const long per_tile = c->np / std::max(1, c->ntiles);
+  const int split = (int)std::max<long>(1, std::min<long>((2048 + ns - 1) / ns, per_tile / 256));
+  const dim3 grid(ns, split);
   if (c->mixed)
-    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, k, pp, k0, k1, tag, stride, c->d_ctr);
+    k_pilot<<<grid, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag, stride,
+                                         c->d_ctr);
   else
-    k_pilot<<<c->ntiles, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, k, pp, k0, k1, tag, stride, c->d_ctr);
+    k_pilot<<<grid, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag, stride,
+                                         c->d_ctr);
   ECS_HIP(hipGetLastError());
   read_ctr(c);
   return {(double)c->h_ctr[7], (double)c->h_ctr[8]};

@@ -200,6 +200,7 @@ struct ecsprt_hip_ctx {
   bool operator_built = false;
   int last_iters = 0;
   double last_rnorm = 0.0, last_bnorm = 0.0;
+  double solve_bytes = 0.0;  // algorithmic HBM bytes of the last potential solve (forcing included)
 
   // timing
   hipEvent_t ev[8] = {};

@@ -22,6 +22,10 @@
 namespace ecsprt {
 
 // ---- host structures ------------------------------------------------------------
+// Algorithmic HBM bytes of the solve: every launch adds its compulsory reads and writes
+// (each distinct array read or written once per point).
+static inline void tally(ecsprt_hip_ctx *c, double bytes) { c->solve_bytes += bytes; }
+
 static int gsz(long n) {
   long b = (n + 255) / 256;
   return (int)std::max(1L, std::min(b, 8192L));
@@ -278,6 +282,7 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
     for (int k = 1; k <= nsw; ++k) {
       T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
       launch_sweep(st, L, in, rhs(l), out, 0, c->o.mg_sweep);
+      tally(c, (double)sizeof(T) * L.n * (in ? 4 : 3));
       in = out;
     }
     cur[l] = const_cast<T *>(in);
@@ -291,6 +296,7 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
     k_residual<T><<<gf.grid, gf.block, 0, st>>>(L.s, L.dn, in, rhs(l), L.r);
     k_restrict<T><<<gc.grid, gc.block, 0, st>>>(L.to_coarse, L.r, mg.L[l + 1].b);
     ECS_HIP(hipGetLastError());
+    tally(c, (double)sizeof(T) * (5 * L.n + mg.L[l + 1].n));
   }
   if (lt < nl) {
     TailArgs<T> a = mg.targs;
@@ -301,6 +307,9 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
     k_tail<T><<<1, TAIL_THREADS, mg.tail_lds, st>>>(a);
     ECS_HIP(hipGetLastError());
     cur[lt] = a.xout;
+    double tb = 2.0 * mg.L[lt].n + (double)mg.L.back().n * mg.L.back().n;
+    for (int l = lt; l < nl; ++l) tb += (double)mg.L[l].n;
+    tally(c, sizeof(T) * tb);
   }
   for (int l = std::min(lt, nl - 1) - 1; l >= 0; --l) {
     Level<T> &L = mg.L[l];
@@ -311,15 +320,18 @@ static void mg_vcycle(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
       // the first post-smoothing sweep adds the coarse correction as it loads x
       T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
       launch_sweep(st, L, in, rhs(l), out, 1, c->o.mg_sweep, (const T *)cur[l + 1], L.to_coarse);
+      tally(c, (double)sizeof(T) * (4 * L.n + mg.L[l + 1].n));
       in = out;
       ++k;
     } else {
       const Launch3 gf = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
       k_prolong_add<T><<<gf.grid, gf.block, 0, st>>>(L.to_coarse, cur[l + 1], cur[l]);
+      tally(c, (double)sizeof(T) * (2 * L.n + mg.L[l + 1].n));
     }
     for (; k <= total; ++k) {
       T *out = ((total - k) % 2 == 0) ? home(l) : tmp(l);
       launch_sweep(st, L, in, rhs(l), out, 1, c->o.mg_sweep);
+      tally(c, (double)sizeof(T) * 4 * L.n);
       in = out;
     }
     ECS_HIP(hipGetLastError());
@@ -332,10 +344,12 @@ static void precond(ecsprt_hip_ctx *c, MG<T> &mg, const T *r, T *z) {
   Level<T> &L = mg.L[0];
   if (pc == ECSPRT_PC_NONE) {
     ECS_HIP(hipMemcpyAsync(z, r, sizeof(T) * L.n, hipMemcpyDeviceToDevice, c->stream));
+    tally(c, 2.0 * sizeof(T) * L.n);
   } else if (pc == ECSPRT_PC_JACOBI) {
     const Launch3 g3 = launch3(L.s.Nx, L.s.Ny, L.s.Nz);
     k_jacobi<T><<<g3.grid, g3.block, 0, c->stream>>>(L.s, L.dn, r, z);
     ECS_HIP(hipGetLastError());
+    tally(c, 3.0 * sizeof(T) * L.n);
   } else {
     mg_vcycle(c, mg, r, z);
   }
@@ -389,6 +403,7 @@ static void dots(ecsprt_hip_ctx *c, Gmres<T> &g, int m, T *const *dV, const T *w
   const int nb = (int)std::min<long>(RED_BLOCKS, (g.n + RED_THREADS - 1) / RED_THREADS);
   k_dot_multi<T><<<nb, RED_THREADS, 0, c->stream>>>(m, dV, w, g.n, g.partial);
   k_reduce_partials<<<1, 256, 0, c->stream>>>(m, nb, g.partial, dst, accumulate);
+  tally(c, (double)sizeof(T) * (m + 1) * g.n);
   ECS_HIP(hipGetLastError());
 }
 
@@ -428,6 +443,7 @@ static void dot_single(ecsprt_hip_ctx *c, double *partial, const T *a, const T *
   const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
   k_dot1<T><<<nb, RED_THREADS, 0, c->stream>>>(a, b, n, partial);
   k_reduce_partials<<<1, 256, 0, c->stream>>>(1, nb, partial, dst, 0);
+  tally(c, (double)sizeof(T) * (a == b ? 1 : 2) * n);
   ECS_HIP(hipGetLastError());
 }
 
@@ -437,6 +453,7 @@ static void remove_mean_dev(ecsprt_hip_ctx *c, double *partial, double *sumdst, 
   k_sum1<T><<<nb, RED_THREADS, 0, c->stream>>>(v, n, partial);
   k_reduce_partials<<<1, 256, 0, c->stream>>>(1, nb, partial, sumdst, 0);
   k_sub_scalar<T><<<gsz(n), 256, 0, c->stream>>>(v, sumdst, n);
+  tally(c, 3.0 * sizeof(T) * n);
   ECS_HIP(hipGetLastError());
 }
 
@@ -467,6 +484,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
   while (true) {
     k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
     ECS_HIP(hipGetLastError());
+    tally(c, 4.0 * sizeof(T) * n);
     dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
     beta = std::sqrt(fetch1(c, g.dh, g.hpin));
     rnorm = beta;
@@ -477,6 +495,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
     if (!std::isfinite(beta)) break;
     if (it >= maxit) break;
     k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.r, g.V[0], g.dh, n, 1);
+    tally(c, 2.0 * sizeof(T) * n);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
     int j = 0;
@@ -485,6 +504,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       precond<T>(c, mg, g.V[j], g.Z[j]);
       k_apply<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, g.Z[j], g.w);
       ECS_HIP(hipGetLastError());
+      tally(c, 3.0 * sizeof(T) * n);
       // Classical Gram-Schmidt, h = V^T w; w -= V h; |w|^2 (PETSc's default), or with
       // gmres_refine CGS2: h2 = V^T w; w -= V h2 before the norm.  Up to 8 basis vectors
       // the subtractions carry the following products in the same pass.
@@ -494,8 +514,10 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
           const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
           k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh, g.w, n, g.partial, 1);
           k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (j + 1), 0);
+          tally(c, (double)sizeof(T) * (j + 3) * n);
         } else {
           k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
+          tally(c, (double)sizeof(T) * (j + 3) * n);
           dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
         }
       } else if (j + 1 <= 8) {
@@ -504,10 +526,12 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
         k_reduce_partials<<<1, 256, 0, st>>>(j + 1, nb, g.partial, g.dh + (m + 2), 0);
         k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n, g.partial, 1);
         k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (j + 1), 0);
+        tally(c, 2.0 * sizeof(T) * (j + 3) * n);
       } else {
         k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
         dots<T>(c, g, j + 1, g.dV, g.w, g.dh + (m + 2), 0);
         k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n);
+        tally(c, 2.0 * sizeof(T) * (j + 3) * n);
         dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
       }
       ECS_HIP(hipGetLastError());
@@ -521,7 +545,10 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       }
       H[(size_t)(j + 1) * m + j] = hn;
       // w/hn -> V[j+1]
-      if (hn > 0) k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.w, g.V[j + 1], g.dh + (j + 1), n, 1);
+      if (hn > 0) {
+        k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.w, g.V[j + 1], g.dh + (j + 1), n, 1);
+        tally(c, 2.0 * sizeof(T) * n);
+      }
       for (int q = 0; q < j; ++q) {
         const double a = H[(size_t)q * m + j], bb = H[(size_t)(q + 1) * m + j];
         H[(size_t)q * m + j] = cs[q] * a + sn[q] * bb;
@@ -551,7 +578,10 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
     }
     for (int q = 0; q < j; ++q) g.ypin[q] = y[q];
     ECS_HIP(hipMemcpyAsync(g.dy, g.ypin, sizeof(double) * std::max(j, 1), hipMemcpyHostToDevice, st));
-    if (j > 0) k_add_multi<T><<<gsz(n), 256, 0, st>>>(j, g.dZ, g.dy, x, n, 1.0);
+    if (j > 0) {
+      k_add_multi<T><<<gsz(n), 256, 0, st>>>(j, g.dZ, g.dy, x, n, 1.0);
+      tally(c, (double)sizeof(T) * (j + 2) * n);
+    }
     ECS_HIP(hipGetLastError());
     ECS_HIP(hipStreamSynchronize(st));
     if (single_cycle) {
@@ -561,6 +591,7 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
     }
     if (it >= maxit) {
       k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
+      tally(c, 4.0 * sizeof(T) * n);
       dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
       rnorm = std::sqrt(fetch1(c, g.dh, g.hpin));
       *converged = rnorm <= tol;
@@ -617,6 +648,7 @@ void solver_rhs(ecsprt_hip_ctx *c, double *b) {
   const Launch3 g3 = launch3(p.Nx, p.Ny, p.Nz);
   k_rhs<<<g3.grid, g3.block, 0, c->stream>>>(k, c->d_moments, b);
   ECS_HIP(hipGetLastError());
+  tally(c, 40.0 * c->nnode);
   solver_remove_mean(c, b);
 }
 
@@ -656,6 +688,7 @@ size_t solver_arena_bytes(const ecsprt_options &o, long n) {
 // the RHS of ComputeFullRHS3D, solution into c->d_phi.  The solver is built and
 // released around the solve (from the arena when the context has one).
 int solver_standalone(ecsprt_hip_ctx *c) {
+  c->solve_bytes = 0.0;
   double *rhs = nullptr;
   sv_alloc(c, &rhs, sizeof(double) * c->nnode);
   solver_create(c);
@@ -685,6 +718,7 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
   if (c->o.initial_guess == 0) {
     k_fill<double><<<gsz(n), 256, 0, c->stream>>>(x, n, 0.0);
     ECS_HIP(hipGetLastError());
+    tally(c, 8.0 * n);
   }
   dot_single<double>(c, s->g64.partial, b, b, n, s->g64.dh);
   const double bnorm = std::sqrt(fetch1(c, s->g64.dh, s->g64.hpin));
@@ -709,6 +743,7 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
       const Launch3 g3 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
       k_residual<double><<<g3.grid, g3.block, 0, c->stream>>>(L0.s, L0.dn, x, b, s->r64);
       ECS_HIP(hipGetLastError());
+      tally(c, 32.0 * n);
       dot_single<double>(c, s->g64.partial, s->r64, s->r64, n, s->g64.dh);
       rnorm = std::sqrt(fetch1(c, s->g64.dh, s->g64.hpin));
       if (rnorm <= tol) {
@@ -719,12 +754,14 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
       if (iters >= maxit) break;
       k_convert<double, float><<<gsz(n), 256, 0, c->stream>>>(s->r64, s->b32, n, 1.0 / rnorm);
       k_fill<float><<<gsz(n), 256, 0, c->stream>>>(s->x32, n, 0.0f);
+      tally(c, 16.0 * n);
       double rn32 = 0.0;
       bool c32 = false;
       iters += gmres<float>(c, s->mg32, s->g32, s->b32, s->x32, inner, maxit - iters, rn32, &c32, true);
       // x += rnorm * e, accumulated in FP64.
       k_convert<float, double><<<gsz(n), 256, 0, c->stream>>>(s->x32, s->r64, n, rnorm);
       k_axpy_from<double><<<gsz(n), 256, 0, c->stream>>>(s->r64, x, n, 1.0);
+      tally(c, 36.0 * n);
       ECS_HIP(hipGetLastError());
     }
   } else {

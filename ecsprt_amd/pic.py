@@ -249,6 +249,12 @@ class Context:
         self._c(self.L.ecsprt_hip_solver_stats(self.ptr, ctypes.byref(it), ctypes.byref(rn), ctypes.byref(bn)))
         return it.value, rn.value, bn.value
 
+    def solver_traffic(self):
+        """Algorithmic HBM bytes of the last potential solve (forcing included)."""
+        b = ctypes.c_double()
+        self._c(self.L.ecsprt_hip_solver_traffic(self.ptr, ctypes.byref(b)))
+        return b.value
+
     def stage_times(self):
         ms = np.zeros(6)
         self._c(self.L.ecsprt_hip_stage_times(self.ptr, _dp(ms)))

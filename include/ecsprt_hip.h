@@ -249,6 +249,10 @@ int ecsprt_hip_collision_stats(ecsprt_hip_ctx *ctx, int *Nc, long *Ns, long *Nf,
 
 /* Last solve: iterations, final true-residual norm, ||b||. */
 int ecsprt_hip_solver_stats(ecsprt_hip_ctx *ctx, int *iters, double *rnorm, double *bnorm);
+/* Algorithmic HBM bytes of the last ComputePotential (forcing, mean removal and KSP
+ * solve): every kernel's compulsory reads and writes, each array once per point (the
+ * numerator of the solve's roofline fraction). */
+int ecsprt_hip_solver_traffic(ecsprt_hip_ctx *ctx, double *bytes);
 /* Per-stage times of the last call that recorded them (ms, HIP events on the context's
  * stream, options.timing = 1).  ecsprt_hip_step: [0] fused particle kernel, [1] set-up
  * before it, [2] re-binning and fold, [4] operator + RHS, [5] KSP solve.  Stage calls:
