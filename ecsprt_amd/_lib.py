@@ -135,6 +135,17 @@ SIGNATURES = {
     "ecsprt_hip_lognormal_moments_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, _P,
                                                             _P]),
+    "ecsprt_hip_output_fluid_hdf5": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "ecsprt_hip_output_swarm_binary": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "ecsprt_hip_load_fluid_quantities": (ctypes.c_int, [_P, _D, ctypes.c_char_p]),
+    "ecsprt_io_hdf5_available": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int]),
+    "ecsprt_io_last_error": (ctypes.c_char_p, []),
+    "ecsprt_io_write_fluid_hdf5": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, _D, _D, _D]),
+    "ecsprt_io_write_hdf5_dataset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, _L, _D]),
+    "ecsprt_io_read_hdf5_dataset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _I, _L, _D, ctypes.c_long]),
+    "ecsprt_io_write_vec_binary": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_long, ctypes.c_int, _D]),
+    "ecsprt_io_read_vec_binary": (ctypes.c_int, [ctypes.c_char_p, _D, ctypes.c_long, _L, _I]),
 }
 
 _lib = None

@@ -25,7 +25,7 @@ HIP_FLAGS = [
     f"-I{os.path.join(ROOT, 'include')}",
 ]
 SOURCES = ["api.hip", "particles.hip", "particles_mixed.hip", "particles2d.hip", "collide.hip", "solver.hip", "batch.hip",
-           "refinit.hip", "noalloc.hip", "bsolve.hip"]
+           "refinit.hip", "noalloc.hip", "bsolve.hip", "fileio.hip"]
 # The FP64 path is built without contraction so that it reproduces the reference's
 # rounding; the mixed-precision kernels are not bit-compatible by design and may fuse.
 EXTRA_FLAGS = {"particles_mixed.hip": ["-ffp-contract=fast"]}

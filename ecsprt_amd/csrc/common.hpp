@@ -198,6 +198,7 @@ struct ecsprt_hip_ctx {
   ecsprt::Solver *solver = nullptr;
   ecsprt::Arena *arena = nullptr;  // null: solver buffers come from hipMalloc
   bool operator_built = false;
+  bool rhs_sinusoidal = false;  // forcing of ComputeSinusoidalRHS3D (LoadFluidQuantities given no file)
   int last_iters = 0;
   double last_rnorm = 0.0, last_bnorm = 0.0;
   double solve_bytes = 0.0;  // algorithmic HBM bytes of the last potential solve (forcing included)

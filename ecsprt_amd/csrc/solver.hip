@@ -646,9 +646,14 @@ void solver_rhs(ecsprt_hip_ctx *c, double *b) {
   const ecsprt_params &p = c->p;
   const RhsConsts k = rhs_consts(p);
   const Launch3 g3 = launch3(p.Nx, p.Ny, p.Nz);
-  k_rhs<<<g3.grid, g3.block, 0, c->stream>>>(k, c->d_moments, b);
+  if (c->rhs_sinusoidal) {
+    k_rhs_sinusoidal<<<g3.grid, g3.block, 0, c->stream>>>(p.Nx, p.Ny, p.dx, p.dy, p.dz, p.n0, p.scale, b);
+    tally(c, 8.0 * c->nnode);
+  } else {
+    k_rhs<<<g3.grid, g3.block, 0, c->stream>>>(k, c->d_moments, b);
+    tally(c, 40.0 * c->nnode);
+  }
   ECS_HIP(hipGetLastError());
-  tally(c, 40.0 * c->nnode);
   solver_remove_mean(c, b);
 }
 

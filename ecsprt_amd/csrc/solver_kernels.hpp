@@ -763,6 +763,17 @@ static __global__ void k_rhs(RhsConsts c, const double *__restrict__ m, double *
   b[q] = rhs_point(c, m, i, j, k);
 }
 
+// ComputeSinusoidalRHS3D body (src/rhs.c:120-133): n0 scale cos(2 pi x) cos(2 pi y)
+// cos(2 pi z) at the cell centres x = (i + 1/2) dx, ...
+static __global__ void k_rhs_sinusoidal(int Nx, int Ny, double dx, double dy, double dz, double n0, double scale,
+                                        double *__restrict__ b) {
+  NODE_IJK(Nx, Ny)
+  const double PI = 3.14159265358979323846;
+  const double x = ((double)i + 0.5) * dx, y = ((double)j + 0.5) * dy, z = ((double)k + 0.5) * dz;
+  const double val = cos(2 * PI * x) * cos(2 * PI * y) * cos(2 * PI * z);
+  b[q] = n0 * val * scale;
+}
+
 // Host: the stencil constants of a level, the forcing constants, and the dense inverse
 // of the coarsest operator plus alpha 1 1^T (Gauss-Jordan with partial pivoting in FP64)
 // from the level's density dnh.

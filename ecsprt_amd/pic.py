@@ -16,6 +16,7 @@ non-zero PetscErrorCode; the reference's Abort conditions (particle count
 changed, ten refused collisions) are such codes.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -215,6 +216,22 @@ class Context:
         m = np.ascontiguousarray(m, dtype=np.float64)
         assert m.shape == self.shape + (4,)
         self._c(self.L.ecsprt_hip_set_moments(self.ptr, _dp(m)))
+
+    # ---- on-disk formats (src/file-io.c) ------------------------------------------------------
+    def output_fluid_hdf5(self, insert=""):
+        """OutputFluidHDF5: "fluid<insert>.hdf" in the working directory."""
+        self._c(self.L.ecsprt_hip_output_fluid_hdf5(self.ptr, insert.encode()))
+
+    def output_swarm_binary(self, insert=""):
+        """OutputSwarmBinary: "position<insert>.bin" and "velocity<insert>.bin"."""
+        self._c(self.L.ecsprt_hip_output_swarm_binary(self.ptr, insert.encode()))
+
+    def load_fluid_quantities(self, flux_scale, inpath):
+        """LoadFluidQuantities: density from "density-kji" of inpath, flux = flux_scale * n;
+        an empty path switches the forcing to ComputeSinusoidalRHS3D."""
+        fs = np.ascontiguousarray(flux_scale, dtype=np.float64)
+        assert fs.shape == (3,)
+        self._c(self.L.ecsprt_hip_load_fluid_quantities(self.ptr, _dp(fs), os.fsencode(inpath) if inpath else b""))
 
     def get_potential(self):
         phi = np.empty(self.shape)

@@ -347,6 +347,49 @@ int ecsprt_hip_lognormal_moments(int Nx, int Ny, int Nz, uint64_t seed, double s
 int ecsprt_hip_lognormal_moments_noalloc(int Nx, int Ny, int Nz, uint64_t seed, double sigma, double smooth,
                                          double flux_scale, double *moments, void *stream);
 
+/* ---- On-disk formats (src/file-io.c) ----------------------------------------------
+ * HDF5 is loaded at run time (ECSPRT_HDF5_LIBRARY alone when set, else libhdf5.so on the
+ * loader's path, else /opt/conda/lib/libhdf5.so.103; release 1.10 or later); without it the HDF5 entry
+ * points return ECSPRT_ERR_STATE.  Files follow PETSc 3.20's viewers: HDF5 datasets in the
+ * root group shaped [Nz][Ny][Nx] ([Ny][Nx] in 2-D), native doubles, attribute
+ * "timestepping" = 0; binary vectors as big-endian int32 classid 1211214, int32 length and
+ * big-endian doubles, with "-vecload_block_size <bs>" in "<file>.info". */
+
+/* OutputFluidHDF5 (src/file-io.c:164-222): writes "fluid<insert>.hdf" in the working
+ * directory with the datasets "density", "x flux", "y flux", "z flux" (the FluidNode
+ * components), "rhs" (the last forcing) and "potential". */
+int ecsprt_hip_output_fluid_hdf5(ecsprt_hip_ctx *ctx, const char *insert);
+
+/* OutputSwarmBinary (src/file-io.c:103-161): writes "position<insert>.bin" and
+ * "velocity<insert>.bin" (ndim values per ion, block size ndim; ions in the library's
+ * order, as the reference's are in its swarm's order). */
+int ecsprt_hip_output_swarm_binary(ecsprt_hip_ctx *ctx, const char *insert);
+
+/* LoadFluidQuantities (src/file-io.c:8-75): reads the density from dataset "density-kji" of
+ * the HDF5 file inpath (shape of the grid) and sets the moments to (n, s0 n, s1 n, s2 n)
+ * with s = flux_scale; the next potential solve builds its operator from them.  An empty
+ * or NULL inpath switches the forcing to ComputeSinusoidalRHS3D (src/rhs.c:86-149)
+ * instead, as the reference does. */
+int ecsprt_hip_load_fluid_quantities(ecsprt_hip_ctx *ctx, const double flux_scale[3], const char *inpath);
+
+/* Host-buffer forms of the same formats, for callers that hold their own arrays.
+ * ecsprt_io_hdf5_available returns 1 when HDF5 loads, else 0 with the reason in msg;
+ * ecsprt_io_last_error is the message of this thread's last failed ecsprt_io call. */
+int ecsprt_io_hdf5_available(char *msg, int cap);
+const char *ecsprt_io_last_error(void);
+/* moments: FluidNode array [N][4]; rhs, phi: [N] (N = Nx Ny Nz, or Nx Ny in 2-D). */
+int ecsprt_io_write_fluid_hdf5(const char *path, int ndim, int Nx, int Ny, int Nz, const double *moments,
+                               const double *rhs, const double *phi);
+/* One dataset of ndims (<= 4) dimensions in a new file, e.g. a "density-kji" input. */
+int ecsprt_io_write_hdf5_dataset(const char *path, const char *name, int ndims, const long *dims, const double *data);
+/* Shape of dataset `name` into ndims and dims[4]; with out, its values (at most cap). */
+int ecsprt_io_read_hdf5_dataset(const char *path, const char *name, int *ndims, long *dims, double *out, long cap);
+/* PETSc binary vector of n values with block size bs (VecView / VecLoad layout). */
+int ecsprt_io_write_vec_binary(const char *path, long n, int bs, const double *v);
+/* Its length into n and, with v, its values (at most cap); bs from the .info file (1 when
+ * absent; may be NULL). */
+int ecsprt_io_read_vec_binary(const char *path, double *v, long cap, long *n, int *bs);
+
 #ifdef __cplusplus
 }
 #endif

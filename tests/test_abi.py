@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "ecsprt_hip.h")).read()
-    return sorted(set(re.findall(r"\b(ecsprt_hip_\w+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(ecsprt_(?:hip|io)_\w+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
