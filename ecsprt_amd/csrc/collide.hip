@@ -343,22 +343,56 @@ __device__ inline int pilot_tile(int j, uint32_t tag, int stride) {
   return j * stride + (int)(h & (uint32_t)(stride - 1));
 }
 
-// ctr[6] attempts, ctr[7] collisions, ctr[8] ions visited.
+// ctr[6] attempts, ctr[7] collisions, ctr[8] ions visited.  The ions with attempts
+// (about one in eight at 128^3 x 100 ions per cell) are queued in LDS and run their
+// attempts with every lane busy, PILOT_CHUNK at a time.
+constexpr int PILOT_CHUNK = 256, PILOT_QUEUE = 2 * PILOT_CHUNK;
+
 template <class P>
-__global__ void k_pilot(P A, const int *base, const int *count, int ntiles, CollideConsts k, PoissonParams pt,
-                        uint32_t key0, uint32_t key1, uint32_t tag, int stride, unsigned long long *ctr) {
+__global__ void __launch_bounds__(PILOT_CHUNK) k_pilot(P A, const int *base, const int *count, int ntiles,
+                                                       CollideConsts k, PoissonParams pt, uint32_t key0, uint32_t key1,
+                                                       uint32_t tag, int stride, unsigned long long *ctr) {
+  __shared__ double qv[3][PILOT_QUEUE];
+  __shared__ int qid[PILOT_QUEUE], qk[PILOT_QUEUE], qn;
   const int t = pilot_tile(blockIdx.x, tag, stride);
   if (t >= ntiles) return;
+  const int tid = threadIdx.x;
   const long b = base[t], n = count[t];
   unsigned natt = 0, nsucc = 0, nfail = 0;
-  const long step = (long)blockDim.x * gridDim.y;
-  for (long q = (long)blockIdx.y * blockDim.x + threadIdx.x; q < n; q += step) {
-    const long s = b + q;
-    double vx, vy, vz;
-    int id;
-    pilot_ion(A, s, vx, vy, vz, id);
-    collide_philox(k, pt, key0, key1, tag, id, vx, vy, vz, natt, nsucc, nfail);
+  auto run = [&](int e) {  // the attempts of queue entry e
+    double vx = qv[0][e], vy = qv[1][e], vz = qv[2][e];
+    collide_attempts<double>(k, key0, key1, tag, qid[e], 0, qk[e], vx, vy, vz, natt, nsucc, nfail);
+  };
+  if (tid == 0) qn = 0;
+  __syncthreads();
+  const long step = (long)PILOT_CHUNK * gridDim.y;
+  for (long c0 = (long)blockIdx.y * PILOT_CHUNK; c0 < n; c0 += step) {
+    const long q = c0 + tid;
+    if (q < n) {
+      double vx, vy, vz;
+      int id;
+      pilot_ion(A, b + q, vx, vy, vz, id);
+      const int na = poisson_attempts(pt, key0, key1, tag, id);
+      if (na > 0) {  // at most PILOT_CHUNK entries join a queue holding fewer than PILOT_CHUNK
+        const int r = atomicAdd(&qn, 1);
+        qv[0][r] = vx;
+        qv[1][r] = vy;
+        qv[2][r] = vz;
+        qid[r] = id;
+        qk[r] = na;
+      }
+    }
+    __syncthreads();
+    const int m = qn;  // every thread reads it before any changes it again
+    __syncthreads();
+    if (m >= PILOT_CHUNK) {
+      run(m - PILOT_CHUNK + tid);
+      __syncthreads();
+      if (tid == 0) qn = m - PILOT_CHUNK;
+      __syncthreads();
+    }
   }
+  if (tid < qn) run(tid);
   const unsigned long long na = wave_sum(natt), ns = wave_sum(nsucc);
   if ((threadIdx.x & 63) == 0) {
     atomicAdd(&ctr[6], na);
@@ -388,11 +422,11 @@ static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int s
   const int split = (int)std::max<long>(1, std::min<long>((2048 + ns - 1) / ns, per_tile / 256));
   const dim3 grid(ns, split);
   if (c->mixed)
-    k_pilot<<<grid, 256, 0, c->stream>>>(c->Am, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag, stride,
-                                         c->d_ctr);
+    k_pilot<<<grid, PILOT_CHUNK, 0, c->stream>>>(c->Am, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag,
+                                                 stride, c->d_ctr);
   else
-    k_pilot<<<grid, 256, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag, stride,
-                                         c->d_ctr);
+    k_pilot<<<grid, PILOT_CHUNK, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag,
+                                                 stride, c->d_ctr);
   ECS_HIP(hipGetLastError());
   read_ctr(c);
   return {(double)c->h_ctr[7], (double)c->h_ctr[8]};
@@ -410,7 +444,7 @@ void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt) {
 }
 
 // The attempt rate of this step, set from the current velocities before the step's
-// draws: two dry-run passes on a sample of the tiles (about 2 * 10^4 expected
+// draws: up to two dry-run passes on a sample of the tiles (about 2 * 10^4 expected
 // collisions, at least 1/64 of the ions), each rescaling lambda by Nc / (predicted
 // collisions), on the same sample so that the second corrects the first's bias.  A rate
 // carried over from the last step lags the state: in a run that cools on cold neutrals
@@ -425,7 +459,11 @@ void collide_predict(ecsprt_hip_ctx *c, int Nc) {
     const PilotResult r = pilot_pass(c, tag, lam, stride);
     if (!(r.collisions > 0.0) || !(r.ions > 0.0)) break;
     const double predicted = r.collisions * (double)c->np / r.ions;
-    lam = std::min(lam * std::min(2.0, std::max(0.5, (double)Nc / predicted)), POISSON_LAMBDA_MAX);
+    const double ratio = std::min(2.0, std::max(0.5, (double)Nc / predicted));
+    lam = std::min(lam * ratio, POISSON_LAMBDA_MAX);
+    // A rescale within 1% leaves a second-order error in the rescale itself, far below
+    // the sample's own noise: the second pass is skipped.
+    if (std::fabs(ratio - 1.0) <= 0.01) break;
   }
   c->coll_lambda = lam;
 }
