@@ -590,7 +590,7 @@ void build_program(LockstepBatch &B) {
       g.out_js = out.js;
       g.prob = q;
       const double by = (double)sizeof(T) * L.n * (k == 1 ? 3 : 4);
-      if (L.n >= SZ_MIN_POINTS) {
+      if (L.n >= SZ_MIN_POINTS_BATCH && L.s.Nz >= 8) {
         g.kz = sweep_kz(L.s.Nx, L.s.Ny, L.s.Nz);
         g.start = bz;
         bz += ((L.s.Nx + SZ_TX - 1) / SZ_TX) * ((L.s.Ny + SZ_TY - 1) / SZ_TY) * ((L.s.Nz + g.kz - 1) / g.kz);

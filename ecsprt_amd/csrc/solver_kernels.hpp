@@ -288,19 +288,22 @@ __global__ void __launch_bounds__(TX * TY) k_sweep2d(StencilConsts s, const T *_
 // and of the density enter LDS rings one at a time with an xy halo of 2 (the next
 // plane's loads are in flight while the current one is relaxed).  The first colour is
 // relaxed on the plane below the newest, over the tile plus a halo of 1, into a third
-// ring; the second colour on the plane below that, over the tile, from the third ring.
+// ring; the second colour three planes below the newest, over the tile, from the third
+// ring (its three first-colour planes were all finished before this plane's barrier).
 // The first colour reads old values everywhere (the 11-point stencil's xy diagonals
 // join nodes of one colour), the second reads the first colour's new values and its
-// own colour's old ones, so the result does not depend on thread order.  Rings of 4, 5
-// and 4 planes keep every plane until no thread can still read it, with two barriers
-// per plane.
+// own colour's old ones, so the result does not depend on thread order.  One barrier
+// per plane: rings of 4, 6 and 4 planes keep every plane until no thread can still
+// read it (a thread storing plane L+1 before its barrier overlaps threads still
+// relaxing after plane L's, which read potential planes L-2..L, density L-4..L and
+// first-colour planes L-4..L-2).
 template <typename T, int TX, int TY>
 __device__ inline void sweep_z(const StencilConsts &s, const T *__restrict__ dn, const T *__restrict__ xin,
                                const T *__restrict__ b, T *__restrict__ xout, int first, int KZ, int bid) {
   constexpr int NT = TX * TY;                            // one thread per tile column
   constexpr int HX = TX + 4, HY = TY + 4, HP = HX * HY;  // potential and density planes
   constexpr int RX = TX + 2, RY = TY + 2, RP = RX * RY;  // first-colour planes
-  constexpr int NXO = 4, ND = 5, NSN = 4;
+  constexpr int NXO = 4, ND = 6, NSN = 4;
   constexpr int LPT = (HP + NT - 1) / NT;
   __shared__ T sxo[NXO * HP], sd[ND * HP], ssn[NSN * RP];
   const int Nx = s.Nx, Ny = s.Ny, Nz = s.Nz;
@@ -330,10 +333,11 @@ __device__ inline void sweep_z(const StencilConsts &s, const T *__restrict__ dn,
       }
   };
   // One plane step: plane L enters LDS from registers, which then take plane L+2, so
-  // two planes of loads are in flight while the relaxations run.
+  // two planes of loads are in flight while the relaxations run.  Step Lend + 1 stores
+  // nothing and only finishes the last plane's second colour.
   auto step = [&](int L, T *rx, T *rd) {
     const int r = L - Lbeg;
-    {
+    if (L <= Lend) {
       T *xo = sxo + (r % NXO) * HP, *dd = sd + (r % ND) * HP;
 #pragma unroll
       for (int u = 0; u < LPT; ++u)
@@ -344,7 +348,7 @@ __device__ inline void sweep_z(const StencilConsts &s, const T *__restrict__ dn,
     }
     __syncthreads();
     if (L + 2 <= Lend) fetch(L + 2, rx, rd);
-    if (r >= 2) {  // first colour on plane L-1
+    if (r >= 2 && L <= Lend) {  // first colour on plane L-1
       const int r1 = r - 1;
       const T *x0 = sxo + ((r1 - 1) % NXO) * HP, *x1 = sxo + (r1 % NXO) * HP, *x2 = sxo + ((r1 + 1) % NXO) * HP;
       const T *d0 = sd + ((r1 - 1) % ND) * HP, *d1 = sd + (r1 % ND) * HP, *d2 = sd + ((r1 + 1) % ND) * HP;
@@ -361,15 +365,13 @@ __device__ inline void sweep_z(const StencilConsts &s, const T *__restrict__ dn,
         sn[q] = v;
       }
     }
-    __syncthreads();
-    if (r >= 4) {  // second colour on plane L-2, written out
-      const int r2 = r - 2;
-      const T *s0 = ssn + ((r2 - 1) % NSN) * RP, *s1 = ssn + (r2 % NSN) * RP, *s2 = ssn + ((r2 + 1) % NSN) * RP;
-      const T *d0 = sd + ((r2 - 1) % ND) * HP, *d1 = sd + (r2 % ND) * HP, *d2 = sd + ((r2 + 1) % ND) * HP;
-      const int gk = L - 2;
-      for (int q = tid; q < TX * TY; q += NT) {
-        const int li = q % TX, lj = q / TX;
-        if (li >= ex || lj >= ey) continue;
+    if (r >= 5) {  // second colour on plane L-3, written out
+      const int r3 = r - 3;
+      const T *s0 = ssn + ((r3 - 1) % NSN) * RP, *s1 = ssn + (r3 % NSN) * RP, *s2 = ssn + ((r3 + 1) % NSN) * RP;
+      const T *d0 = sd + ((r3 - 1) % ND) * HP, *d1 = sd + (r3 % ND) * HP, *d2 = sd + ((r3 + 1) % ND) * HP;
+      const int gk = L - 3;
+      const int li = tid % TX, lj = tid / TX;  // TX * TY == NT: one node per thread
+      if (li < ex && lj < ey) {
         const int pd = (lj + 2) * HX + (li + 2), ps = (lj + 1) * RX + (li + 1);
         const int gi = i0 + li, gj = j0 + lj;
         const long g = gk * plane + (long)gj * Nx + gi;
@@ -382,9 +384,9 @@ __device__ inline void sweep_z(const StencilConsts &s, const T *__restrict__ dn,
   T rxa[LPT], rda[LPT], rxb[LPT], rdb[LPT];
   fetch(Lbeg, rxa, rda);
   fetch(Lbeg + 1, rxb, rdb);  // Lend >= Lbeg + 4
-  for (int L = Lbeg; L <= Lend; L += 2) {
+  for (int L = Lbeg; L <= Lend + 1; L += 2) {
     step(L, rxa, rda);
-    if (L + 1 <= Lend) step(L + 1, rxb, rdb);
+    if (L + 1 <= Lend + 1) step(L + 1, rxb, rdb);
   }
 }
 
@@ -399,6 +401,9 @@ constexpr int SW_TX = 16, SW_TY = 8, SW_TZ = 4;  // box of the one-shot sweep
 constexpr int S2_TX = 32, S2_TY = 16;             // tile of the single-plane sweep
 constexpr int SZ_TX = 32, SZ_TY = 16;            // xy tile of the marching sweep
 constexpr long SZ_MIN_POINTS = 1L << 21;         // levels at least this large march
+// In the lockstep batch every launch spans all problems, so the marching sweep's lower
+// load redundancy pays from much smaller levels (given 8 planes or more).
+constexpr long SZ_MIN_POINTS_BATCH = 1L << 18;
 constexpr int DENSE_MAX = 1024;                // largest coarsest level solved by a dense inverse
 constexpr size_t TAIL_LDS_MAX = 140 * 1024;    // LDS of the single-workgroup tail
 
