@@ -343,21 +343,23 @@ __device__ inline int pilot_tile(int j, uint32_t tag, int stride) {
   return j * stride + (int)(h & (uint32_t)(stride - 1));
 }
 
-// ctr[6] attempts, ctr[7] collisions, ctr[8] ions visited.  The ions with attempts
-// (about one in eight at 128^3 x 100 ions per cell) are queued in LDS and run their
-// attempts with every lane busy, PILOT_CHUNK at a time.
+// Per-workgroup sums (attempts, collisions, ions visited) go to part[3 * workgroup]: the
+// thousands of workgroups of one pass would otherwise end on the same three global
+// atomics, serialised after the last of them.  The ions with attempts (about one in
+// eight at 128^3 x 100 ions per cell) are queued in LDS and run their attempts with every
+// lane busy, PILOT_CHUNK at a time.
 constexpr int PILOT_CHUNK = 256, PILOT_QUEUE = 2 * PILOT_CHUNK;
 
 template <class P>
 __global__ void __launch_bounds__(PILOT_CHUNK) k_pilot(P A, const int *base, const int *count, int ntiles,
                                                        CollideConsts k, PoissonParams pt, uint32_t key0, uint32_t key1,
-                                                       uint32_t tag, int stride, unsigned long long *ctr) {
+                                                       uint32_t tag, int stride, unsigned long long *part) {
   __shared__ double qv[3][PILOT_QUEUE];
   __shared__ int qid[PILOT_QUEUE], qk[PILOT_QUEUE], qn;
+  __shared__ unsigned long long wsum[PILOT_CHUNK / 64][2];
   const int t = pilot_tile(blockIdx.x, tag, stride);
-  if (t >= ntiles) return;
   const int tid = threadIdx.x;
-  const long b = base[t], n = count[t];
+  const long b = t < ntiles ? base[t] : 0, n = t < ntiles ? count[t] : 0;
   unsigned natt = 0, nsucc = 0, nfail = 0;
   auto run = [&](int e) {  // the attempts of queue entry e
     double vx = qv[0][e], vy = qv[1][e], vz = qv[2][e];
@@ -394,15 +396,43 @@ __global__ void __launch_bounds__(PILOT_CHUNK) k_pilot(P A, const int *base, con
   }
   if (tid < qn) run(tid);
   const unsigned long long na = wave_sum(natt), ns = wave_sum(nsucc);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&ctr[6], na);
-    atomicAdd(&ctr[7], ns);
+  if ((tid & 63) == 0) {
+    wsum[tid >> 6][0] = na;
+    wsum[tid >> 6][1] = ns;
   }
-  if (threadIdx.x == 0 && blockIdx.y == 0) atomicAdd(&ctr[8], (unsigned long long)n);
+  __syncthreads();
+  if (tid == 0) {
+    unsigned long long sa = 0, sc = 0;
+    for (int w = 0; w < PILOT_CHUNK / 64; ++w) {
+      sa += wsum[w][0];
+      sc += wsum[w][1];
+    }
+    unsigned long long *o = part + 3 * ((long)blockIdx.y * gridDim.x + blockIdx.x);
+    o[0] = sa;
+    o[1] = sc;
+    o[2] = blockIdx.y == 0 ? (unsigned long long)n : 0ull;
+  }
 }
 
-__global__ void k_zero_pilot_ctr(unsigned long long *ctr) {
-  if (threadIdx.x < 3) ctr[6 + threadIdx.x] = 0ull;
+// ctr[6] attempts, ctr[7] collisions, ctr[8] ions visited: the sums of the nwg partials.
+__global__ void k_pilot_reduce(const unsigned long long *part, int nwg, unsigned long long *ctr) {
+  __shared__ unsigned long long s[3][256];
+  unsigned long long a = 0, c = 0, v = 0;
+  for (int i = threadIdx.x; i < nwg; i += blockDim.x) {
+    a += part[3 * i];
+    c += part[3 * i + 1];
+    v += part[3 * i + 2];
+  }
+  s[0][threadIdx.x] = a;
+  s[1][threadIdx.x] = c;
+  s[2][threadIdx.x] = v;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h)
+      for (int j = 0; j < 3; ++j) s[j][threadIdx.x] += s[j][threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) ctr[6 + threadIdx.x] = s[threadIdx.x][0];
 }
 
 struct PilotResult {
@@ -411,7 +441,6 @@ struct PilotResult {
 
 // One dry run of the Poisson process at rate lam on the tiles of sample `tag`.
 static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int stride) {
-  k_zero_pilot_ctr<<<1, 64, 0, c->stream>>>(c->d_ctr);
   CollideConsts k = collide_consts_of(c->p);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
   const PoissonParams pp = poisson_params(lam);
@@ -420,13 +449,23 @@ static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int s
   const int ns = (c->ntiles + stride - 1) / stride;
   const long per_tile = c->np / std::max(1, c->ntiles);
   const int split = (int)std::max<long>(1, std::min<long>((2048 + ns - 1) / ns, per_tile / 256));
+  const long nwg = (long)ns * split;
+  if (nwg > c->pilot_cap) {
+    (void)hipFree(c->d_pilot);
+    c->d_pilot = nullptr;
+    c->pilot_cap = 0;
+    if (hipMalloc(&c->d_pilot, sizeof(unsigned long long) * 3 * nwg) != hipSuccess)
+      throw Error{ECSPRT_ERR_NOMEM, "collision pilot partial sums"};
+    c->pilot_cap = nwg;
+  }
   const dim3 grid(ns, split);
   if (c->mixed)
     k_pilot<<<grid, PILOT_CHUNK, 0, c->stream>>>(c->Am, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag,
-                                                 stride, c->d_ctr);
+                                                 stride, c->d_pilot);
   else
     k_pilot<<<grid, PILOT_CHUNK, 0, c->stream>>>(c->A, c->d_base, c->d_count, c->ntiles, k, pp, k0, k1, tag,
-                                                 stride, c->d_ctr);
+                                                 stride, c->d_pilot);
+  k_pilot_reduce<<<1, 256, 0, c->stream>>>(c->d_pilot, (int)nwg, c->d_ctr);
   ECS_HIP(hipGetLastError());
   read_ctr(c);
   return {(double)c->h_ctr[7], (double)c->h_ctr[8]};

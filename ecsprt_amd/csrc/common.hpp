@@ -147,6 +147,8 @@ struct ecsprt_hip_ctx {
   // collisions, ions visited); CTR_N in all
   unsigned long long *d_ctr = nullptr;
   unsigned long long *h_ctr = nullptr;  // pinned mirror
+  unsigned long long *d_pilot = nullptr;  // per-workgroup partial sums of a pilot pass
+  long pilot_cap = 0;                     // workgroups they have room for
 
   // grid vectors
   long nnode = 0;

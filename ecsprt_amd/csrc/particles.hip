@@ -9,6 +9,8 @@
 #include <numeric>
 #include <type_traits>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "philox.hpp"
 #include "pic_math.hpp"
@@ -423,12 +425,8 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       }
       particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
     }
-    unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
-    if (__lane_id() == 0) {
-      if (s0) atomicAdd(&a.ctr[3], s0);
-      if (s1) atomicAdd(&a.ctr[4], s1);
-      if (s2) atomicAdd(&a.ctr[5], s2);
-    }
+    __shared__ unsigned long long s_red[16][3];
+    block_add3(s_red, natt, nsucc, nfail, a.ctr + 3);
   }
   if (MODE & M_DEPOSIT) {
     __syncthreads();
@@ -776,15 +774,16 @@ static void ensure_keys(ecsprt_hip_ctx *c, long np) {
 
 template <int MODE, int TX, int TY, int TZ, int HH>
 static void set_lds_attr() {
-  static bool done = false;
-  if (!done) {
+  // Once per kernel instance and process, also when contexts on several host threads
+  // launch their first step at the same time.
+  static std::once_flag once;
+  std::call_once(once, [] {
     // 160 KB of LDS per workgroup on gfx950, less what the kernel declares statically.
     hipFuncAttributes fa;
     ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, TX, TY, TZ, HH>));
     const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
     ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, TX, TY, TZ, HH>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-    done = true;
-  }
+  });
 }
 
 template <int MODE>
@@ -905,6 +904,9 @@ void tiles_free(ecsprt_hip_ctx *c) {
   (void)hipFree(c->d_fold);
   (void)hipFree(c->d_ovf);
   (void)hipFree(c->d_ctr);
+  (void)hipFree(c->d_pilot);
+  c->d_pilot = nullptr;
+  c->pilot_cap = 0;
   if (c->h_ctr) (void)hipHostFree(c->h_ctr);
   (void)hipFree(c->d_base);
   (void)hipFree(c->d_cap);

@@ -10,6 +10,8 @@
 //
 // The tile structure, re-binning protocol and collision deferral are those of the
 // FP64 tile kernel (particles.hip); only the particle arithmetic differs.
+#include <mutex>
+
 #include "common.hpp"
 #include "philox.hpp"
 #include "pic_math.hpp"
@@ -412,12 +414,8 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
       tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
                                     pid);
     }
-    unsigned long long s0 = wave_sum(natt), s1 = wave_sum(nsucc), s2 = wave_sum(nfail);
-    if (__lane_id() == 0) {
-      if (s0) atomicAdd(&a.ctr[3], s0);
-      if (s1) atomicAdd(&a.ctr[4], s1);
-      if (s2) atomicAdd(&a.ctr[5], s2);
-    }
+    __shared__ unsigned long long s_red[16][3];
+    block_add3(s_red, natt, nsucc, nfail, a.ctr + 3);
   }
   if (MODE & M_DEPOSIT) {
     __syncthreads();
@@ -613,15 +611,16 @@ void mx_alloc(ParticlesMix &P, long n) {
 
 template <int MODE, int TX, int TY, int TZ, int HH>
 static void set_attr_mx() {
-  static bool done = false;
-  if (!done) {
+  // Once per kernel instance and process, also when contexts on several host threads
+  // launch their first step at the same time.
+  static std::once_flag once;
+  std::call_once(once, [] {
     hipFuncAttributes fa;
     ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile_mx<MODE, TX, TY, TZ, HH>));
     const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
     ECS_HIP(hipFuncSetAttribute((const void *)k_tile_mx<MODE, TX, TY, TZ, HH>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
-    done = true;
-  }
+  });
 }
 
 template <int MODE>

@@ -16,6 +16,8 @@
 #include <cmath>
 #include <type_traits>
 
+#include <mutex>
+
 #include "common.hpp"
 #include "solver_kernels.hpp"
 
@@ -174,8 +176,12 @@ static void mg_create(ecsprt_hip_ctx *c, MG<T> &mg) {
         a.dn[l - t] = mg.L[l].dn;
         off += 5 * (size_t)mg.L[l].n;
       }
-      ECS_HIP(hipFuncSetAttribute((const void *)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)TAIL_LDS_MAX));
+      // Once per process: slot-pool batches create solvers on several threads at once.
+      static std::once_flag once;
+      std::call_once(once, [] {
+        ECS_HIP(hipFuncSetAttribute((const void *)k_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)TAIL_LDS_MAX));
+      });
     }
   }
 }

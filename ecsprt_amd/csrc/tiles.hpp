@@ -22,6 +22,25 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+// Adds the workgroup's sums of a, b, c to ctr[0..2] with one device atomic each (wave sums
+// through red, one row per wave; at most 16 waves): thousands of workgroups ending on
+// per-wave atomics to the same three addresses would serialise there.  Every thread calls it.
+__device__ inline void block_add3(unsigned long long (*red)[3], unsigned a, unsigned b, unsigned c,
+                                  unsigned long long *ctr) {
+  const unsigned long long s0 = wave_sum(a), s1 = wave_sum(b), s2 = wave_sum(c);
+  if (__lane_id() == 0) {
+    red[threadIdx.x >> 6][0] = s0;
+    red[threadIdx.x >> 6][1] = s1;
+    red[threadIdx.x >> 6][2] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned long long t = 0;
+    for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) t += red[w][threadIdx.x];
+    if (t) atomicAdd(&ctr[threadIdx.x], t);
+  }
+}
+
 // Tile offset d = b - a on a periodic axis of n tiles folded to -1, 0, +1; 2 when
 // the destination is further than one tile.
 __device__ inline int tile_step(int d, int n) {

@@ -27,7 +27,7 @@ def one(path_glob):
     return hits[0] if hits else None
 
 
-def stats(d, out, command, top, step_marker, drop):
+def stats(d, out, command, top, step_marker, drop, by_grid=""):
     path = one(os.path.join(d, "**", "*kernel_stats.csv"))
     if path is None:
         raise SystemExit(f"no kernel_stats.csv under {d}")
@@ -42,6 +42,8 @@ def stats(d, out, command, top, step_marker, drop):
                      f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['MinNs']) / 1e3:9.2f} "
                      f"{float(r['MaxNs']) / 1e3:9.2f} {float(r['Percentage']):6.2f}")
     trace = one(os.path.join(d, "**", "*kernel_trace.csv"))
+    if trace and by_grid:
+        lines += [""] + grid_lines(trace, by_grid)
     if trace and step_marker:
         lines += [""] + step_lines(trace, step_marker)
     with open(out, "w") as f:
@@ -49,6 +51,27 @@ def stats(d, out, command, top, step_marker, drop):
     if drop:
         for p in glob.glob(os.path.join(d, "**", "*.csv"), recursive=True):
             os.remove(p)
+
+
+def grid_lines(trace, prefixes):
+    """Launches of the kernels whose names start with one of `prefixes`, grouped by launch
+    shape (one workload's launches of a kernel share it), in order of first launch."""
+    rows = list(csv.DictReader(open(trace)))
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    groups = collections.OrderedDict()
+    for r in rows:
+        k = short(r["Kernel_Name"], 400)
+        if not any(k.startswith(p) for p in prefixes.split(",")):
+            continue
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        block = int(r["Workgroup_Size_X"]) * int(r["Workgroup_Size_Y"]) * int(r["Workgroup_Size_Z"])
+        g = groups.setdefault((short(r["Kernel_Name"]), grid // max(block, 1), block), [])
+        g.append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    out = ["launches by shape (workgroups x threads), in order of first launch:",
+           f"{'kernel':60s} {'workgroups':>10s} {'threads':>7s} {'calls':>6s} {'avg_us':>10s} {'min_us':>9s} {'max_us':>9s}"]
+    for (k, wg, b), v in groups.items():
+        out.append(f"{k[:60]:60s} {wg:10d} {b:7d} {len(v):6d} {sum(v) / len(v):10.2f} {min(v):9.2f} {max(v):9.2f}")
+    return out
 
 
 def step_lines(trace, marker):
@@ -105,10 +128,11 @@ def main():
     ap.add_argument("--command", required=True, help="the command that produced the profile")
     ap.add_argument("--top", type=int, default=25)
     ap.add_argument("--step-marker", default="", help="comma-separated kernel-name prefixes that start a step")
+    ap.add_argument("--by-grid", default="", help="comma-separated kernel-name prefixes to list per launch shape")
     ap.add_argument("--drop", action="store_true", help="delete the CSVs after summarising")
     a = ap.parse_args()
     if a.mode == "stats":
-        stats(a.dir, a.out, a.command, a.top, a.step_marker, a.drop)
+        stats(a.dir, a.out, a.command, a.top, a.step_marker, a.drop, a.by_grid)
     else:
         pmc(a.dir, a.out, a.command, a.drop)
 
