@@ -483,15 +483,16 @@ void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt) {
 }
 
 // The attempt rate of this step, set from the current velocities before the step's
-// draws: up to two dry-run passes on a sample of the tiles (about 2 * 10^4 expected
-// collisions, at least 1/64 of the ions), each rescaling lambda by Nc / (predicted
-// collisions), on the same sample so that the second corrects the first's bias.  A rate
+// draws: up to two dry-run passes on a sample of the tiles (at least 32 tiles and about
+// 2 * 10^4 expected collisions, at least 1/64 of the ions), each rescaling lambda by
+// Nc / (predicted collisions), on the same sample so that the second corrects the first's
+// bias.  A rate
 // carried over from the last step lags the state: in a run that cools on cold neutrals
 // the acceptance falls every step and the collisions would fall short of Nc.
 void collide_predict(ecsprt_hip_ctx *c, int Nc) {
   if (Nc <= 0 || c->np <= 0 || c->coll_lambda <= 0.0) return;
   int stride = 1;
-  while (stride < 64 && (double)Nc / (2 * stride) >= 2.0e4 && c->ntiles / (2 * stride) >= 64) stride *= 2;
+  while (stride < 64 && (double)Nc / (2 * stride) >= 2.0e4 && c->ntiles / (2 * stride) >= 32) stride *= 2;
   const uint32_t tag = 0x5A17E000u ^ (uint32_t)c->step_index;
   double lam = c->coll_lambda;
   for (int it = 0; it < 2; ++it) {
