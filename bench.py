@@ -13,7 +13,7 @@ The default run measures every workload of the hot path and prints one JSON line
   the oracle's FP64 GMRES(30)+multigrid on the same field on the host;
 * legs "c0" (32^3 x 64 ions/cell FP64, cold neutrals, 10 steps), "c3" (256 standalone
   solves of smoothed log-normal fields, grid edges from {32, ..., 192}) and "c4" (16
-  independent mixed-precision runs of uneven size, stepped concurrently).
+  independent mixed-precision runs of uneven size, stepped as one lockstep ensemble).
 
 Every leg carries the HBM roofline of its dominant kernel (HIP events on the stream the
 kernel runs on) and a CPU baseline: the oracle on a bounded sample of the same workload,
@@ -544,9 +544,10 @@ def leg_c3(args, pic, _lib, cores):
 # ----------------------------------------------------------------------------------- c4
 def leg_c4(args, pic, _lib, cores):
     """16 independent PIC runs of uneven size, mixed precision (particles and solve, rtol
-    1e-10 as in the 128^3 workload), stepped concurrently (one stream and host thread per
-    run).  CPU baseline: per run, one oracle step on 1/16 of its ions plus the full
-    RHS + solve on the run's own final moments."""
+    1e-10 as in the 128^3 workload), stepped as one lockstep ensemble (one host thread and
+    stream, one launch per stage for all runs).  Also timed: the runs stepped concurrently
+    on one stream and host thread each, and one by one.  CPU baseline: per run, one oracle
+    step on 1/16 of its ions plus the full RHS + solve on the run's own final moments."""
     import oracle
     from ecsprt_amd import configs
     runs = configs.ensemble_runs()
@@ -565,26 +566,38 @@ def leg_c4(args, pic, _lib, cores):
     nps = [r["ppc"] * r["N"] ** 3 for r in runs]
     total = sum(nps)
     nsteps = args.profile_steps or args.steps
+    ens = pic.Ensemble(ctxs)
     for _ in range(0 if args.profile_steps else args.warmup):
-        pic.ensemble_step(ctxs, 1.0)
-    for c in ctxs:
-        c.synchronize()
+        ens.advance(1.0)
+    stages = np.zeros(4)
+    sbytes_ens = 0.0
     t0 = time.perf_counter()
     for _ in range(nsteps):
         if args.sequential and args.profile_steps:
             for c in ctxs:
                 c.step(1.0)
         else:
-            pic.ensemble_step(ctxs, 1.0)
-    for c in ctxs:
-        c.synchronize()
+            ens.advance(1.0)
+            if not args.profile_steps:
+                stages += ens.stage_times() / nsteps
+                sbytes_ens += ens.traffic() / nsteps
     el = time.perf_counter() - t0
     if args.profile_steps:
+        ens.close()
         for c in ctxs:
             c.close()
         return {"profile_steps": nsteps, "seconds": el, "sequential": bool(args.sequential)}
-    # The same runs stepped one after another on this thread, for the concurrency gain and
-    # for each kernel's own duration (HIP events on the run's stream, nothing else running).
+    ens.close()
+    # The runs stepped concurrently, one stream and host thread per run.
+    nt = max(2, nsteps // 4)
+    t0 = time.perf_counter()
+    for _ in range(nt):
+        pic.ensemble_step(ctxs, 1.0)
+    for c in ctxs:
+        c.synchronize()
+    el_thr = (time.perf_counter() - t0) / nt
+    # The same runs stepped one after another on this thread, for each kernel's own
+    # duration (HIP events on the run's stream, nothing else running).
     kms = np.zeros(len(runs))
     sms = np.zeros(len(runs))
     sbytes = np.zeros(len(runs))
@@ -623,14 +636,22 @@ def leg_c4(args, pic, _lib, cores):
         "value": total * nsteps / el, "unit": "particle-steps/s", "steps": nsteps,
         "ms_per_step": 1e3 * el / nsteps, "dtype": "f32 particles, f64 fold, mixed solve",
         "config": {"workload": "ensemble_16_runs", "runs": runs, "particles_total": total,
-                   "concurrency": "one stream and host thread per run"},
-        "sequential_ms_per_step": 1e3 * el_seq, "concurrency_speedup": el_seq / (el / nsteps),
-        "roofline": roofline(alg, float(kms.sum()), "k_tile_mx (16 launches per step, one per run)",
-                             note="sum over the runs of algorithmic bytes / sum of the runs' k_tile_mx durations, "
-                                  "each measured in the sequential pass (duration-weighted mean over launches of "
-                                  "different size)"),
-        "solve_roofline": roofline(float(sbytes.sum()), float(sms.sum()),
-                                   "potential solves, mixed (16 per step, sequential pass, summed spans)"),
+                   "stepping": "lockstep ensemble: one host thread and stream, one launch per stage for all runs"},
+        "stage_ms_mean": {"tile_passes": stages[0], "pre": stages[1], "rebin_fold": stages[2],
+                          "rhs_and_solve": stages[3]},
+        "threads_ms_per_step": 1e3 * el_thr, "sequential_ms_per_step": 1e3 * el_seq,
+        "speedup_vs_threads": el_thr / (el / nsteps), "speedup_vs_sequential": el_seq / (el / nsteps),
+        "roofline": roofline(alg, float(stages[0]), "k_tile_mx_runs (one launch per tile shape: 4x4x4 and 8x8x8)",
+                             note="algorithmic bytes of all runs' ions and nodes / the span of the step's tile-pass "
+                                  "launches (HIP events)"),
+        "sequential_roofline": roofline(alg, float(kms.sum()), "k_tile_mx (16 launches per step, one per run)",
+                                        note="the runs stepped one by one: sum of the runs' k_tile_mx durations"),
+        "solve_roofline": roofline(sbytes_ens, float(stages[3]),
+                                   "lockstep solve of the 16 runs (forcing, GMRES/GMG, all launches)",
+                                   note="algorithmic bytes of the lockstep solve / its span (forcing included in "
+                                        "the span, not in the bytes)"),
+        "sequential_solve_roofline": roofline(float(sbytes.sum()), float(sms.sum()),
+                                              "potential solves, mixed (16 per step, sequential pass, summed spans)"),
         "errors_vs_fp64_one_step": errs,
         "phi_rel_err_vs_fp64_max": max(e["phi_rel_max"] for e in errs),
     }

@@ -130,6 +130,12 @@ SIGNATURES = {
     "ecsprt_hip_batch_last_error": (ctypes.c_char_p, [_P]),
     "ecsprt_hip_batch_traffic": (ctypes.c_int, [_P, _D]),
     "ecsprt_hip_ensemble_step": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_double]),
+    "ecsprt_hip_ensemble_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
+    "ecsprt_hip_ensemble_advance": (ctypes.c_int, [_P, ctypes.c_double]),
+    "ecsprt_hip_ensemble_destroy": (ctypes.c_int, [_P]),
+    "ecsprt_hip_ensemble_last_error": (ctypes.c_char_p, [_P]),
+    "ecsprt_hip_ensemble_traffic": (ctypes.c_int, [_P, _D]),
+    "ecsprt_hip_ensemble_stage_times": (ctypes.c_int, [_P, _D]),
     "ecsprt_hip_lognormal_moments": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, _P]),
     "ecsprt_hip_lognormal_moments_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,

@@ -358,4 +358,175 @@ int ecsprt_hip_ensemble_step(ecsprt_hip_ctx *const *ctxs, int n, double dt) {
   return ECSPRT_OK;
 }
 
+// ---- lockstep ensembles --------------------------------------------------------------
+
+struct ecsprt_hip_ensemble {
+  std::vector<ecsprt_hip_ctx *> ctx;
+  ecsprt::LockstepBatch *solve = nullptr;
+  hipStream_t st = nullptr;
+  void *h_args = nullptr, *d_args = nullptr;
+  std::vector<char> op_set;  // the batch holds the run's current operator
+  double solve_bytes = 0.0;
+  std::string last_error;
+  // start, tile passes begin, tile passes end, particle stages end, solve end
+  hipEvent_t ev[5] = {};
+  bool timed = false;  // the events of the last advance are complete
+  ~ecsprt_hip_ensemble() {
+    for (hipEvent_t v : ev)
+      if (v) (void)hipEventDestroy(v);
+    if (solve) ecsprt::lockstep_destroy(solve);
+    if (h_args) (void)hipHostFree(h_args);
+    (void)hipFree(d_args);
+  }
+};
+
+int ecsprt_hip_ensemble_create(ecsprt_hip_ctx *const *ctxs, int n, ecsprt_hip_ensemble **out) {
+  using ecsprt::Error;
+  if (!out) return ECSPRT_ERR_ARG;
+  *out = nullptr;
+  auto *e = new ecsprt_hip_ensemble();
+  try {
+    ECS_CHECK(ctxs && n > 0, ECSPRT_ERR_ARG, "ensemble: no contexts");
+    int device = 0;
+    ECS_HIP(hipGetDevice(&device));
+    std::vector<ecsprt_params> pp;
+    for (int q = 0; q < n; ++q) {
+      ecsprt_hip_ctx *c = ctxs[q];
+      ECS_CHECK(c != nullptr, ECSPRT_ERR_ARG, "ensemble: null context");
+      for (int r = 0; r < q; ++r) ECS_CHECK(ctxs[r] != c, ECSPRT_ERR_ARG, "ensemble: a context appears twice");
+      const ecsprt_options &o = c->o, &o0 = ctxs[0]->o;
+      ECS_CHECK(c->p.ndim != 2, ECSPRT_ERR_ARG, "ensemble: three-dimensional runs only");
+      ECS_CHECK(c->device == device, ECSPRT_ERR_ARG, "ensemble: every context on the current device");
+      ECS_CHECK(o.collide_mode != ECSPRT_COLLIDE_PREDRAWN, ECSPRT_ERR_ARG,
+                "ensemble: Philox collisions or none (pre-drawn records replay one run)");
+      ECS_CHECK(o.pc_type == ECSPRT_PC_GMG && o.gmres_refine == 0 && o.initial_guess == 0, ECSPRT_ERR_ARG,
+                "ensemble: multigrid preconditioning, classical Gram-Schmidt and a zero initial guess");
+      ECS_CHECK(o.solve_precision == o0.solve_precision && o.ksp_restart == o0.ksp_restart &&
+                    o.ksp_max_it == o0.ksp_max_it && o.ksp_rtol == o0.ksp_rtol && o.ksp_atol == o0.ksp_atol &&
+                    o.mg_pre == o0.mg_pre && o.mg_post == o0.mg_post && o.inner_rtol == o0.inner_rtol,
+                ECSPRT_ERR_ARG, "ensemble: the runs' solver options differ");
+      pp.push_back(c->p);
+      e->ctx.push_back(c);
+    }
+    e->solve = ecsprt::lockstep_create_runs(pp, ctxs[0]->o);
+    e->st = ecsprt::lockstep_stream(*e->solve);
+    const size_t bytes = ((ecsprt::tile_args_bytes() + 255) & ~(size_t)255) * n;
+    ECS_HIP(hipHostMalloc(&e->h_args, bytes, hipHostMallocDefault));
+    if (hipMalloc(&e->d_args, bytes) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "ensemble: argument table"};
+    e->op_set.assign(n, 0);
+    for (hipEvent_t &v : e->ev) ECS_HIP(hipEventCreate(&v));
+  } catch (const Error &x) {
+    ecsprt::set_global_error(x.msg);
+    delete e;
+    return x.code;
+  } catch (const std::bad_alloc &) {
+    ecsprt::set_global_error("host allocation failed");
+    delete e;
+    return ECSPRT_ERR_NOMEM;
+  }
+  *out = e;
+  return ECSPRT_OK;
+}
+
+int ecsprt_hip_ensemble_destroy(ecsprt_hip_ensemble *e) {
+  delete e;
+  return ECSPRT_OK;
+}
+
+const char *ecsprt_hip_ensemble_last_error(const ecsprt_hip_ensemble *e) { return e ? e->last_error.c_str() : ""; }
+
+int ecsprt_hip_ensemble_traffic(const ecsprt_hip_ensemble *e, double *bytes) {
+  if (!e || !bytes) return ECSPRT_ERR_ARG;
+  *bytes = e->solve_bytes;
+  return ECSPRT_OK;
+}
+
+int ecsprt_hip_ensemble_stage_times(const ecsprt_hip_ensemble *e, double *ms) {
+  if (!e || !ms) return ECSPRT_ERR_ARG;
+  if (!e->timed) return ECSPRT_ERR_STATE;
+  const int pairs[4][2] = {{1, 2}, {0, 1}, {2, 3}, {3, 4}};
+  for (int i = 0; i < 4; ++i) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, e->ev[pairs[i][0]], e->ev[pairs[i][1]]) != hipSuccess) return ECSPRT_ERR_HIP;
+    ms[i] = t;
+  }
+  return ECSPRT_OK;
+}
+
+// One timestep of every run in lockstep: the particle stages of all runs (one launch per
+// stage; see run_fused_particles_runs), then each run's forcing, then one GMRES/multigrid
+// solve of all runs with one launch per phase.  The contexts' streams are set to the
+// ensemble's for the call.
+int ecsprt_hip_ensemble_advance(ecsprt_hip_ensemble *e, double dt) {
+  using ecsprt::Error;
+  if (!e) return ECSPRT_ERR_ARG;
+  const int n = (int)e->ctx.size();
+  std::vector<hipStream_t> saved(n);
+  for (int q = 0; q < n; ++q) saved[q] = e->ctx[q]->stream;
+  std::vector<int> rc(n, ECSPRT_OK);
+  int code = ECSPRT_OK;
+  try {
+    for (int q = 0; q < n; ++q) {
+      ECS_HIP(hipStreamSynchronize(saved[q]));
+      ECS_CHECK(e->ctx[q]->np > 0, ECSPRT_ERR_STATE, "ensemble: a run has no particles");
+    }
+    for (int q = 0; q < n; ++q) e->ctx[q]->stream = e->st;
+    e->timed = false;
+    ECS_HIP(hipEventRecord(e->ev[0], e->st));
+    ecsprt::run_fused_particles_runs(e->ctx, e->st, dt, e->h_args, e->d_args, rc, e->ev[1], e->ev[2]);
+    ECS_HIP(hipEventRecord(e->ev[3], e->st));
+    std::vector<char> take(n, 0);
+    std::vector<const double *> dens(n, nullptr), rhs(n, nullptr);
+    std::vector<double *> phi(n, nullptr);
+    for (int q = 0; q < n; ++q) {
+      ecsprt_hip_ctx *c = e->ctx[q];
+      rhs[q] = c->d_rhs;
+      phi[q] = c->d_phi;
+      if (rc[q] != ECSPRT_OK) continue;
+      take[q] = 1;
+      // The operator is built at the first solve and kept (src/pic.c:192), or rebuilt
+      // every solve in the rebuild mode: the batch copies it when the run's changes.
+      if (!c->operator_built || c->o.operator_mode == ECSPRT_OPERATOR_REBUILD) {
+        ecsprt::solver_build_operator(c);
+        e->op_set[q] = 0;
+      }
+      if (!e->op_set[q]) dens[q] = ecsprt::solver_density(c);
+      ecsprt::solver_rhs(c, c->d_rhs);
+    }
+    ecsprt::lockstep_solve_given(*e->solve, dens.data(), rhs.data(), phi.data(), take);
+    ECS_HIP(hipEventRecord(e->ev[4], e->st));
+    ECS_HIP(hipEventSynchronize(e->ev[4]));
+    e->timed = true;
+    std::vector<int> iters(n);
+    std::vector<double> rnorm(n), bnorm(n);
+    ecsprt::lockstep_stats(*e->solve, iters.data(), rnorm.data(), &e->solve_bytes);
+    ecsprt::lockstep_bnorm(*e->solve, bnorm.data());
+    const double rtol = e->ctx[0]->o.ksp_rtol > 0 ? e->ctx[0]->o.ksp_rtol : 1e-5;
+    const double atol = e->ctx[0]->o.ksp_atol > 0 ? e->ctx[0]->o.ksp_atol : 1e-50;
+    for (int q = 0; q < n; ++q) {
+      if (!take[q]) continue;
+      ecsprt_hip_ctx *c = e->ctx[q];
+      if (dens[q]) e->op_set[q] = 1;
+      c->last_iters = iters[q];
+      c->last_rnorm = rnorm[q];
+      c->last_bnorm = bnorm[q];
+      if (!(rnorm[q] <= std::max(rtol * bnorm[q], atol))) {
+        rc[q] = ECSPRT_ERR_NOT_CONVERGED;
+        c->last_error = "KSP did not converge: ||r|| = " + std::to_string(rnorm[q]) + " after " +
+                        std::to_string(iters[q]) + " iterations";
+      }
+    }
+    for (int q = 0; q < n && code == ECSPRT_OK; ++q) code = rc[q];
+    if (code != ECSPRT_OK) e->last_error = "ensemble: a run failed (see its context's last error)";
+  } catch (const Error &x) {
+    e->last_error = x.msg;
+    code = x.code;
+  } catch (const std::bad_alloc &) {
+    e->last_error = "host allocation failed";
+    code = ECSPRT_ERR_NOMEM;
+  }
+  for (int q = 0; q < n; ++q) e->ctx[q]->stream = saved[q];
+  return code;
+}
+
 }  // extern "C"

@@ -331,8 +331,10 @@ __global__ void __launch_bounds__(BT) kb_sub_mean(const BVec<double> *vs, int ns
 }
 
 // Density of each problem's moments into its level-0 density; the forcing into b64.
+// With dens set instead (the frozen operators of an ensemble), that density is copied.
 struct BProbIn {
   const double *moments;
+  const double *dens;
   double *rhs;
   RhsConsts rc;
   float *dn32;
@@ -351,6 +353,15 @@ __global__ void __launch_bounds__(BT) kb_density_rhs(const BProbIn *pin, int nse
   const int Nx = p.rc.Nx, Ny = p.rc.Ny;
   const int i = (int)(q % Nx), j = (int)((q / Nx) % Ny), k = (int)(q / ((long)Nx * Ny));
   p.rhs[q] = rhs_point(p.rc, p.moments, i, j, k);
+}
+
+__global__ void __launch_bounds__(BT) kb_density_copy(const BProbIn *pin, int nseg) {
+  const BProbIn p = pin[seg_index(pin, nseg)];
+  const long q = (long)(blockIdx.x - p.start) * BT + threadIdx.x;
+  if (!p.dens || q >= p.n) return;
+  const double d = p.dens[q];
+  if (p.dn64) p.dn64[q] = d;
+  if (p.dn32) p.dn32[q] = (float)d;
 }
 
 }  // namespace ecsprt
@@ -419,7 +430,7 @@ struct BArena {
 // The whole batch in lockstep: per precision the problems' hierarchies, the V-cycle
 // program, the Krylov segment tables; the FP64 level-0 data of every problem.
 struct LockstepBatch {
-  ecsprt_params p;
+  std::vector<ecsprt_params> pp;  // per problem (grid and physics)
   ecsprt_options o;
   hipStream_t st = nullptr;
   int P = 0, m = 30, pre = 2, post = 2;
@@ -452,7 +463,7 @@ struct LockstepBatch {
   int DH = 0;
   double bytes = 0.0;  // algorithmic bytes of the last solve
   std::vector<int> iters;
-  std::vector<double> rnorm;
+  std::vector<double> rnorm, bnorm;
   std::vector<double *> phi_ptr;
   std::vector<const double *> mom_ptr;
   ~LockstepBatch();
@@ -474,7 +485,7 @@ BProb<float> *&probs<float>(LockstepBatch &B) {
 // Hierarchy of problem q (the coarsening and tail rules of the single-problem solver).
 template <typename T>
 void build_levels(LockstepBatch &B, int q, BProb<T> &P) {
-  const ecsprt_params &p = B.p;
+  const ecsprt_params &p = B.pp[q];
   int Nx = B.dims[3 * q], Ny = B.dims[3 * q + 1], Nz = B.dims[3 * q + 2];
   double dx = p.dx, dy = p.dy, dz = p.dz;
   const double k2 = 1 + p.kappa_e * p.kappa_e;
@@ -942,6 +953,7 @@ void outer_residual(LockstepBatch &B, const std::vector<char> &run, std::vector<
   for (int q = 0; q < B.P; ++q) norms[q] = run[q] ? std::sqrt(std::max(B.h_dh[(size_t)q * B.DH], 0.0)) : 0.0;
 }
 
+// The hierarchies' densities and coarsest inverses of the problems whose flags are set.
 template <typename T>
 void setup_operators(LockstepBatch &B) {
   BProb<T> *PP = probs<T>(B);
@@ -949,6 +961,7 @@ void setup_operators(LockstepBatch &B) {
   // dense inverses of the coarsest operators
   std::vector<std::vector<T>> dn(B.P);
   for (int q = 0; q < B.P; ++q) {
+    if (!B.act_now[q]) continue;
     const BLevel<T> &C = PP[q].L.back();
     dn[q].resize(C.n);
     ECS_HIP(hipMemcpyAsync(dn[q].data(), C.dn, sizeof(T) * C.n, hipMemcpyDeviceToHost, B.st));
@@ -956,6 +969,7 @@ void setup_operators(LockstepBatch &B) {
   ECS_HIP(hipStreamSynchronize(B.st));
   std::vector<T> inv;
   for (int q = 0; q < B.P; ++q) {
+    if (!B.act_now[q]) continue;
     const BLevel<T> &C = PP[q].L.back();
     const std::vector<double> I = coarse_inverse(C.s, dn[q]);
     inv.resize(I.size());
@@ -1075,7 +1089,8 @@ static void lockstep_layout(LockstepBatch &B) {
       s.Z = P.Z;
       s.w = P.w;
     } else {
-      s.s = stencil_consts(B.p, B.dims[3 * q], B.dims[3 * q + 1], B.dims[3 * q + 2], B.p.dx, B.p.dy, B.p.dz);
+      const ecsprt_params &pq = B.pp[q];
+      s.s = stencil_consts(pq, B.dims[3 * q], B.dims[3 * q + 1], B.dims[3 * q + 2], pq.dx, pq.dy, pq.dz);
       s.dn = B.dn64[q];
     }
   }
@@ -1103,11 +1118,7 @@ static void lockstep_layout(LockstepBatch &B) {
   std::vector<BProbIn> pin(B.P);
   int start = 0;
   for (int q = 0; q < B.P; ++q) {
-    ecsprt_params pq = B.p;
-    pq.Nx = B.dims[3 * q];
-    pq.Ny = B.dims[3 * q + 1];
-    pq.Nz = B.dims[3 * q + 2];
-    pin[q].rc = rhs_consts(pq);
+    pin[q].rc = rhs_consts(B.pp[q]);
     pin[q].rhs = B.rhs64[q];
     pin[q].n = B.n[q];
     pin[q].start = start;
@@ -1132,13 +1143,19 @@ static void lockstep_layout(LockstepBatch &B) {
   }
 }
 
-LockstepBatch *lockstep_create(const ecsprt_params &p, const ecsprt_options &o, int nprob, const int *dims) {
+LockstepBatch *lockstep_create_runs(const std::vector<ecsprt_params> &pp, const ecsprt_options &o) {
   auto *B = new LockstepBatch();
   try {
-    B->p = p;
+    const int nprob = (int)pp.size();
+    B->pp = pp;
     B->o = o;
     B->P = nprob;
-    B->dims.assign(dims, dims + 3 * nprob);
+    B->dims.resize(3 * nprob);
+    for (int q = 0; q < nprob; ++q) {
+      B->dims[3 * q] = pp[q].Nx;
+      B->dims[3 * q + 1] = pp[q].Ny;
+      B->dims[3 * q + 2] = pp[q].Nz;
+    }
     B->m = o.ksp_restart > 0 ? o.ksp_restart : 30;
     B->pre = o.mg_pre > 0 ? o.mg_pre : 2;
     B->post = o.mg_post > 0 ? o.mg_post : 2;
@@ -1146,7 +1163,7 @@ LockstepBatch *lockstep_create(const ecsprt_params &p, const ecsprt_options &o, 
     ECS_CHECK(o.pc_type == ECSPRT_PC_GMG, ECSPRT_ERR_ARG, "lockstep batch: multigrid preconditioning only");
     B->DH = 2 * (B->m + 2);
     B->n.resize(nprob);
-    for (int q = 0; q < nprob; ++q) B->n[q] = (long)dims[3 * q] * dims[3 * q + 1] * dims[3 * q + 2];
+    for (int q = 0; q < nprob; ++q) B->n[q] = (long)pp[q].Nx * pp[q].Ny * pp[q].Nz;
     if (B->mixed) {
       B->p32 = new BProb<float>[nprob];
       for (int q = 0; q < nprob; ++q) build_levels<float>(*B, q, B->p32[q]);
@@ -1177,44 +1194,27 @@ LockstepBatch *lockstep_create(const ecsprt_params &p, const ecsprt_options &o, 
   return B;
 }
 
+LockstepBatch *lockstep_create(const ecsprt_params &p, const ecsprt_options &o, int nprob, const int *dims) {
+  std::vector<ecsprt_params> pp(nprob, p);
+  for (int q = 0; q < nprob; ++q) {
+    pp[q].Nx = dims[3 * q];
+    pp[q].Ny = dims[3 * q + 1];
+    pp[q].Nz = dims[3 * q + 2];
+  }
+  return lockstep_create_runs(pp, o);
+}
+
 void lockstep_destroy(LockstepBatch *B) { delete B; }
 
-// All problems: operator and forcing from moments[q], solution into phi[q].
-int lockstep_solve(LockstepBatch &B, double *const *moments, double *const *phi) {
+hipStream_t lockstep_stream(const LockstepBatch &B) { return B.st; }
+
+// GMRES with mixed refinement or in FP64 for the problems in take, from the x already in
+// place, each to rtol times its initial residual; then the solutions' means removed.
+static int lockstep_iterate(LockstepBatch &B, const std::vector<char> &take) {
   const int P = B.P;
-  B.bytes = 0.0;
-  B.iters.assign(P, 0);
-  B.rnorm.assign(P, 0.0);
-  // per-solve pointers: moments into the input table, phi as each FP64 x
-  {
-    std::vector<BProbIn> pin(P);
-    ECS_HIP(hipMemcpy(pin.data(), B.pin, sizeof(BProbIn) * P, hipMemcpyDeviceToHost));
-    std::vector<BVec<double>> v64(P);
-    ECS_HIP(hipMemcpy(v64.data(), B.vsegs64, sizeof(BVec<double>) * P, hipMemcpyDeviceToHost));
-    for (int q = 0; q < P; ++q) {
-      pin[q].moments = moments[q];
-      v64[q].x = phi[q];
-    }
-    ECS_HIP(hipMemcpy(B.pin, pin.data(), sizeof(BProbIn) * P, hipMemcpyHostToDevice));
-    ECS_HIP(hipMemcpy(B.vsegs64, v64.data(), sizeof(BVec<double>) * P, hipMemcpyHostToDevice));
-  }
-  std::vector<char> run(P, 1);
-  set_act(B, run);
-  kb_density_rhs<<<B.nblocks_in, BT, 0, B.st>>>(B.pin, P);
-  ECS_HIP(hipGetLastError());
-  for (int q = 0; q < P; ++q) B.bytes += 48.0 * B.n[q];
-  if (B.mixed) setup_operators<float>(B);
-  else setup_operators<double>(B);
-  // the forcing's mean removed
-  kb_sum<<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1);
-  kb_reduce<double><<<P, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1, 0);
-  kb_sub_mean<<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1);
-  ECS_HIP(hipGetLastError());
-  B.bytes += vec_bytes<double>(B, 3);
+  std::vector<char> run(take);
   const double rtol = B.o.ksp_rtol > 0 ? B.o.ksp_rtol : 1e-5, atol = B.o.ksp_atol > 0 ? B.o.ksp_atol : 1e-50;
   const int maxit = B.o.ksp_max_it > 0 ? B.o.ksp_max_it : 10000;
-  kb_fill_x<double><<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act);
-  ECS_HIP(hipGetLastError());
   std::vector<double> norms(P), tol(P), tol_in(P), beta(P), est(P);
   std::vector<int> steps(P);
   std::vector<char> conv(P, 0);
@@ -1225,12 +1225,15 @@ int lockstep_solve(LockstepBatch &B, double *const *moments, double *const *phi)
     inner = std::pow(10.0, -lr / cycles);
   }
   for (int outer = 0; outer < 200; ++outer) {
-    for (int q = 0; q < P; ++q) run[q] = !conv[q];
+    for (int q = 0; q < P; ++q) run[q] = take[q] && !conv[q];
     outer_residual(B, run, norms);
     bool any = false;
     for (int q = 0; q < P; ++q) {
       if (!run[q]) continue;
-      if (outer == 0) tol[q] = std::max(rtol * norms[q], atol);
+      if (outer == 0) {
+        tol[q] = std::max(rtol * norms[q], atol);
+        B.bnorm[q] = norms[q];
+      }
       B.rnorm[q] = norms[q];
       if (norms[q] <= tol[q]) {
         conv[q] = 1;
@@ -1264,8 +1267,7 @@ int lockstep_solve(LockstepBatch &B, double *const *moments, double *const *phi)
       if (run[q]) B.iters[q] += steps[q];
   }
   // mean removal of every solution
-  for (int q = 0; q < P; ++q) run[q] = 1;
-  set_act(B, run);
+  set_act(B, take);
   kb_sum<<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 0);
   kb_reduce<double><<<P, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1, 0);
   kb_sub_mean<<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 0);
@@ -1274,8 +1276,84 @@ int lockstep_solve(LockstepBatch &B, double *const *moments, double *const *phi)
   ECS_HIP(hipStreamSynchronize(B.st));
   int rc = ECSPRT_OK;
   for (int q = 0; q < P; ++q)
-    if (!conv[q]) rc = ECSPRT_ERR_NOT_CONVERGED;
+    if (take[q] && !conv[q]) rc = ECSPRT_ERR_NOT_CONVERGED;
   return rc;
+}
+
+// Solution pointers x = phi[q] and forcing pointers b = rhs[q] (rhs null: the batch's own
+// forcing buffers) into the FP64 vector table, densities into the input table.
+static void set_pointers(LockstepBatch &B, const double *const *moments, const double *const *dens,
+                         const double *const *rhs, double *const *phi) {
+  const int P = B.P;
+  std::vector<BProbIn> pin(P);
+  ECS_HIP(hipMemcpy(pin.data(), B.pin, sizeof(BProbIn) * P, hipMemcpyDeviceToHost));
+  std::vector<BVec<double>> v64(P);
+  ECS_HIP(hipMemcpy(v64.data(), B.vsegs64, sizeof(BVec<double>) * P, hipMemcpyDeviceToHost));
+  for (int q = 0; q < P; ++q) {
+    pin[q].moments = moments ? moments[q] : nullptr;
+    pin[q].dens = dens ? dens[q] : nullptr;
+    v64[q].x = phi[q];
+    v64[q].b = rhs ? rhs[q] : B.rhs64[q];
+  }
+  ECS_HIP(hipMemcpy(B.pin, pin.data(), sizeof(BProbIn) * P, hipMemcpyHostToDevice));
+  ECS_HIP(hipMemcpy(B.vsegs64, v64.data(), sizeof(BVec<double>) * P, hipMemcpyHostToDevice));
+}
+
+// All problems: operator and forcing from moments[q], solution into phi[q].
+int lockstep_solve(LockstepBatch &B, double *const *moments, double *const *phi) {
+  const int P = B.P;
+  B.bytes = 0.0;
+  B.iters.assign(P, 0);
+  B.rnorm.assign(P, 0.0);
+  B.bnorm.assign(P, 0.0);
+  set_pointers(B, moments, nullptr, nullptr, phi);
+  std::vector<char> run(P, 1);
+  set_act(B, run);
+  kb_density_rhs<<<B.nblocks_in, BT, 0, B.st>>>(B.pin, P);
+  ECS_HIP(hipGetLastError());
+  for (int q = 0; q < P; ++q) B.bytes += 48.0 * B.n[q];
+  if (B.mixed) setup_operators<float>(B);
+  else setup_operators<double>(B);
+  // the forcing's mean removed
+  kb_sum<<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1);
+  kb_reduce<double><<<P, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1, 0);
+  kb_sub_mean<<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act, 1);
+  ECS_HIP(hipGetLastError());
+  B.bytes += vec_bytes<double>(B, 3);
+  kb_fill_x<double><<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act);
+  ECS_HIP(hipGetLastError());
+  return lockstep_iterate(B, run);
+}
+
+// The problems in take: forcing rhs[q] (mean already removed), solution into phi[q]
+// from a zero start.  A problem with dens[q] set first takes its operator from that
+// level-0 density; the others keep theirs.
+int lockstep_solve_given(LockstepBatch &B, const double *const *dens, const double *const *rhs, double *const *phi,
+                         const std::vector<char> &take) {
+  const int P = B.P;
+  B.bytes = 0.0;
+  B.iters.assign(P, 0);
+  B.rnorm.assign(P, 0.0);
+  B.bnorm.assign(P, 0.0);
+  set_pointers(B, nullptr, dens, rhs, phi);
+  std::vector<char> ops(P, 0);
+  bool any = false;
+  for (int q = 0; q < P; ++q) any |= (ops[q] = dens[q] != nullptr);
+  if (any) {
+    set_act(B, ops);
+    kb_density_copy<<<B.nblocks_in, BT, 0, B.st>>>(B.pin, P);
+    ECS_HIP(hipGetLastError());
+    if (B.mixed) setup_operators<float>(B);
+    else setup_operators<double>(B);
+  }
+  set_act(B, take);
+  kb_fill_x<double><<<B.vblocks, BT, 0, B.st>>>(B.vsegs64, P, B.d_act);
+  ECS_HIP(hipGetLastError());
+  return lockstep_iterate(B, take);
+}
+
+void lockstep_bnorm(const LockstepBatch &B, double *bnorm) {
+  for (int q = 0; q < B.P; ++q) bnorm[q] = q < (int)B.bnorm.size() ? B.bnorm[q] : 0.0;
 }
 
 void lockstep_stats(const LockstepBatch &B, int *iters, double *rnorm, double *bytes) {

@@ -439,8 +439,10 @@ struct PilotResult {
   double collisions, ions;
 };
 
-// One dry run of the Poisson process at rate lam on the tiles of sample `tag`.
-static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int stride) {
+// One dry run of the Poisson process at rate lam on the tiles of sample `tag`, queued on
+// the context's stream; its sums reach the pinned counter mirror (pilot_result) once the
+// stream has synchronised.
+static void pilot_enqueue(ecsprt_hip_ctx *c, uint32_t tag, double lam, int stride) {
   CollideConsts k = collide_consts_of(c->p);
   const uint32_t k0 = (uint32_t)c->o.seed, k1 = (uint32_t)(c->o.seed >> 32);
   const PoissonParams pp = poisson_params(lam);
@@ -467,8 +469,15 @@ static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int s
                                                  stride, c->d_pilot);
   k_pilot_reduce<<<1, 256, 0, c->stream>>>(c->d_pilot, (int)nwg, c->d_ctr);
   ECS_HIP(hipGetLastError());
-  read_ctr(c);
-  return {(double)c->h_ctr[7], (double)c->h_ctr[8]};
+  ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * CTR_N, hipMemcpyDeviceToHost, c->stream));
+}
+
+static PilotResult pilot_result(const ecsprt_hip_ctx *c) { return {(double)c->h_ctr[7], (double)c->h_ctr[8]}; }
+
+static PilotResult pilot_pass(ecsprt_hip_ctx *c, uint32_t tag, double lam, int stride) {
+  pilot_enqueue(c, tag, lam, stride);
+  ECS_HIP(hipStreamSynchronize(c->stream));
+  return pilot_result(c);
 }
 
 // Initial attempt rate: dry runs of the Poisson process over every ion, each rescaling
@@ -485,27 +494,51 @@ void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt) {
 // The attempt rate of this step, set from the current velocities before the step's
 // draws: up to two dry-run passes on a sample of the tiles (at least 32 tiles and about
 // 2 * 10^4 expected collisions, at least 1/64 of the ions), each rescaling lambda by
-// Nc / (predicted collisions), on the same sample so that the second corrects the first's
-// bias.  A rate
-// carried over from the last step lags the state: in a run that cools on cold neutrals
-// the acceptance falls every step and the collisions would fall short of Nc.
-void collide_predict(ecsprt_hip_ctx *c, int Nc) {
+// Nc / (predicted collisions), on the same sample so that the second corrects the
+// first's bias.  A rate carried over from the last step lags the state: in a run that
+// cools on cold neutrals the acceptance falls every step and the collisions would fall
+// short of Nc.  predict_begin sets up the passes, predict_enqueue queues one and
+// predict_collect (after the stream has synchronised) applies it and says whether
+// another is due, so that an ensemble can run the passes of all its runs together.
+void predict_begin(ecsprt_hip_ctx *c, int Nc, PredictState &s) {
+  s = PredictState{};
   if (Nc <= 0 || c->np <= 0 || c->coll_lambda <= 0.0) return;
-  int stride = 1;
-  while (stride < 64 && (double)Nc / (2 * stride) >= 2.0e4 && c->ntiles / (2 * stride) >= 32) stride *= 2;
-  const uint32_t tag = 0x5A17E000u ^ (uint32_t)c->step_index;
-  double lam = c->coll_lambda;
-  for (int it = 0; it < 2; ++it) {
-    const PilotResult r = pilot_pass(c, tag, lam, stride);
-    if (!(r.collisions > 0.0) || !(r.ions > 0.0)) break;
-    const double predicted = r.collisions * (double)c->np / r.ions;
-    const double ratio = std::min(2.0, std::max(0.5, (double)Nc / predicted));
-    lam = std::min(lam * ratio, POISSON_LAMBDA_MAX);
-    // A rescale within 1% leaves a second-order error in the rescale itself, far below
-    // the sample's own noise: the second pass is skipped.
-    if (std::fabs(ratio - 1.0) <= 0.01) break;
+  s.began = s.active = true;
+  s.Nc = Nc;
+  s.stride = 1;
+  while (s.stride < 64 && (double)Nc / (2 * s.stride) >= 2.0e4 && c->ntiles / (2 * s.stride) >= 32) s.stride *= 2;
+  s.tag = 0x5A17E000u ^ (uint32_t)c->step_index;
+  s.lam = c->coll_lambda;
+}
+
+void predict_enqueue(ecsprt_hip_ctx *c, const PredictState &s) { pilot_enqueue(c, s.tag, s.lam, s.stride); }
+
+bool predict_collect(ecsprt_hip_ctx *c, PredictState &s) {
+  const PilotResult r = pilot_result(c);
+  s.active = false;
+  if (!(r.collisions > 0.0) || !(r.ions > 0.0)) return false;
+  const double predicted = r.collisions * (double)c->np / r.ions;
+  const double ratio = std::min(2.0, std::max(0.5, (double)s.Nc / predicted));
+  s.lam = std::min(s.lam * ratio, POISSON_LAMBDA_MAX);
+  // A rescale within 1% leaves a second-order error in the rescale itself, far below
+  // the sample's own noise: the second pass is skipped.
+  s.active = ++s.it < 2 && std::fabs(ratio - 1.0) > 0.01;
+  return s.active;
+}
+
+void predict_end(ecsprt_hip_ctx *c, const PredictState &s) {
+  if (s.began) c->coll_lambda = s.lam;
+}
+
+void collide_predict(ecsprt_hip_ctx *c, int Nc) {
+  PredictState s;
+  predict_begin(c, Nc, s);
+  while (s.active) {
+    predict_enqueue(c, s);
+    ECS_HIP(hipStreamSynchronize(c->stream));
+    predict_collect(c, s);
   }
-  c->coll_lambda = lam;
+  predict_end(c, s);
 }
 
 }  // namespace ecsprt

@@ -238,6 +238,12 @@ void run_push(ecsprt_hip_ctx *c, double dt);
 void run_drift_rebin(ecsprt_hip_ctx *c, double dt);
 void run_deposit(ecsprt_hip_ctx *c);
 void run_fused_particles(ecsprt_hip_ctx *c, double dt);
+// One fused particle step of several 3-D runs on stream st, one launch per stage for all
+// of them; the kernel arguments are staged through h_args (pinned) and d_args, each at
+// least tile_args_bytes() rounded up to 256, per run.
+void run_fused_particles_runs(const std::vector<ecsprt_hip_ctx *> &cs, hipStream_t st, double dt, void *h_args,
+                              void *d_args, std::vector<int> &rc, hipEvent_t ev_pass0, hipEvent_t ev_pass1);
+size_t tile_args_bytes();
 void run_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
 void run_apply_bc_aos(ecsprt_hip_ctx *c, long np, double *pos);
 void ensure_stage(ecsprt_hip_ctx *c, long np);
@@ -246,6 +252,16 @@ void ensure_stage(ecsprt_hip_ctx *c, long np);
 void run_collide_predrawn(ecsprt_hip_ctx *c, double dt);
 void run_collide_philox_pilot(ecsprt_hip_ctx *c, double dt);
 void collide_predict(ecsprt_hip_ctx *c, int Nc);
+struct PredictState {
+  bool began = false, active = false;
+  int Nc = 0, stride = 1, it = 0;
+  uint32_t tag = 0;
+  double lam = 0.0;
+};
+void predict_begin(ecsprt_hip_ctx *c, int Nc, PredictState &s);
+void predict_enqueue(ecsprt_hip_ctx *c, const PredictState &s);
+bool predict_collect(ecsprt_hip_ctx *c, PredictState &s);
+void predict_end(ecsprt_hip_ctx *c, const PredictState &s);
 void run_collide_philox(ecsprt_hip_ctx *c, double dt);  // particles.hip
 int collision_count(const ecsprt_params *p, long np, double dt);
 
@@ -259,9 +275,15 @@ hipError_t lognormal_launch(hipStream_t st, int Nx, int Ny, int Nz, uint64_t see
 // bsolve.hip: the batched solves in lockstep (one launch per phase for the whole batch).
 struct LockstepBatch;
 LockstepBatch *lockstep_create(const ecsprt_params &p, const ecsprt_options &o, int nprob, const int *dims);
+// One problem per params entry (its grid and physics), the shared solver options o.
+LockstepBatch *lockstep_create_runs(const std::vector<ecsprt_params> &pp, const ecsprt_options &o);
 void lockstep_destroy(LockstepBatch *b);
+hipStream_t lockstep_stream(const LockstepBatch &b);
 int lockstep_solve(LockstepBatch &b, double *const *moments, double *const *phi);
+int lockstep_solve_given(LockstepBatch &b, const double *const *dens, const double *const *rhs, double *const *phi,
+                         const std::vector<char> &take);
 void lockstep_stats(const LockstepBatch &b, int *iters, double *rnorm, double *bytes);
+void lockstep_bnorm(const LockstepBatch &b, double *bnorm);
 
 // api.hip: the message ecsprt_hip_last_error(NULL) returns on this thread.
 void set_global_error(const std::string &msg);
@@ -271,6 +293,7 @@ void solver_create(ecsprt_hip_ctx *c);
 void solver_destroy(ecsprt_hip_ctx *c);
 void solver_build_operator(ecsprt_hip_ctx *c);
 void solver_rhs(ecsprt_hip_ctx *c, double *b);
+const double *solver_density(const ecsprt_hip_ctx *c);
 void solver_matmult(ecsprt_hip_ctx *c, const double *x, double *y);
 void solver_pcapply(ecsprt_hip_ctx *c, const double *r, double *z);
 void solver_remove_mean(ecsprt_hip_ctx *c, double *v);

@@ -224,19 +224,18 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
   }
 }
 
+// The fused pass of workgroup wi of one run (arguments at ap): phase 1 pushes, decides
+// collisions and finishes the ions without attempts; phases 2 and 3 collide the deferred
+// ones; the deposit region goes to the run's fold buffer.
 template <int MODE, int TX, int TY, int TZ, int HH>
-__global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
+__device__ __attribute__((always_inline)) inline void tile_f64(ConstPtr<TileArgs> ap, int wi) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
+  const TileArgs &a = fresh(ap);
   extern __shared__ double smem[];
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
   __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
   const TileGeom g = a.g;
-  const int nwg = gridDim.x;
-  int wi = blockIdx.x;
-  // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
-  // contiguous run of tiles so neighbouring tiles share an L2.
-  if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
   constexpr int NEX = TX + 1, NEY = TY + 1, NE3 = NEX * NEY * (TZ + 1);
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH;
   TileWG w;
@@ -298,7 +297,6 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
   const long cnt = a.count[w.t];
   const long beg = a.base[w.t] + cnt * s / g.split;
   const long end = a.base[w.t] + cnt * (s + 1) / g.split;
-  const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
   unsigned natt = 0, nsucc = 0, nfail = 0;
 
   // Phase 1.  Software pipeline: the next batch's particles load while this one computes.
@@ -315,6 +313,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     nid = a.src.id(p);
   }
   for (long p0 = beg; p0 < end; p0 += blockDim.x) {
+    const TileArgs &A = fresh(ap);
     const long p = p0 + tid;
     const bool valid = p < end;
     double px = nxp, py = nyp, pz = nzp, vx = nvx, vy = nvy, vz = nvz;
@@ -322,50 +321,50 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     {
       const long pn = p + blockDim.x;
       if (pn < end) {
-        nxp = a.src.x(pn);
-        nyp = a.src.y(pn);
-        nzp = a.src.z(pn);
-        nvx = a.src.vx(pn);
-        nvy = a.src.vy(pn);
-        nvz = a.src.vz(pn);
-        nid = a.src.id(pn);
+        nxp = A.src.x(pn);
+        nyp = A.src.y(pn);
+        nzp = A.src.z(pn);
+        nvx = A.src.vx(pn);
+        nvy = A.src.vy(pn);
+        nvz = A.src.vz(pn);
+        nid = A.src.id(pn);
       }
     }
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body, src/velocities.c:342-356.
       // DifferenceVector3D at x = pos/d: il = (int)x, w = x - il (vector-math.c:607-617).
       double E[3];
-      const double x0 = cell_coord(px, dx, a.pc.inv[0], a.pc.pow2);
-      const double y0 = cell_coord(py, dy, a.pc.inv[1], a.pc.pow2);
-      const double z0 = cell_coord(pz, dz, a.pc.inv[2], a.pc.pow2);
+      const double x0 = cell_coord(px, A.pc.dx, A.pc.inv[0], A.pc.pow2);
+      const double y0 = cell_coord(py, A.pc.dy, A.pc.inv[1], A.pc.pow2);
+      const double z0 = cell_coord(pz, A.pc.dz, A.pc.inv[2], A.pc.pow2);
       const int ixl = (int)x0, iyl = (int)y0, izl = (int)z0;
       const int la = ixl - w.i0, lb = iyl - w.j0, lc = izl - w.k0;
       if ((unsigned)la < TX && (unsigned)lb < TY && (unsigned)lc < TZ && ixl < g.Nx && iyl < g.Ny && izl < g.Nz)
         gather_nodes<double, NEX, NEY, NE3>(sE, (lc * NEY + lb) * NEX + la, x0 - (double)ixl, y0 - (double)iyl,
                                             z0 - (double)izl, E);
       else  // a coordinate that rounded to N: the node differences differ by corner role
-        diff_vec3d_core<double>(gphi, g.Nx, g.Ny, g.Nz, ixl, iyl, izl, x0 - (double)ixl, y0 - (double)iyl,
-                                z0 - (double)izl, E);
-      double vxm = vx + a.pc.Escale[0] * E[0];
-      double vym = vy + a.pc.Escale[1] * E[1];
-      double vzm = vz + a.pc.Escale[2] * E[2];
-      double vxt = vxm + vym * a.pc.t;
-      double vyp = vym - vxt * a.pc.s;
-      double vxp = vxt + vyp * a.pc.t;
+        diff_vec3d_core<double>(PhiGlobal{A.phi, g.Nx, g.Ny, g.Nz}, g.Nx, g.Ny, g.Nz, ixl, iyl, izl,
+                                x0 - (double)ixl, y0 - (double)iyl, z0 - (double)izl, E);
+      double vxm = vx + A.pc.Escale[0] * E[0];
+      double vym = vy + A.pc.Escale[1] * E[1];
+      double vzm = vz + A.pc.Escale[2] * E[2];
+      double vxt = vxm + vym * A.pc.t;
+      double vyp = vym - vxt * A.pc.s;
+      double vxp = vxt + vyp * A.pc.t;
       vx = vxp;
       vy = vyp;
       vz = vzm;
     }
     bool defer = false;
-    if ((MODE & M_COLLIDE) && valid && poisson_any(a.pt, a.key0, a.key1, a.step, pid)) {
+    if ((MODE & M_COLLIDE) && valid && poisson_any(A.pt, A.key0, A.key1, A.step, pid)) {
       // About one ion in seven has a collision attempt this step.  It is set aside
       // (post-push state, in this workgroup's own slot range) and finished in
       // phase 2 with every lane busy.
       defer = true;
       const int q = atomicAdd(&s_ndef, 1);
-      store_particle(a.def, beg + q, px, py, pz, vx, vy, vz, pid);
+      store_particle(A.def, beg + q, px, py, pz, vx, vy, vz, pid);
     }
-    particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
+    particle_tail<MODE, TX, TY, TZ, HH>(A, w, valid && !defer, p, px, py, pz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
     // Phase 2: the deferred ions, first attempt each (every lane busy), then drifted,
@@ -375,6 +374,7 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     const int ndef = s_ndef;
     const long room = (end - beg) - ndef;
     for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
+      const TileArgs &A = fresh(ap);
       const int q = q0 + tid;
       const bool valid = q < ndef;
       bool done = valid;
@@ -382,48 +382,49 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
       int pid = 0;
       if (valid) {
         const long d = beg + q;
-        px = a.def.x(d);
-        py = a.def.y(d);
-        pz = a.def.z(d);
-        vx = a.def.vx(d);
-        vy = a.def.vy(d);
-        vz = a.def.vz(d);
-        pid = a.def.id(d);
-        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
-        collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
+        px = A.def.x(d);
+        py = A.def.y(d);
+        pz = A.def.z(d);
+        vx = A.def.vx(d);
+        vy = A.def.vy(d);
+        vz = A.def.vz(d);
+        pid = A.def.id(d);
+        const int k = poisson_attempts(A.pt, A.key0, A.key1, A.step, pid);
+        collide_attempts<double>(A.cc, A.key0, A.key1, A.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
         if (k > 1) {
           const int r = atomicAdd(&s_n3, 1);
           if (r < room) {
-            store_particle(a.def, beg + ndef + r, px, py, pz, vx, vy, vz, pid);
+            store_particle(A.def, beg + ndef + r, px, py, pz, vx, vy, vz, pid);
             done = false;
           } else {
-            collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
+            collide_attempts<double>(A.cc, A.key0, A.key1, A.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
           }
         }
       }
-      particle_tail<MODE, TX, TY, TZ, HH>(a, w, done, 0, px, py, pz, vx, vy, vz, pid);
+      particle_tail<MODE, TX, TY, TZ, HH>(A, w, done, 0, px, py, pz, vx, vy, vz, pid);
     }
     // Phase 3: the remaining attempts of the parked ions.
     __syncthreads();
     const int n3 = (int)min((long)s_n3, room);
     for (int q0 = 0; q0 < n3; q0 += blockDim.x) {
+      const TileArgs &A = fresh(ap);
       const int q = q0 + tid;
       const bool valid = q < n3;
       double px = 0, py = 0, pz = 0, vx = 0, vy = 0, vz = 0;
       int pid = 0;
       if (valid) {
         const long d = beg + ndef + q;
-        px = a.def.x(d);
-        py = a.def.y(d);
-        pz = a.def.z(d);
-        vx = a.def.vx(d);
-        vy = a.def.vy(d);
-        vz = a.def.vz(d);
-        pid = a.def.id(d);
-        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
-        collide_attempts<double>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
+        px = A.def.x(d);
+        py = A.def.y(d);
+        pz = A.def.z(d);
+        vx = A.def.vx(d);
+        vy = A.def.vy(d);
+        vz = A.def.vz(d);
+        pid = A.def.id(d);
+        const int k = poisson_attempts(A.pt, A.key0, A.key1, A.step, pid);
+        collide_attempts<double>(A.cc, A.key0, A.key1, A.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
-      particle_tail<MODE, TX, TY, TZ, HH>(a, w, valid, 0, px, py, pz, vx, vy, vz, pid);
+      particle_tail<MODE, TX, TY, TZ, HH>(A, w, valid, 0, px, py, pz, vx, vy, vz, pid);
     }
     __shared__ unsigned long long s_red[16][3];
     block_add3(s_red, natt, nsucc, nfail, a.ctr + 3);
@@ -433,6 +434,20 @@ __global__ void __launch_bounds__(1024) k_tile(TileArgs a) {
     long long *out = a.fold + (size_t)wi * 4 * w.RA3;
     for (int q = tid; q < 4 * w.RA3; q += blockDim.x) out[q] = w.sacc[q];
   }
+}
+
+template <int MODE, int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(1024) k_tile(TileArgs) {
+  tile_f64<MODE, TX, TY, TZ, HH>(kernarg<TileArgs>(), xcd_order(blockIdx.x, gridDim.x));
+}
+
+// One launch over the tiles of several runs of the same tile shape and block size.
+template <int MODE, int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(1024) k_tile_runs(RunTable) {
+  const ConstPtr<RunTable> t = kernarg<RunTable>();
+  const int gi = xcd_order(blockIdx.x, gridDim.x);
+  const int r = run_of(t, gi);
+  tile_f64<MODE, TX, TY, TZ, HH>((ConstPtr<TileArgs>)t->args + r, gi - t->start[r]);
 }
 
 // Deposit of the ions listed for tile t: those that arrived from beyond their source
@@ -783,6 +798,11 @@ static void set_lds_attr() {
     ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile<MODE, TX, TY, TZ, HH>));
     const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
     ECS_HIP(hipFuncSetAttribute((const void *)k_tile<MODE, TX, TY, TZ, HH>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    if constexpr ((MODE & (M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN)) == (M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN)) {
+      ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile_runs<MODE, TX, TY, TZ, HH>));
+      ECS_HIP(hipFuncSetAttribute((const void *)k_tile_runs<MODE, TX, TY, TZ, HH>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - (int)fa.sharedSizeBytes));
+    }
   });
 }
 
@@ -1093,8 +1113,9 @@ static void finish_rebin(ecsprt_hip_ctx *c) {
   std::swap(c->Am, c->Bm);
 }
 
-static void check_after_rebin(ecsprt_hip_ctx *c) {
-  read_ctr(c);
+// The conservation check and spill recovery of a re-binning pass, on counters already
+// read into h_ctr.
+static void check_counters(ecsprt_hip_ctx *c) {
   const long lost = (long)c->h_ctr[2];
   const long nspill = (long)c->h_ctr[1];
   if (lost > 0) {
@@ -1103,6 +1124,11 @@ static void check_after_rebin(ecsprt_hip_ctx *c) {
                 "total number of ions changed (" + std::to_string(-lost) + ") under periodic boundaries"};
   }
   if (nspill > 0) rebuild_from_spill(c, nspill);
+}
+
+static void check_after_rebin(ecsprt_hip_ctx *c) {
+  read_ctr(c);
+  check_counters(c);
 }
 
 // The attempt rate of this step's Philox collisions, from the ions' state at the start
@@ -1211,6 +1237,16 @@ static CollideConsts collide_consts(const ecsprt_params &p) {
 
 CollideConsts collide_consts_of(const ecsprt_params &p) { return collide_consts(p); }
 
+// The step's collision counters (read into h_ctr) into the context; ten refusals are
+// the reference's Abort (src/collisions.c:141-148).
+static void collision_totals(ecsprt_hip_ctx *c) {
+  const long natt = (long)c->h_ctr[3], ns = (long)c->h_ctr[4], nf = (long)c->h_ctr[5];
+  c->last_attempts = natt;
+  c->last_Ns = ns;
+  c->last_Nf = nf;
+  ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
+}
+
 void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
   zero_ctr(c);
   clear_overflow(c);
@@ -1242,14 +1278,168 @@ void run_fused_particles(ecsprt_hip_ctx *c, double dt) {
   launch_fold(c);
   if (c->o.timing) ECS_HIP(hipEventRecord(c->ev[2], c->stream));
   check_after_rebin(c);
-  if (coll) {
-    const long natt = (long)c->h_ctr[3], ns = (long)c->h_ctr[4], nf = (long)c->h_ctr[5];
-    c->last_attempts = natt;
-    c->last_Ns = ns;
-    c->last_Nf = nf;
-    ECS_CHECK(nf < 10, ECSPRT_ERR_COLLISION, "failed to collide " + std::to_string(nf) + " ion-neutral pairs");
-  }
+  if (coll) collision_totals(c);
   c->step_index++;
+}
+
+template <int MODE, int TX, int TY, int TZ>
+static void launch_tile_runs(hipStream_t st, const RunTable &t, int block, size_t lds) {
+  k_tile_runs<MODE, TX, TY, TZ, 2><<<t.start[t.nrun], block, lds, st>>>(t);
+  ECS_HIP(hipGetLastError());
+}
+
+// The fused FP64 pass over the runs cs (same tile shape and block size; with collisions
+// when coll, then every run has them) in one launch on stream st; arguments staged at
+// h_args (pinned) and copied to d_args first.
+static void tile_runs_f64(hipStream_t st, const std::vector<ecsprt_hip_ctx *> &cs, double dt, bool coll,
+                          const std::vector<PoissonParams> &pt, void *h_args, void *d_args) {
+  const int n = (int)cs.size();
+  TileArgs *h = static_cast<TileArgs *>(h_args);
+  RunTable t{};
+  t.args = d_args;
+  t.nrun = n;
+  long total = 0;
+  for (int r = 0; r < n; ++r) {
+    ecsprt_hip_ctx *c = cs[r];
+    TileArgs a = make_args(c);
+    a.pc = push_consts(c->p, dt);
+    a.dc = drift_consts(c->p, dt);
+    if (coll) {
+      a.cc = collide_consts(c->p);
+      a.key0 = (uint32_t)c->o.seed;
+      a.key1 = (uint32_t)(c->o.seed >> 32);
+      a.step = (uint32_t)c->step_index;
+      a.pt = pt[r];
+    }
+    h[r] = a;
+    t.start[r] = (int)total;
+    total += (long)c->ntiles * c->g.split;
+  }
+  ECS_CHECK(total < (1L << 31), ECSPRT_ERR_ARG, "multi-run tile pass: too many workgroups");
+  t.start[n] = (int)total;
+  ECS_HIP(hipMemcpyAsync(d_args, h, sizeof(TileArgs) * n, hipMemcpyHostToDevice, st));
+  constexpr int F = M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN, FC = F | M_COLLIDE;
+  const size_t lds = lds_for(cs[0]->g, coll ? FC : F, cs[0]->block);
+  const int block = cs[0]->block, tx = cs[0]->g.Tx, tz = cs[0]->g.Tz;
+  if (tx == 8 && tz == 8) {
+    if (coll) launch_tile_runs<FC, 8, 8, 8>(st, t, block, lds);
+    else launch_tile_runs<F, 8, 8, 8>(st, t, block, lds);
+  } else if (tx == 8 && tz == 4) {
+    if (coll) launch_tile_runs<FC, 8, 8, 4>(st, t, block, lds);
+    else launch_tile_runs<F, 8, 8, 4>(st, t, block, lds);
+  } else {
+    if (coll) launch_tile_runs<FC, 4, 4, 4>(st, t, block, lds);
+    else launch_tile_runs<F, 4, 4, 4>(st, t, block, lds);
+  }
+}
+
+size_t tile_args_bytes() { return std::max(sizeof(TileArgs), mx_args_bytes()); }
+
+// One fused particle step of every run of cs (3-D contexts whose stream is st for the
+// call): the stages of run_fused_particles, each queued for all runs before the next.
+// The rate predictions of all runs share each host synchronisation; the tile passes of
+// runs with the same precision, tile shape, block size and collision setting run as one
+// launch (at most RUNS_MAX runs each); the counters of all runs come back with one
+// synchronisation.  Every run's arithmetic and draws are those of run_fused_particles,
+// so its ions and moments are the same bit for bit.  A run whose checks fail gets its
+// error code in rc and its message in last_error and takes no further part; a HIP error
+// is thrown.  Events ev_pass0/ev_pass1 (may be null) bracket the tile passes.
+void run_fused_particles_runs(const std::vector<ecsprt_hip_ctx *> &cs, hipStream_t st, double dt, void *h_args,
+                              void *d_args, std::vector<int> &rc, hipEvent_t ev_pass0, hipEvent_t ev_pass1) {
+  const int n = (int)cs.size();
+  std::vector<char> coll(n, 0);
+  std::vector<PoissonParams> pt(n);
+  std::vector<PredictState> ps(n);
+  for (int r = 0; r < n; ++r) {
+    ecsprt_hip_ctx *c = cs[r];
+    zero_ctr(c);
+    clear_overflow(c);
+    coll[r] = c->o.collide_mode == ECSPRT_COLLIDE_PHILOX && c->p.mn > 0.0;
+    if (!coll[r]) continue;
+    const int Nc = collision_count(&c->p, c->np, dt);
+    c->last_Nc = Nc;
+    if (c->coll_lambda <= 0.0) run_collide_philox_pilot(c, dt);
+    else predict_begin(c, Nc, ps[r]);
+    c->lambda_step = c->step_index;
+  }
+  for (;;) {
+    bool any = false;
+    for (int r = 0; r < n; ++r)
+      if (ps[r].active) {
+        predict_enqueue(cs[r], ps[r]);
+        any = true;
+      }
+    if (!any) break;
+    ECS_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < n; ++r)
+      if (ps[r].active) predict_collect(cs[r], ps[r]);
+  }
+  for (int r = 0; r < n; ++r) {
+    predict_end(cs[r], ps[r]);
+    if (coll[r]) pt[r] = poisson_params(cs[r]->last_Nc > 0 ? cs[r]->coll_lambda : 0.0);
+  }
+  // groups of runs that share one launch
+  if (ev_pass0) ECS_HIP(hipEventRecord(ev_pass0, st));
+  std::vector<char> done(n, 0);
+  size_t off = 0;
+  const size_t abytes = (tile_args_bytes() + 255) & ~(size_t)255;
+  for (int r0 = 0; r0 < n; ++r0) {
+    if (done[r0]) continue;
+    const ecsprt_hip_ctx *c0 = cs[r0];
+    std::vector<int> g;
+    for (int r = r0; r < n && (int)g.size() < RUNS_MAX; ++r) {
+      const ecsprt_hip_ctx *c = cs[r];
+      if (!done[r] && c->mixed == c0->mixed && c->g.Tx == c0->g.Tx && c->g.Tz == c0->g.Tz && c->block == c0->block &&
+          coll[r] == coll[r0]) {
+        g.push_back(r);
+        done[r] = 1;
+      }
+    }
+    std::vector<ecsprt_hip_ctx *> gc;
+    std::vector<PoissonParams> gpt;
+    for (int r : g) {
+      gc.push_back(cs[r]);
+      gpt.push_back(pt[r]);
+    }
+    void *h = static_cast<char *>(h_args) + off, *d = static_cast<char *>(d_args) + off;
+    off += abytes * g.size();
+    if (c0->mixed) {
+      std::vector<const PoissonParams *> pp;
+      std::vector<uint32_t> k0, k1, sp;
+      for (size_t i = 0; i < g.size(); ++i) {
+        const ecsprt_hip_ctx *c = gc[i];
+        pp.push_back(coll[r0] ? &gpt[i] : nullptr);
+        k0.push_back(coll[r0] ? (uint32_t)c->o.seed : 0u);
+        k1.push_back(coll[r0] ? (uint32_t)(c->o.seed >> 32) : 0u);
+        sp.push_back(coll[r0] ? (uint32_t)c->step_index : 0u);
+      }
+      mx_launch_runs(st, gc, dt, pp, k0, k1, sp, h, d);
+    } else {
+      tile_runs_f64(st, gc, dt, coll[r0], gpt, h, d);
+    }
+  }
+  if (ev_pass1) ECS_HIP(hipEventRecord(ev_pass1, st));
+  for (int r = 0; r < n; ++r) {
+    ecsprt_hip_ctx *c = cs[r];
+    finish_rebin(c);
+    launch_late(c);
+    launch_fold(c);
+    ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * CTR_N, hipMemcpyDeviceToHost, st));
+  }
+  ECS_HIP(hipStreamSynchronize(st));
+  for (int r = 0; r < n; ++r) {
+    ecsprt_hip_ctx *c = cs[r];
+    try {
+      check_counters(c);
+      if (coll[r]) collision_totals(c);
+      c->step_index++;
+      rc[r] = ECSPRT_OK;
+    } catch (const Error &e) {
+      if (e.code == ECSPRT_ERR_HIP) throw;
+      c->last_error = e.msg;
+      rc[r] = e.code;
+    }
+  }
 }
 
 // ScatterElastic (src/collisions.c:9-167) in the Philox statistical mode outside the

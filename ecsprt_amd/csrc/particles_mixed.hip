@@ -262,15 +262,16 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
   }
 }
 
+// The fused pass of workgroup wi of one run (arguments at ap): phase 1 pushes, decides
+// collisions and finishes the ions without attempts; phases 2 and 3 collide the deferred
+// ones; the deposit region goes to the run's fold buffer.
 template <int MODE, int TX, int TY, int TZ, int HH>
-__global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
+__device__ __attribute__((always_inline)) inline void tile_mx(ConstPtr<TileArgsMx> ap, int wi) {
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
+  const TileArgsMx &a = fresh(ap);
   extern __shared__ long long smemq[];
   __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
   const TileGeom g = a.g;
-  const int nwg = gridDim.x;
-  int wi = blockIdx.x;
-  if ((nwg & 7) == 0) wi = (blockIdx.x & 7) * (nwg >> 3) + (blockIdx.x >> 3);
   constexpr int NEX = TX + 1, NEY = TY + 1, NE3 = NEX * NEY * (TZ + 1);
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
   TileWGMx w;
@@ -334,12 +335,13 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
   // Phase 1.  Two workgroups share a CU (64 VGPRs), so one's loads overlap the
   // other's arithmetic without a software prefetch.
   for (long p0 = beg; p0 < end; p0 += blockDim.x) {
+    const TileArgsMx &A = fresh(ap);
     const long p = p0 + tid;
     const bool valid = p < end;
     uint32_t cell = 0;
     float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
     int pid = 0;
-    if (valid) load_mx(a.src, p, cell, ox, oy, oz, vx, vy, vz, pid);
+    if (valid) load_mx(A.src, p, cell, ox, oy, oz, vx, vy, vz, pid);
     const int ix = mx_i(cell), iy = mx_j(cell), iz = mx_k(cell);
     if ((MODE & M_PUSH) && valid) {
       // BorisMoverBz3D body (src/velocities.c:342-356) in FP32.
@@ -347,23 +349,23 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
       // the staged (T+1)^3 block.
       float E[3];
       gather_nodes<float, NEX, NEY, NE3>(sE, ((iz - w.k0) * NEY + (iy - w.j0)) * NEX + (ix - w.i0), ox, oy, oz, E);
-      const float vxm = vx + a.k.Escale[0] * E[0];
-      const float vym = vy + a.k.Escale[1] * E[1];
-      const float vzm = vz + a.k.Escale[2] * E[2];
-      const float vxt = vxm + vym * a.k.t;
-      const float vyp = vym - vxt * a.k.s;
-      const float vxp = vxt + vyp * a.k.t;
+      const float vxm = vx + A.k.Escale[0] * E[0];
+      const float vym = vy + A.k.Escale[1] * E[1];
+      const float vzm = vz + A.k.Escale[2] * E[2];
+      const float vxt = vxm + vym * A.k.t;
+      const float vyp = vym - vxt * A.k.s;
+      const float vxp = vxt + vyp * A.k.t;
       vx = vxp;
       vy = vyp;
       vz = vzm;
     }
     bool defer = false;
-    if ((MODE & M_COLLIDE) && valid && poisson_any(a.pt, a.key0, a.key1, a.step, pid)) {
+    if ((MODE & M_COLLIDE) && valid && poisson_any(A.pt, A.key0, A.key1, A.step, pid)) {
       defer = true;
       const int q = atomicAdd(&s_ndef, 1);
-      store_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
+      store_mx(A.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
     }
-    tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid && !defer, p, ix, iy, iz, ox, oy, oz, vx, vy, vz, pid);
+    tail_mx<MODE, TX, TY, TZ, HH>(A, w, valid && !defer, p, ix, iy, iz, ox, oy, oz, vx, vy, vz, pid);
   }
   if (MODE & M_COLLIDE) {
     // Phase 2: deferred ions, first attempt each (every lane busy).  The scattering
@@ -374,6 +376,7 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
     const int ndef = s_ndef;
     const long room = (end - beg) - ndef;
     for (int q0 = 0; q0 < ndef; q0 += blockDim.x) {
+      const TileArgsMx &A = fresh(ap);
       const int q = q0 + tid;
       const bool valid = q < ndef;
       bool done = valid;
@@ -381,37 +384,38 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
       float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
       int pid = 0;
       if (valid) {
-        load_mx(a.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
-        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
-        collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
+        load_mx(A.def, beg + q, cell, ox, oy, oz, vx, vy, vz, pid);
+        const int k = poisson_attempts(A.pt, A.key0, A.key1, A.step, pid);
+        collide_attempts<float>(A.cc, A.key0, A.key1, A.step, pid, 0, 1, vx, vy, vz, natt, nsucc, nfail);
         if (k > 1) {
           const int r = atomicAdd(&s_n3, 1);
           if (r < room) {
-            store_mx(a.def, beg + ndef + r, cell, ox, oy, oz, vx, vy, vz, pid);
+            store_mx(A.def, beg + ndef + r, cell, ox, oy, oz, vx, vy, vz, pid);
             done = false;
           } else {
-            collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
+            collide_attempts<float>(A.cc, A.key0, A.key1, A.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
           }
         }
       }
-      tail_mx<MODE, TX, TY, TZ, HH>(a, w, done, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
+      tail_mx<MODE, TX, TY, TZ, HH>(A, w, done, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
                                     pid);
     }
     // Phase 3: the remaining attempts of the parked ions.
     __syncthreads();
     const int n3 = (int)min((long)s_n3, room);
     for (int q0 = 0; q0 < n3; q0 += blockDim.x) {
+      const TileArgsMx &A = fresh(ap);
       const int q = q0 + tid;
       const bool valid = q < n3;
       uint32_t cell = 0;
       float ox = 0, oy = 0, oz = 0, vx = 0, vy = 0, vz = 0;
       int pid = 0;
       if (valid) {
-        load_mx(a.def, beg + ndef + q, cell, ox, oy, oz, vx, vy, vz, pid);
-        const int k = poisson_attempts(a.pt, a.key0, a.key1, a.step, pid);
-        collide_attempts<float>(a.cc, a.key0, a.key1, a.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
+        load_mx(A.def, beg + ndef + q, cell, ox, oy, oz, vx, vy, vz, pid);
+        const int k = poisson_attempts(A.pt, A.key0, A.key1, A.step, pid);
+        collide_attempts<float>(A.cc, A.key0, A.key1, A.step, pid, 1, k, vx, vy, vz, natt, nsucc, nfail);
       }
-      tail_mx<MODE, TX, TY, TZ, HH>(a, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
+      tail_mx<MODE, TX, TY, TZ, HH>(A, w, valid, 0, mx_i(cell), mx_j(cell), mx_k(cell), ox, oy, oz, vx, vy, vz,
                                     pid);
     }
     __shared__ unsigned long long s_red[16][3];
@@ -422,6 +426,20 @@ __global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx a) {
     long long *out = a.fold + (size_t)wi * 4 * RA3;
     for (int q = tid; q < 4 * RA3; q += blockDim.x) out[q] = w.sacc[q];
   }
+}
+
+template <int MODE, int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(1024, 8) k_tile_mx(TileArgsMx) {
+  tile_mx<MODE, TX, TY, TZ, HH>(kernarg<TileArgsMx>(), xcd_order(blockIdx.x, gridDim.x));
+}
+
+// One launch over the tiles of several runs of the same tile shape and block size.
+template <int MODE, int TX, int TY, int TZ, int HH>
+__global__ void __launch_bounds__(1024, 8) k_tile_mx_runs(RunTable) {
+  const ConstPtr<RunTable> t = kernarg<RunTable>();
+  const int gi = xcd_order(blockIdx.x, gridDim.x);
+  const int r = run_of(t, gi);
+  tile_mx<MODE, TX, TY, TZ, HH>((ConstPtr<TileArgsMx>)t->args + r, gi - t->start[r]);
 }
 
 // Emigrants further than one tile -> destination tiles.
@@ -620,6 +638,11 @@ static void set_attr_mx() {
     const int dyn = 160 * 1024 - (int)fa.sharedSizeBytes;
     ECS_HIP(hipFuncSetAttribute((const void *)k_tile_mx<MODE, TX, TY, TZ, HH>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, dyn));
+    if constexpr ((MODE & (M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN)) == (M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN)) {
+      ECS_HIP(hipFuncGetAttributes(&fa, (const void *)k_tile_mx_runs<MODE, TX, TY, TZ, HH>));
+      ECS_HIP(hipFuncSetAttribute((const void *)k_tile_mx_runs<MODE, TX, TY, TZ, HH>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - (int)fa.sharedSizeBytes));
+    }
   });
 }
 
@@ -657,8 +680,9 @@ static void launch_mx(ecsprt_hip_ctx *c, const TileArgsMx &a) {
   ECS_HIP(hipGetLastError());
 }
 
-void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonParams *pt, uint32_t key0, uint32_t key1,
-               uint32_t step) {
+// Arguments of a tile pass of context c (pt null: no collisions).
+static TileArgsMx mx_args(ecsprt_hip_ctx *c, double dt, const PoissonParams *pt, uint32_t key0, uint32_t key1,
+                          uint32_t step) {
   TileArgsMx a{};
   a.g = c->g;
   a.src = c->Am;
@@ -685,6 +709,12 @@ void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonParams *pt, 
   a.key0 = key0;
   a.key1 = key1;
   a.step = step;
+  return a;
+}
+
+void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonParams *pt, uint32_t key0, uint32_t key1,
+               uint32_t step) {
+  const TileArgsMx a = mx_args(c, dt, pt, key0, key1, step);
   switch (mode) {
     case M_PUSH: launch_mx<M_PUSH>(c, a); break;
     case M_DEPOSIT: launch_mx<M_DEPOSIT>(c, a); break;
@@ -696,6 +726,55 @@ void mx_launch(ecsprt_hip_ctx *c, int mode, double dt, const PoissonParams *pt, 
     default: throw Error{ECSPRT_ERR_ARG, "unsupported tile pass"};
   }
 }
+
+template <int MODE, int TX, int TY, int TZ>
+static void launch_mx_runs(hipStream_t st, const RunTable &t, int block, size_t lds) {
+  k_tile_mx_runs<MODE, TX, TY, TZ, 2><<<t.start[t.nrun], block, lds, st>>>(t);
+  ECS_HIP(hipGetLastError());
+}
+
+// The fused pass (with collisions when pt[0] is non-null; then every run has them) over
+// the runs cs, which share the tile shape and block size, in one launch on stream st.
+// The arguments are staged at h_args (pinned) and copied to d_args first.
+void mx_launch_runs(hipStream_t st, const std::vector<ecsprt_hip_ctx *> &cs, double dt,
+                    const std::vector<const PoissonParams *> &pt, const std::vector<uint32_t> &key0,
+                    const std::vector<uint32_t> &key1, const std::vector<uint32_t> &step, void *h_args, void *d_args) {
+  const int n = (int)cs.size();
+  ECS_CHECK(n >= 1 && n <= RUNS_MAX, ECSPRT_ERR_ARG, "multi-run tile pass: 1 to 64 runs");
+  TileArgsMx *h = static_cast<TileArgsMx *>(h_args);
+  RunTable t{};
+  t.args = d_args;
+  t.nrun = n;
+  long total = 0;
+  for (int r = 0; r < n; ++r) {
+    ecsprt_hip_ctx *c = cs[r];
+    ECS_CHECK(c->g.Tx == cs[0]->g.Tx && c->g.Tz == cs[0]->g.Tz && c->block == cs[0]->block, ECSPRT_ERR_ARG,
+              "multi-run tile pass: tile shapes or block sizes differ");
+    h[r] = mx_args(c, dt, pt[r], key0[r], key1[r], step[r]);
+    t.start[r] = (int)total;
+    total += (long)c->ntiles * c->g.split;
+  }
+  ECS_CHECK(total < (1L << 31), ECSPRT_ERR_ARG, "multi-run tile pass: too many workgroups");
+  t.start[n] = (int)total;
+  ECS_HIP(hipMemcpyAsync(d_args, h, sizeof(TileArgsMx) * n, hipMemcpyHostToDevice, st));
+  const bool coll = pt[0] != nullptr;
+  const int mode = coll ? (M_PUSH | M_COLLIDE | M_DRIFT | M_DEPOSIT | M_REBIN) : (M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN);
+  const size_t lds = lds_mx(cs[0]->g, mode);
+  const int block = cs[0]->block, tx = cs[0]->g.Tx, tz = cs[0]->g.Tz;
+  constexpr int F = M_PUSH | M_DRIFT | M_DEPOSIT | M_REBIN, FC = F | M_COLLIDE;
+  if (tx == 8 && tz == 8) {
+    if (coll) launch_mx_runs<FC, 8, 8, 8>(st, t, block, lds);
+    else launch_mx_runs<F, 8, 8, 8>(st, t, block, lds);
+  } else if (tx == 8 && tz == 4) {
+    if (coll) launch_mx_runs<FC, 8, 8, 4>(st, t, block, lds);
+    else launch_mx_runs<F, 8, 8, 4>(st, t, block, lds);
+  } else {
+    if (coll) launch_mx_runs<FC, 4, 4, 4>(st, t, block, lds);
+    else launch_mx_runs<F, 4, 4, 4>(st, t, block, lds);
+  }
+}
+
+size_t mx_args_bytes() { return sizeof(TileArgsMx); }
 
 // The Philox collide pass of the CollideFunc slot on mixed ions: the fused pass's
 // phases 2 and 3 (FP32 scattering on the same draws, in this file's arithmetic), applied

@@ -648,6 +648,9 @@ void solver_build_operator(ecsprt_hip_ctx *c) {
   c->operator_built = true;
 }
 
+// The FP64 level-0 density of the operator as last built.
+const double *solver_density(const ecsprt_hip_ctx *c) { return c->solver->mg64.L[0].dn; }
+
 void solver_rhs(ecsprt_hip_ctx *c, double *b) {
   const ecsprt_params &p = c->p;
   const RhsConsts k = rhs_consts(p);

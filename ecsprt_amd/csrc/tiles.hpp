@@ -9,6 +9,54 @@ namespace ecsprt {
 
 enum : int { M_PUSH = 1, M_DRIFT = 2, M_DEPOSIT = 4, M_REBIN = 8, M_COLLIDE = 16 };
 
+// Argument structs read afresh.  A by-value argument struct is loaded into scalar
+// registers at kernel entry and each field stays live to its last use; in the fused
+// tile kernels that is about twice the scalar register budget, and the compiler spills
+// the excess to vector-register lanes, one VALU instruction per read inside the particle
+// loops.  Read through a constant-address-space pointer that an empty asm statement
+// redefines at each call, the fields are not carried between loop iterations: they
+// become scalar loads, which the scalar cache serves.  The struct must stay unchanged
+// while the kernel runs.
+template <typename T>
+using ConstPtr = const __attribute__((address_space(4))) T *;
+
+template <typename T>
+__device__ inline const T &fresh(ConstPtr<T> p) {
+  asm volatile("" : "+s"(p));
+  return *(const T *)p;
+}
+
+// The kernel's argument T, for a kernel whose sole explicit argument is a T (the
+// code-object ABI places it at offset 0 of the kernel-argument segment).
+template <typename T>
+__device__ inline ConstPtr<T> kernarg() {
+  return (ConstPtr<T>)__builtin_amdgcn_kernarg_segment_ptr();
+}
+
+// Workgroups of one launch over the tiles of several runs: run r owns workgroups
+// [start[r], start[r + 1]) and its argument struct is args[r] (device memory).
+constexpr int RUNS_MAX = 64;
+struct RunTable {
+  const void *args;
+  int nrun;
+  int start[RUNS_MAX + 1];
+};
+
+// Run of workgroup gi (binary search over the starts, read from the kernel argument).
+__device__ inline int run_of(ConstPtr<RunTable> t, int gi) {
+  int lo = 0, hi = t->nrun - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t->start[mid] <= gi) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; with a grid of a multiple of 8
+// each XCD takes a contiguous run of workgroup indices, so neighbouring tiles share an L2.
+__device__ inline int xcd_order(int b, int nwg) { return (nwg & 7) == 0 ? (b & 7) * (nwg >> 3) + (b >> 3) : b; }
+
 __device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg) {
   int ci = (int)xg, cj = (int)yg, ck = (int)zg;
   if (ci >= g.Nx) ci = g.Nx - 1;
@@ -133,6 +181,10 @@ void mx_collide_philox(ecsprt_hip_ctx *c, const PoissonParams &pt, const Collide
                        uint32_t key1, uint32_t step);
 double mx_vmax(const ecsprt_params &p);
 int fx_bits(const ecsprt_hip_ctx *c);
+void mx_launch_runs(hipStream_t st, const std::vector<ecsprt_hip_ctx *> &cs, double dt,
+                    const std::vector<const PoissonParams *> &pt, const std::vector<uint32_t> &key0,
+                    const std::vector<uint32_t> &key1, const std::vector<uint32_t> &step, void *h_args, void *d_args);
+size_t mx_args_bytes();
 
 // Fill kernels launched on the context's stream (particles.hip).
 void fill_f64(ecsprt_hip_ctx *c, double *p, long n, double v);

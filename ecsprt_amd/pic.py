@@ -345,6 +345,52 @@ def ensemble_step(contexts, dt):
         raise HipError(rc, "")
 
 
+class Ensemble:
+    """Lockstep ensemble over contexts (ecsprt_hip_ensemble_*): each advance is one
+    timestep of every run, driven from one thread with one launch per stage for all runs;
+    every run's state equals that of stepping it alone."""
+
+    def __init__(self, contexts):
+        L = _lib.load()
+        self.contexts = list(contexts)
+        n = len(self.contexts)
+        arr = (ctypes.c_void_p * n)(*[c.ptr.value for c in self.contexts])
+        self.ptr = ctypes.c_void_p()
+        rc = L.ecsprt_hip_ensemble_create(arr, n, ctypes.byref(self.ptr))
+        if rc != _lib.OK:
+            msg = L.ecsprt_hip_last_error(None)
+            raise HipError(rc, msg.decode() if msg else "")
+
+    def advance(self, dt):
+        L = _lib.load()
+        rc = L.ecsprt_hip_ensemble_advance(self.ptr, dt)
+        if rc != _lib.OK:
+            for c in self.contexts:
+                msg = L.ecsprt_hip_last_error(c.ptr)
+                if msg:
+                    raise HipError(rc, msg.decode())
+            msg = L.ecsprt_hip_ensemble_last_error(self.ptr)
+            raise HipError(rc, msg.decode() if msg else "")
+
+    def traffic(self):
+        """Algorithmic HBM bytes of the last advance's solve."""
+        b = ctypes.c_double()
+        _check(_lib.load().ecsprt_hip_ensemble_traffic(self.ptr, ctypes.byref(b)))
+        return b.value
+
+    def stage_times(self):
+        """ms of the last advance: tile passes, stages before them, re-binning and folds,
+        forcing and solve."""
+        ms = np.zeros(4)
+        _check(_lib.load().ecsprt_hip_ensemble_stage_times(self.ptr, _dp(ms)))
+        return ms
+
+    def close(self):
+        if self.ptr:
+            _lib.load().ecsprt_hip_ensemble_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+
 class Batch:
     """Batched standalone potential solves (one per grid in `dims`): on `slots`
     concurrent streams, or with slots = 0 in lockstep, every kernel launched once for

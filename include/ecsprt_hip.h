@@ -327,6 +327,32 @@ int ecsprt_hip_batch_traffic(const ecsprt_hip_batch *b, double *bytes);
  * non-zero code; each context's message is in its ecsprt_hip_last_error. */
 int ecsprt_hip_ensemble_step(ecsprt_hip_ctx *const *ctxs, int n, double dt);
 
+/* Lockstep ensemble: the same timestep on each of n contexts (three-dimensional, Philox
+ * collisions or none, equal solver options, zero initial guess, multigrid
+ * preconditioning), driven from one host thread on one stream with one launch per stage
+ * for all runs.  The fused particle passes of runs that share a precision, tile shape
+ * and block size run as one kernel; the rate predictions and counter checks of all runs
+ * share each host synchronisation; the potentials are solved by one lockstep
+ * GMRES/multigrid over all runs (the lockstep batch solver).  From the same state, each
+ * run's ions and moments equal those of ecsprt_hip_step bit for bit; its potential
+ * solves the same system to the same tolerance (the same method with its partial sums
+ * in another order).  The contexts keep their own state and may still be stepped alone
+ * between ensemble steps.  No counterpart in the reference.  advance returns the first
+ * non-zero code of a run (its message in that context's ecsprt_hip_last_error) or of
+ * the ensemble (ensemble_last_error); runs that failed in the particle stage are left
+ * out of that step's solve. */
+typedef struct ecsprt_hip_ensemble ecsprt_hip_ensemble;
+int ecsprt_hip_ensemble_create(ecsprt_hip_ctx *const *ctxs, int n, ecsprt_hip_ensemble **out);
+int ecsprt_hip_ensemble_advance(ecsprt_hip_ensemble *e, double dt);
+int ecsprt_hip_ensemble_destroy(ecsprt_hip_ensemble *e);
+const char *ecsprt_hip_ensemble_last_error(const ecsprt_hip_ensemble *e);
+/* Algorithmic HBM bytes of the last advance's lockstep solve (every run). */
+int ecsprt_hip_ensemble_traffic(const ecsprt_hip_ensemble *e, double *bytes);
+/* HIP-event times (ms) of the last advance: [0] the fused tile passes (all launches),
+ * [1] the stages before them (counters, rate predictions), [2] re-binning, late deposits
+ * and folds with the counter checks, [3] the forcing and the lockstep solve. */
+int ecsprt_hip_ensemble_stage_times(const ecsprt_hip_ensemble *e, double *ms);
+
 /* Synthetic density field on the device: n = exp(sigma g - sigma^2/2), g ~ N(0,1) per node
  * (Philox4x32-10 keyed by seed), smoothed by a periodic Gaussian of `smooth` cells (radius
  * ceil(4 smooth)), normalised to mean 1; written as FluidNode moments (n, s n, s n, s n) with
