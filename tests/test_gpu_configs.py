@@ -338,6 +338,36 @@ def test_c4_ensemble_equals_sequential_steps():
         b.close()
 
 
+def test_c4_lockstep_ensemble_equals_single_steps():
+    """c4 as the bench steps it, one lockstep ensemble: per step from a common state, each
+    run's ions, moments and collision counts equal its own single step bit for bit, and its
+    potential (the lockstep solver's) is within 1e-8 of the single solve's."""
+    runs = configs.ensemble_runs()
+    say("c4 lockstep ensemble vs single steps")
+    ens = [_ensemble_member(r, MIXED) for r in runs]
+    seq = [_ensemble_member(r, MIXED) for r in runs]
+    e = pic.Ensemble(ens)
+    for step in range(2):
+        e.advance(1.0)
+        worst = 0.0
+        for a, b in zip(ens, seq):
+            b.step(1.0)
+            pa, va = a.get_particles()
+            pb, vb = b.get_particles()
+            np.testing.assert_array_equal(pa, pb)
+            np.testing.assert_array_equal(va, vb)
+            np.testing.assert_array_equal(a.get_moments(), b.get_moments())
+            assert a.collision_stats() == b.collision_stats() and a.collision_stats()[1] > 0
+            fa = a.get_potential()
+            worst = max(worst, rel(fa, b.get_potential()))
+            b.set_potential(fa)
+        say(f"  step {step}: potential vs single solves {worst:.1e}")
+        assert worst <= 1e-8
+    e.close()
+    for c in ens + seq:
+        c.close()
+
+
 def test_c4_per_run_mixed_errors_against_fp64():
     """Each of c4's runs: one mixed step against one FP64 step from the same initial ions
     (collisions off): density, flux and potential within 1e-6 of each field's maximum."""
