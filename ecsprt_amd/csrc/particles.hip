@@ -85,7 +85,7 @@ __device__ inline void deposit_direct(double *ovf, const TileGeom &g, const Push
 __device__ inline void late_append(double *late, int *late_cnt, double *ovf, const TileGeom &g, const PushConsts &pc,
                                    int dt, int base, int cap, double px, double py, double pz, double vx, double vy,
                                    double vz) {
-  const int li = atomicAdd(&late_cnt[dt], 1);
+  const int li = atomicAdd(&late_cnt[tile_ctr(dt)], 1);
   if (li < (cap >> 2)) {
     double2 *e = reinterpret_cast<double2 *>(late + 6 * ((long)(base >> 2) + li));
     e[0] = make_double2(px, py);
@@ -201,7 +201,7 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
     __syncthreads();
     if (tid < 27) {
       const int n = cnt[tid];
-      if (n > 0) bas[tid] = atomicAdd(&a.next[w.s_nbr[tid]], n);
+      if (n > 0) bas[tid] = atomicAdd(&a.next[tile_ctr(w.s_nbr[tid])], n);
     } else if (tid >= 64 && tid < 64 + 27) {
       w.s_cnt[27 * (w.par ^ 1) + tid - 64] = 0;
     }
@@ -465,9 +465,9 @@ __global__ void __launch_bounds__(256) k_late(TileGeom g, const double *__restri
   constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
   __shared__ long long acc[4 * N3];
   const int t = blockIdx.x;
-  const int n = min(late_cnt[t], cap[t] >> 2);
+  const int n = min(late_cnt[tile_ctr(t)], cap[t] >> 2);
   __syncthreads();
-  if (threadIdx.x == 0) late_cnt[t] = 0;
+  if (threadIdx.x == 0) late_cnt[tile_ctr(t)] = 0;
   if (n == 0) return;
   const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
   for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0;
@@ -525,7 +525,7 @@ __global__ void k_rebin(TileGeom g, ParticlesF64 emi, ParticlesF64 dst, Particle
     const double x = emi.x(e), y = emi.y(e), z = emi.z(e), vx = emi.vx(e), vy = emi.vy(e), vz = emi.vz(e);
     const int tagged = emi.id(e), id = tagged & ~EMI_DEPOSIT;
     const int t = tile_of(g, x / pc.dx, y / pc.dy, z / pc.dz);
-    const int slot = atomicAdd(&next[t], 1);
+    const int slot = atomicAdd(&next[tile_ctr(t)], 1);
     if (slot < cap[t]) {
       store_particle(dst, (long)base[t] + slot, x, y, z, vx, vy, vz, id);
     } else {
@@ -540,8 +540,8 @@ __global__ void k_rebin(TileGeom g, ParticlesF64 emi, ParticlesF64 dst, Particle
 __global__ void k_finalize_counts(int ntiles, int *count, int *next, const int *cap) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < ntiles) {
-    count[t] = min(next[t], cap[t]);
-    next[t] = 0;
+    count[t] = min(next[tile_ctr(t)], cap[t]);
+    next[tile_ctr(t)] = 0;
   }
 }
 
@@ -899,11 +899,11 @@ void tiles_init(ecsprt_hip_ctx *c) {
   ECS_HIP(hipMalloc(&c->d_base, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_cap, sizeof(int) * c->ntiles));
   ECS_HIP(hipMalloc(&c->d_count, sizeof(int) * c->ntiles));
-  ECS_HIP(hipMalloc(&c->d_next, sizeof(int) * c->ntiles));
-  ECS_HIP(hipMalloc(&c->d_late_cnt, sizeof(int) * c->ntiles));
-  fill_i32(c, c->d_late_cnt, c->ntiles, 0);
+  ECS_HIP(hipMalloc(&c->d_next, sizeof(int) * tile_ctr(c->ntiles)));
+  ECS_HIP(hipMalloc(&c->d_late_cnt, sizeof(int) * tile_ctr(c->ntiles)));
+  fill_i32(c, c->d_late_cnt, tile_ctr(c->ntiles), 0);
   fill_i32(c, c->d_count, c->ntiles, 0);
-  fill_i32(c, c->d_next, c->ntiles, 0);
+  fill_i32(c, c->d_next, tile_ctr(c->ntiles), 0);
   clear_overflow(c);
   zero_ctr(c);
   set_lds_attrs<M_PUSH>();
@@ -949,13 +949,13 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
   const TileGeom &g = c->g;
   const ecsprt_params &p = c->p;
   zero_ctr(c);
-  fill_i32(c, c->d_next, c->ntiles, 0);
+  fill_i32(c, c->d_count, c->ntiles, 0);
   DriftConsts dc = drift_consts(p, 0.0);
-  k_bin_count<<<grid_for(np, 256), 256, 0, c->stream>>>(g, c->d_stage, np, p.dx, p.dy, p.dz, dc, c->d_key, c->d_next,
+  k_bin_count<<<grid_for(np, 256), 256, 0, c->stream>>>(g, c->d_stage, np, p.dx, p.dy, p.dz, dc, c->d_key, c->d_count,
                                                          c->d_ctr);
   ECS_HIP(hipGetLastError());
   std::vector<int> cnt(c->ntiles);
-  ECS_HIP(hipMemcpyAsync(cnt.data(), c->d_next, sizeof(int) * c->ntiles, hipMemcpyDeviceToHost, c->stream));
+  ECS_HIP(hipMemcpyAsync(cnt.data(), c->d_count, sizeof(int) * c->ntiles, hipMemcpyDeviceToHost, c->stream));
   read_ctr(c);
   const long lost = (long)c->h_ctr[2];
   // Capacities with slack for immigrants; bases in tile order.
@@ -1009,8 +1009,7 @@ void particles_bin_from_stage(ecsprt_hip_ctx *c, long np, bool) {
   }
   ECS_HIP(hipMemcpyAsync(c->d_base, c->h_base.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice, c->stream));
   ECS_HIP(hipMemcpyAsync(c->d_cap, c->h_cap.data(), sizeof(int) * c->ntiles, hipMemcpyHostToDevice, c->stream));
-  ECS_HIP(hipMemcpyAsync(c->d_count, c->d_next, sizeof(int) * c->ntiles, hipMemcpyDeviceToDevice, c->stream));
-  fill_i32(c, c->d_next, c->ntiles, 0);
+  fill_i32(c, c->d_next, tile_ctr(c->ntiles), 0);
   if (c->mixed) {
     mx_bin_scatter(c, np);
   } else {

@@ -158,7 +158,7 @@ __device__ inline void deposit_direct_mx(const TileArgsMx &a, int ix, int iy, in
 __device__ inline void late_append_mx(uint4 *late, int *late_cnt, double *ovf, const TileGeom &g, int dt, int base,
                                       int cap, int ix, int iy, int iz, float ox, float oy, float oz, float vx, float vy,
                                       float vz) {
-  const int li = atomicAdd(&late_cnt[dt], 1);
+  const int li = atomicAdd(&late_cnt[tile_ctr(dt)], 1);
   if (li < (cap >> 2)) {
     uint4 *e = late + 2 * ((long)(base >> 2) + li);
     e[0] = make_uint4(mx_pack(ix, iy, iz), __float_as_uint(ox), __float_as_uint(oy), __float_as_uint(oz));
@@ -237,7 +237,7 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
     __syncthreads();
     if (tid < 27) {
       const int n = cnt[tid];
-      if (n > 0) bas[tid] = atomicAdd(&a.next[w.s_nbr[tid]], n);
+      if (n > 0) bas[tid] = atomicAdd(&a.next[tile_ctr(w.s_nbr[tid])], n);
     } else if (tid >= 64 && tid < 64 + 27) {
       w.s_cnt[27 * (w.par ^ 1) + tid - 64] = 0;
     }
@@ -457,9 +457,9 @@ __global__ void __launch_bounds__(256) k_late_mx(TileGeom g, const uint4 *__rest
   constexpr int NX = TX + 1, NY = TY + 1, NZ = TZ + 1, N3 = NX * NY * NZ;
   __shared__ long long acc[4 * N3];
   const int t = blockIdx.x;
-  const int n = min(late_cnt[t], cap[t] >> 2);
+  const int n = min(late_cnt[tile_ctr(t)], cap[t] >> 2);
   __syncthreads();
-  if (threadIdx.x == 0) late_cnt[t] = 0;
+  if (threadIdx.x == 0) late_cnt[tile_ctr(t)] = 0;
   if (n == 0) return;
   const int i0 = (t % g.ntx) * TX, j0 = ((t / g.ntx) % g.nty) * TY, k0 = (t / (g.ntx * g.nty)) * TZ;
   for (int q = threadIdx.x; q < 4 * N3; q += blockDim.x) acc[q] = 0;
@@ -516,7 +516,7 @@ __global__ void k_rebin_mx(TileGeom g, ParticlesMix emi, ParticlesMix dst, Parti
     load_mx(emi, (long)e, cell, ox, oy, oz, vx, vy, vz, tagged);
     const int id = (int)((uint32_t)tagged & ~EMI_DEPOSIT_MX);
     const int t = ((mx_k(cell) / g.Tz) * g.nty + mx_j(cell) / g.Ty) * g.ntx + mx_i(cell) / g.Tx;
-    const int slot = atomicAdd(&next[t], 1);
+    const int slot = atomicAdd(&next[tile_ctr(t)], 1);
     if (slot < cap[t]) {
       store_mx(dst, (long)base[t] + slot, cell, ox, oy, oz, vx, vy, vz, id);
     } else {

@@ -89,6 +89,13 @@ __device__ inline void block_add3(unsigned long long (*red)[3], unsigned a, unsi
   }
 }
 
+// The per-tile slot counters (next) and late-list lengths (late_cnt) take device
+// atomics from the 27 neighbouring workgroups every batch.  Packed 32 to a 128-byte
+// line, the atomics on one line serialise as if on one counter; each tile's counter
+// sits on a line of its own.
+constexpr int TILE_CTR_STRIDE = 32;
+__host__ __device__ inline long tile_ctr(int t) { return (long)t * TILE_CTR_STRIDE; }
+
 // Tile offset d = b - a on a periodic axis of n tiles folded to -1, 0, +1; 2 when
 // the destination is further than one tile.
 __device__ inline int tile_step(int d, int n) {
