@@ -371,9 +371,13 @@ struct ecsprt_hip_ensemble {
   // start, tile passes begin, tile passes end, particle stages end, solve end
   hipEvent_t ev[5] = {};
   bool timed = false;  // the events of the last advance are complete
+  ecsprt::RunStreams rs;  // the runs' own streams during a step (events owned here)
   ~ecsprt_hip_ensemble() {
     for (hipEvent_t v : ev)
       if (v) (void)hipEventDestroy(v);
+    for (hipEvent_t v : rs.done)
+      if (v) (void)hipEventDestroy(v);
+    if (rs.fork) (void)hipEventDestroy(rs.fork);
     if (solve) ecsprt::lockstep_destroy(solve);
     if (h_args) (void)hipHostFree(h_args);
     (void)hipFree(d_args);
@@ -415,6 +419,9 @@ int ecsprt_hip_ensemble_create(ecsprt_hip_ctx *const *ctxs, int n, ecsprt_hip_en
     if (hipMalloc(&e->d_args, bytes) != hipSuccess) throw Error{ECSPRT_ERR_NOMEM, "ensemble: argument table"};
     e->op_set.assign(n, 0);
     for (hipEvent_t &v : e->ev) ECS_HIP(hipEventCreate(&v));
+    ECS_HIP(hipEventCreateWithFlags(&e->rs.fork, hipEventDisableTiming));
+    e->rs.done.assign(n, nullptr);
+    for (hipEvent_t &v : e->rs.done) ECS_HIP(hipEventCreateWithFlags(&v, hipEventDisableTiming));
   } catch (const Error &x) {
     ecsprt::set_global_error(x.msg);
     delete e;
@@ -471,9 +478,10 @@ int ecsprt_hip_ensemble_advance(ecsprt_hip_ensemble *e, double dt) {
       ECS_CHECK(e->ctx[q]->np > 0, ECSPRT_ERR_STATE, "ensemble: a run has no particles");
     }
     for (int q = 0; q < n; ++q) e->ctx[q]->stream = e->st;
+    e->rs.st = saved;
     e->timed = false;
     ECS_HIP(hipEventRecord(e->ev[0], e->st));
-    ecsprt::run_fused_particles_runs(e->ctx, e->st, dt, e->h_args, e->d_args, rc, e->ev[1], e->ev[2]);
+    ecsprt::run_fused_particles_runs(e->ctx, e->st, dt, e->h_args, e->d_args, rc, e->ev[1], e->ev[2], &e->rs);
     ECS_HIP(hipEventRecord(e->ev[3], e->st));
     std::vector<char> take(n, 0);
     std::vector<const double *> dens(n, nullptr), rhs(n, nullptr);

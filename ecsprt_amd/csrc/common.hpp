@@ -238,11 +238,34 @@ void run_push(ecsprt_hip_ctx *c, double dt);
 void run_drift_rebin(ecsprt_hip_ctx *c, double dt);
 void run_deposit(ecsprt_hip_ctx *c);
 void run_fused_particles(ecsprt_hip_ctx *c, double dt);
+// Streams of the runs of a lockstep step, one per run, with an event each and one for the
+// fork: the runs' small per-run launches (collision-rate passes, re-binning, late
+// deposits, folds) go out on them, concurrently, between a fork from and a join into the
+// step's stream.
+struct RunStreams {
+  std::vector<hipStream_t> st;
+  std::vector<hipEvent_t> done;
+  hipEvent_t fork = nullptr;
+};
+// Run r's launches go to its own stream (after everything queued on st so far) from
+// leave(r) to back(r), when rs is given; join() makes st wait for the runs that left.
+struct RunFork {
+  const std::vector<ecsprt_hip_ctx *> &cs;
+  hipStream_t st;
+  const RunStreams *rs;
+  std::vector<char> out;
+  RunFork(const std::vector<ecsprt_hip_ctx *> &c, hipStream_t s, const RunStreams *r);
+  void leave(int r);
+  void back(int r);
+  void join();
+};
 // One fused particle step of several 3-D runs on stream st, one launch per stage for all
 // of them; the kernel arguments are staged through h_args (pinned) and d_args, each at
-// least tile_args_bytes() rounded up to 256, per run.
+// least tile_args_bytes() rounded up to 256, per run.  With rs, the per-run launches run
+// on the runs' own streams.
 void run_fused_particles_runs(const std::vector<ecsprt_hip_ctx *> &cs, hipStream_t st, double dt, void *h_args,
-                              void *d_args, std::vector<int> &rc, hipEvent_t ev_pass0, hipEvent_t ev_pass1);
+                              void *d_args, std::vector<int> &rc, hipEvent_t ev_pass0, hipEvent_t ev_pass1,
+                              const RunStreams *rs = nullptr);
 size_t tile_args_bytes();
 void run_cell_counts(ecsprt_hip_ctx *c, int *d_counts);
 void run_apply_bc_aos(ecsprt_hip_ctx *c, long np, double *pos);

@@ -1347,8 +1347,30 @@ size_t tile_args_bytes() { return std::max(sizeof(TileArgs), mx_args_bytes()); }
 // so its ions and moments are the same bit for bit.  A run whose checks fail gets its
 // error code in rc and its message in last_error and takes no further part; a HIP error
 // is thrown.  Events ev_pass0/ev_pass1 (may be null) bracket the tile passes.
+RunFork::RunFork(const std::vector<ecsprt_hip_ctx *> &c, hipStream_t s, const RunStreams *r)
+    : cs(c), st(s), rs(r), out(c.size(), 0) {
+  if (rs) ECS_HIP(hipEventRecord(rs->fork, st));
+}
+void RunFork::leave(int r) {
+  if (!rs) return;
+  ECS_HIP(hipStreamWaitEvent(rs->st[r], rs->fork, 0));
+  cs[r]->stream = rs->st[r];
+  out[r] = 1;
+}
+void RunFork::back(int r) {
+  if (!rs || !out[r]) return;
+  ECS_HIP(hipEventRecord(rs->done[r], rs->st[r]));
+  cs[r]->stream = st;
+}
+void RunFork::join() {
+  if (!rs) return;
+  for (size_t r = 0; r < out.size(); ++r)
+    if (out[r]) ECS_HIP(hipStreamWaitEvent(st, rs->done[r], 0));
+}
+
 void run_fused_particles_runs(const std::vector<ecsprt_hip_ctx *> &cs, hipStream_t st, double dt, void *h_args,
-                              void *d_args, std::vector<int> &rc, hipEvent_t ev_pass0, hipEvent_t ev_pass1) {
+                              void *d_args, std::vector<int> &rc, hipEvent_t ev_pass0, hipEvent_t ev_pass1,
+                              const RunStreams *rs) {
   const int n = (int)cs.size();
   std::vector<char> coll(n, 0);
   std::vector<PoissonParams> pt(n);
@@ -1367,12 +1389,16 @@ void run_fused_particles_runs(const std::vector<ecsprt_hip_ctx *> &cs, hipStream
   }
   for (;;) {
     bool any = false;
+    for (int r = 0; r < n && !any; ++r) any = ps[r].active;
+    if (!any) break;
+    RunFork f(cs, st, rs);
     for (int r = 0; r < n; ++r)
       if (ps[r].active) {
+        f.leave(r);
         predict_enqueue(cs[r], ps[r]);
-        any = true;
+        f.back(r);
       }
-    if (!any) break;
+    f.join();
     ECS_HIP(hipStreamSynchronize(st));
     for (int r = 0; r < n; ++r)
       if (ps[r].active) predict_collect(cs[r], ps[r]);
@@ -1422,12 +1448,19 @@ void run_fused_particles_runs(const std::vector<ecsprt_hip_ctx *> &cs, hipStream
     }
   }
   if (ev_pass1) ECS_HIP(hipEventRecord(ev_pass1, st));
-  for (int r = 0; r < n; ++r) {
-    ecsprt_hip_ctx *c = cs[r];
-    finish_rebin(c);
-    launch_late(c);
-    launch_fold(c);
-    ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * CTR_N, hipMemcpyDeviceToHost, st));
+  {
+    RunFork f(cs, st, rs);
+    for (int r = 0; r < n; ++r) {
+      ecsprt_hip_ctx *c = cs[r];
+      f.leave(r);
+      finish_rebin(c);
+      launch_late(c);
+      launch_fold(c);
+      ECS_HIP(hipMemcpyAsync(c->h_ctr, c->d_ctr, sizeof(unsigned long long) * CTR_N, hipMemcpyDeviceToHost,
+                             c->stream));
+      f.back(r);
+    }
+    f.join();
   }
   ECS_HIP(hipStreamSynchronize(st));
   for (int r = 0; r < n; ++r) {
