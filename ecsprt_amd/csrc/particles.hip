@@ -1343,7 +1343,10 @@ size_t tile_args_bytes() { return std::max(sizeof(TileArgs), mx_args_bytes()); }
 // The rate predictions of all runs share each host synchronisation; the tile passes of
 // runs with the same precision, tile shape, block size and collision setting run as one
 // launch (at most RUNS_MAX runs each); the counters of all runs come back with one
-// synchronisation.  Every run's arithmetic and draws are those of run_fused_particles,
+// synchronisation.  With rs, each run's pilot passes and its re-binning, late deposits,
+// fold and counter copy go on the run's own stream (rs->st[r]) between a fork and a join
+// event on st, so that the runs' small launches overlap; the tile passes stay on st.
+// Every run's arithmetic and draws are those of run_fused_particles,
 // so its ions and moments are the same bit for bit.  A run whose checks fail gets its
 // error code in rc and its message in last_error and takes no further part; a HIP error
 // is thrown.  Events ev_pass0/ev_pass1 (may be null) bracket the tile passes.

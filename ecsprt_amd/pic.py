@@ -347,8 +347,10 @@ def ensemble_step(contexts, dt):
 
 class Ensemble:
     """Lockstep ensemble over contexts (ecsprt_hip_ensemble_*): each advance is one
-    timestep of every run, driven from one thread with one launch per stage for all runs;
-    every run's state equals that of stepping it alone."""
+    timestep of every run, driven from one thread: one tile-pass launch per group of
+    runs that share precision and tile shape, each run's small launches on its own
+    stream, one lockstep solve of all potentials; every run's ions and moments equal
+    those of stepping it alone."""
 
     def __init__(self, contexts):
         L = _lib.load()

@@ -329,10 +329,10 @@ int ecsprt_hip_ensemble_step(ecsprt_hip_ctx *const *ctxs, int n, double dt);
 
 /* Lockstep ensemble: the same timestep on each of n contexts (three-dimensional, Philox
  * collisions or none, equal solver options, zero initial guess, multigrid
- * preconditioning), driven from one host thread on one stream with one launch per stage
- * for all runs.  The fused particle passes of runs that share a precision, tile shape
- * and block size run as one kernel; the rate predictions and counter checks of all runs
- * share each host synchronisation; the potentials are solved by one lockstep
+ * preconditioning), driven from one host thread.  The fused particle passes of runs that
+ * share a precision, tile shape and block size run as one kernel; each run's rate
+ * prediction passes and re-binning go on that context's own stream, concurrently, and
+ * the rate predictions and counter checks of all runs share each host synchronisation; the potentials are solved by one lockstep
  * GMRES/multigrid over all runs (the lockstep batch solver).  From the same state, each
  * run's ions and moments equal those of ecsprt_hip_step bit for bit; its potential
  * solves the same system to the same tolerance (the same method with its partial sums
