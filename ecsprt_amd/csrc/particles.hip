@@ -105,19 +105,17 @@ struct TileWG {
   int t, tx, ty, tz, i0, j0, k0;
   int RA, RA3, RAx, RAy, RAz, H;
   long long *sacc;
-  int *s_cnt, *s_base, *s_nbr;  // counters and bases: two buffers of 27
-  int *s_nslot, *s_ncap;         // slot offset and capacity of each neighbour tile
-  int par;                       // buffer of the current batch
+  int *s_nbr;             // the 27 neighbour tiles
+  int *s_nslot, *s_ncap;  // slot offset and capacity of each neighbour tile
 };
 
-// Drift + wrap, deposit, and re-binning of one particle per thread.  Every thread
-// of the workgroup calls this in uniform control flow (re-binning uses barriers).
+// Drift + wrap, deposit, and re-binning of one particle per thread.  Every lane of a
+// wave calls this in uniform control flow (re-binning uses wave-wide ballots).
 template <int MODE, int TX, int TY, int TZ, int HH>
 __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, long p, double px, double py,
                                      double pz, double vx, double vy, double vz, int pid) {
   const TileGeom &g = a.g;
   const double dx = a.pc.dx, dy = a.pc.dy, dz = a.pc.dz;
-  const int tid = threadIdx.x;
   bool lost = false, late = false;
   if ((MODE & M_DRIFT) && valid) {
     // UpdatePositions drift (src/positions.c:455-459) and ApplyBC (src/boundaries.c:19-28).
@@ -193,22 +191,10 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
       const int oz = tile_step(ck / TZ - w.tz, g.ntz);
       code = (ox < 2 && oy < 2 && oz < 2) ? ((oz + 1) * 3 + (oy + 1)) * 3 + (ox + 1) : 27;
     }
-    // Counters and bases are double-buffered across batches: the buffer used by the
-    // previous batch is cleared here, after the barrier every thread passed once it
-    // finished with it, so two barriers per batch suffice.
-    int *cnt = w.s_cnt + 27 * w.par, *bas = w.s_base + 27 * w.par;
-    const int rank = rebin_rank(cnt, code);
-    __syncthreads();
-    if (tid < 27) {
-      const int n = cnt[tid];
-      if (n > 0) bas[tid] = atomicAdd(&a.next[tile_ctr(w.s_nbr[tid])], n);
-    } else if (tid >= 64 && tid < 64 + 27) {
-      w.s_cnt[27 * (w.par ^ 1) + tid - 64] = 0;
-    }
-    __syncthreads();
+    const int slot = wave_reserve(code, w.s_nbr, a.next);
     if (code < 27) {
       const int nb = w.s_nslot[code], nc = w.s_ncap[code];
-      place(a, nb, nc, bas[code] + rank, px, py, pz, vx, vy, vz, pid);
+      place(a, nb, nc, slot, px, py, pz, vx, vy, vz, pid);
       if (late) late_append(a.late, a.late_cnt, a.ovf, g, a.pc, w.s_nbr[code], nb, nc, px, py, pz, vx, vy, vz);
     } else if (code == 27) {
       // further than one tile: k_rebin places it, and lists its deposit when owed
@@ -216,7 +202,6 @@ __device__ inline void particle_tail(const TileArgs &a, TileWG &w, bool valid, l
       store_particle(a.emi, (long)e, px, py, pz, vx, vy, vz, late ? (pid | EMI_DEPOSIT) : pid);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
-    w.par ^= 1;
   } else if ((MODE & M_PUSH) && valid) {
     a.src.vx(p) = vx;
     a.src.vy(p) = vy;
@@ -234,7 +219,7 @@ __device__ __attribute__((always_inline)) inline void tile_f64(ConstPtr<TileArgs
   extern __shared__ double smem[];
   // Re-binning bookkeeping: particles per destination among the 27 neighbour tiles,
   // the slot range reserved in each; number of ions deferred for collisions.
-  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
+  __shared__ int s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
   const TileGeom g = a.g;
   constexpr int NEX = TX + 1, NEY = TY + 1, NE3 = NEX * NEY * (TZ + 1);
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH;
@@ -253,9 +238,6 @@ __device__ __attribute__((always_inline)) inline void tile_f64(ConstPtr<TileArgs
   w.RAx = min(TX, g.Nx - w.i0) + 1 + 2 * HH;
   w.RAy = min(TY, g.Ny - w.j0) + 1 + 2 * HH;
   w.RAz = min(TZ, g.Nz - w.k0) + 1 + 2 * HH;
-  w.s_cnt = s_cnt;
-  w.s_base = s_base;
-  w.par = 0;
   w.s_nbr = s_nbr;
   w.s_nslot = s_nslot;
   w.s_ncap = s_ncap;
@@ -279,7 +261,6 @@ __device__ __attribute__((always_inline)) inline void tile_f64(ConstPtr<TileArgs
   if (MODE & M_DEPOSIT) {
     for (int q = tid; q < 4 * w.RA3; q += blockDim.x) w.sacc[q] = 0;
   }
-  if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
   if ((MODE & M_REBIN) && tid < 27) {
     const int ox = tid % 3 - 1, oy = (tid / 3) % 3 - 1, oz = tid / 9 - 1;
     const int nb = (wrap_index(w.tz + oz, g.ntz) * g.nty + wrap_index(w.ty + oy, g.nty)) * g.ntx +

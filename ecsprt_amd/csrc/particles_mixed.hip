@@ -176,18 +176,16 @@ struct TileWGMx {
   int t, tx, ty, tz, i0, j0, k0;
   int RAx, RAy, RAz;
   long long *sacc;
-  int *s_cnt, *s_base, *s_nbr;
+  int *s_nbr;             // the 27 neighbour tiles
   int *s_nslot, *s_ncap;  // slot offset and capacity of each neighbour tile
-  int par;
 };
 
 // Drift + wrap, deposit and re-binning of one ion per thread, in uniform control
-// flow across the workgroup (re-binning uses barriers).
+// flow across each wave (re-binning uses wave-wide ballots).
 template <int MODE, int TX, int TY, int TZ, int HH>
 __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, long p, int ix, int iy, int iz, float ox,
                                float oy, float oz, float vx, float vy, float vz, int pid) {
   const TileGeom &g = a.g;
-  const int tid = threadIdx.x;
   bool lost = false, late = false;
   if ((MODE & M_DRIFT) && valid) {
     // UpdatePositions (src/positions.c:455-459) with the wrap of src/boundaries.c:19-28.
@@ -232,19 +230,10 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
       const int sz = tile_step(iz / TZ - w.tz, g.ntz);
       code = (sx < 2 && sy < 2 && sz < 2) ? ((sz + 1) * 3 + (sy + 1)) * 3 + (sx + 1) : 27;
     }
-    int *cnt = w.s_cnt + 27 * w.par, *bas = w.s_base + 27 * w.par;
-    const int rank = rebin_rank(cnt, code);
-    __syncthreads();
-    if (tid < 27) {
-      const int n = cnt[tid];
-      if (n > 0) bas[tid] = atomicAdd(&a.next[tile_ctr(w.s_nbr[tid])], n);
-    } else if (tid >= 64 && tid < 64 + 27) {
-      w.s_cnt[27 * (w.par ^ 1) + tid - 64] = 0;
-    }
-    __syncthreads();
+    const int slot = wave_reserve(code, w.s_nbr, a.next);
     if (code < 27) {
       const int dt = w.s_nbr[code], nb = w.s_nslot[code], nc = w.s_ncap[code];
-      place_mx(a, nb, nc, bas[code] + rank, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
+      place_mx(a, nb, nc, slot, mx_pack(ix, iy, iz), ox, oy, oz, vx, vy, vz, pid);
       if (late) late_append_mx(a.late, a.late_cnt, a.ovf, g, dt, nb, nc, ix, iy, iz, ox, oy, oz, vx, vy, vz);
     } else if (code == 27) {
       // further than one tile: k_rebin_mx places it, and lists its deposit when owed
@@ -253,7 +242,6 @@ __device__ inline void tail_mx(const TileArgsMx &a, TileWGMx &w, bool valid, lon
                late ? (int)((uint32_t)pid | EMI_DEPOSIT_MX) : pid);
     }
     if (valid && lost) atomicAdd(&a.ctr[2], 1ull);
-    w.par ^= 1;
   } else if ((MODE & M_PUSH) && valid) {
     uint32_t *b = mx_at(a.src, p);
     b[64 * MX_VX] = __float_as_uint(vx);
@@ -270,7 +258,7 @@ __device__ __attribute__((always_inline)) inline void tile_mx(ConstPtr<TileArgsM
   static_assert(!(MODE & M_COLLIDE) || (MODE & M_REBIN), "collisions run only in the re-binning pass");
   const TileArgsMx &a = fresh(ap);
   extern __shared__ long long smemq[];
-  __shared__ int s_cnt[2 * 27], s_base[2 * 27], s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
+  __shared__ int s_nbr[27], s_nslot[27], s_ncap[27], s_ndef, s_n3;
   const TileGeom g = a.g;
   constexpr int NEX = TX + 1, NEY = TY + 1, NE3 = NEX * NEY * (TZ + 1);
   constexpr int RAX = TX + 1 + 2 * HH, RAY = TY + 1 + 2 * HH, RAZ = TZ + 1 + 2 * HH, RA3 = RAX * RAY * RAZ;
@@ -286,12 +274,9 @@ __device__ __attribute__((always_inline)) inline void tile_mx(ConstPtr<TileArgsM
   w.RAx = min(TX, g.Nx - w.i0) + 1 + 2 * HH;
   w.RAy = min(TY, g.Ny - w.j0) + 1 + 2 * HH;
   w.RAz = min(TZ, g.Nz - w.k0) + 1 + 2 * HH;
-  w.s_cnt = s_cnt;
-  w.s_base = s_base;
   w.s_nbr = s_nbr;
   w.s_nslot = s_nslot;
   w.s_ncap = s_ncap;
-  w.par = 0;
   // LDS: the accumulators first (8-byte aligned), then the gather's difference
   // numerators at the tile's (T+1)^3 corner nodes (FP64 differences rounded to FP32).
   w.sacc = smemq;
@@ -312,7 +297,6 @@ __device__ __attribute__((always_inline)) inline void tile_mx(ConstPtr<TileArgsM
   if (MODE & M_DEPOSIT) {
     for (int q = tid; q < 4 * RA3; q += blockDim.x) w.sacc[q] = 0;
   }
-  if ((MODE & M_REBIN) && tid < 2 * 27) s_cnt[tid] = 0;
   if ((MODE & M_REBIN) && tid < 27) {
     const int sx = tid % 3 - 1, sy = (tid / 3) % 3 - 1, sz = tid / 9 - 1;
     const int nb = (wrap_index(w.tz + sz, g.ntz) * g.nty + wrap_index(w.ty + sy, g.nty)) * g.ntx +

@@ -117,23 +117,27 @@ __device__ inline void lds_addfx(long long *p, double scaled) {
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Rank of this ion among the workgroup's ions bound for destination code (0..26; codes
-// 27 and 28 take none), from the LDS counters cnt.  Most ions stay in their tile (code
-// 13): a wave reserves their ranks with one atomic and numbers them by lane, where 64
-// atomics on one LDS address would be serialised.  Called by every lane of the wave.
-__device__ inline int rebin_rank(int *cnt, int code) {
-  const unsigned long long stay = __ballot(code == 13);
-  int rank = 0;
-  if (code < 27 && code != 13) rank = atomicAdd(&cnt[code], 1);
-  if (stay) {
-    const int below =
-        (int)__builtin_amdgcn_mbcnt_hi((unsigned)(stay >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)stay, 0u));
-    int b = 0;
-    if (code == 13 && below == 0) b = atomicAdd(&cnt[13], (int)__popcll(stay));
-    b = __builtin_amdgcn_readlane(b, __ffsll((long long)stay) - 1);
-    if (code == 13) rank = b + below;
+// Slot of this ion in the list of its destination tile, code 0..26 among the 27
+// neighbours nbr (codes 27 and 28 take none), reserved by its wave alone: the lanes
+// bound for one destination take consecutive slots after one device atomic on that
+// tile's counter, made by the lowest of them, so no workgroup barrier is needed.  One
+// pass per distinct destination in the wave (about ten).  Called by every lane of the
+// wave.
+__device__ inline int wave_reserve(int code, const int *nbr, int *next) {
+  unsigned long long todo = __ballot(code < 27);
+  unsigned long long mine = 0;
+  while (todo) {
+    const int c = __builtin_amdgcn_readlane(code, (int)__builtin_ctzll(todo));
+    const unsigned long long m = __ballot(code == c);
+    if (code == c) mine = m;
+    todo &= ~m;
   }
-  return rank;
+  const int below =
+      (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+  int base = 0;
+  if (code < 27 && below == 0) base = atomicAdd(&next[tile_ctr(nbr[code])], (int)__popcll(mine));
+  base = __shfl(base, mine ? (int)__builtin_ctzll(mine) : 0);
+  return base + below;
 }
 
 // Periodic index for i within one period of [0, n) (no integer division).
