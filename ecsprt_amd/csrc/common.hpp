@@ -324,4 +324,8 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x);
 size_t solver_arena_bytes(const ecsprt_options &o, long n);
 int solver_standalone(ecsprt_hip_ctx *c);
 
+// Workgroups are dispatched round-robin over the 8 XCDs; with a grid of a multiple of 8
+// each XCD takes a contiguous run of workgroup indices, so neighbouring tiles share an L2.
+__device__ inline int xcd_order(int b, int nwg) { return (nwg & 7) == 0 ? (b & 7) * (nwg >> 3) + (b >> 3) : b; }
+
 }  // namespace ecsprt

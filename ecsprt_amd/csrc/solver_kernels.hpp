@@ -214,7 +214,7 @@ __device__ inline void sweep_box(const StencilConsts &s, const T *__restrict__ d
 template <typename T, int TX, int TY, int TZ>
 __global__ void __launch_bounds__(512) k_sweep(StencilConsts s, const T *__restrict__ dn, const T *__restrict__ xin,
                                               const T *__restrict__ b, T *__restrict__ xout, int first) {
-  sweep_box<T, TX, TY, TZ>(s, dn, xin, b, xout, first, blockIdx.x);
+  sweep_box<T, TX, TY, TZ>(s, dn, xin, b, xout, first, xcd_order(blockIdx.x, gridDim.x));
 }
 
 // Gauss-Seidel value of a node from three planes of density (d0 below, d1 at, d2 above
@@ -394,7 +394,7 @@ template <typename T, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) k_sweep_z(StencilConsts s, const T *__restrict__ dn,
                                                     const T *__restrict__ xin, const T *__restrict__ b,
                                                     T *__restrict__ xout, int first, int KZ) {
-  sweep_z<T, TX, TY>(s, dn, xin, b, xout, first, KZ, blockIdx.x);
+  sweep_z<T, TX, TY>(s, dn, xin, b, xout, first, KZ, xcd_order(blockIdx.x, gridDim.x));
 }
 
 constexpr int SW_TX = 16, SW_TY = 8, SW_TZ = 4;  // box of the one-shot sweep

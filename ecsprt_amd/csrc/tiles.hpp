@@ -53,9 +53,6 @@ __device__ inline int run_of(ConstPtr<RunTable> t, int gi) {
   return lo;
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs; with a grid of a multiple of 8
-// each XCD takes a contiguous run of workgroup indices, so neighbouring tiles share an L2.
-__device__ inline int xcd_order(int b, int nwg) { return (nwg & 7) == 0 ? (b & 7) * (nwg >> 3) + (b >> 3) : b; }
 
 __device__ inline int tile_of(const TileGeom &g, double xg, double yg, double zg) {
   int ci = (int)xg, cj = (int)yg, ck = (int)zg;
