@@ -89,7 +89,11 @@ struct Gmres {
   std::vector<T *> V, Z;
   T **dV = nullptr, **dZ = nullptr;
   T *w = nullptr, *r = nullptr;
-  double *partial = nullptr, *dh = nullptr, *hpin = nullptr, *dy = nullptr, *ypin = nullptr;
+  double *partial = nullptr, *dh = nullptr, *dy = nullptr, *ypin = nullptr;
+  // Pinned: two slots of one Hessenberg column (2 (m+2) doubles each, alternating
+  // between Arnoldi steps), then one double for a norm read back late.
+  double *hpin = nullptr, *bpin = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};  // each column slot's copy done
 };
 
 struct Solver {
@@ -383,8 +387,11 @@ static void gmres_create(ecsprt_hip_ctx *c, Gmres<T> &g, int m, long n) {
   sv_alloc(c, &g.partial, sizeof(double) * RED_BLOCKS * (m + 2));
   sv_alloc(c, &g.dh, sizeof(double) * 2 * (m + 2));
   sv_alloc(c, &g.dy, sizeof(double) * (m + 1));
-  sv_host_alloc(c, &g.hpin, sizeof(double) * 2 * (m + 2));
+  sv_host_alloc(c, &g.hpin, sizeof(double) * (4 * (m + 2) + 1));
+  g.bpin = g.hpin + 4 * (m + 2);
   sv_host_alloc(c, &g.ypin, sizeof(double) * (m + 1));
+  if (m > 0)
+    for (hipEvent_t &e : g.ev) ECS_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 template <typename T>
@@ -400,6 +407,8 @@ static void gmres_free(ecsprt_hip_ctx *c, Gmres<T> &g) {
   sv_free(c, g.dy);
   sv_host_free(c, g.hpin);
   sv_host_free(c, g.ypin);
+  for (hipEvent_t e : g.ev)
+    if (e) (void)hipEventDestroy(e);
   g = Gmres<T>{};
 }
 
@@ -469,92 +478,150 @@ static double fetch1(ecsprt_hip_ctx *c, const double *d, double *pin) {
   return pin[0];
 }
 
-// Right-preconditioned GMRES(m) with classical Gram-Schmidt and one
-// re-orthogonalisation; convergence on the true residual ||b - A x|| <= tol.
+// Right-preconditioned GMRES(m) with classical Gram-Schmidt (optionally one
+// re-orthogonalisation); convergence on the true residual ||b - A x|| <= tol.
 // Returns iterations; rnorm receives the final true residual norm.  single_cycle: one
 // restart cycle stopped by the Arnoldi residual estimate (the inner solve of the
 // mixed-precision refinement, whose FP32 true residual has a rounding floor).
+// x_zero: x is zero on entry, so the first residual is b itself; beta0 >= 0 is then
+// ||b||, with b.b already in g.dh[0], else ||b|| is read back with the first column.
+//
+// The Arnoldi steps are pipelined one deep: step j+1 is queued before the host waits
+// for step j's Hessenberg column whenever the residual estimates so far say that step
+// j cannot reach tol (the last estimate times the fastest reduction seen in this
+// cycle, or 0.1 before one is seen, still 10 times above tol).  A step queued this way
+// after all is never used: the solution only takes the columns the host processed, so
+// the iterates do not depend on the guess.
 template <typename T>
 static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, double tol, int maxit,
-                 double &rnorm, bool *converged, bool single_cycle = false) {
+                 double &rnorm, bool *converged, bool single_cycle = false, bool x_zero = false,
+                 double beta0 = -1.0) {
   hipStream_t st = c->stream;
   Level<T> &L0 = mg.L[0];
   const long n = L0.n;
   const int m = g.m;
   const bool refine = c->o.gmres_refine != 0;
   const Launch3 g0 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
+  const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
+  const size_t col = 2 * (size_t)(m + 2);  // doubles of one column slot
+  constexpr double SPEC_MARGIN = 10.0, SPEC_RATIO0 = 0.1;
   std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m), sn(m), gv(m + 1);
+  // Arnoldi step q: Z[q] = M^-1 V[q], w = A Z[q], w orthogonalised against V[0..q]
+  // with its column in g.dh, the column copied to slot q & 1 (event ev[q & 1]), then
+  // V[q+1] = w / |w| on the device (zero when |w| = 0, and then never used).
+  auto enqueue = [&](int q) {
+    precond<T>(c, mg, g.V[q], g.Z[q]);
+    k_apply<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, g.Z[q], g.w);
+    ECS_HIP(hipGetLastError());
+    tally(c, 3.0 * sizeof(T) * n);
+    // Classical Gram-Schmidt, h = V^T w; w -= V h; |w|^2 (PETSc's default), or with
+    // gmres_refine CGS2: h2 = V^T w; w -= V h2 before the norm.  Up to 8 basis vectors
+    // the subtractions carry the following products in the same pass.
+    dots<T>(c, g, q + 1, g.dV, g.w, g.dh, 0);
+    if (!refine) {
+      if (q + 1 <= 8) {
+        k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(q + 1, g.dV, g.dh, g.w, n, g.partial, 1);
+        k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (q + 1), 0);
+        tally(c, (double)sizeof(T) * (q + 3) * n);
+      } else {
+        k_sub_multi<T><<<gsz(n), 256, 0, st>>>(q + 1, g.dV, g.dh, g.w, n);
+        tally(c, (double)sizeof(T) * (q + 3) * n);
+        dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (q + 1));
+      }
+    } else if (q + 1 <= 8) {
+      k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(q + 1, g.dV, g.dh, g.w, n, g.partial, 0);
+      k_reduce_partials<<<1, 256, 0, st>>>(q + 1, nb, g.partial, g.dh + (m + 2), 0);
+      k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(q + 1, g.dV, g.dh + (m + 2), g.w, n, g.partial, 1);
+      k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (q + 1), 0);
+      tally(c, 2.0 * sizeof(T) * (q + 3) * n);
+    } else {
+      k_sub_multi<T><<<gsz(n), 256, 0, st>>>(q + 1, g.dV, g.dh, g.w, n);
+      dots<T>(c, g, q + 1, g.dV, g.w, g.dh + (m + 2), 0);
+      k_sub_multi<T><<<gsz(n), 256, 0, st>>>(q + 1, g.dV, g.dh + (m + 2), g.w, n);
+      tally(c, 2.0 * sizeof(T) * (q + 3) * n);
+      dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (q + 1));
+    }
+    ECS_HIP(hipGetLastError());
+    ECS_HIP(hipMemcpyAsync(g.hpin + (q & 1) * col, g.dh, sizeof(double) * col, hipMemcpyDeviceToHost, st));
+    ECS_HIP(hipEventRecord(g.ev[q & 1], st));
+    k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.w, g.V[q + 1], g.dh + (q + 1), n, 1);
+    ECS_HIP(hipGetLastError());
+    tally(c, 2.0 * sizeof(T) * n);
+  };
   int it = 0;
   *converged = false;
   double beta = 0.0;
+  bool first = true;
   while (true) {
-    k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
-    ECS_HIP(hipGetLastError());
-    tally(c, 4.0 * sizeof(T) * n);
-    dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
-    beta = std::sqrt(fetch1(c, g.dh, g.hpin));
-    rnorm = beta;
-    if (beta <= tol) {
-      *converged = true;
-      break;
+    const T *r0 = g.r;
+    bool beta_late = false;
+    if (first && x_zero) {
+      r0 = b;
+      if (beta0 >= 0) {
+        beta = beta0;
+      } else {
+        dot_single<T>(c, g.partial, b, b, n, g.dh);
+        ECS_HIP(hipMemcpyAsync(g.bpin, g.dh, sizeof(double), hipMemcpyDeviceToHost, st));
+        beta_late = true;
+      }
+    } else {
+      k_residual<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, x, b, g.r);
+      ECS_HIP(hipGetLastError());
+      tally(c, 4.0 * sizeof(T) * n);
+      dot_single<T>(c, g.partial, g.r, g.r, n, g.dh);
+      beta = std::sqrt(fetch1(c, g.dh, g.hpin));
     }
-    if (!std::isfinite(beta)) break;
+    first = false;
+    if (!beta_late) {
+      rnorm = beta;
+      if (beta <= tol) {
+        *converged = true;
+        break;
+      }
+      if (!std::isfinite(beta)) break;
+    }
     if (it >= maxit) break;
-    k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.r, g.V[0], g.dh, n, 1);
+    k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(r0, g.V[0], g.dh, n, 1);
     tally(c, 2.0 * sizeof(T) * n);
     std::fill(gv.begin(), gv.end(), 0.0);
     gv[0] = beta;
-    int j = 0;
-    bool done = false;
+    int j = 0, queued = 1;
+    bool done = false, stop = false;
+    double ratio = SPEC_RATIO0;  // fastest reduction of the residual estimate per step
+    enqueue(0);
     for (; j < m && it < maxit; ++j, ++it) {
-      precond<T>(c, mg, g.V[j], g.Z[j]);
-      k_apply<T><<<g0.grid, g0.block, 0, st>>>(L0.s, L0.dn, g.Z[j], g.w);
-      ECS_HIP(hipGetLastError());
-      tally(c, 3.0 * sizeof(T) * n);
-      // Classical Gram-Schmidt, h = V^T w; w -= V h; |w|^2 (PETSc's default), or with
-      // gmres_refine CGS2: h2 = V^T w; w -= V h2 before the norm.  Up to 8 basis vectors
-      // the subtractions carry the following products in the same pass.
-      dots<T>(c, g, j + 1, g.dV, g.w, g.dh, 0);
-      if (!refine) {
-        if (j + 1 <= 8) {
-          const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
-          k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh, g.w, n, g.partial, 1);
-          k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (j + 1), 0);
-          tally(c, (double)sizeof(T) * (j + 3) * n);
-        } else {
-          k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
-          tally(c, (double)sizeof(T) * (j + 3) * n);
-          dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
-        }
-      } else if (j + 1 <= 8) {
-        const int nb = (int)std::min<long>(RED_BLOCKS, (n + RED_THREADS - 1) / RED_THREADS);
-        k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh, g.w, n, g.partial, 0);
-        k_reduce_partials<<<1, 256, 0, st>>>(j + 1, nb, g.partial, g.dh + (m + 2), 0);
-        k_sub_dot_multi<T><<<nb, RED_THREADS, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n, g.partial, 1);
-        k_reduce_partials<<<1, 256, 0, st>>>(1, nb, g.partial, g.dh + (j + 1), 0);
-        tally(c, 2.0 * sizeof(T) * (j + 3) * n);
-      } else {
-        k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh, g.w, n);
-        dots<T>(c, g, j + 1, g.dV, g.w, g.dh + (m + 2), 0);
-        k_sub_multi<T><<<gsz(n), 256, 0, st>>>(j + 1, g.dV, g.dh + (m + 2), g.w, n);
-        tally(c, 2.0 * sizeof(T) * (j + 3) * n);
-        dot_single<T>(c, g.partial, g.w, g.w, n, g.dh + (j + 1));
+      const bool more = j + 1 < m && it + 1 < maxit;
+      // the residual estimate before step j (unknown, so far above tol, while the
+      // norm of b is still on its way)
+      const double est = beta_late ? HUGE_VAL : std::fabs(gv[j]);
+      if (more && queued == j + 1 && est * ratio > SPEC_MARGIN * tol) {
+        enqueue(j + 1);
+        ++queued;
       }
-      ECS_HIP(hipGetLastError());
-      ECS_HIP(hipMemcpyAsync(g.hpin, g.dh, sizeof(double) * 2 * (m + 2), hipMemcpyDeviceToHost, st));
-      ECS_HIP(hipStreamSynchronize(st));
-      for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = g.hpin[q] + (refine ? g.hpin[(m + 2) + q] : 0.0);
-      const double hn = std::sqrt(std::max(g.hpin[j + 1], 0.0));
+      ECS_HIP(hipEventSynchronize(g.ev[j & 1]));
+      if (beta_late) {  // step 0: the norm of b arrived before its column
+        beta_late = false;
+        beta = std::sqrt(g.bpin[0]);
+        rnorm = beta;
+        gv[0] = beta;
+        if (beta <= tol) {
+          *converged = true;
+          stop = true;
+          break;
+        }
+        if (!std::isfinite(beta)) {
+          stop = true;
+          break;
+        }
+      }
+      const double *hp = g.hpin + (j & 1) * col;
+      for (int q = 0; q <= j; ++q) H[(size_t)q * m + j] = hp[q] + (refine ? hp[(m + 2) + q] : 0.0);
+      const double hn = std::sqrt(std::max(hp[j + 1], 0.0));
       if (!std::isfinite(hn)) {
         it = maxit;
         break;
       }
       H[(size_t)(j + 1) * m + j] = hn;
-      // w/hn -> V[j+1]
-      if (hn > 0) {
-        k_scale_by_inv<T><<<gsz(n), 256, 0, st>>>(g.w, g.V[j + 1], g.dh + (j + 1), n, 1);
-        tally(c, 2.0 * sizeof(T) * n);
-      }
       for (int q = 0; q < j; ++q) {
         const double a = H[(size_t)q * m + j], bb = H[(size_t)(q + 1) * m + j];
         H[(size_t)q * m + j] = cs[q] * a + sn[q] * bb;
@@ -574,7 +641,13 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
         done = true;
         break;
       }
+      ratio = std::min(ratio, std::fabs(sn[j]));  // the estimate fell by |sn[j]| in step j
+      if (more && queued == j + 1) {
+        enqueue(j + 1);
+        ++queued;
+      }
     }
+    if (stop) break;
     // Back substitution for y (j x j) and x += Z y.
     std::vector<double> y(j, 0.0);
     for (int q = j - 1; q >= 0; --q) {
@@ -583,13 +656,13 @@ static int gmres(ecsprt_hip_ctx *c, MG<T> &mg, Gmres<T> &g, const T *b, T *x, do
       y[q] = H[(size_t)q * m + q] != 0 ? s / H[(size_t)q * m + q] : 0.0;
     }
     for (int q = 0; q < j; ++q) g.ypin[q] = y[q];
-    ECS_HIP(hipMemcpyAsync(g.dy, g.ypin, sizeof(double) * std::max(j, 1), hipMemcpyHostToDevice, st));
     if (j > 0) {
+      // ypin is next written after a later synchronisation of this stream
+      ECS_HIP(hipMemcpyAsync(g.dy, g.ypin, sizeof(double) * j, hipMemcpyHostToDevice, st));
       k_add_multi<T><<<gsz(n), 256, 0, st>>>(j, g.dZ, g.dy, x, n, 1.0);
       tally(c, (double)sizeof(T) * (j + 2) * n);
     }
     ECS_HIP(hipGetLastError());
-    ECS_HIP(hipStreamSynchronize(st));
     if (single_cycle) {
       rnorm = std::fabs(gv[j]);
       *converged = done;
@@ -729,7 +802,8 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
   const double rtol = c->o.ksp_rtol > 0 ? c->o.ksp_rtol : 1e-5;
   const double atol = c->o.ksp_atol > 0 ? c->o.ksp_atol : 1e-50;
   const int maxit = c->o.ksp_max_it > 0 ? c->o.ksp_max_it : 10000;
-  if (c->o.initial_guess == 0) {
+  const bool x_zero = c->o.initial_guess == 0;
+  if (x_zero) {
     k_fill<double><<<gsz(n), 256, 0, c->stream>>>(x, n, 0.0);
     ECS_HIP(hipGetLastError());
     tally(c, 8.0 * n);
@@ -754,24 +828,30 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
     }
     Level<double> &L0 = s->mg64.L[0];
     for (int outer = 0; outer < 100; ++outer) {
-      const Launch3 g3 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
-      k_residual<double><<<g3.grid, g3.block, 0, c->stream>>>(L0.s, L0.dn, x, b, s->r64);
-      ECS_HIP(hipGetLastError());
-      tally(c, 32.0 * n);
-      dot_single<double>(c, s->g64.partial, s->r64, s->r64, n, s->g64.dh);
-      rnorm = std::sqrt(fetch1(c, s->g64.dh, s->g64.hpin));
+      const double *r = s->r64;
+      if (outer == 0 && x_zero) {
+        r = b;  // x is zero: the residual is b itself
+        rnorm = bnorm;
+      } else {
+        const Launch3 g3 = launch3(L0.s.Nx, L0.s.Ny, L0.s.Nz);
+        k_residual<double><<<g3.grid, g3.block, 0, c->stream>>>(L0.s, L0.dn, x, b, s->r64);
+        ECS_HIP(hipGetLastError());
+        tally(c, 32.0 * n);
+        dot_single<double>(c, s->g64.partial, s->r64, s->r64, n, s->g64.dh);
+        rnorm = std::sqrt(fetch1(c, s->g64.dh, s->g64.hpin));
+      }
       if (rnorm <= tol) {
         conv = true;
         break;
       }
       if (!std::isfinite(rnorm)) break;
       if (iters >= maxit) break;
-      k_convert<double, float><<<gsz(n), 256, 0, c->stream>>>(s->r64, s->b32, n, 1.0 / rnorm);
+      k_convert<double, float><<<gsz(n), 256, 0, c->stream>>>(r, s->b32, n, 1.0 / rnorm);
       k_fill<float><<<gsz(n), 256, 0, c->stream>>>(s->x32, n, 0.0f);
       tally(c, 16.0 * n);
       double rn32 = 0.0;
       bool c32 = false;
-      iters += gmres<float>(c, s->mg32, s->g32, s->b32, s->x32, inner, maxit - iters, rn32, &c32, true);
+      iters += gmres<float>(c, s->mg32, s->g32, s->b32, s->x32, inner, maxit - iters, rn32, &c32, true, true);
       // x += rnorm * e, accumulated in FP64.
       k_convert<float, double><<<gsz(n), 256, 0, c->stream>>>(s->x32, s->r64, n, rnorm);
       k_axpy_from<double><<<gsz(n), 256, 0, c->stream>>>(s->r64, x, n, 1.0);
@@ -779,7 +859,7 @@ int solver_solve(ecsprt_hip_ctx *c, const double *b, double *x) {
       ECS_HIP(hipGetLastError());
     }
   } else {
-    iters = gmres<double>(c, s->mg64, s->g64, b, x, tol, maxit, rnorm, &conv);
+    iters = gmres<double>(c, s->mg64, s->g64, b, x, tol, maxit, rnorm, &conv, false, x_zero, bnorm);
   }
   solver_remove_mean(c, x);
   ECS_HIP(hipStreamSynchronize(c->stream));
