@@ -99,6 +99,44 @@ def test_uniform_density_fft_solution():
     ctx.close()
 
 
+@pytest.mark.parametrize("prec,rtol,tol", [(_lib.SOLVE_FP64, 1e-10, 1e-9), (_lib.SOLVE_MIXED, 1e-6, 1e-6)])
+def test_ksp_first_step_solution_drops_the_queued_step(prec, rtol, tol):
+    """A Fourier mode is an eigenvector of the uniform-density operator, so
+    unpreconditioned GMRES meets its tolerance in its first Arnoldi step; the solver has
+    by then queued the second step on the residual estimates, and that step must not
+    reach the solution, which is b / lambda.  In the mixed solve this is the first FP32
+    cycle (its tolerance 1e-3; the FP32 operator's rounding on this smooth mode leaves
+    about 3e-5, which a second refinement cycle removes)."""
+    say(f"first-step solve, precision {prec}")
+    N = 16
+    p, po, m = moments_case(N, 7)
+    m[..., 0] = 64.0
+    m[..., 1:] = 0.0
+    ctx = pic.Context(p, pic.default_options(operator_mode=_lib.OPERATOR_REBUILD, pc_type=_lib.PC_NONE,
+                                             ksp_rtol=rtol, solve_precision=prec))
+    ctx.set_moments(m)
+    ctx.lhs()
+    i = np.arange(N)
+    b = np.broadcast_to(np.cos(2 * np.pi * i / N)[None, None, :], (N, N, N)).copy()
+    lam = po.scale * 64.0 * (2 * np.cos(2 * np.pi / N) - 2)
+    # the mode is an eigenvector of the oracle's operator
+    assert rel(oracle.lhs_apply(po, m, b), lam * b) < 1e-12
+    n = b.size
+    bd, xd = pic.DeviceVector(n), pic.DeviceVector(n)
+    bd.upload(b.reshape(-1))
+    ctx.ksp_solve(bd, xd)
+    x = xd.download().reshape(b.shape)
+    it, rn, bn = ctx.solver_stats()
+    say(f"  iters={it} ||r||/||b||={rn / bn:.2e}")
+    if prec == _lib.SOLVE_FP64:
+        assert it == 1
+    assert rn <= rtol * bn
+    assert rel(x, b / lam) < tol
+    bd.free()
+    xd.free()
+    ctx.close()
+
+
 def test_frozen_operator_semantics():
     """KSPSetComputeOperators: the operator is built at the first solve only."""
     say("frozen operator")
