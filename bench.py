@@ -544,8 +544,9 @@ def leg_c3(args, pic, _lib, cores):
 # ----------------------------------------------------------------------------------- c4
 def leg_c4(args, pic, _lib, cores):
     """16 independent PIC runs of uneven size, mixed precision (particles and solve, rtol
-    1e-10 as in the 128^3 workload), stepped as one lockstep ensemble (one host thread and
-    stream, one launch per stage for all runs).  Also timed: the runs stepped concurrently
+    1e-10 as in the 128^3 workload), stepped as one lockstep ensemble (one host thread: the
+    tile passes and the solve phases one launch for all runs, each run's small launches on
+    its own stream).  Also timed: the runs stepped concurrently
     on one stream and host thread each, and one by one.  CPU baseline: per run, one oracle
     step on 1/16 of its ions plus the full RHS + solve on the run's own final moments."""
     import oracle
@@ -636,7 +637,8 @@ def leg_c4(args, pic, _lib, cores):
         "value": total * nsteps / el, "unit": "particle-steps/s", "steps": nsteps,
         "ms_per_step": 1e3 * el / nsteps, "dtype": "f32 particles, f64 fold, mixed solve",
         "config": {"workload": "ensemble_16_runs", "runs": runs, "particles_total": total,
-                   "stepping": "lockstep ensemble: one host thread and stream, one launch per stage for all runs"},
+                   "stepping": "lockstep ensemble: one host thread; tile passes and solve phases one launch "
+                               "for all runs, each run's small launches on its own stream"},
         "stage_ms_mean": {"tile_passes": stages[0], "pre": stages[1], "rebin_fold": stages[2],
                           "rhs_and_solve": stages[3]},
         "threads_ms_per_step": 1e3 * el_thr, "sequential_ms_per_step": 1e3 * el_seq,

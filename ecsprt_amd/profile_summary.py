@@ -61,7 +61,7 @@ def grid_lines(trace, prefixes):
     groups = collections.OrderedDict()
     for r in rows:
         k = short(r["Kernel_Name"], 400)
-        if not any(k.startswith(p) for p in prefixes.split(",")):
+        if not any(k.startswith(p) for p in prefixes.split(";")):
             continue
         grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
         block = int(r["Workgroup_Size_X"]) * int(r["Workgroup_Size_Y"]) * int(r["Workgroup_Size_Z"])
@@ -78,7 +78,7 @@ def step_lines(trace, marker):
     rows = list(csv.DictReader(open(trace)))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     out = []
-    for m in marker.split(","):
+    for m in marker.split(";"):
         idx = [i for i, r in enumerate(rows) if short(r["Kernel_Name"], 400).startswith(m)]
         if len(idx) < 2:
             continue
@@ -127,8 +127,8 @@ def main():
     ap.add_argument("out")
     ap.add_argument("--command", required=True, help="the command that produced the profile")
     ap.add_argument("--top", type=int, default=25)
-    ap.add_argument("--step-marker", default="", help="comma-separated kernel-name prefixes that start a step")
-    ap.add_argument("--by-grid", default="", help="comma-separated kernel-name prefixes to list per launch shape")
+    ap.add_argument("--step-marker", default="", help="kernel-name prefixes that start a step, separated by ';'")
+    ap.add_argument("--by-grid", default="", help="kernel-name prefixes to list per launch shape, separated by ';'")
     ap.add_argument("--drop", action="store_true", help="delete the CSVs after summarising")
     a = ap.parse_args()
     if a.mode == "stats":
