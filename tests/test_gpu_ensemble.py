@@ -1,5 +1,6 @@
-"""Lockstep ensembles (ecsprt_hip_ensemble_*): one host thread and stream, one launch per
-stage for all runs.  From the same state, every run's ions, moments and collision counts
+"""Lockstep ensembles (ecsprt_hip_ensemble_*): one host thread; the tile passes and the
+solve phases one launch for all runs, each run's small launches on its own stream between
+a fork and a join.  From the same state, every run's ions, moments and collision counts
 equal those of stepping its context alone (ecsprt_hip_step) bit for bit, and its
 potential meets the solve tolerance against the oracle's operator and forcing."""
 import numpy as np
