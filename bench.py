@@ -897,8 +897,10 @@ def main():
         "solve": {solve_name: {"ms": r["solve_total_ms"], "iterations": r["iters"],
                                "roofline": roofline(r["solve_bytes"], r["solve_total_ms"],
                                                     f"potential solve (rhs + GMRES/GMG, {solve_name}, all launches)",
+                                                    committed_traffic("solve_" + solve_name, wl.name, "all", args),
                                                     note="algorithmic bytes of every launch of one solve / the "
-                                                         "solve's span between HIP events")}},
+                                                         "solve's span between HIP events; traffic: calibrated HBM "
+                                                         "bytes of all the solve's launches")}},
     }
     if not args.no_mixed and not mixed_only:
         # The same workload on the mixed-precision path (FP32 particles, mixed solve).
@@ -928,7 +930,8 @@ def main():
         out["solve"]["mixed"] = {"ms": rm["solve_total_ms"], "iterations": rm["iters"],
                                  "speedup_vs_fp64": r["solve_total_ms"] / max(rm["solve_total_ms"], 1e-9),
                                  "roofline": roofline(rm["solve_bytes"], rm["solve_total_ms"],
-                                                      "potential solve (rhs + GMRES/GMG, mixed, all launches)")}
+                                                      "potential solve (rhs + GMRES/GMG, mixed, all launches)",
+                                                      committed_traffic("solve_mixed", wl.name, "all", args))}
     if tuned:
         out["autotune"] = tuned
     if not args.no_cpu:
