@@ -644,8 +644,9 @@ def leg_c4(args, pic, _lib, cores):
         "threads_ms_per_step": 1e3 * el_thr, "sequential_ms_per_step": 1e3 * el_seq,
         "speedup_vs_threads": el_thr / (el / nsteps), "speedup_vs_sequential": el_seq / (el / nsteps),
         "roofline": roofline(alg, float(stages[0]), "k_tile_mx_runs (one launch per tile shape: 4x4x4 and 8x8x8)",
+                             committed_traffic("k_tile_mx_runs", "ensemble_16_runs", "all", args),
                              note="algorithmic bytes of all runs' ions and nodes / the span of the step's tile-pass "
-                                  "launches (HIP events)"),
+                                  "launches (HIP events); traffic: calibrated HBM bytes of both launches"),
         "sequential_roofline": roofline(alg, float(kms.sum()), "k_tile_mx (16 launches per step, one per run)",
                                         note="the runs stepped one by one: sum of the runs' k_tile_mx durations"),
         "solve_roofline": roofline(sbytes_ens, float(stages[3]),
