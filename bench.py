@@ -496,7 +496,10 @@ def leg_c3(args, pic, _lib, cores):
                   "speedup_vs_fp64": fs / ms, "phi_rel_err_vs_fp64_max": max(errs),
                   "phi_rel_err_vs_fp64_median": float(np.median(errs))},
         "roofline": roofline(res["mixed"]["traffic"], 1e3 * ms, "batched solve, mixed (all launches)",
-                             note="algorithmic bytes of every launch of one batch pass / the pass's wall time"),
+                             committed_traffic("batch_mixed", "batched_solves_256_fields", "all", args)
+                             if nprob == 256 and args.slots == 0 else None,
+                             note="algorithmic bytes of every launch of one batch pass / the pass's wall time; "
+                                  "traffic: calibrated HBM bytes of the pass's launches"),
         "field_generation_s": gen_s,
     }
     if args.no_cpu:
